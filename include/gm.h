@@ -1,0 +1,163 @@
+/*
+ * gm.h - C ABI of the MI355X Crank-Nicolson step of the gravitational-molecule
+ * solver.
+ *
+ * Every entry point mirrors one member of the reference class
+ * CrankNicolsonModes (include/CrankNicolsonModes.hpp:60-455 of
+ * rdellamonica/boson-cloud-binary) and is cited beside its declaration. The
+ * handle owns the device state (psi, the Krylov work vectors, the Gaunt and
+ * radial tables, the preconditioner factors) on one GPU; the caller only ever
+ * passes plain host pointers and sizes.
+ *
+ * Complex arrays are interleaved (re, im) doubles, mode-major: element
+ * j*Nr + i is mode j at radial node i (include/CrankNicolsonModes.hpp:10-13).
+ *
+ * Status codes mirror the reference's exception types:
+ *   GM_OK, GM_E_INVALID (std::invalid_argument), GM_E_LOGIC (std::logic_error),
+ *   GM_E_RUNTIME (std::runtime_error), GM_E_DEVICE (HIP failure or a missing
+ *   device). gm_last_error() returns the message of the last failure.
+ *
+ * Threading: one host thread drives a handle; handles are independent.
+ */
+#ifndef GM_H
+#define GM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GM_OK = 0,
+    GM_E_INVALID = 1,
+    GM_E_LOGIC = 2,
+    GM_E_RUNTIME = 3,
+    GM_E_DEVICE = 4
+};
+
+enum {
+    GM_PRECISION_FP64 = 0,  /* plain FP64 drop-in */
+    GM_PRECISION_MIXED = 1  /* FP32 coupling / preconditioner inside FP64 BiCGSTAB */
+};
+
+typedef struct gm_solver gm_solver;
+
+/* Constructor arguments, CrankNicolsonModes::CrankNicolsonModes
+ * (include/CrankNicolsonModes.hpp:84, src/CrankNicolsonModes.cpp:11-50). */
+typedef struct gm_params {
+    int32_t Nr;
+    double L;
+    double dt;
+    double epsilon;
+    int32_t l_max;
+    double mu;
+    double M_c;
+    double q;
+    double a;
+    double R;
+    int32_t counter;
+    int32_t absorption;
+    int32_t save_every;
+    double ramp_periods;
+    double epsilon_smear;
+} gm_params;
+
+/* Public flags of the class (include/CrankNicolsonModes.hpp:137-156) plus the
+ * rolling-average window (set_power_average_orbits, :153) and the arithmetic
+ * mode of this build. */
+typedef struct gm_flags {
+    int32_t evolve_orbit;             /* default 1 */
+    int32_t ramp_secondary_potential; /* default 1 */
+    double steps_per_period;          /* default 50 */
+    int32_t use_block_preconditioner; /* default 1 */
+    double power_average_orbits;      /* default 2 */
+    int32_t precision;                /* GM_PRECISION_FP64 (default) or _MIXED */
+} gm_flags;
+
+/* Orbit state: the 13 doubles of the checkpoint (restart.cpp:153-163) in the
+ * same order, followed by solver statistics (include/CrankNicolsonModes.hpp:157-158). */
+typedef struct gm_scalars {
+    double t, theta, phase, R, M1, M2, q, R1, R2, Omega, P, dt, p_bin;
+    double Omega_0;
+    double last_solver_residual;
+    int32_t last_solver_iterations;
+    int32_t sink_active;
+} gm_scalars;
+
+/* Diagnostics appended by one step (src/CrankNicolsonModes.cpp:197-226,
+ * src/CrankNicolsonModes_extraction.cpp:8-129). The per-mode arrays are
+ * caller-owned, of length n_modes, and may be NULL. */
+typedef struct gm_diag {
+    double energy;       /* get_energy */
+    double p_bin;        /* rolling average written to p_bin.txt */
+    double p_bin_inst;   /* get_binary_power */
+    double dV_bin_dt;    /* get_dVexpectation_dt */
+    double cloud_mass;   /* mu * dr * |psi|^2 after the step */
+    double absorbed;     /* mu * dr * dt * Re<mid|Gamma|mid>, CN sink balance */
+    double *flux;        /* get_flux */
+    double *energy_flux; /* get_energy_flux */
+    double *norms;       /* get_norms */
+    double *energies;    /* get_energy_per_mode */
+} gm_diag;
+
+/* Library version string and whether a GPU is usable. */
+const char *gm_version(void);
+int gm_device_count(void);
+
+/* CrankNicolsonModes::CrankNicolsonModes (include/CrankNicolsonModes.hpp:84). */
+int gm_create(const gm_params *params, gm_solver **out);
+/* ~CrankNicolsonModes. */
+void gm_destroy(gm_solver *solver);
+/* Message of the last failed call on this handle (or of gm_create when solver is NULL). */
+const char *gm_last_error(const gm_solver *solver);
+
+/* n_modes and the (l, m) list (include/CrankNicolsonModes.hpp:91,174);
+ * lm receives 2*n_modes int32. */
+int gm_n_modes(const gm_solver *solver);
+int gm_modes(const gm_solver *solver, int32_t *lm);
+/* r (include/CrankNicolsonModes.hpp:161); receives Nr doubles. */
+int gm_grid(const gm_solver *solver, double *r);
+
+/* Public flags (include/CrankNicolsonModes.hpp:137-156) and set_power_average_orbits (:153). */
+int gm_set_flags(gm_solver *solver, const gm_flags *flags);
+int gm_get_flags(const gm_solver *solver, gm_flags *flags);
+
+/* configure_isolated_black_hole (include/CrankNicolsonModes.hpp:193). */
+int gm_configure_isolated_black_hole(gm_solver *solver, double x);
+/* set_initial_state_full (include/CrankNicolsonModes.hpp:171): n = Nr*n_modes
+ * complex values; the state is scaled by sqrt(M_c/mu). */
+int gm_set_initial_state_full(gm_solver *solver, const double *psi, int64_t n);
+/* initialize (include/CrankNicolsonModes.hpp:183). */
+int gm_initialize(gm_solver *solver);
+/* build_damping_mask / build_cap (include/CrankNicolsonModes.hpp:230,238). */
+int gm_build_damping_mask(gm_solver *solver, double L_factor, double W, int32_t n);
+int gm_build_cap(gm_solver *solver, double L_factor, double W, int32_t n);
+
+/* The compute part of run() (src/CrankNicolsonModes.cpp:236-259) without its
+ * file output: set_extraction_radius, t = 0, and the refresh of H(0). */
+int gm_begin_run(gm_solver *solver, double r_ex, int32_t n_steps);
+/* step (src/CrankNicolsonModes.cpp:145-234) without the text output; the
+ * diagnostics that step() writes are returned in diag (may be NULL). */
+int gm_step(gm_solver *solver, int32_t step_id, gm_diag *diag);
+
+/* psi (include/CrankNicolsonModes.hpp:162): read, or overwrite without scaling. */
+int gm_get_state(const gm_solver *solver, double *psi);
+int gm_set_state(gm_solver *solver, const double *psi);
+/* Orbit scalars: read, or restore for resume (restart.cpp:179-233). */
+int gm_get_scalars(const gm_solver *solver, gm_scalars *out);
+int gm_set_scalars(gm_solver *solver, const gm_scalars *in);
+
+/* Operators on host buffers of Nr*n_modes complex values, with the current H:
+ * H*psi (include/CrankNicolsonModes.hpp:197), apply_sink (:210),
+ * schrodinger_rhs (:213) and sink_strength (:216). */
+int gm_apply_H(gm_solver *solver, const double *in, double *out);
+int gm_apply_sink(gm_solver *solver, const double *in, double *out);
+int gm_schrodinger_rhs(gm_solver *solver, const double *in, double *out);
+double gm_sink_strength(const gm_solver *solver, int32_t body);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GM_H */
