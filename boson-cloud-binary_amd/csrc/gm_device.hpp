@@ -1,0 +1,87 @@
+// gm_device.hpp - data shared between the host solver and the gfx950 kernels.
+//
+// Layout in HBM (one solver):
+//   complex vectors   K*Nr double2, mode-major (mode j at node i is j*Nr + i),
+//                     the reference's stacking (include/CrankNicolsonModes.hpp:10-13);
+//   E table           2 * (2 l_max + 1) * Nr doubles: the radial multipole
+//                     profiles c_l* (f1 + sign prf f2) for even (index 0) and
+//                     odd (index 1) m*-parity pairs;
+//   G table           per (l, l1) block, [m][m1][s] doubles: the Gaunt
+//                     coefficient of modes (l, m), (l1, m1) at l* = |l-l1| + 2s;
+//   preconditioner    K*Nr double2 inverse pivots of each mode's radial tridiagonal.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gmd {
+
+constexpr int TILE = 64;   // radial nodes per wavefront in the operator apply
+constexpr int CH = 8;      // rows (modes of one l) per wavefront in the operator apply
+constexpr int MAXLM = 40;  // largest supported l_max + 1
+constexpr int PT = 256;    // threads of a preconditioner workgroup
+constexpr int PE = 8;      // consecutive nodes per preconditioner thread
+constexpr int RED_BLOCKS = 1024;  // max partial-sum slots per reduction
+
+struct Chunk {
+    int l, m0, cnt, pad;
+};
+
+// Constant geometry and tables of one solver (device pointers).
+struct Geometry {
+    int N, K, lmax, Ls;        // Ls = 2 l_max + 1
+    int ntiles, nchunks;       // node tiles of TILE, row chunks
+    double dr, eps, mu, off;   // off = -1/(2 mu dr^2), the stencil off-diagonal of H0
+    const Chunk *chunks;
+    const int *goff;           // (l_max+1)^2 offsets into G
+    const double *G;
+    const double *E;           // 2 * Ls * N
+    const double2 *cap;        // Nr diagonal CAP or nullptr
+};
+
+// One s-wave sink restricted to its radial support [lo, lo+n).
+struct SinkDev {
+    int lo, n;
+    const double2 *v;  // K * n (mode-major over the support)
+    const double2 *W;  // full-length P^-1 v, or nullptr
+    int wlo, whi;      // nodes where W can be non-zero
+};
+
+// Per-step scalars, resident in device memory so that a captured graph can
+// replay a step with new values.
+struct StepParams {
+    double2 ph[2 * MAXLM + 1];  // exp(i m theta), index m + l_max
+    double half_dt;             // dt / 2
+    double dt;
+    double g[2];                // sink strengths g_a (0 when inactive)
+    int nsink;                  // active sinks in the solve (0..2), in order
+    int sink_body[2];           // body (0 or 1) of each active sink
+    int idx_ex;                 // extraction index
+    int pad;
+};
+
+// BiCGSTAB state, device resident (src/CrankNicolsonModes_matrices.cpp:47-91).
+struct Krylov {
+    double2 rho, rho_old, alpha, w, beta;
+    double r2, tol2, r0_sq, restart_thr;
+    double rhs_nrm;             // |rhs| of the CN system
+    double res2;                // |A x - b|^2 of the acceptance check
+    double2 c[2];               // Woodbury coefficients of the last preconditioner apply
+    double2 coefA[2];           // rank-one coefficients (dt/4) g_a (v_a^H x) for the next apply
+    double2 gq[2];              // g_a (v_a^H psi) for Gamma psi
+    double2 qold[2];            // v_a^H psi at the start of the step
+    double2 qnew[2];            // v_a^H psi after the solve, before the mask
+    double2 qpost[2];           // v_a^H psi after the mask
+    double2 Ginv[4];            // inverse of the k x k Woodbury capacitance
+    double2 G[4];
+    int wrange[4];              // [lo, hi) of each Woodbury column W_a
+    int iter, maxit, done, restart, eigen_mode, restarts, zero_rhs, accepted;
+    unsigned counter;
+    int pad;
+};
+
+// Output of the diagnostics reduction.
+struct DiagOut {
+    double energy, pbin_sum, dv, mass;
+};
+
+}  // namespace gmd

@@ -1,0 +1,1048 @@
+// gm_kernels.hip - gfx950 kernels of the Crank-Nicolson step.
+//
+// Kernels (all FP64, complex values as double2):
+//   k_build_E   radial multipole profiles (src/CrankNicolsonModes_hamiltonian.cpp:12-42,179-185)
+//   k_apply     H, A = 1 + i dt/2 H and their fused reductions: three-point
+//               radial stencil plus the Gaunt coupling formed per node from the
+//               E and G tables (hamiltonian.cpp:44-66,176-243, matrices.cpp:27-40)
+//   k_factor    Thomas pivots of each mode's radial tridiagonal
+//               (include/RadialBlockPreconditioner.hpp:43-76)
+//   k_precond   forward/backward substitution as tile scans of affine maps,
+//               fused with the BiCGSTAB p / s updates and the Woodbury sink
+//               correction (include/RadialBlockPreconditioner.hpp:79-97,
+//               matrices.cpp:128-157)
+//   k_update    x += alpha y + w z, r = s - w t with fused dot products and the
+//               loop control of matrices.cpp:65-89 / Eigen BiCGSTAB
+//   k_sink      projections v_a^H x over the absorber supports (absorption.cpp:100-125)
+//   k_diag_final diagnostics of extraction.cpp:8-129
+//
+// Reductions are deterministic: every block writes one partial per quantity
+// and the last block to finish sums the partials in block order.
+#include "gm_kernels.hpp"
+
+#include <cmath>
+
+namespace gmk {
+
+using gmd::Chunk;
+using gmd::Geometry;
+using gmd::Krylov;
+using gmd::SinkDev;
+using gmd::StepParams;
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 cdotm(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b)
+{
+    const double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double cnorm2(double2 a) { return a.x * a.x + a.y * a.y; }
+// i * h * a for real h
+__device__ __forceinline__ double2 imul(double h, double2 a) { return make_double2(-h * a.y, h * a.x); }
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ double vload(const double *p) { return *(const volatile double *)p; }
+
+__device__ __forceinline__ double2 vload2(const double2 *p)
+{
+    const volatile double *q = (const volatile double *)p;
+    return make_double2(q[0], q[1]);
+}
+
+// Sum NQ per-thread values over the block; every block stores its totals in
+// partial[q * nblocks + block]. Returns true in the last block to finish, whose
+// threads then see every partial. blockDim.x must be a multiple of 64, <= 1024.
+template <int NQ>
+__device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned *counter)
+{
+    __shared__ double ws[16][NQ];
+    __shared__ bool last;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) ws[wv][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NQ; ++q) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += ws[w][q];
+            partial[(size_t)q * gridDim.x + blockIdx.x] = s;
+        }
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// In the last block: sum the partials of quantity q in block order.
+__device__ double sum_partials(const double *partial, int q)
+{
+    __shared__ double ws[16];
+    const int nb = gridDim.x;
+    double s = 0.0;
+    // contiguous slice per thread keeps the order fixed for a fixed grid
+    const int per = (nb + blockDim.x - 1) / blockDim.x;
+    const int b0 = threadIdx.x * per;
+    for (int b = b0; b < b0 + per && b < nb; ++b) s += vload(partial + (size_t)q * nb + b);
+    // fixed-order combine across threads
+    s = wave_sum(s);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) ws[wv] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += ws[w];
+    __syncthreads();
+    return t;
+}
+
+// ---------------------------------------------------------------- E table
+
+__global__ void k_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
+                          double mu, double prf, double *E)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Ls * N) return;
+    const int ls = idx / N, i = idx - ls * N;
+    double e0 = 0.0, e1 = 0.0;
+    if (i > 0 && i < N - 1) {
+        const double r = eps + i * dr;
+        double f1, f2;
+        if (r >= R1) f1 = 1 / (1 + q) * pow(R1, (double)ls) / pow(r, (double)(ls + 1));
+        else f1 = 1 / (1 + q) * pow(r, (double)ls) / pow(R1, (double)(ls + 1));
+        if (r >= R2) f2 = q / (1 + q) * pow(R2, (double)ls) / pow(r, (double)(ls + 1));
+        else f2 = q / (1 + q) * pow(r, (double)ls) / pow(R2, (double)(ls + 1));
+        const double c = -mu * Mt * 4.0 * M_PI / (2.0 * ls + 1.0);
+        e0 = c * (f1 + prf * f2);
+        e1 = c * (f1 - prf * f2);
+    }
+    E[(size_t)ls * N + i] = e0;
+    E[(size_t)(Ls + ls) * N + i] = e1;
+}
+
+// ---------------------------------------------------------------- operator apply
+
+template <bool CORR>
+__device__ __forceinline__ double2 load_x(const ApplyArgs &a, const Krylov *Kr, int nsink, const double2 (&c)[2],
+                                          int k, int i)
+{
+    double2 x = a.X[(size_t)k * a.g.N + i];
+    if (CORR) {
+        for (int s = 0; s < nsink; ++s) {
+            const int wlo = Kr->wrange[2 * s], whi = Kr->wrange[2 * s + 1];
+            if (i >= wlo && i < whi) x = csub(x, cmul(c[s], a.sk[s].W[(size_t)k * a.g.N + i]));
+        }
+    }
+    return x;
+}
+
+// Rank-one sink term sum_a coef_a v_a at (mode j, node i).
+__device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, const double2 (&coef)[2], int j, int i)
+{
+    double2 t = make_double2(0.0, 0.0);
+    for (int s = 0; s < nsink; ++s) {
+        const SinkDev &sk = a.sk[s];
+        const int d = i - sk.lo;
+        if (d >= 0 && d < sk.n) t = cadd(t, cmul(coef[s], sk.v[(size_t)j * sk.n + d]));
+    }
+    return t;
+}
+
+template <int SMAX, int MODE, bool CORR>
+__global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
+{
+    const Geometry &g = a.g;
+    Krylov *Kr = a.Kr;
+    if ((MODE == AM_V || MODE == AM_T) && Kr->done) return;
+    const StepParams *P = a.P;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * a.per + (b >> 3);
+    const int item = lb * 4 + wv;
+    const int N = g.N, lmax = g.lmax;
+    const int nsink = P->nsink;
+    double2 cw[2], coef[2];
+    for (int s = 0; s < 2; ++s) {
+        cw[s] = Kr->c[s];
+        coef[s] = (MODE == AM_RHS || MODE == AM_DIAG) ? Kr->gq[s] : Kr->coefA[s];
+    }
+    double red[3] = {0.0, 0.0, 0.0};
+    if (item < a.nitems) {
+        const int tile = item / g.nchunks, chn = item - tile * g.nchunks;
+        const Chunk C = g.chunks[chn];
+        const int l = C.l;
+        const int i = tile * gmd::TILE + lane;
+        const bool valid = i < N;
+        const int ic = valid ? i : N - 1;
+        double2 acc[gmd::CH];
+#pragma unroll
+        for (int c = 0; c < gmd::CH; ++c) acc[c] = make_double2(0.0, 0.0);
+        for (int l1 = 0; l1 <= lmax; ++l1) {
+            const int S = (l < l1 ? l : l1) + 1;
+            const int lo = l > l1 ? l - l1 : l1 - l;
+            const int p = (l + l1) & 1;
+            double e[SMAX];
+#pragma unroll
+            for (int s = 0; s < SMAX; ++s)
+                e[s] = s < S ? g.E[(size_t)(p * g.Ls + lo + 2 * s) * N + ic] : 0.0;
+            const double *Gb = g.G + g.goff[l * (lmax + 1) + l1];
+            const int kb = l1 * (l1 + 1) / 2;
+            const int rs = (l1 + 1) * S;
+            for (int m1i = 0; m1i <= l1; ++m1i) {
+                const double2 xv = load_x<CORR>(a, Kr, nsink, cw, kb + m1i, ic);
+                const double2 xr = cmul(P->ph[lmax - l1 + 2 * m1i], xv);
+                const double *Gr = Gb + (size_t)(C.m0 * (l1 + 1) + m1i) * S;
+#pragma unroll
+                for (int c = 0; c < gmd::CH; ++c) {
+                    if (c < C.cnt) {
+                        const double *Gc = Gr + (size_t)c * rs;
+                        double w = 0.0;
+#pragma unroll
+                        for (int s = 0; s < SMAX; ++s)
+                            if (s < S) w = fma(Gc[s], e[s], w);
+                        acc[c].x = fma(w, xr.x, acc[c].x);
+                        acc[c].y = fma(w, xr.y, acc[c].y);
+                    }
+                }
+            }
+        }
+        // epilogue per row
+        const double alpha2 = 1.0 / (g.dr * g.dr);
+        const double hdt = P->half_dt;
+        const bool interior = i > 0 && i < N - 1;
+        const double2 capv = g.cap && valid ? g.cap[i] : make_double2(0.0, 0.0);
+        double rp[6] = {0, 0, 0, 0, 0, 0};
+        const int ix = P->idx_ex;
+#pragma unroll
+        for (int c = 0; c < gmd::CH; ++c) {
+            if (c >= C.cnt) continue;
+            const int mi = C.m0 + c;
+            const int m = -l + 2 * mi;
+            const int j = l * (l + 1) / 2 + mi;
+            const double2 Cc = cdotm(P->ph[m + lmax], acc[c]);
+            if (valid) {
+                const size_t row = (size_t)j * N + i;
+                const double2 xi = load_x<CORR>(a, Kr, nsink, cw, j, i);
+                if (CORR) a.Xc[row] = xi;
+                double2 Hx;
+                if (interior) {
+                    const double ri = g.eps + i * g.dr;
+                    const double h0d = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * g.mu);
+                    const double2 xm = load_x<CORR>(a, Kr, nsink, cw, j, i - 1);
+                    const double2 xp = load_x<CORR>(a, Kr, nsink, cw, j, i + 1);
+                    Hx = cadd(cscale(g.off, cadd(xm, xp)), cadd(cmul(make_double2(h0d + capv.x, capv.y), xi), Cc));
+                } else {
+                    Hx = cmul(capv, xi);
+                }
+                if (MODE == AM_H) {
+                    a.Y[row] = Hx;
+                } else if (MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_A) {
+                    double2 out = cadd(xi, imul(hdt, Hx));
+                    if (nsink) out = cadd(out, sink_term(a, nsink, coef, j, i));
+                    if (MODE == AM_V) {
+                        a.Y[row] = out;
+                        const double2 d = cdotm(a.R0[row], out);
+                        red[0] += d.x;
+                        red[1] += d.y;
+                    } else if (MODE == AM_T) {
+                        a.Y[row] = out;
+                        const double2 d = cdotm(out, a.R0[row]);
+                        red[0] += d.x;
+                        red[1] += d.y;
+                        red[2] += cnorm2(out);
+                    } else if (MODE == AM_A) {
+                        a.Y[row] = out;
+                    } else {
+                        red[0] += cnorm2(csub(out, a.R0[row]));
+                    }
+                } else if (MODE == AM_RHS) {
+                    const double2 sHx = imul(hdt, Hx);
+                    double2 st = make_double2(0.0, 0.0);
+                    if (nsink) st = cscale(0.5 * hdt, sink_term(a, nsink, coef, j, i));  // (dt/4) Gamma psi
+                    const double2 rhs = csub(csub(xi, sHx), st);
+                    a.Y[row] = rhs;
+                    a.Y2[row] = cadd(cadd(xi, sHx), st);
+                    red[0] += cnorm2(rhs);
+                } else if (MODE == AM_DIAG) {
+                    double2 dpsi = make_double2(Hx.y, -Hx.x);  // -i H psi
+                    if (nsink) dpsi = csub(dpsi, cscale(0.5, sink_term(a, nsink, coef, j, i)));
+                    const double2 eh = cdotm(xi, Hx);
+                    const double n2 = cnorm2(xi);
+                    if (i < ix) {
+                        rp[0] += n2;
+                        rp[1] += eh.x;
+                    }
+                    if (i <= ix) {
+                        rp[2] += eh.x;
+                        rp[3] += m * cdotm(xi, Cc).y;
+                        rp[4] += cdotm(dpsi, Cc).x;
+                    }
+                    rp[5] += n2;
+                    if (i == ix) a.dpsi_ix[j] = dpsi;
+                }
+            }
+            if (MODE == AM_DIAG) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const double s = wave_sum(rp[q]);
+                    if (lane == 0) a.rowpart[((size_t)tile * g.K + j) * 6 + q] = s;
+                    rp[q] = 0.0;
+                }
+            }
+        }
+    }
+    if (MODE == AM_V) {
+        if (block_partials<2>({red[0], red[1]}, a.partial, &Kr->counter)) {
+            const double dr_ = sum_partials(a.partial, 0), di = sum_partials(a.partial, 1);
+            if (threadIdx.x == 0) {
+                Kr->alpha = cdiv(Kr->rho, make_double2(dr_, di));
+                Kr->counter = 0;
+            }
+        }
+    } else if (MODE == AM_T) {
+        if (block_partials<3>({red[0], red[1], red[2]}, a.partial, &Kr->counter)) {
+            const double tr = sum_partials(a.partial, 0), ti = sum_partials(a.partial, 1);
+            const double tt = sum_partials(a.partial, 2);
+            if (threadIdx.x == 0) {
+                Kr->w = tt > 0.0 ? make_double2(tr / tt, ti / tt) : make_double2(0.0, 0.0);
+                Kr->counter = 0;
+            }
+        }
+    } else if (MODE == AM_RHS) {
+        if (block_partials<1>({red[0]}, a.partial, &Kr->counter)) {
+            const double s = sum_partials(a.partial, 0);
+            if (threadIdx.x == 0) {
+                Kr->rhs_nrm = sqrt(s);
+                Kr->counter = 0;
+            }
+        }
+    } else if (MODE == AM_RES) {
+        if (block_partials<1>({red[0]}, a.partial, &Kr->counter)) {
+            const double s = sum_partials(a.partial, 0);
+            if (threadIdx.x == 0) {
+                Kr->res2 = s;
+                Kr->counter = 0;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- Thomas factor
+
+__global__ void k_factor(FactorArgs f)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= f.K) return;
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= j) ++l;
+    const int mi = j - l * (l + 1) / 2;
+    const double *Gjj = f.G + f.goff[l * (f.lmax + 1) + l] + (size_t)(mi * (l + 1) + mi) * (l + 1);
+    const double alpha2 = 1.0 / (f.dr * f.dr);
+    const double2 ell = make_double2(0.0, f.half_dt * f.off);
+    double2 prev = make_double2(0.0, 0.0);
+    int failed = 0;
+    const int N = f.N;
+    for (int i = 0; i < N; ++i) {
+        double2 h = make_double2(0.0, 0.0);
+        if (i > 0 && i < N - 1) {
+            const double ri = f.eps + i * f.dr;
+            double v = 0.0;
+            for (int s = 0; s <= l; ++s) v += Gjj[s] * f.E[(size_t)(2 * s) * N + i];
+            h.x = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * f.mu) + v;
+        }
+        if (f.cap) h = cadd(h, f.cap[i]);
+        double2 piv = make_double2(1.0 - f.half_dt * h.y, f.half_dt * h.x);
+        if (i >= 2 && i <= N - 2) piv = csub(piv, cmul(cmul(ell, prev), ell));
+        const double ap = sqrt(cnorm2(piv));
+        double2 inv;
+        if (ap == 0.0 || !isfinite(ap)) {
+            failed = 1;
+            inv = make_double2(0.0, 0.0);
+        } else {
+            inv = cdiv(make_double2(1.0, 0.0), piv);
+        }
+        f.inv[(size_t)j * N + i] = inv;
+        prev = inv;
+    }
+    if (failed) atomicOr(f.failed, 1);
+}
+
+// ---------------------------------------------------------------- Thomas solve
+
+struct Aff {
+    double2 A, F;
+};
+
+// first, then second
+__device__ __forceinline__ Aff compose(const Aff &first, const Aff &second)
+{
+    Aff r;
+    r.A = cmul(second.A, first.A);
+    r.F = cadd(cmul(second.A, first.F), second.F);
+    return r;
+}
+
+__device__ __forceinline__ Aff shfl_aff(const Aff &v, int src)
+{
+    Aff r;
+    r.A.x = __shfl(v.A.x, src);
+    r.A.y = __shfl(v.A.y, src);
+    r.F.x = __shfl(v.F.x, src);
+    r.F.y = __shfl(v.F.y, src);
+    return r;
+}
+
+// Exclusive scan of affine maps over the PT threads of a block in the order
+// of increasing (REV = false) or decreasing (REV = true) thread index.
+// Returns the thread's exclusive prefix; `total` receives the block total.
+template <bool REV>
+__device__ Aff block_scan(Aff v, Aff &total)
+{
+    __shared__ Aff wt[gmd::PT / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int vl = REV ? 63 - lane : lane;  // position in scan order within the wave
+    Aff inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int src = REV ? lane + d : lane - d;
+        const Aff o = shfl_aff(inc, src < 0 ? 0 : (src > 63 ? 63 : src));
+        if (vl >= d) inc = compose(o, inc);
+    }
+    Aff ex = shfl_aff(inc, REV ? (lane < 63 ? lane + 1 : 63) : (lane > 0 ? lane - 1 : 0));
+    if (vl == 0) {
+        ex.A = make_double2(1.0, 0.0);
+        ex.F = make_double2(0.0, 0.0);
+    }
+    if (vl == 63) wt[wv] = inc;
+    __syncthreads();
+    const int nw = gmd::PT / 64;
+    Aff pre;
+    pre.A = make_double2(1.0, 0.0);
+    pre.F = make_double2(0.0, 0.0);
+    Aff tot = pre;
+    for (int k = 0; k < nw; ++k) {
+        const int w = REV ? nw - 1 - k : k;
+        if (w == wv) pre = tot;
+        tot = compose(tot, wt[w]);
+    }
+    __syncthreads();
+    total = tot;
+    return compose(pre, ex);
+}
+
+__device__ __forceinline__ int lds_slot(int pos) { return pos + pos / gmd::PE; }
+
+template <int VAR>
+__global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
+{
+    Krylov *Kr = a.Kr;
+    if (VAR != PV_PLAIN && Kr->done) return;
+    constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
+    __shared__ double2 sa[TE + TE / E];
+    __shared__ double2 sf[TE + TE / E];
+    const int j = blockIdx.x;
+    const int N = a.N;
+    const int tid = threadIdx.x;
+    const size_t base_j = (size_t)j * N;
+    const double2 ell = make_double2(0.0, a.half_dt * a.off);
+    const int ntile = (N + TE - 1) / TE;
+    double2 beta = make_double2(0, 0), wv = make_double2(0, 0), alpha = make_double2(0, 0);
+    if (VAR == PV_P) {
+        beta = Kr->beta;
+        wv = Kr->w;
+    }
+    if (VAR == PV_S) alpha = Kr->alpha;
+    // forward sweep
+    double2 carry = make_double2(0.0, 0.0);
+    for (int t = 0; t < ntile; ++t) {
+        const int t0 = t * TE;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int pos = tid + T * e;
+            const int idx = t0 + pos;
+            double2 av = make_double2(1.0, 0.0), fv = make_double2(0.0, 0.0);
+            if (idx < N) {
+                const size_t q = base_j + idx;
+                double2 in;
+                if (VAR == PV_P) {
+                    in = cadd(a.R[q], cmul(beta, csub(a.Pv[q], cmul(wv, a.V[q]))));
+                    a.Pv[q] = in;
+                } else if (VAR == PV_S) {
+                    in = csub(a.R[q], cmul(alpha, a.V[q]));
+                    a.Pv[q] = in;
+                } else {
+                    in = a.X[q];
+                }
+                const double2 iv = a.inv[q];
+                fv = cmul(iv, in);
+                av = (idx >= 1 && idx <= N - 2) ? make_double2(-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x))
+                                                : make_double2(0.0, 0.0);
+            }
+            sa[lds_slot(pos)] = av;
+            sf[lds_slot(pos)] = fv;
+        }
+        __syncthreads();
+        Aff loc;
+        loc.A = make_double2(1.0, 0.0);
+        loc.F = make_double2(0.0, 0.0);
+        double2 ae[E], fe[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            ae[e] = sa[lds_slot(tid * E + e)];
+            fe[e] = sf[lds_slot(tid * E + e)];
+            loc.A = cmul(ae[e], loc.A);
+            loc.F = cadd(cmul(ae[e], loc.F), fe[e]);
+        }
+        Aff tot;
+        const Aff ex = block_scan<false>(loc, tot);
+        double2 y = cadd(cmul(ex.A, carry), ex.F);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            y = cadd(cmul(ae[e], y), fe[e]);
+            sf[lds_slot(tid * E + e)] = y;
+        }
+        carry = cadd(cmul(tot.A, carry), tot.F);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int pos = tid + T * e;
+            const int idx = t0 + pos;
+            if (idx < N) a.Y[base_j + idx] = sf[lds_slot(pos)];
+        }
+        __syncthreads();
+    }
+    // backward sweep
+    carry = make_double2(0.0, 0.0);
+    for (int t = ntile - 1; t >= 0; --t) {
+        const int t0 = t * TE;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int pos = tid + T * e;
+            const int idx = t0 + pos;
+            double2 cv = make_double2(1.0, 0.0), fv = make_double2(0.0, 0.0);
+            if (idx < N) {
+                const size_t q = base_j + idx;
+                const double2 iv = a.inv[q];
+                fv = a.Y[q];
+                cv = (idx >= 1 && idx <= N - 2) ? make_double2(-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x))
+                                                : make_double2(0.0, 0.0);
+            }
+            sa[lds_slot(pos)] = cv;
+            sf[lds_slot(pos)] = fv;
+        }
+        __syncthreads();
+        Aff loc;
+        loc.A = make_double2(1.0, 0.0);
+        loc.F = make_double2(0.0, 0.0);
+        double2 ce[E], fe[E];
+#pragma unroll
+        for (int e = E - 1; e >= 0; --e) {
+            ce[e] = sa[lds_slot(tid * E + e)];
+            fe[e] = sf[lds_slot(tid * E + e)];
+            loc.A = cmul(ce[e], loc.A);
+            loc.F = cadd(cmul(ce[e], loc.F), fe[e]);
+        }
+        Aff tot;
+        const Aff ex = block_scan<true>(loc, tot);
+        double2 y = cadd(cmul(ex.A, carry), ex.F);
+#pragma unroll
+        for (int e = E - 1; e >= 0; --e) {
+            y = cadd(cmul(ce[e], y), fe[e]);
+            sf[lds_slot(tid * E + e)] = y;
+        }
+        carry = cadd(cmul(tot.A, carry), tot.F);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int pos = tid + T * e;
+            const int idx = t0 + pos;
+            if (idx < N) a.Y[base_j + idx] = sf[lds_slot(pos)];
+        }
+        __syncthreads();
+    }
+    if (!a.wood) return;
+    // Woodbury epilogue in the last block (matrices.cpp:146-153):
+    // c = Ginv (V^H y), then the rank-one coefficients of A applied to
+    // y - W c: coefA_a = (dt/4) g_a v_a^H (y - W c).
+    __shared__ bool last;
+    if (tid == 0) {
+        __threadfence();
+        last = atomicAdd(&Kr->counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const StepParams *P = a.Ps;
+    const int ns = P->nsink;
+    double v[4] = {0, 0, 0, 0};
+    for (int s = 0; s < ns; ++s) {
+        const SinkDev &sk = a.sk[s];
+        const int tot = a.K * sk.n;
+        for (int q = tid; q < tot; q += T) {
+            const int k = q / sk.n, d = q - k * sk.n;
+            const double2 d2 = cdotm(sk.v[q], vload2(a.Y + (size_t)k * N + sk.lo + d));
+            v[2 * s] += d2.x;
+            v[2 * s + 1] += d2.y;
+        }
+    }
+    __shared__ double red[4];
+    __shared__ double ws[gmd::PT / 64][4];
+    for (int q = 0; q < 4; ++q) {
+        const double s = wave_sum(v[q]);
+        if ((tid & 63) == 0) ws[tid >> 6][q] = s;
+    }
+    __syncthreads();
+    if (tid == 0)
+        for (int q = 0; q < 4; ++q) {
+            double s = 0;
+            for (int w = 0; w < T / 64; ++w) s += ws[w][q];
+            red[q] = s;
+        }
+    __syncthreads();
+    double2 proj[2] = {make_double2(red[0], red[1]), make_double2(red[2], red[3])};
+    double2 c[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int s = 0; s < ns; ++s)
+        for (int u = 0; u < ns; ++u) c[s] = cadd(c[s], cmul(Kr->Ginv[s * 2 + u], proj[u]));
+    __syncthreads();
+    for (int q = 0; q < 4; ++q) v[q] = 0.0;
+    for (int s = 0; s < ns; ++s) {
+        const SinkDev &sk = a.sk[s];
+        const int tot = a.K * sk.n;
+        for (int q = tid; q < tot; q += T) {
+            const int k = q / sk.n, d = q - k * sk.n;
+            const size_t gi = (size_t)k * N + sk.lo + d;
+            double2 y = vload2(a.Y + gi);
+            for (int u = 0; u < ns; ++u) y = csub(y, cmul(c[u], a.sk[u].W[gi]));
+            const double2 d2 = cdotm(sk.v[q], y);
+            v[2 * s] += d2.x;
+            v[2 * s + 1] += d2.y;
+        }
+    }
+    for (int q = 0; q < 4; ++q) {
+        const double s = wave_sum(v[q]);
+        if ((tid & 63) == 0) ws[tid >> 6][q] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 0; q < 4; ++q) {
+            double s = 0;
+            for (int w = 0; w < T / 64; ++w) s += ws[w][q];
+            red[q] = s;
+        }
+        for (int s = 0; s < ns; ++s) {
+            Kr->c[s] = c[s];
+            Kr->coefA[s] = cscale(0.5 * P->half_dt * P->g[s], make_double2(red[2 * s], red[2 * s + 1]));
+        }
+        Kr->counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------- BiCGSTAB vector update
+
+__device__ void loop_top(Krylov *Kr, double r2, double2 rho_new)
+{
+    // while (r^2 > tol^2 |b|^2 && i < maxit): rho_old = rho; rho = r0.r; restart check; beta
+    Kr->r2 = r2;
+    if (!(r2 > Kr->tol2 && Kr->iter < Kr->maxit)) {
+        Kr->done = 1;
+        return;
+    }
+    Kr->rho_old = Kr->rho;
+    Kr->rho = rho_new;
+    const double thr = Kr->eigen_mode ? 4.930380657631324e-32 * Kr->r0_sq : Kr->restart_thr;
+    if (sqrt(cnorm2(rho_new)) < thr) {
+        Kr->restart = 1;
+        Kr->done = 1;
+        return;
+    }
+    Kr->beta = cmul(cdiv(Kr->rho, Kr->rho_old), cdiv(Kr->alpha, Kr->w));
+}
+
+__global__ __launch_bounds__(256) void k_update(UpdateArgs a)
+{
+    Krylov *Kr = a.Kr;
+    if (Kr->done) return;
+    const double2 al = Kr->alpha, w = Kr->w;
+    double v[3] = {0, 0, 0};
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (size_t)gridDim.x * blockDim.x) {
+        const double2 x = cadd(a.X[q], cadd(cmul(al, a.Yc[q]), cmul(w, a.Zc[q])));
+        a.X[q] = x;
+        const double2 r = csub(a.S[q], cmul(w, a.T[q]));
+        a.R[q] = r;
+        const double2 d = cdotm(a.R0[q], r);
+        v[0] += d.x;
+        v[1] += d.y;
+        v[2] += cnorm2(r);
+    }
+    if (block_partials<3>(v, a.partial, &Kr->counter)) {
+        const double rr = sum_partials(a.partial, 0), ri = sum_partials(a.partial, 1);
+        const double r2 = sum_partials(a.partial, 2);
+        if (threadIdx.x == 0) {
+            Kr->iter += 1;
+            loop_top(Kr, r2, make_double2(rr, ri));
+            Kr->counter = 0;
+        }
+    }
+}
+
+// b = rhs/|rhs|, x = psi/|rhs|, r = (rhs - A psi)/|rhs|, r0 = r, p = v = 0.
+__global__ __launch_bounds__(256) void k_init(InitArgs a)
+{
+    Krylov *Kr = a.Kr;
+    const double nrm = Kr->rhs_nrm;
+    double v[2] = {0, 0};
+    if (nrm != 0.0) {
+        for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (size_t)gridDim.x * blockDim.x) {
+            const double2 rhs = a.B[q];
+            const double2 b = make_double2(rhs.x / nrm, rhs.y / nrm);
+            a.B[q] = b;
+            const double2 x = a.Psi[q];
+            a.X[q] = make_double2(x.x / nrm, x.y / nrm);
+            const double2 ax = a.AX[q];
+            const double2 r = make_double2((rhs.x - ax.x) / nrm, (rhs.y - ax.y) / nrm);
+            a.R[q] = r;
+            a.R0[q] = r;
+            a.Pv[q] = make_double2(0.0, 0.0);
+            a.V[q] = make_double2(0.0, 0.0);
+            v[0] += cnorm2(r);
+            v[1] += cnorm2(b);
+        }
+    }
+    if (block_partials<2>(v, a.partial, &Kr->counter)) {
+        const double r2 = sum_partials(a.partial, 0), b2 = sum_partials(a.partial, 1);
+        if (threadIdx.x == 0) {
+            Kr->counter = 0;
+            Kr->iter = 0;
+            Kr->maxit = a.maxit;
+            Kr->eigen_mode = a.eigen_mode;
+            Kr->restarts = 0;
+            Kr->restart = 0;
+            Kr->accepted = 0;
+            Kr->done = 0;
+            Kr->zero_rhs = nrm == 0.0;
+            Kr->rho = make_double2(1.0, 0.0);
+            Kr->alpha = make_double2(1.0, 0.0);
+            Kr->w = make_double2(1.0, 0.0);
+            Kr->r0_sq = r2;
+            Kr->restart_thr = r2 * 1e-32;
+            Kr->tol2 = a.tol * a.tol * b2;
+            if (nrm == 0.0) {
+                Kr->done = 1;
+            } else if (b2 == 0.0) {
+                Kr->done = 1;
+            } else {
+                loop_top(Kr, r2, make_double2(r2, 0.0));
+            }
+        }
+    }
+}
+
+// Acceptance (matrices.cpp:118-121,166-170): psi = mask * |rhs| x when
+// |A x - b| <= 1e-9, or 0 for a zero right-hand side.
+__global__ __launch_bounds__(256) void k_accept(AcceptArgs a)
+{
+    const Krylov *Kr = a.Kr;
+    const double nrm = Kr->rhs_nrm;
+    const double res = sqrt(Kr->res2);
+    const bool zero = Kr->zero_rhs;
+    const bool ok = zero || res <= 1e-9;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.status[0] = ok ? 1 : 0;
+        a.resid[0] = zero ? 0.0 : res;
+    }
+    if (!ok) return;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (size_t)gridDim.x * blockDim.x) {
+        double2 x = zero ? make_double2(0.0, 0.0) : cscale(nrm, a.X[q]);
+        if (a.mask) x = cscale(a.mask[q % a.N], x);
+        a.Psi[q] = x;
+    }
+}
+
+// ---------------------------------------------------------------- sinks
+
+// One workgroup. mode SK_PROJ: out[a] = scale * v_a^H X (stored per the dest
+// selector); SK_SETUP: Woodbury capacitance G = diag(1/((dt/4) g)) + V^H W and its inverse.
+__global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
+{
+    Krylov *Kr = a.Kr;
+    const StepParams *P = a.P;
+    const int ns = P->nsink;
+    const int tid = threadIdx.x;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int N = a.N;
+    for (int s = 0; s < ns; ++s) {
+        const SinkDev &sk = a.sk[s];
+        const int tot = a.K * sk.n;
+        for (int q = tid; q < tot; q += blockDim.x) {
+            const int k = q / sk.n, d = q - k * sk.n;
+            const size_t gi = (size_t)k * N + sk.lo + d;
+            if (a.mode == SK_PROJ) {
+                const double2 d2 = cdotm(sk.v[q], a.X[gi]);
+                v[2 * s] += d2.x;
+                v[2 * s + 1] += d2.y;
+            } else {
+                for (int u = 0; u < ns; ++u) {
+                    const double2 d2 = cdotm(sk.v[q], a.sk[u].W[gi]);
+                    v[4 * s + 2 * u] += d2.x;
+                    v[4 * s + 2 * u + 1] += d2.y;
+                }
+            }
+        }
+    }
+    __shared__ double ws[16][8];
+    __shared__ double red[8];
+    for (int q = 0; q < 8; ++q) {
+        const double s = wave_sum(v[q]);
+        if ((tid & 63) == 0) ws[tid >> 6][q] = s;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    for (int q = 0; q < 8; ++q) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += ws[w][q];
+        red[q] = s;
+    }
+    if (a.mode == SK_PROJ) {
+        for (int s = 0; s < ns; ++s) {
+            const double sc = a.scale * (a.scale_by_nrm ? Kr->rhs_nrm : 1.0);
+            const double2 pr = cscale(sc, make_double2(red[2 * s], red[2 * s + 1]));
+            if (a.dest == SD_START) {
+                Kr->qold[s] = pr;
+                Kr->gq[s] = cscale(P->g[s], pr);
+            } else if (a.dest == SD_QNEW) {
+                Kr->qnew[s] = pr;
+            } else if (a.dest == SD_POST) {
+                Kr->qpost[s] = pr;
+                Kr->gq[s] = cscale(P->g[s], pr);
+            } else {
+                Kr->coefA[s] = cscale(0.5 * P->half_dt * P->g[s], pr);
+            }
+        }
+    } else {
+        double2 G[4];
+        for (int s = 0; s < ns; ++s)
+            for (int u = 0; u < ns; ++u) {
+                G[s * 2 + u] = make_double2(red[4 * s + 2 * u], red[4 * s + 2 * u + 1]);
+                if (s == u) G[s * 2 + u].x += 1.0 / (0.5 * P->half_dt * P->g[s]);
+            }
+        for (int q = 0; q < 4; ++q) Kr->G[q] = G[q];
+        if (ns == 1) {
+            Kr->Ginv[0] = cdiv(make_double2(1.0, 0.0), G[0]);
+        } else if (ns == 2) {
+            const double2 det = csub(cmul(G[0], G[3]), cmul(G[2], G[1]));
+            const double2 id = cdiv(make_double2(1.0, 0.0), det);
+            Kr->Ginv[0] = cmul(G[3], id);
+            Kr->Ginv[1] = cmul(make_double2(-G[1].x, -G[1].y), id);
+            Kr->Ginv[2] = cmul(make_double2(-G[2].x, -G[2].y), id);
+            Kr->Ginv[3] = cmul(G[0], id);
+        }
+    }
+}
+
+// Expand a compact sink projector into a full-length vector.
+__global__ void k_expand(int N, int K, SinkDev sk, double2 *out)
+{
+    const size_t n = (size_t)N * K;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / N), i = (int)(q - (size_t)k * N);
+        const int d = i - sk.lo;
+        out[q] = (d >= 0 && d < sk.n) ? sk.v[(size_t)k * sk.n + d] : make_double2(0.0, 0.0);
+    }
+}
+
+// Node range [lo, hi) where W may be non-zero (exact zeros outside).
+__global__ void k_wrange(int N, int K, const double2 *W, int *range)
+{
+    int lo = N, hi = 0;
+    const size_t n = (size_t)N * K;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const double2 w = W[q];
+        if (w.x != 0.0 || w.y != 0.0) {
+            const int i = (int)(q % N);
+            lo = min(lo, i);
+            hi = max(hi, i + 1);
+        }
+    }
+    atomicMin(range, lo);
+    atomicMax(range + 1, hi);
+}
+
+// ---------------------------------------------------------------- diagnostics
+
+__global__ __launch_bounds__(256) void k_diag_final(DiagArgs a)
+{
+    const int K = a.K, N = a.N, ix = a.ix, nt = a.ntiles;
+    const double dr = a.dr, rix = a.eps + ix * dr;
+    double v[4] = {0, 0, 0, 0};
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        double n0 = 0, e1 = 0;
+        for (int t = 0; t < nt; ++t) {
+            const double *rp = a.rowpart + ((size_t)t * K + j) * 6;
+            n0 += rp[0];
+            e1 += rp[1];
+            v[0] += rp[2];
+            v[1] += rp[3];
+            v[2] += rp[4];
+            v[3] += rp[5];
+        }
+        const double2 um = a.Psi[(size_t)j * N + ix - 1], u0 = a.Psi[(size_t)j * N + ix],
+                      up = a.Psi[(size_t)j * N + ix + 1];
+        const double2 dU = csub(cscale(1.0 / (2.0 * dr), csub(up, um)), make_double2(u0.x / rix, u0.y / rix));
+        a.out[j] = cdotm(u0, dU).y;
+        a.out[K + j] = cdotm(a.dpsi_ix[j], dU).x / a.mu;
+        a.out[2 * K + j] = n0 * dr;
+        a.out[3 * K + j] = e1 * dr;
+    }
+    __shared__ double ws[4][4];
+    for (int q = 0; q < 4; ++q) {
+        const double s = wave_sum(v[q]);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) {
+            double s = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += ws[w][q];
+            a.out[4 * K + q] = s;
+        }
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <int SMAX, int MODE>
+static hipError_t launch_apply_s(const ApplyArgs &a, bool corr, int grid, hipStream_t st)
+{
+    if (corr) hipLaunchKernelGGL((k_apply<SMAX, MODE, true>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_apply<SMAX, MODE, false>), dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_apply_m(const ApplyArgs &a, bool corr, int grid, hipStream_t st)
+{
+    const int smax = a.g.lmax + 1;
+    if (smax <= 8) return launch_apply_s<8, MODE>(a, corr, grid, st);
+    if (smax <= 16) return launch_apply_s<16, MODE>(a, corr, grid, st);
+    if (smax <= 24) return launch_apply_s<24, MODE>(a, corr, grid, st);
+    if (smax <= 32) return launch_apply_s<32, MODE>(a, corr, grid, st);
+    return launch_apply_s<40, MODE>(a, corr, grid, st);
+}
+
+int apply_grid(int nitems, int *per)
+{
+    const int nb = (nitems + 3) / 4;
+    *per = (nb + 7) / 8;
+    return *per * 8;
+}
+
+hipError_t launch_apply(ApplyArgs a, int mode, bool corr, hipStream_t st)
+{
+    const int grid = apply_grid(a.nitems, &a.per);
+    switch (mode) {
+    case AM_H: return launch_apply_m<AM_H>(a, corr, grid, st);
+    case AM_A: return launch_apply_m<AM_A>(a, corr, grid, st);
+    case AM_V: return launch_apply_m<AM_V>(a, corr, grid, st);
+    case AM_T: return launch_apply_m<AM_T>(a, corr, grid, st);
+    case AM_RHS: return launch_apply_m<AM_RHS>(a, corr, grid, st);
+    case AM_RES: return launch_apply_m<AM_RES>(a, corr, grid, st);
+    case AM_DIAG: return launch_apply_m<AM_DIAG>(a, corr, grid, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt, double mu,
+                          double prf, double *E, hipStream_t st)
+{
+    const int n = Ls * N;
+    hipLaunchKernelGGL(k_build_E, dim3((n + 255) / 256), dim3(256), 0, st, N, Ls, dr, eps, R1, R2, q, Mt, mu, prf, E);
+    return hipGetLastError();
+}
+
+hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_factor, dim3((f.K + 63) / 64), dim3(64), 0, st, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st)
+{
+    if (var == PV_P) hipLaunchKernelGGL(k_precond<PV_P>, dim3(a.K), dim3(gmd::PT), 0, st, a);
+    else if (var == PV_S) hipLaunchKernelGGL(k_precond<PV_S>, dim3(a.K), dim3(gmd::PT), 0, st, a);
+    else hipLaunchKernelGGL(k_precond<PV_PLAIN>, dim3(a.K), dim3(gmd::PT), 0, st, a);
+    return hipGetLastError();
+}
+
+static int stream_grid(size_t n)
+{
+    size_t g = (n + 255) / 256;
+    if (g > (size_t)gmd::RED_BLOCKS) g = gmd::RED_BLOCKS;
+    return (int)g;
+}
+
+hipError_t launch_update(const UpdateArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_update, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_init(const InitArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_accept(const AcceptArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_accept, dim3(stream_grid(a.n)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sink(const SinkArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sink, dim3(1), dim3(1024), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(int N, int K, const SinkDev &sk, double2 *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_expand, dim3(stream_grid((size_t)N * K)), dim3(256), 0, st, N, K, sk, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wrange(int N, int K, const double2 *W, int *range, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wrange, dim3(stream_grid((size_t)N * K)), dim3(256), 0, st, N, K, W, range);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace gmk

@@ -1,0 +1,120 @@
+// gm_kernels.hpp - launch interface of the gfx950 kernels (gm_kernels.hip).
+#pragma once
+#include "gm_device.hpp"
+
+namespace gmk {
+
+enum ApplyMode { AM_H = 0, AM_A = 1, AM_V = 2, AM_T = 3, AM_RHS = 4, AM_RES = 5, AM_DIAG = 6 };
+enum PrecVar { PV_PLAIN = 0, PV_P = 1, PV_S = 2 };
+enum SinkMode { SK_PROJ = 0, SK_SETUP = 1 };
+enum SinkDest { SD_START = 0, SD_QNEW = 1, SD_POST = 2, SD_COEFA = 3 };
+
+struct ApplyArgs {
+    gmd::Geometry g;
+    const gmd::StepParams *P;
+    gmd::Krylov *Kr;
+    const double2 *X;    // input vector
+    double2 *Y;          // output
+    double2 *Y2;         // second output (AM_RHS: A psi)
+    double2 *Xc;         // corrected input (Woodbury), written when corr
+    const double2 *R0;   // r0 (AM_V), s (AM_T), b (AM_RES)
+    gmd::SinkDev sk[2];
+    double *partial;     // reduction partials (3 * grid doubles)
+    double *rowpart;     // AM_DIAG: 6 doubles per (tile, mode)
+    double2 *dpsi_ix;    // AM_DIAG: dpsi/dt at the extraction node per mode
+    int nitems;          // node tiles x row chunks
+    int per;             // set by the launcher
+};
+
+struct FactorArgs {
+    int N, K, lmax;
+    double dr, eps, mu, off, half_dt;
+    const int *goff;
+    const double *G;
+    const double *E;
+    const double2 *cap;
+    double2 *inv;
+    int *failed;
+};
+
+struct PrecArgs {
+    int N, K;
+    double half_dt, off;
+    const double2 *inv;
+    gmd::Krylov *Kr;
+    const double2 *R;
+    double2 *Pv;         // p (PV_P, updated in place) or s (PV_S, written)
+    const double2 *V;
+    const double2 *X;    // PV_PLAIN input
+    double2 *Y;          // output
+    int wood;            // run the Woodbury epilogue
+    gmd::SinkDev sk[2];
+    const gmd::StepParams *Ps;
+};
+
+struct UpdateArgs {
+    size_t n;
+    gmd::Krylov *Kr;
+    double2 *X;
+    const double2 *Yc, *Zc, *S, *T, *R0;
+    double2 *R;
+    double *partial;
+};
+
+struct InitArgs {
+    size_t n;
+    gmd::Krylov *Kr;
+    double2 *B;          // rhs in, b out
+    const double2 *Psi, *AX;
+    double2 *X, *R, *R0, *Pv, *V;
+    double tol;
+    int maxit, eigen_mode;
+    double *partial;
+};
+
+struct AcceptArgs {
+    size_t n;
+    int N;
+    const gmd::Krylov *Kr;
+    const double2 *X;
+    const double *mask;
+    double2 *Psi;
+    int *status;
+    double *resid;
+};
+
+struct SinkArgs {
+    int N, K;
+    int mode, dest;
+    double scale;
+    int scale_by_nrm;    // multiply by |rhs| (device value)
+    const double2 *X;
+    gmd::SinkDev sk[2];
+    gmd::Krylov *Kr;
+    const gmd::StepParams *P;
+};
+
+struct DiagArgs {
+    int N, K, ix, ntiles;
+    double dr, eps, mu;
+    const double2 *Psi;
+    const double2 *dpsi_ix;
+    const double *rowpart;
+    double *out;         // flux, energy_flux, norms, energies (K each), then 4 sums
+};
+
+int apply_grid(int nitems, int *per);
+hipError_t launch_apply(ApplyArgs a, int mode, bool corr, hipStream_t st);
+hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
+                          double mu, double prf, double *E, hipStream_t st);
+hipError_t launch_factor(const FactorArgs &f, hipStream_t st);
+hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st);
+hipError_t launch_update(const UpdateArgs &a, hipStream_t st);
+hipError_t launch_init(const InitArgs &a, hipStream_t st);
+hipError_t launch_accept(const AcceptArgs &a, hipStream_t st);
+hipError_t launch_sink(const SinkArgs &a, hipStream_t st);
+hipError_t launch_expand(int N, int K, const gmd::SinkDev &sk, double2 *out, hipStream_t st);
+hipError_t launch_wrange(int N, int K, const double2 *W, int *range, hipStream_t st);
+hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st);
+
+}  // namespace gmk

@@ -1,0 +1,1155 @@
+// gm_solver.cpp - the solver handle behind include/gm.h.
+//
+// The host side restates the orbit bookkeeping of the reference's
+// CrankNicolsonModes::step (src/CrankNicolsonModes.cpp:145-234) and keeps the
+// Gaunt and absorber tables; all per-node work runs in the gfx950 kernels of
+// gm_kernels.hip on one HIP stream. psi and every Krylov vector stay resident
+// in HBM; a step returns to the host only the solver status and the
+// diagnostics that step() writes (O(n_modes) doubles).
+#include "gm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gm_kernels.hpp"
+#include "gm_math.hpp"
+
+namespace {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCK(x) hip_check((x), #x)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        release();
+        if (count == 0) return;
+        HIPCK(hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+template <class T>
+struct PinBuf {
+    T *p = nullptr;
+    void alloc(size_t count)
+    {
+        release();
+        HIPCK(hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault));
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+    }
+    ~PinBuf() { release(); }
+};
+
+struct Profile {
+    double radius = -1.0, width = -1.0;
+    bool valid = false;
+    int lo = 0, n = 0;           // interior support nodes [lo, lo+n)
+    std::vector<double> mult;    // (2 l_max + 1) * n, over the support
+};
+
+struct Sink {
+    std::vector<double2> v;      // K * n over the profile support
+    int lo = 0, n = 0;
+    double strength = 0.0;
+    double norm2 = 0.0;
+    DevBuf<double2> dv;          // device copy of v
+    DevBuf<double2> W;           // P^-1 v, full length
+    PinBuf<double2> stage;       // pinned staging of v
+    size_t stage_n = 0;
+};
+
+}  // namespace
+
+struct gm_solver {
+    // ---- constructor arguments and public members (include/CrankNicolsonModes.hpp:88-164)
+    int Nr = 0, save_every = 0, l_max = 0, K = 0;
+    double epsilon = 0, mu = 0, q = 0, R = 0, theta = 0, phase = 0, Omega = 0, Omega_0 = 0, P = 0;
+    double R1 = 0, R2 = 0, M1 = 0, M2 = 0, a = 0, L = 0, dr = 0, dt = 0;
+    bool counter = false;
+    double M_c = 0;
+    bool isolated = false;
+    double ramp_periods = 0, initial_period = 0;
+    bool absorption = false;
+    double epsilon_smear = 0;
+    gm_flags flags{};
+    bool initialized = false;
+    std::vector<int> ml, mm;
+    std::vector<double> r;
+    // ---- time-dependent state
+    double t_i = 0, ramp_factor = 0, prf = 0, Omega_V = 0, p_bin = 0;
+    double Ah1 = 0, Ah2 = 0;
+    bool sink_active = false;
+    int mode00 = -1;
+    double r_ex = 0;
+    int idx_ex = 1;
+    double last_residual = 0;
+    int last_iterations = 0;
+    std::unique_ptr<gmh::RollingPhaseAverage> avg;
+    Profile prof[2];
+    Sink sink[2];
+    // ---- mask and CAP
+    bool mask = false, cap = false;
+    std::vector<double> damping;
+    std::vector<double2> capv;
+    int cap_version = 0;
+    // ---- device
+    hipStream_t st = nullptr;
+    std::vector<int> goff_h;
+    std::vector<double> G_h;
+    std::vector<gmd::Chunk> chunks_h;
+    DevBuf<int> goff;
+    DevBuf<double> G, E, dmask;
+    DevBuf<double2> dcap;
+    DevBuf<gmd::Chunk> chunks;
+    DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, yc, zc, inv, tmp;
+    DevBuf<double> partial, rowpart, diagout;
+    DevBuf<double2> dpsi_ix;
+    DevBuf<gmd::StepParams> dP;
+    DevBuf<gmd::Krylov> dK;
+    DevBuf<int> dflags;  // [0] factor failure, [1] accept status
+    DevBuf<double> dresid;
+    PinBuf<gmd::StepParams> hP;
+    PinBuf<gmd::Krylov> hK;
+    PinBuf<double> hdiag;
+    PinBuf<int> hflags;
+    PinBuf<double> hresid;
+    PinBuf<int> hwr;
+    int ntiles = 0, nchunks = 0;
+    // caches
+    bool E_valid = false;
+    double E_key[6] = {0, 0, 0, 0, 0, 0};
+    bool fac_valid = false;
+    double fac_key[3] = {0, 0, 0};
+    int fac_cap_version = -1;
+    int prev_iters = 4;
+    std::string err;
+
+    size_t nvec() const { return static_cast<size_t>(Nr) * K; }
+    double ramp_end_time() const { return ramp_periods * initial_period; }
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+void ensure_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw DeviceError("no HIP device is available");
+}
+
+// ------------------------------------------------------------------ tables
+
+// Gaunt blocks: for each (l, l1), [m][m1][s] with l* = |l - l1| + 2 s, the
+// angular coefficient of build_coupling_cache (hamiltonian.cpp:77-95).
+void build_gaunt(gm_solver &S)
+{
+    const int Lm = S.l_max + 1;
+    S.goff_h.assign(Lm * Lm, 0);
+    S.G_h.clear();
+    for (int l = 0; l <= S.l_max; ++l)
+        for (int l1 = 0; l1 <= S.l_max; ++l1) {
+            S.goff_h[l * Lm + l1] = static_cast<int>(S.G_h.size());
+            const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
+            for (int mi = 0; mi <= l; ++mi)
+                for (int m1i = 0; m1i <= l1; ++m1i)
+                    for (int s = 0; s < Sl; ++s) {
+                        const int m = -l + 2 * mi, m1 = -l1 + 2 * m1i, ls = lo + 2 * s;
+                        const int ms = m1 - m;
+                        double ang = 0.0;
+                        if (std::abs(ms) <= ls) {
+                            const double w1 = gmh::wigner3j(ls, l1, l, 0, 0, 0);
+                            const double w2 = gmh::wigner3j(ls, l1, l, -ms, m1, -m);
+                            double yv = gmh::sph_plm0(ls, std::abs(ms));
+                            if (ms < 0 && std::abs(ms) % 2 != 0) yv = -yv;
+                            const double pref = (m1 % 2 == 0 ? 1.0 : -1.0) *
+                                                std::sqrt((2.0 * l + 1) * (2.0 * l1 + 1) * (2.0 * ls + 1) / (4.0 * M_PI));
+                            ang = pref * w1 * w2 * yv;
+                        }
+                        S.G_h.push_back(ang);
+                    }
+        }
+    S.chunks_h.clear();
+    for (int l = 0; l <= S.l_max; ++l)
+        for (int m0 = 0; m0 <= l; m0 += gmd::CH)
+            S.chunks_h.push_back({l, m0, std::min(gmd::CH, l + 1 - m0), 0});
+    S.nchunks = static_cast<int>(S.chunks_h.size());
+    S.ntiles = (S.Nr + gmd::TILE - 1) / gmd::TILE;
+}
+
+double angular(const gm_solver &S, int j, int k, int ls)
+{
+    const int l = S.ml[j], l1 = S.ml[k];
+    const int mi = (S.mm[j] + l) / 2, m1i = (S.mm[k] + l1) / 2;
+    const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
+    const int s = (ls - lo) / 2;
+    if (ls < lo || s >= Sl || (ls - lo) % 2) return 0.0;
+    return S.G_h[S.goff_h[l * (S.l_max + 1) + l1] + (mi * (l1 + 1) + m1i) * Sl + s];
+}
+
+gmd::Geometry geometry(const gm_solver &S)
+{
+    gmd::Geometry g{};
+    g.N = S.Nr;
+    g.K = S.K;
+    g.lmax = S.l_max;
+    g.Ls = 2 * S.l_max + 1;
+    g.ntiles = S.ntiles;
+    g.nchunks = S.nchunks;
+    g.dr = S.dr;
+    g.eps = S.epsilon;
+    g.mu = S.mu;
+    g.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
+    g.chunks = S.chunks.p;
+    g.goff = S.goff.p;
+    g.G = S.G.p;
+    g.E = S.E.p;
+    g.cap = S.cap ? S.dcap.p : nullptr;
+    return g;
+}
+
+// ------------------------------------------------------------------ absorbers
+
+// update_absorption_profile (src/CrankNicolsonModes_absorption.cpp:12-81),
+// kept on the body's support only.
+void update_profile(gm_solver &S, Profile &pr, double radius, double width)
+{
+    if (pr.valid && pr.radius == radius && pr.width == width) return;
+    pr.valid = true;
+    pr.radius = radius;
+    pr.width = width;
+    pr.mult.clear();
+    pr.lo = 0;
+    pr.n = 0;
+    if (width < S.dr || width <= 0.0 || radius < 0.0) return;
+    const int degree = 2 * S.l_max, N = S.Nr;
+    int lo = -1, hi = -1;
+    for (int i = 1; i < N - 1; ++i)
+        if (std::abs(S.r[i] - radius) < width) {
+            if (lo < 0) lo = i;
+            hi = i + 1;
+        }
+    if (lo < 0) return;
+    const int n = hi - lo;
+    pr.lo = lo;
+    pr.n = n;
+    pr.mult.assign(static_cast<size_t>(degree + 1) * n, 0.0);
+    const int nq = std::max(128, degree + 16);
+    std::vector<double> nodes, weights;
+    gmh::gauss_legendre01(nq, nodes, weights);
+    for (int d = 0; d < n; ++d) {
+        const int i = lo + d;
+        const double delta = S.r[i] - radius;
+        if (std::abs(delta) >= width) continue;
+        if (radius == 0.0) {
+            const double xx = S.r[i] / width;
+            pr.mult[d] = 4.0 * M_PI * std::exp(-1.0 / (1.0 - xx * xx));
+            continue;
+        }
+        const double product = 2.0 * S.r[i] * radius;
+        const double capw = std::min(2.0, (width * width - delta * delta) / product);
+        for (int k = 0; k < nq; ++k) {
+            const double z = capw * nodes[k];
+            const double u = 1.0 - z;
+            const double dist2 = delta * delta + product * z;
+            const double weight = 2.0 * M_PI * capw * weights[k] * std::exp(-1.0 / (1.0 - dist2 / (width * width)));
+            double previous = 1.0, current = u;
+            pr.mult[d] += weight;
+            if (degree > 0) pr.mult[n + d] += weight * current;
+            for (int l = 2; l <= degree; ++l) {
+                const double next = ((2 * l - 1) * u * current - (l - 1) * previous) / l;
+                pr.mult[static_cast<size_t>(l) * n + d] += weight * next;
+                previous = current;
+                current = next;
+            }
+        }
+    }
+    double norm = 0.0;
+    for (int d = 0; d < n; ++d) norm += S.dr * S.r[lo + d] * S.r[lo + d] * pr.mult[d];
+    if (norm > 0.0)
+        for (double &value : pr.mult) value /= norm;
+}
+
+// build_swave_projector (absorption.cpp:127-151): the (j, 00) pair has the
+// single multipole l* = l_j.
+void build_projector(gm_solver &S, const Profile &pr, bool secondary, Sink &sk)
+{
+    sk.lo = pr.lo;
+    sk.n = pr.n;
+    sk.v.assign(static_cast<size_t>(S.K) * pr.n, make_double2(0.0, 0.0));
+    sk.norm2 = 0.0;
+    if (S.mode00 < 0 || pr.n == 0) return;
+    const double sqrt4pi = std::sqrt(4.0 * M_PI);
+    for (int j = 0; j < S.K; ++j) {
+        const int ms = 0 - S.mm[j];
+        const double ang = angular(S, j, S.mode00, S.ml[j]);
+        if (ang == 0.0) continue;
+        const double ph_re = std::cos(ms * S.theta), ph_im = std::sin(ms * S.theta);
+        const double parity = secondary && ms % 2 != 0 ? -1.0 : 1.0;
+        for (int d = 0; d < pr.n; ++d) {
+            const double m = pr.mult[static_cast<size_t>(S.ml[j]) * pr.n + d] * ang;
+            const double sc = S.r[pr.lo + d] * sqrt4pi * parity;
+            double2 val = make_double2(sc * (m * ph_re), sc * (m * ph_im));
+            sk.v[static_cast<size_t>(j) * pr.n + d] = val;
+            sk.norm2 += val.x * val.x + val.y * val.y;
+        }
+    }
+    const size_t cnt = std::max<size_t>(sk.v.size(), 1);
+    if (sk.dv.n < cnt) {
+        sk.dv.alloc(cnt);
+        sk.stage.alloc(cnt);
+        sk.stage_n = cnt;
+    }
+    if (!sk.v.empty()) {
+        std::memcpy(sk.stage.p, sk.v.data(), sk.v.size() * sizeof(double2));
+        HIPCK(hipMemcpyAsync(sk.dv.p, sk.stage.p, sk.v.size() * sizeof(double2), hipMemcpyHostToDevice, S.st));
+    }
+}
+
+// build_spherical_absorption (absorption.cpp:153-182).
+void build_absorption(gm_solver &S)
+{
+    update_profile(S, S.prof[0], S.R1, S.epsilon_smear * S.M1);
+    update_profile(S, S.prof[1], S.R2, S.epsilon_smear * S.M2);
+    double ramp = 1.0;
+    if (!S.isolated && S.t_i < S.ramp_end_time()) ramp = 0.0;
+    else if (!S.isolated && S.t_i < S.ramp_end_time() + S.initial_period) {
+        const double xx = (S.t_i - S.ramp_end_time()) / S.initial_period;
+        ramp = 3.0 * xx * xx - 2.0 * xx * xx * xx;
+    }
+    S.Ah1 = 16.0 * M_PI * S.M1 * S.M1 * ramp;
+    S.Ah2 = 16.0 * M_PI * S.M2 * S.M2 * ramp;
+    build_projector(S, S.prof[0], false, S.sink[0]);
+    S.sink[0].strength = S.Ah1 * S.dr;
+    build_projector(S, S.prof[1], true, S.sink[1]);
+    S.sink[1].strength = S.Ah2 * S.dr;
+}
+
+// ------------------------------------------------------------------ per-step operator
+
+// build_time_dependent_perturbation (hamiltonian.cpp:245-268) on the device tables.
+void build_perturbation(gm_solver &S, double time)
+{
+    S.ramp_factor = 1.0;
+    if (S.isolated) {
+        S.Omega = 0.0;
+    } else if (time < S.ramp_end_time()) {
+        const double xx = time / S.ramp_end_time();
+        S.ramp_factor = 3.0 * xx * xx - 2.0 * xx * xx * xx;
+        S.Omega = S.Omega_0 * S.ramp_factor;
+    } else {
+        S.Omega = std::sqrt((S.M1 + S.M2) / std::pow(S.R, 3));
+    }
+    S.prf = S.flags.ramp_secondary_potential ? S.ramp_factor : 1.0;
+    // radial profiles: rebuilt only when the orbit or the companion weight changed
+    const double key[6] = {S.R1, S.R2, S.q, S.M1 + S.M2, S.prf, S.mu};
+    if (!S.E_valid || std::memcmp(key, S.E_key, sizeof(key)) != 0) {
+        HIPCK(gmk::launch_build_E(S.Nr, 2 * S.l_max + 1, S.dr, S.epsilon, S.R1, S.R2, S.q, S.M1 + S.M2, S.mu, S.prf,
+                                  S.E.p, S.st));
+        std::memcpy(S.E_key, key, sizeof(key));
+        S.E_valid = true;
+        S.fac_valid = false;
+    }
+    gmd::StepParams &P = *S.hP.p;
+    for (int m = -S.l_max; m <= S.l_max; ++m)
+        P.ph[m + S.l_max] = make_double2(std::cos(m * S.theta), std::sin(m * S.theta));
+    S.Omega_V = S.Omega;
+    if (S.absorption) build_absorption(S);
+    S.sink_active = S.absorption && time >= S.ramp_end_time();
+    P.half_dt = S.dt * 0.5;
+    P.dt = S.dt;
+    P.idx_ex = S.idx_ex;
+    P.nsink = 0;
+    P.g[0] = P.g[1] = 0.0;
+    if (S.sink_active)
+        for (int bdy = 0; bdy < 2; ++bdy)
+            if (S.sink[bdy].strength != 0.0 && S.sink[bdy].norm2 > 0.0) {
+                P.sink_body[P.nsink] = bdy;
+                P.g[P.nsink] = S.sink[bdy].strength;
+                P.nsink++;
+            }
+    HIPCK(hipMemcpyAsync(S.dP.p, S.hP.p, sizeof(gmd::StepParams), hipMemcpyHostToDevice, S.st));
+}
+
+void fill_sinks(const gm_solver &S, gmd::SinkDev (&sk)[2])
+{
+    const gmd::StepParams &P = *S.hP.p;
+    for (int s = 0; s < 2; ++s) {
+        sk[s] = gmd::SinkDev{};
+        if (s < P.nsink) {
+            const Sink &src = S.sink[P.sink_body[s]];
+            sk[s].lo = src.lo;
+            sk[s].n = src.n;
+            sk[s].v = src.dv.p;
+            sk[s].W = src.W.p;
+        }
+    }
+}
+
+gmk::ApplyArgs apply_args(gm_solver &S, const double2 *X, double2 *Y)
+{
+    gmk::ApplyArgs a{};
+    a.g = geometry(S);
+    a.P = S.dP.p;
+    a.Kr = S.dK.p;
+    a.X = X;
+    a.Y = Y;
+    fill_sinks(S, a.sk);
+    a.partial = S.partial.p;
+    a.rowpart = S.rowpart.p;
+    a.dpsi_ix = S.dpsi_ix.p;
+    a.nitems = S.ntiles * S.nchunks;
+    return a;
+}
+
+gmk::PrecArgs prec_args(gm_solver &S)
+{
+    gmk::PrecArgs a{};
+    a.N = S.Nr;
+    a.K = S.K;
+    a.half_dt = S.dt * 0.5;
+    a.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
+    a.inv = S.inv.p;
+    a.Kr = S.dK.p;
+    fill_sinks(S, a.sk);
+    a.Ps = S.dP.p;
+    return a;
+}
+
+// Thomas pivots, recomputed when dt, the E table or the CAP changed.
+void factorize(gm_solver &S)
+{
+    const double key[3] = {S.dt, S.E_key[4], S.E_key[3]};
+    if (S.fac_valid && std::memcmp(key, S.fac_key, sizeof(key)) == 0 && S.fac_cap_version == S.cap_version) return;
+    gmk::FactorArgs f{};
+    f.N = S.Nr;
+    f.K = S.K;
+    f.lmax = S.l_max;
+    f.dr = S.dr;
+    f.eps = S.epsilon;
+    f.mu = S.mu;
+    f.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
+    f.half_dt = S.dt * 0.5;
+    f.goff = S.goff.p;
+    f.G = S.G.p;
+    f.E = S.E.p;
+    f.cap = S.cap ? S.dcap.p : nullptr;
+    f.inv = S.inv.p;
+    f.failed = S.dflags.p;
+    HIPCK(gmk::launch_factor(f, S.st));
+    std::memcpy(S.fac_key, key, sizeof(key));
+    S.fac_cap_version = S.cap_version;
+    S.fac_valid = true;
+}
+
+void sink_proj(gm_solver &S, const double2 *X, int dest, double scale, int by_nrm)
+{
+    gmk::SinkArgs a{};
+    a.N = S.Nr;
+    a.K = S.K;
+    a.mode = gmk::SK_PROJ;
+    a.dest = dest;
+    a.scale = scale;
+    a.scale_by_nrm = by_nrm;
+    a.X = X;
+    fill_sinks(S, a.sk);
+    a.Kr = S.dK.p;
+    a.P = S.dP.p;
+    HIPCK(gmk::launch_sink(a, S.st));
+}
+
+// Woodbury columns W_a = P^-1 v_a and the capacitance (matrices.cpp:133-145).
+void woodbury_setup(gm_solver &S)
+{
+    const gmd::StepParams &P = *S.hP.p;
+    gmk::PrecArgs pa = prec_args(S);
+    for (int s = 0; s < P.nsink; ++s) {
+        Sink &sk = S.sink[P.sink_body[s]];
+        if (sk.W.n != S.nvec()) sk.W.alloc(S.nvec());
+        gmd::SinkDev sd{};
+        sd.lo = sk.lo;
+        sd.n = sk.n;
+        sd.v = sk.dv.p;
+        HIPCK(gmk::launch_expand(S.Nr, S.K, sd, S.tmp.p, S.st));
+        pa.X = S.tmp.p;
+        pa.Y = sk.W.p;
+        pa.wood = 0;
+        HIPCK(gmk::launch_precond(pa, gmk::PV_PLAIN, S.st));
+    }
+    // exact non-zero node range of each column
+    S.hwr.p[0] = S.hwr.p[2] = S.Nr;
+    S.hwr.p[1] = S.hwr.p[3] = 0;
+    gmd::Krylov *dk = S.dK.p;
+    HIPCK(hipMemcpyAsync(&dk->wrange[0], S.hwr.p, 4 * sizeof(int), hipMemcpyHostToDevice, S.st));
+    for (int s = 0; s < P.nsink; ++s)
+        HIPCK(gmk::launch_wrange(S.Nr, S.K, S.sink[P.sink_body[s]].W.p, &dk->wrange[2 * s], S.st));
+    gmk::SinkArgs a{};
+    a.N = S.Nr;
+    a.K = S.K;
+    a.mode = gmk::SK_SETUP;
+    fill_sinks(S, a.sk);
+    a.Kr = S.dK.p;
+    a.P = S.dP.p;
+    HIPCK(gmk::launch_sink(a, S.st));
+}
+
+void enqueue_iteration(gm_solver &S, bool wood)
+{
+    gmk::PrecArgs pa = prec_args(S);
+    pa.wood = wood;
+    // y = M^-1 p with p = r + beta (p - w v)
+    pa.R = S.rr.p;
+    pa.Pv = S.p.p;
+    pa.V = S.v.p;
+    pa.Y = S.y.p;
+    HIPCK(gmk::launch_precond(pa, gmk::PV_P, S.st));
+    gmk::ApplyArgs a = apply_args(S, S.y.p, S.v.p);
+    a.Xc = S.yc.p;
+    a.R0 = S.r0.p;
+    HIPCK(gmk::launch_apply(a, gmk::AM_V, wood, S.st));
+    // z = M^-1 s with s = r - alpha v
+    pa.Pv = S.s.p;
+    pa.Y = S.z.p;
+    HIPCK(gmk::launch_precond(pa, gmk::PV_S, S.st));
+    a = apply_args(S, S.z.p, S.t.p);
+    a.Xc = S.zc.p;
+    a.R0 = S.s.p;
+    HIPCK(gmk::launch_apply(a, gmk::AM_T, wood, S.st));
+    gmk::UpdateArgs u{};
+    u.n = S.nvec();
+    u.Kr = S.dK.p;
+    u.X = S.x.p;
+    u.Yc = wood ? S.yc.p : S.y.p;
+    u.Zc = wood ? S.zc.p : S.z.p;
+    u.S = S.s.p;
+    u.T = S.t.p;
+    u.R0 = S.r0.p;
+    u.R = S.rr.p;
+    u.partial = S.partial.p;
+    HIPCK(gmk::launch_update(u, S.st));
+}
+
+void sync_krylov(gm_solver &S)
+{
+    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
+    HIPCK(hipStreamSynchronize(S.st));
+}
+
+// solve_CN (matrices.cpp:94-215) with the block preconditioner. Returns
+// false when the residual check fails.
+bool solve_cn(gm_solver &S)
+{
+    const gmd::StepParams &P = *S.hP.p;
+    const bool wood = P.nsink > 0;
+    factorize(S);
+    if (wood) woodbury_setup(S);
+    gmd::Krylov *dk = S.dK.p;
+    // rhs = B psi - (dt/4) Gamma psi and A psi in one apply
+    if (wood) sink_proj(S, S.psi.p, gmk::SD_START, 1.0, 0);
+    {
+        gmk::ApplyArgs a = apply_args(S, S.psi.p, S.b.p);
+        a.Y2 = S.ax.p;
+        HIPCK(gmk::launch_apply(a, gmk::AM_RHS, false, S.st));
+    }
+    // Eigen's BiCGSTAB without sinks, the reference's own recurrence with them
+    const int maxit = 500, eigen_mode = wood ? 0 : 1;
+    {
+        gmk::InitArgs ia{};
+        ia.n = S.nvec();
+        ia.Kr = dk;
+        ia.B = S.b.p;
+        ia.Psi = S.psi.p;
+        ia.AX = S.ax.p;
+        ia.X = S.x.p;
+        ia.R = S.rr.p;
+        ia.R0 = S.r0.p;
+        ia.Pv = S.p.p;
+        ia.V = S.v.p;
+        ia.tol = 1e-10;
+        ia.maxit = maxit;
+        ia.eigen_mode = eigen_mode;
+        ia.partial = S.partial.p;
+        HIPCK(gmk::launch_init(ia, S.st));
+    }
+    int batch = std::max(1, S.prev_iters);
+    int enqueued = 0;
+    for (;;) {
+        for (int k = 0; k < batch; ++k) enqueue_iteration(S, wood);
+        enqueued += batch;
+        sync_krylov(S);
+        const gmd::Krylov &K = *S.hK.p;
+        if (K.restart) throw std::runtime_error("BiCGSTAB lost bi-orthogonality; the restart path is not available");
+        if (K.done) break;
+        if (enqueued > 2 * maxit) throw std::runtime_error("BiCGSTAB did not report convergence");
+        batch = 2;
+    }
+    S.last_iterations = S.hK.p->iter;
+    S.prev_iters = std::max(1, S.last_iterations);
+    // acceptance: |A_full x - b| with the rank-one sinks of x
+    if (wood) sink_proj(S, S.x.p, gmk::SD_COEFA, 1.0, 0);
+    {
+        gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
+        a.R0 = S.b.p;
+        HIPCK(gmk::launch_apply(a, gmk::AM_RES, false, S.st));
+    }
+    if (wood) sink_proj(S, S.x.p, gmk::SD_QNEW, 1.0, 1);
+    gmk::AcceptArgs ac{};
+    ac.n = S.nvec();
+    ac.N = S.Nr;
+    ac.Kr = dk;
+    ac.X = S.x.p;
+    ac.mask = S.mask ? S.dmask.p : nullptr;
+    ac.Psi = S.psi.p;
+    ac.status = S.dflags.p + 1;
+    ac.resid = S.dresid.p;
+    HIPCK(gmk::launch_accept(ac, S.st));
+    return true;
+}
+
+void alloc_device(gm_solver &S)
+{
+    const size_t n = S.nvec();
+    for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.ax, &S.yc, &S.zc,
+                     &S.inv, &S.tmp})
+        if (bf->n != n) bf->alloc(n);
+    S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
+    int per = 0;
+    const int grid = gmk::apply_grid(S.ntiles * S.nchunks, &per);
+    S.partial.alloc(static_cast<size_t>(3) * std::max(grid, gmd::RED_BLOCKS));
+    S.rowpart.alloc(static_cast<size_t>(S.ntiles) * S.K * 6);
+    S.dpsi_ix.alloc(S.K);
+    S.diagout.alloc(static_cast<size_t>(4) * S.K + 4);
+    S.hdiag.alloc(static_cast<size_t>(4) * S.K + 4);
+    S.goff.alloc(S.goff_h.size());
+    S.G.alloc(std::max<size_t>(S.G_h.size(), 1));
+    S.chunks.alloc(S.chunks_h.size());
+    HIPCK(hipMemcpyAsync(S.goff.p, S.goff_h.data(), S.goff_h.size() * sizeof(int), hipMemcpyHostToDevice, S.st));
+    if (!S.G_h.empty())
+        HIPCK(hipMemcpyAsync(S.G.p, S.G_h.data(), S.G_h.size() * sizeof(double), hipMemcpyHostToDevice, S.st));
+    HIPCK(hipMemcpyAsync(S.chunks.p, S.chunks_h.data(), S.chunks_h.size() * sizeof(gmd::Chunk), hipMemcpyHostToDevice,
+                         S.st));
+    HIPCK(hipMemsetAsync(S.dK.p, 0, sizeof(gmd::Krylov), S.st));
+    HIPCK(hipMemsetAsync(S.dflags.p, 0, 2 * sizeof(int), S.st));
+}
+
+// ------------------------------------------------------------------ C ABI plumbing
+
+template <class F>
+int guarded(gm_solver *S, F &&f)
+{
+    try {
+        f();
+        return GM_OK;
+    } catch (const std::invalid_argument &e) {
+        if (S) S->err = e.what();
+        else g_create_err = e.what();
+        return GM_E_INVALID;
+    } catch (const std::logic_error &e) {
+        if (S) S->err = e.what();
+        else g_create_err = e.what();
+        return GM_E_LOGIC;
+    } catch (const DeviceError &e) {
+        if (S) S->err = e.what();
+        else g_create_err = e.what();
+        return GM_E_DEVICE;
+    } catch (const std::exception &e) {
+        if (S) S->err = e.what();
+        else g_create_err = e.what();
+        return GM_E_RUNTIME;
+    }
+}
+
+void to_host(gm_solver &S, const double2 *d, double *h)
+{
+    HIPCK(hipMemcpyAsync(h, d, S.nvec() * sizeof(double2), hipMemcpyDeviceToHost, S.st));
+    HIPCK(hipStreamSynchronize(S.st));
+}
+
+void to_dev(gm_solver &S, const double *h, double2 *d)
+{
+    HIPCK(hipMemcpyAsync(d, h, S.nvec() * sizeof(double2), hipMemcpyHostToDevice, S.st));
+    HIPCK(hipStreamSynchronize(S.st));
+}
+
+void host_sink(const gm_solver &S, const double *in, double *out, double factor)
+{
+    // out += factor * sum_a g_a (v_a^H x) v_a on the supports (apply_sink, absorption.cpp:100-109)
+    if (!S.sink_active) return;
+    for (const Sink &sk : S.sink) {
+        if (sk.strength == 0.0 || sk.n == 0) continue;
+        double pr = 0.0, pi = 0.0;
+        for (int j = 0; j < S.K; ++j)
+            for (int d = 0; d < sk.n; ++d) {
+                const double2 v = sk.v[static_cast<size_t>(j) * sk.n + d];
+                const size_t q = static_cast<size_t>(j) * S.Nr + sk.lo + d;
+                pr += v.x * in[2 * q] + v.y * in[2 * q + 1];
+                pi += v.x * in[2 * q + 1] - v.y * in[2 * q];
+            }
+        const double cr = factor * sk.strength * pr, ci = factor * sk.strength * pi;
+        for (int j = 0; j < S.K; ++j)
+            for (int d = 0; d < sk.n; ++d) {
+                const double2 v = sk.v[static_cast<size_t>(j) * sk.n + d];
+                const size_t q = static_cast<size_t>(j) * S.Nr + sk.lo + d;
+                out[2 * q] += cr * v.x - ci * v.y;
+                out[2 * q + 1] += cr * v.y + ci * v.x;
+            }
+    }
+}
+
+void apply_H_host(gm_solver &S, const double *in, double *out)
+{
+    to_dev(S, in, S.tmp.p);
+    gmk::ApplyArgs a = apply_args(S, S.tmp.p, S.t.p);
+    HIPCK(gmk::launch_apply(a, gmk::AM_H, false, S.st));
+    to_host(S, S.t.p, out);
+}
+
+void require_initialized(const gm_solver &S)
+{
+    if (!S.initialized) throw std::logic_error("initialize() must be called first");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gm_version(void) { return "gm-mi355x-0.1"; }
+
+int gm_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *gm_last_error(const gm_solver *S) { return S ? S->err.c_str() : g_create_err.c_str(); }
+
+int gm_create(const gm_params *p, gm_solver **out)
+{
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        if (p->Nr < 3 || p->l_max < 0 || p->L <= 0.0 || p->dt <= 0.0 || p->save_every <= 0)
+            throw std::invalid_argument("Require Nr >= 3, l_max >= 0, and positive L, dt, save_every");
+        if (p->l_max + 1 > gmd::MAXLM)
+            throw std::invalid_argument("l_max above the largest supported value (" + std::to_string(gmd::MAXLM - 1) + ")");
+        ensure_device();
+        auto S = std::make_unique<gm_solver>();
+        S->Nr = p->Nr;
+        S->save_every = p->save_every;
+        S->l_max = p->l_max;
+        S->epsilon = p->epsilon;
+        S->mu = p->mu;
+        S->q = p->q;
+        S->R = p->R;
+        S->a = p->a;
+        S->L = p->L;
+        S->dr = p->L / (p->Nr - 1);
+        S->dt = p->dt;
+        S->counter = p->counter != 0;
+        S->M_c = p->M_c;
+        S->ramp_periods = p->ramp_periods;
+        S->initial_period = 2.0 * M_PI / std::sqrt((1 + p->q) / std::pow(p->R, 3));
+        S->absorption = p->absorption != 0;
+        S->epsilon_smear = p->epsilon_smear;
+        S->Omega = 0.0;
+        S->Omega_0 = std::sqrt((1.0 + p->q) / std::pow(p->R, 3));
+        S->P = S->initial_period;
+        S->R1 = p->q / (1 + p->q) * p->R;
+        S->R2 = 1 / (1 + p->q) * p->R;
+        S->M1 = 1.0;
+        S->M2 = p->q;
+        S->flags = gm_flags{1, 1, 50.0, 1, 2.0, GM_PRECISION_FP64};
+        S->avg = std::make_unique<gmh::RollingPhaseAverage>(2.0);
+        S->r.resize(S->Nr);
+        for (int i = 0; i < S->Nr; ++i) S->r[i] = S->epsilon + i * S->dr;
+        for (int l = 0; l <= S->l_max; ++l)
+            for (int m = -l; m <= l; ++m)
+                if (std::abs(l - m) % 2 == 0) {
+                    S->ml.push_back(l);
+                    S->mm.push_back(m);
+                }
+        S->K = static_cast<int>(S->ml.size());
+        HIPCK(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+        S->dP.alloc(1);
+        S->dK.alloc(1);
+        S->dflags.alloc(2);
+        S->dresid.alloc(1);
+        S->hP.alloc(1);
+        S->hK.alloc(1);
+        S->hflags.alloc(2);
+        S->hresid.alloc(1);
+        S->hwr.alloc(4);
+        std::memset(S->hP.p, 0, sizeof(gmd::StepParams));
+        build_gaunt(*S);
+        alloc_device(*S);
+        HIPCK(hipMemsetAsync(S->psi.p, 0, S->nvec() * sizeof(double2), S->st));
+        HIPCK(hipStreamSynchronize(S->st));
+        *out = S.release();
+    });
+}
+
+void gm_destroy(gm_solver *S)
+{
+    if (!S) return;
+    if (S->st) {
+        (void)hipStreamSynchronize(S->st);
+        (void)hipStreamDestroy(S->st);
+    }
+    delete S;
+}
+
+int gm_n_modes(const gm_solver *S) { return S->K; }
+
+int gm_modes(const gm_solver *S, int32_t *lm)
+{
+    for (int j = 0; j < S->K; ++j) {
+        lm[2 * j] = S->ml[j];
+        lm[2 * j + 1] = S->mm[j];
+    }
+    return GM_OK;
+}
+
+int gm_grid(const gm_solver *S, double *r)
+{
+    std::memcpy(r, S->r.data(), sizeof(double) * S->Nr);
+    return GM_OK;
+}
+
+int gm_set_flags(gm_solver *S, const gm_flags *f)
+{
+    return guarded(S, [&] {
+        if (!(f->power_average_orbits > 0.0)) throw std::invalid_argument("RollingPhaseAverage: n_orbits must be positive");
+        if (f->precision != GM_PRECISION_FP64 && f->precision != GM_PRECISION_MIXED)
+            throw std::invalid_argument("precision must be GM_PRECISION_FP64 or GM_PRECISION_MIXED");
+        if (f->power_average_orbits != S->flags.power_average_orbits)
+            S->avg = std::make_unique<gmh::RollingPhaseAverage>(f->power_average_orbits);
+        if (f->ramp_secondary_potential != S->flags.ramp_secondary_potential) S->E_valid = false;
+        S->flags = *f;
+    });
+}
+
+int gm_get_flags(const gm_solver *S, gm_flags *f)
+{
+    *f = S->flags;
+    return GM_OK;
+}
+
+int gm_configure_isolated_black_hole(gm_solver *S, double x)
+{
+    return guarded(S, [&] {
+        if (!std::isfinite(x) || std::abs(x) >= S->L)
+            throw std::invalid_argument("The isolated BH must lie inside the radial domain");
+        if (S->initialized) throw std::logic_error("Configure the isolated BH before initialize()");
+        S->isolated = true;
+        S->flags.evolve_orbit = 0;
+        S->M1 = 1.0;
+        S->M2 = S->q = 0.0;
+        S->R = S->R1 = std::abs(x);
+        S->R2 = 0.0;
+        S->theta = x < 0.0 ? M_PI : 0.0;
+        S->phase = 0.0;
+        S->Omega = S->Omega_0 = 0.0;
+        S->ramp_periods = 0.0;
+    });
+}
+
+int gm_set_initial_state_full(gm_solver *S, const double *psi, int64_t n)
+{
+    return guarded(S, [&] {
+        if (n != static_cast<int64_t>(S->nvec()))
+            throw std::invalid_argument("Initial state size does not match the radial grid and modes");
+        const double scale = std::sqrt(S->M_c / S->mu);
+        std::vector<double> tmp(psi, psi + 2 * S->nvec());
+        for (double &v : tmp) v *= scale;
+        to_dev(*S, tmp.data(), S->psi.p);
+    });
+}
+
+int gm_set_state(gm_solver *S, const double *psi)
+{
+    return guarded(S, [&] { to_dev(*S, psi, S->psi.p); });
+}
+
+int gm_get_state(const gm_solver *S, double *psi)
+{
+    gm_solver *M = const_cast<gm_solver *>(S);
+    return guarded(M, [&] { to_host(*M, M->psi.p, psi); });
+}
+
+int gm_initialize(gm_solver *S)
+{
+    return guarded(S, [&] {
+        if (S->absorption) {
+            S->prof[0] = Profile{};
+            S->prof[1] = Profile{};
+            S->mode00 = -1;
+            for (int j = 0; j < S->K; ++j)
+                if (S->ml[j] == 0 && S->mm[j] == 0) S->mode00 = j;
+            for (Sink &sk : S->sink) {
+                sk.v.clear();
+                sk.n = 0;
+                sk.strength = 0.0;
+                sk.norm2 = 0.0;
+            }
+        }
+        S->t_i = 0.0;
+        build_perturbation(*S, 0.0);
+        S->initialized = true;
+        HIPCK(hipStreamSynchronize(S->st));
+    });
+}
+
+int gm_build_damping_mask(gm_solver *S, double f, double W, int32_t n)
+{
+    return guarded(S, [&] {
+        S->damping.assign(S->Nr, 1.0);
+        S->mask = true;
+        const int idx = static_cast<int>(std::floor(S->L * f / S->dr));
+        const double r_mask = f * S->L, L_mask = (1 - f) * S->L;
+        for (int i = 0; i < S->Nr; ++i)
+            if (i >= idx) S->damping[i] = std::exp(-W * std::pow((S->r[i] - r_mask) / L_mask, n));
+        S->dmask.alloc(S->Nr);
+        HIPCK(hipMemcpyAsync(S->dmask.p, S->damping.data(), sizeof(double) * S->Nr, hipMemcpyHostToDevice, S->st));
+        HIPCK(hipStreamSynchronize(S->st));
+    });
+}
+
+int gm_build_cap(gm_solver *S, double f, double W, int32_t n)
+{
+    return guarded(S, [&] {
+        S->capv.assign(S->Nr, make_double2(0.0, 0.0));
+        S->cap = true;
+        const int idx = static_cast<int>(std::floor(S->L * f / S->dr));
+        const double r_cap = f * S->L, L_cap = (1 - f) * S->L;
+        for (int i = 0; i < S->Nr; ++i)
+            if (i >= idx) S->capv[i] = make_double2(0.0, -W * std::pow((S->r[i] - r_cap) / (S->r[i] - L_cap), n));
+        S->dcap.alloc(S->Nr);
+        HIPCK(hipMemcpyAsync(S->dcap.p, S->capv.data(), sizeof(double2) * S->Nr, hipMemcpyHostToDevice, S->st));
+        S->cap_version++;
+        HIPCK(hipStreamSynchronize(S->st));
+    });
+}
+
+int gm_begin_run(gm_solver *S, double r_ex, int32_t n_steps)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        S->r_ex = r_ex;
+        S->idx_ex = static_cast<int>(std::floor(r_ex / S->dr));
+        if (n_steps < 1 || S->idx_ex < 1 || S->idx_ex >= S->Nr - 1)
+            throw std::invalid_argument("Require at least one time sample and an interior extraction radius");
+        S->t_i = 0.0;
+        build_perturbation(*S, S->t_i);
+        HIPCK(hipStreamSynchronize(S->st));
+    });
+}
+
+int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
+{
+    (void)step_id;
+    return guarded(S, [&] {
+        require_initialized(*S);
+        if (!S->flags.use_block_preconditioner)
+            throw std::runtime_error("ILUT preconditioner is not available in this build");
+        const double step_start = S->t_i;
+        S->t_i += S->dt;
+        S->theta += (S->counter ? -1.0 : 1.0) * S->Omega * S->dt;
+        S->phase += S->Omega * S->dt;
+        build_perturbation(*S, S->t_i);
+        const gmd::StepParams &P = *S->hP.p;
+        solve_cn(*S);
+        // diagnostics on the post-mask state (extraction.cpp), with Gamma psi
+        if (P.nsink) sink_proj(*S, S->psi.p, gmk::SD_POST, 1.0, 0);
+        {
+            gmk::ApplyArgs a = apply_args(*S, S->psi.p, nullptr);
+            HIPCK(gmk::launch_apply(a, gmk::AM_DIAG, false, S->st));
+            gmk::DiagArgs da{};
+            da.N = S->Nr;
+            da.K = S->K;
+            da.ix = S->idx_ex;
+            da.ntiles = S->ntiles;
+            da.dr = S->dr;
+            da.eps = S->epsilon;
+            da.mu = S->mu;
+            da.Psi = S->psi.p;
+            da.dpsi_ix = S->dpsi_ix.p;
+            da.rowpart = S->rowpart.p;
+            da.out = S->diagout.p;
+            HIPCK(gmk::launch_diag_final(da, S->st));
+        }
+        HIPCK(hipMemcpyAsync(S->hdiag.p, S->diagout.p, sizeof(double) * (4 * S->K + 4), hipMemcpyDeviceToHost, S->st));
+        HIPCK(hipMemcpyAsync(S->hflags.p, S->dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S->st));
+        HIPCK(hipMemcpyAsync(S->hresid.p, S->dresid.p, sizeof(double), hipMemcpyDeviceToHost, S->st));
+        HIPCK(hipMemcpyAsync(S->hK.p, S->dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S->st));
+        HIPCK(hipStreamSynchronize(S->st));
+        if (S->hflags.p[0]) {
+            S->fac_valid = false;
+            int zero = 0;
+            HIPCK(hipMemcpy(S->dflags.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+            throw std::runtime_error("Radial preconditioner found a zero pivot; the ILUT fallback is not available");
+        }
+        S->last_residual = S->hresid.p[0];
+        if (!S->hflags.p[1]) throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
+        const gmd::Krylov &K = *S->hK.p;
+        // absorbed mass: mu dr dt sum_a g_a |v_a^H psi_mid|^2 (CN balance)
+        double absorbed = 0.0, Mdot[2] = {0.0, 0.0};
+        for (int s = 0; s < P.nsink; ++s) {
+            const double mr = 0.5 * (K.qold[s].x + K.qnew[s].x), mi = 0.5 * (K.qold[s].y + K.qnew[s].y);
+            absorbed += S->mu * S->dr * S->dt * P.g[s] * (mr * mr + mi * mi);
+            // update_time_dependent_increment uses the post-mask midpoint (evolution.cpp:20-30)
+            const double pr = 0.5 * (K.qpost[s].x + K.qold[s].x), pi = 0.5 * (K.qpost[s].y + K.qold[s].y);
+            Mdot[P.sink_body[s]] = S->mu * S->dr * P.g[s] * (pr * pr + pi * pi);
+        }
+        const double Omega_diag = S->Omega_V;
+        if (!S->isolated && S->flags.evolve_orbit && S->t_i > S->ramp_end_time()) {
+            // update_time_dependent_increment (evolution.cpp:17-72)
+            const double orbit_dt = S->t_i - std::max(step_start, S->ramp_end_time());
+            const double Mdot1 = S->absorption ? Mdot[0] : 0.0, Mdot2 = S->absorption ? Mdot[1] : 0.0;
+            double ramp = 1.0;
+            if (S->t_i < S->ramp_end_time()) ramp = 0.0;
+            else if (S->t_i < S->ramp_end_time() + S->initial_period) {
+                const double xx = (S->t_i - S->ramp_end_time()) / S->initial_period;
+                ramp = 3.0 * xx * xx - 2.0 * xx * xx * xx;
+            }
+            const double dR_gw = -(64.0 / 5.0) * S->M1 * S->M2 * (S->M1 + S->M2) / std::pow(S->R, 3);
+            const double dR_cloud = -2.0 * S->R * S->R * S->p_bin / (S->M1 * S->M2);
+            const double dRdt = (dR_gw + dR_cloud) * ramp;
+            S->R += dRdt * orbit_dt;
+            S->M1 += Mdot1 * orbit_dt;
+            S->M2 += Mdot2 * orbit_dt;
+            S->q = S->M2 / S->M1;
+            const double Mt = S->M1 + S->M2;
+            S->R1 = S->M2 / Mt * S->R;
+            S->R2 = S->M1 / Mt * S->R;
+            S->Omega = std::sqrt(Mt / std::pow(S->R, 3));
+            S->P = 2.0 * M_PI / S->Omega;
+            if (S->ramp_factor == 1.0) S->dt = S->P / S->flags.steps_per_period;
+        }
+        const double *h = S->hdiag.p;
+        const int Kn = S->K;
+        const double sign = S->counter ? -1.0 : 1.0;
+        const double p_bin_last = 2.0 * sign * Omega_diag * h[4 * Kn + 1] * S->dr;
+        S->p_bin = S->isolated ? 0.0 : S->avg->update(S->t_i, S->phase, p_bin_last);
+        if (d) {
+            d->energy = h[4 * Kn + 0] * S->dr;
+            d->p_bin = S->p_bin;
+            d->p_bin_inst = p_bin_last;
+            d->dV_bin_dt = h[4 * Kn + 2] * S->dr;
+            d->cloud_mass = S->mu * S->dr * h[4 * Kn + 3];
+            d->absorbed = absorbed;
+            if (d->flux) std::memcpy(d->flux, h, sizeof(double) * Kn);
+            if (d->energy_flux) std::memcpy(d->energy_flux, h + Kn, sizeof(double) * Kn);
+            if (d->norms) std::memcpy(d->norms, h + 2 * Kn, sizeof(double) * Kn);
+            if (d->energies) std::memcpy(d->energies, h + 3 * Kn, sizeof(double) * Kn);
+        }
+    });
+}
+
+int gm_get_scalars(const gm_solver *S, gm_scalars *o)
+{
+    o->t = S->t_i;
+    o->theta = S->theta;
+    o->phase = S->phase;
+    o->R = S->R;
+    o->M1 = S->M1;
+    o->M2 = S->M2;
+    o->q = S->q;
+    o->R1 = S->R1;
+    o->R2 = S->R2;
+    o->Omega = S->Omega;
+    o->P = S->P;
+    o->dt = S->dt;
+    o->p_bin = S->p_bin;
+    o->Omega_0 = S->Omega_0;
+    o->last_solver_residual = S->last_residual;
+    o->last_solver_iterations = S->last_iterations;
+    o->sink_active = S->sink_active;
+    return GM_OK;
+}
+
+int gm_set_scalars(gm_solver *S, const gm_scalars *o)
+{
+    S->t_i = o->t;
+    S->theta = o->theta;
+    S->phase = o->phase;
+    S->R = o->R;
+    S->M1 = o->M1;
+    S->M2 = o->M2;
+    S->q = o->q;
+    S->R1 = o->R1;
+    S->R2 = o->R2;
+    S->Omega = o->Omega;
+    S->P = o->P;
+    S->dt = o->dt;
+    S->p_bin = o->p_bin;
+    return GM_OK;
+}
+
+int gm_apply_H(gm_solver *S, const double *in, double *out)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        apply_H_host(*S, in, out);
+    });
+}
+
+int gm_apply_sink(gm_solver *S, const double *in, double *out)
+{
+    return guarded(S, [&] {
+        std::fill(out, out + 2 * S->nvec(), 0.0);
+        host_sink(*S, in, out, 1.0);
+    });
+}
+
+int gm_schrodinger_rhs(gm_solver *S, const double *in, double *out)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        apply_H_host(*S, in, out);
+        for (size_t q = 0; q < S->nvec(); ++q) {
+            const double re = out[2 * q], im = out[2 * q + 1];
+            out[2 * q] = im;
+            out[2 * q + 1] = -re;
+        }
+        host_sink(*S, in, out, -0.5);
+    });
+}
+
+double gm_sink_strength(const gm_solver *S, int32_t body)
+{
+    const Sink &sk = S->sink[body == 1 ? 0 : 1];
+    if (!S->sink_active || sk.strength == 0.0) return 0.0;
+    return sk.strength * sk.norm2;
+}
+
+}  // extern "C"
