@@ -1,0 +1,221 @@
+"""Benchmark of the MI355X Crank-Nicolson step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+
+Workload (one GPU): the plain FP64 drop-in at scale -- binary q=0.5, R=50 on a
+fixed orbit with both s-wave absorbers on, boson mass mu=0.15, N=16384 radial
+nodes over L=4000, l_max=20 (231 modes), dt=0.25, damping mask (0.85, 1, 6),
+r_ex=0.8 L; initial state the 2p atom on the primary translated to the centre
+of mass plus a seeded 1e-3 complex Gaussian perturbation (synthetic inputs).
+
+A step is one CrankNicolsonModes::step: the CN solve with its BiCGSTAB, the
+mask and every diagnostic. The metric is mode.node-steps/s = K*N*steps/time.
+The JSON line also carries the dominant kernel's roofline (HIP-event kernel
+times on the solver's stream) and a CPU baseline timed on the host: the FP64
+oracle on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "boson-cloud-binary_amd"))
+
+import gm  # noqa: E402
+from gm import initial  # noqa: E402
+
+METRIC = "CN mode·node-steps/s, mixed vs FP64 (≤1e-8 rel err); p99 served step latency"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+WORKLOAD = dict(Nr=16384, L=4000.0, dt=0.25, epsilon=0.0, l_max=20, mu=0.15, M_c=0.01, q=0.5, a=0.0, R=50.0,
+                counter=False, absorption=True, save_every=50, ramp_periods=0.0, epsilon_smear=2.0)
+MASK = (0.85, 1.0, 6)
+PROF_NAMES = ["apply_krylov", "apply_other", "precond", "precond_setup", "update", "factor", "small"]
+
+
+def initial_state(w, seed=0):
+    """Translated 2p atom + perturbation, cached in /tmp by parameters."""
+    key = "gm_init_%d_%g_%d_%g_%g_%g_%d.npy" % (w["Nr"], w["L"], w["l_max"], w["mu"], w["q"], w["R"], seed)
+    path = os.path.join("/tmp", key)
+    if os.path.exists(path):
+        return np.load(path)
+    dr = w["L"] / (w["Nr"] - 1)
+    r = w["epsilon"] + dr * np.arange(w["Nr"])
+    R1 = w["q"] / (1 + w["q"]) * w["R"]
+    base = initial.translated_atom(r, w["l_max"], w["mu"], R1)
+    base /= math.sqrt(np.vdot(base, base).real * dr)
+    K = (w["l_max"] + 1) * (w["l_max"] + 2) // 2
+    psi = initial.perturbed_state(base, w["Nr"], K, rel=1e-3, seed=seed, dr=dr)
+    try:
+        np.save(path, psi)
+    except OSError:
+        pass
+    return psi
+
+
+def coupling_terms(l_max):
+    """Gaunt terms with |m*| <= l* (the multiply-adds the coupling needs per node)."""
+    n = 0
+    modes = initial.modes_for(l_max)
+    ls_m = {}
+    for (l, m) in modes:
+        for (l1, m1) in modes:
+            lo = max(abs(l - l1), abs(m1 - m))
+            if (lo + l + l1) % 2:
+                lo += 1
+            if lo <= l + l1:
+                n += (l + l1 - lo) // 2 + 1
+    return n
+
+
+def make_solver(w, precision):
+    s = gm.CrankNicolsonModes(**w)
+    s.set_flags(evolve_orbit=0, precision=precision)
+    s.set_initial_state_full(initial_state(w))
+    s.initialize()
+    s.build_damping_mask(*MASK)
+    s.begin_run(0.8 * w["L"], 1000)
+    return s
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def profile(s, on):
+    lib = gm.lib()
+    lib.gm_profile_enable.argtypes = [C.c_void_p, C.c_int32]
+    lib.gm_profile_enable.restype = C.c_int
+    s._check(lib.gm_profile_enable(s._h, int(on)))
+
+
+def profile_read(s):
+    lib = gm.lib()
+    ms = (C.c_double * 7)()
+    n = (C.c_int64 * 7)()
+    lib.gm_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
+    lib.gm_profile_read(s._h, ms, n, 7)
+    return {PROF_NAMES[c]: (ms[c], n[c]) for c in range(7)}
+
+
+def run_gpu(w, precision, steps, warmup):
+    s = make_solver(w, precision)
+    for k in range(1, warmup + 1):
+        s.step(k, diag=True)
+    iters = []
+    profile(s, True)
+    device_sync()
+    t0 = time.perf_counter()
+    for k in range(warmup + 1, warmup + steps + 1):
+        s.step(k, diag=True)
+        iters.append(s.last_solver_iterations)
+    device_sync()
+    t1 = time.perf_counter()
+    prof = profile_read(s)
+    profile(s, False)
+    return s, t1 - t0, prof, iters
+
+
+def cpu_baseline(w, budget_s=20.0):
+    """The FP64 oracle on the host cores: one step of the same workload on a
+    radial sub-grid of the same spacing (all 231 modes, fewer nodes)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # sub-grid: keep dr, l_max and the physics, cut the node count
+    Nsub = 2048
+    ws = dict(w)
+    ws["Nr"] = Nsub
+    ws["L"] = w["L"] / (w["Nr"] - 1) * (Nsub - 1)
+    o = oracle.Oracle(**ws)
+    o.set_flags(evolve_orbit=0)
+    psi = initial_state(w).reshape(-1, w["Nr"])[:, :Nsub].copy()
+    psi[:, -1] = 0
+    o.set_initial_state_full(psi.ravel())
+    o.initialize()
+    o.build_damping_mask(*MASK)
+    o.begin_run(0.8 * ws["L"], 10)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        o.step(n + 1, diag=True)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 3:
+            break
+    dt = time.perf_counter() - t0
+    K = o.n_modes
+    return {"value": K * Nsub * n / dt, "unit": "mode·node-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 oracle step(s), all {K} modes on a {Nsub}-node sub-grid of the same dr",
+            "iterations_per_step": o.last_solver_iterations}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every box here has one MI355X and this build "
+              "has no multi-GPU path (ensembles would shard whole members per GPU).", file=sys.stderr)
+        sys.exit(2)
+    w = dict(WORKLOAD)
+    precision = gm.GM_PRECISION_MIXED if args.precision == "mixed" else gm.GM_PRECISION_FP64
+    s, elapsed, prof, iters = run_gpu(w, precision, args.steps, args.warmup)
+    K, N = s.n_modes, s.Nr
+    value = K * N * args.steps / elapsed
+    # roofline of the dominant kernel class
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, cnt = prof[dom]
+    avg_s = ms / max(cnt, 1) * 1e-3
+    nt = coupling_terms(w["l_max"])
+    if dom.startswith("apply"):
+        flops = (N - 2) * (2.0 * nt + 4.0 * K * K + 20.0 * K)
+        roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "kernel": "k_apply (" + dom + ")", "per_launch": "%.4g flop" % flops}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    else:
+        passes = {"precond": 6, "precond_setup": 3, "update": 8}.get(dom, 2)
+        byts = passes * 16.0 * K * N
+        roof = {"bound": "hbm", "achieved": byts / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "kernel": dom, "per_launch": "%.4g B" % byts}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["avg_launch_ms"] = ms / max(cnt, 1)
+    out = {
+        "metric": METRIC, "value": value, "unit": "mode·node-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if precision == gm.GM_PRECISION_FP64 else "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "plain FP64 CN step, l_max=20 (K=231), N=16384, L=4000, q=0.5, R=50, mu=0.15, "
+                               "dt=0.25, fixed orbit, absorbers on, mask", "K": K, "N": N,
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": roof,
+        "iterations_per_step": float(np.mean(iters)) if iters else None,
+        "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
+        "kernel_launches": {k: int(v[1]) for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(w)
+        except Exception as e:  # the baseline never hides the GPU number
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -153,6 +153,16 @@ struct gm_solver {
     int fac_cap_version = -1;
     int prev_iters = 4;
     std::string err;
+    // profiling (gm_profile_enable)
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending {
+        int cat;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    double prof_ms[GM_PROF_NCAT] = {};
+    int64_t prof_n[GM_PROF_NCAT] = {};
 
     size_t nvec() const { return static_cast<size_t>(Nr) * K; }
     double ramp_end_time() const { return ramp_periods * initial_period; }
@@ -166,6 +176,50 @@ void ensure_device()
 {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw DeviceError("no HIP device is available");
+}
+
+
+// ------------------------------------------------------------------ profiling
+
+hipEvent_t take_event(gm_solver &S)
+{
+    if (S.ev_pool.empty()) {
+        hipEvent_t e;
+        HIPCK(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = S.ev_pool.back();
+    S.ev_pool.pop_back();
+    return e;
+}
+
+// Launch `f` (returning hipError_t) on the solver stream, timed when profiling.
+template <class F>
+void launch(gm_solver &S, int cat, F &&f)
+{
+    if (!S.prof_on) {
+        HIPCK(f());
+        return;
+    }
+    hipEvent_t a = take_event(S), b = take_event(S);
+    HIPCK(hipEventRecord(a, S.st));
+    HIPCK(f());
+    HIPCK(hipEventRecord(b, S.st));
+    S.pending.push_back({cat, a, b});
+}
+
+// After a stream synchronisation: fold the recorded intervals into the totals.
+void prof_collect(gm_solver &S)
+{
+    for (auto &p : S.pending) {
+        float ms = 0.f;
+        HIPCK(hipEventElapsedTime(&ms, p.a, p.b));
+        S.prof_ms[p.cat] += ms;
+        S.prof_n[p.cat] += 1;
+        S.ev_pool.push_back(p.a);
+        S.ev_pool.push_back(p.b);
+    }
+    S.pending.clear();
 }
 
 // ------------------------------------------------------------------ tables
@@ -374,8 +428,10 @@ void build_perturbation(gm_solver &S, double time)
     // radial profiles: rebuilt only when the orbit or the companion weight changed
     const double key[6] = {S.R1, S.R2, S.q, S.M1 + S.M2, S.prf, S.mu};
     if (!S.E_valid || std::memcmp(key, S.E_key, sizeof(key)) != 0) {
-        HIPCK(gmk::launch_build_E(S.Nr, 2 * S.l_max + 1, S.dr, S.epsilon, S.R1, S.R2, S.q, S.M1 + S.M2, S.mu, S.prf,
-                                  S.E.p, S.st));
+        launch(S, GM_PROF_FACTOR, [&] {
+            return gmk::launch_build_E(S.Nr, 2 * S.l_max + 1, S.dr, S.epsilon, S.R1, S.R2, S.q, S.M1 + S.M2, S.mu,
+                                       S.prf, S.E.p, S.st);
+        });
         std::memcpy(S.E_key, key, sizeof(key));
         S.E_valid = true;
         S.fac_valid = false;
@@ -466,7 +522,7 @@ void factorize(gm_solver &S)
     f.cap = S.cap ? S.dcap.p : nullptr;
     f.inv = S.inv.p;
     f.failed = S.dflags.p;
-    HIPCK(gmk::launch_factor(f, S.st));
+    launch(S, GM_PROF_FACTOR, [&] { return gmk::launch_factor(f, S.st); });
     std::memcpy(S.fac_key, key, sizeof(key));
     S.fac_cap_version = S.cap_version;
     S.fac_valid = true;
@@ -485,7 +541,7 @@ void sink_proj(gm_solver &S, const double2 *X, int dest, double scale, int by_nr
     fill_sinks(S, a.sk);
     a.Kr = S.dK.p;
     a.P = S.dP.p;
-    HIPCK(gmk::launch_sink(a, S.st));
+    launch(S, GM_PROF_SMALL, [&] { return gmk::launch_sink(a, S.st); });
 }
 
 // Woodbury columns W_a = P^-1 v_a and the capacitance (matrices.cpp:133-145).
@@ -500,11 +556,11 @@ void woodbury_setup(gm_solver &S)
         sd.lo = sk.lo;
         sd.n = sk.n;
         sd.v = sk.dv.p;
-        HIPCK(gmk::launch_expand(S.Nr, S.K, sd, S.tmp.p, S.st));
+        launch(S, GM_PROF_SMALL, [&] { return gmk::launch_expand(S.Nr, S.K, sd, S.tmp.p, S.st); });
         pa.X = S.tmp.p;
         pa.Y = sk.W.p;
         pa.wood = 0;
-        HIPCK(gmk::launch_precond(pa, gmk::PV_PLAIN, S.st));
+        launch(S, GM_PROF_PRECOND_SETUP, [&] { return gmk::launch_precond(pa, gmk::PV_PLAIN, S.st); });
     }
     // exact non-zero node range of each column
     S.hwr.p[0] = S.hwr.p[2] = S.Nr;
@@ -512,7 +568,7 @@ void woodbury_setup(gm_solver &S)
     gmd::Krylov *dk = S.dK.p;
     HIPCK(hipMemcpyAsync(&dk->wrange[0], S.hwr.p, 4 * sizeof(int), hipMemcpyHostToDevice, S.st));
     for (int s = 0; s < P.nsink; ++s)
-        HIPCK(gmk::launch_wrange(S.Nr, S.K, S.sink[P.sink_body[s]].W.p, &dk->wrange[2 * s], S.st));
+        launch(S, GM_PROF_SMALL, [&] { return gmk::launch_wrange(S.Nr, S.K, S.sink[P.sink_body[s]].W.p, &dk->wrange[2 * s], S.st); });
     gmk::SinkArgs a{};
     a.N = S.Nr;
     a.K = S.K;
@@ -520,7 +576,7 @@ void woodbury_setup(gm_solver &S)
     fill_sinks(S, a.sk);
     a.Kr = S.dK.p;
     a.P = S.dP.p;
-    HIPCK(gmk::launch_sink(a, S.st));
+    launch(S, GM_PROF_SMALL, [&] { return gmk::launch_sink(a, S.st); });
 }
 
 void enqueue_iteration(gm_solver &S, bool wood)
@@ -532,19 +588,19 @@ void enqueue_iteration(gm_solver &S, bool wood)
     pa.Pv = S.p.p;
     pa.V = S.v.p;
     pa.Y = S.y.p;
-    HIPCK(gmk::launch_precond(pa, gmk::PV_P, S.st));
+    launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_P, S.st); });
     gmk::ApplyArgs a = apply_args(S, S.y.p, S.v.p);
     a.Xc = S.yc.p;
     a.R0 = S.r0.p;
-    HIPCK(gmk::launch_apply(a, gmk::AM_V, wood, S.st));
+    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_V, wood, S.st); });
     // z = M^-1 s with s = r - alpha v
     pa.Pv = S.s.p;
     pa.Y = S.z.p;
-    HIPCK(gmk::launch_precond(pa, gmk::PV_S, S.st));
+    launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_S, S.st); });
     a = apply_args(S, S.z.p, S.t.p);
     a.Xc = S.zc.p;
     a.R0 = S.s.p;
-    HIPCK(gmk::launch_apply(a, gmk::AM_T, wood, S.st));
+    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_T, wood, S.st); });
     gmk::UpdateArgs u{};
     u.n = S.nvec();
     u.Kr = S.dK.p;
@@ -556,13 +612,14 @@ void enqueue_iteration(gm_solver &S, bool wood)
     u.R0 = S.r0.p;
     u.R = S.rr.p;
     u.partial = S.partial.p;
-    HIPCK(gmk::launch_update(u, S.st));
+    launch(S, GM_PROF_UPDATE, [&] { return gmk::launch_update(u, S.st); });
 }
 
 void sync_krylov(gm_solver &S)
 {
     HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
     HIPCK(hipStreamSynchronize(S.st));
+    prof_collect(S);
 }
 
 // solve_CN (matrices.cpp:94-215) with the block preconditioner. Returns
@@ -579,7 +636,7 @@ bool solve_cn(gm_solver &S)
     {
         gmk::ApplyArgs a = apply_args(S, S.psi.p, S.b.p);
         a.Y2 = S.ax.p;
-        HIPCK(gmk::launch_apply(a, gmk::AM_RHS, false, S.st));
+        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RHS, false, S.st); });
     }
     // Eigen's BiCGSTAB without sinks, the reference's own recurrence with them
     const int maxit = 500, eigen_mode = wood ? 0 : 1;
@@ -599,7 +656,7 @@ bool solve_cn(gm_solver &S)
         ia.maxit = maxit;
         ia.eigen_mode = eigen_mode;
         ia.partial = S.partial.p;
-        HIPCK(gmk::launch_init(ia, S.st));
+        launch(S, GM_PROF_SMALL, [&] { return gmk::launch_init(ia, S.st); });
     }
     int batch = std::max(1, S.prev_iters);
     int enqueued = 0;
@@ -620,7 +677,7 @@ bool solve_cn(gm_solver &S)
     {
         gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
         a.R0 = S.b.p;
-        HIPCK(gmk::launch_apply(a, gmk::AM_RES, false, S.st));
+        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RES, false, S.st); });
     }
     if (wood) sink_proj(S, S.x.p, gmk::SD_QNEW, 1.0, 1);
     gmk::AcceptArgs ac{};
@@ -632,7 +689,7 @@ bool solve_cn(gm_solver &S)
     ac.Psi = S.psi.p;
     ac.status = S.dflags.p + 1;
     ac.resid = S.dresid.p;
-    HIPCK(gmk::launch_accept(ac, S.st));
+    launch(S, GM_PROF_SMALL, [&] { return gmk::launch_accept(ac, S.st); });
     return true;
 }
 
@@ -825,7 +882,31 @@ void gm_destroy(gm_solver *S)
         (void)hipStreamSynchronize(S->st);
         (void)hipStreamDestroy(S->st);
     }
+    for (auto &p : S->pending) S->ev_pool.insert(S->ev_pool.end(), {p.a, p.b});
+    for (hipEvent_t e : S->ev_pool) (void)hipEventDestroy(e);
     delete S;
+}
+
+int gm_profile_enable(gm_solver *S, int32_t on)
+{
+    return guarded(S, [&] {
+        HIPCK(hipStreamSynchronize(S->st));
+        prof_collect(*S);
+        S->prof_on = on != 0;
+        for (int c = 0; c < GM_PROF_NCAT; ++c) {
+            S->prof_ms[c] = 0.0;
+            S->prof_n[c] = 0;
+        }
+    });
+}
+
+int gm_profile_read(const gm_solver *S, double *ms, int64_t *launches, int32_t ncat)
+{
+    for (int c = 0; c < ncat && c < GM_PROF_NCAT; ++c) {
+        ms[c] = S->prof_ms[c];
+        launches[c] = S->prof_n[c];
+    }
+    return GM_OK;
 }
 
 int gm_n_modes(const gm_solver *S) { return S->K; }
@@ -992,7 +1073,7 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
         if (P.nsink) sink_proj(*S, S->psi.p, gmk::SD_POST, 1.0, 0);
         {
             gmk::ApplyArgs a = apply_args(*S, S->psi.p, nullptr);
-            HIPCK(gmk::launch_apply(a, gmk::AM_DIAG, false, S->st));
+            launch(*S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_DIAG, false, S->st); });
             gmk::DiagArgs da{};
             da.N = S->Nr;
             da.K = S->K;
@@ -1005,13 +1086,14 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
             da.dpsi_ix = S->dpsi_ix.p;
             da.rowpart = S->rowpart.p;
             da.out = S->diagout.p;
-            HIPCK(gmk::launch_diag_final(da, S->st));
+            launch(*S, GM_PROF_SMALL, [&] { return gmk::launch_diag_final(da, S->st); });
         }
         HIPCK(hipMemcpyAsync(S->hdiag.p, S->diagout.p, sizeof(double) * (4 * S->K + 4), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipMemcpyAsync(S->hflags.p, S->dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipMemcpyAsync(S->hresid.p, S->dresid.p, sizeof(double), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipMemcpyAsync(S->hK.p, S->dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipStreamSynchronize(S->st));
+        prof_collect(*S);
         if (S->hflags.p[0]) {
             S->fac_valid = false;
             int zero = 0;

@@ -156,6 +156,22 @@ int gm_apply_sink(gm_solver *solver, const double *in, double *out);
 int gm_schrodinger_rhs(gm_solver *solver, const double *in, double *out);
 double gm_sink_strength(const gm_solver *solver, int32_t body);
 
+/* Measurement hook (no counterpart in the reference; for benchmarks): device
+ * time per kernel class, recorded with HIP events on the solver's stream.
+ * gm_profile_enable resets the counters; gm_profile_read fills ncat entries. */
+enum {
+    GM_PROF_APPLY_KRYLOV = 0, /* operator apply inside BiCGSTAB (v = A y, t = A z) */
+    GM_PROF_APPLY_OTHER = 1,  /* rhs, acceptance and diagnostics applies */
+    GM_PROF_PRECOND = 2,      /* Thomas solves inside BiCGSTAB */
+    GM_PROF_PRECOND_SETUP = 3,/* Woodbury columns P^-1 v */
+    GM_PROF_UPDATE = 4,       /* x, r update and loop control */
+    GM_PROF_FACTOR = 5,       /* Thomas pivots and radial profiles */
+    GM_PROF_SMALL = 6,        /* sink projections, init, accept, diagnostics sums */
+    GM_PROF_NCAT = 7
+};
+int gm_profile_enable(gm_solver *solver, int32_t on);
+int gm_profile_read(const gm_solver *solver, double *ms, int64_t *launches, int32_t ncat);
+
 #ifdef __cplusplus
 }
 #endif
