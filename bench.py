@@ -95,19 +95,11 @@ def device_sync():
 
 
 def profile(s, on):
-    lib = gm.lib()
-    lib.gm_profile_enable.argtypes = [C.c_void_p, C.c_int32]
-    lib.gm_profile_enable.restype = C.c_int
-    s._check(lib.gm_profile_enable(s._h, int(on)))
+    s.profile(on)
 
 
 def profile_read(s):
-    lib = gm.lib()
-    ms = (C.c_double * 7)()
-    n = (C.c_int64 * 7)()
-    lib.gm_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
-    lib.gm_profile_read(s._h, ms, n, 7)
-    return {PROF_NAMES[c]: (ms[c], n[c]) for c in range(7)}
+    return {PROF_NAMES[c]: v for c, v in enumerate(s.profile_read())}
 
 
 def run_gpu(w, precision, steps, warmup):
@@ -182,9 +174,10 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     ms, cnt = prof[dom]
     avg_s = ms / max(cnt, 1) * 1e-3
-    nt = coupling_terms(w["l_max"])
+    terms, pairs = s.coupling_work()
     if dom.startswith("apply"):
-        flops = (N - 2) * (2.0 * nt + 4.0 * K * K + 20.0 * K)
+        # Gaunt terms (1 FMA each), pair matvec (complex x real, 2 FMA), stencil/CN (~10 FMA per mode.node)
+        flops = 2.0 * terms + 4.0 * pairs + 20.0 * K * N
         roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "kernel": "k_apply (" + dom + ")", "per_launch": "%.4g flop" % flops}
         roof["frac"] = roof["achieved"] / roof["peak"]
@@ -208,6 +201,8 @@ def main():
         "iterations_per_step": float(np.mean(iters)) if iters else None,
         "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
         "kernel_launches": {k: int(v[1]) for k, v in prof.items()},
+        "coupling_work": {"terms_per_apply": terms, "pairs_per_apply": pairs,
+                          "untruncated_terms_per_apply": float(coupling_terms(w["l_max"])) * (N - 2)},
     }
     if not args.no_cpu_baseline:
         try:

@@ -6,7 +6,7 @@
 //   E table           2 * (2 l_max + 1) * Nr doubles: the radial multipole
 //                     profiles c_l* (f1 + sign prf f2) for even (index 0) and
 //                     odd (index 1) m*-parity pairs;
-//   G table           per (l, l1) block, [m][m1][s] doubles: the Gaunt
+//   G table           per (l, l1) block, [m1][m][s] doubles: the Gaunt
 //                     coefficient of modes (l, m), (l1, m1) at l* = |l-l1| + 2s;
 //   preconditioner    K*Nr double2 inverse pivots of each mode's radial tridiagonal.
 #pragma once
@@ -35,6 +35,7 @@ struct Geometry {
     const int *goff;           // (l_max+1)^2 offsets into G
     const double *G;
     const double *E;           // 2 * Ls * N
+    const int *lcut;           // per node tile: largest multipole l* evaluated
     const double2 *cap;        // Nr diagonal CAP or nullptr
 };
 

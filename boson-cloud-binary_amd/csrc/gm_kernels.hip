@@ -144,6 +144,30 @@ __global__ void k_build_E(int N, int Ls, double dr, double eps, double R1, doubl
     E[(size_t)(Ls + ls) * N + i] = e1;
 }
 
+// Largest multipole l* of each node tile whose profile exceeds tau times the
+// tile's leading profile (one wavefront per tile). tau = 0 keeps every l*.
+__global__ __launch_bounds__(64) void k_lcut(int N, int Ls, double tau, const double *E, int *lcut)
+{
+    const int tile = blockIdx.x;
+    const int i = tile * gmd::TILE + threadIdx.x;
+    double gmax = 0.0;
+    for (int ls = 0; ls < Ls; ++ls) {
+        double v = 0.0;
+        if (i < N) v = fmax(fabs(E[(size_t)ls * N + i]), fabs(E[(size_t)(Ls + ls) * N + i]));
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+        gmax = fmax(gmax, v);
+    }
+    int keep = tau > 0.0 ? 0 : Ls - 1;
+    if (tau > 0.0)
+        for (int ls = 0; ls < Ls; ++ls) {
+            double v = 0.0;
+            if (i < N) v = fmax(fabs(E[(size_t)ls * N + i]), fabs(E[(size_t)(Ls + ls) * N + i]));
+            for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+            if (v > tau * gmax) keep = ls;
+        }
+    if (threadIdx.x == 0) lcut[tile] = keep;
+}
+
 // ---------------------------------------------------------------- operator apply
 
 template <bool CORR>
@@ -172,17 +196,69 @@ __device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, cons
     return t;
 }
 
-template <int SMAX, int MODE, bool CORR>
+// Coupling of the rows of chunk C with the modes of l1 at node ic, for a
+// (l, l1) block with S multipole slots: W = sum_s G[m][m1][s] E_s(r) formed
+// per node, then acc[m] += W * exp(i m1 theta) x_(l1,m1).
+template <int S, bool CORR>
+__device__ __forceinline__ void coupling_block(const ApplyArgs &a, const Krylov *Kr, int nsink, const double2 (&cw)[2],
+                                               const StepParams *P, const Chunk &C, int l, int l1, int ic, int Sfull,
+                                               double2 (&acc)[gmd::CH])
+{
+    const Geometry &g = a.g;
+    const int N = g.N, lmax = g.lmax;
+    const int lo = l > l1 ? l - l1 : l1 - l;
+    const int p = (l + l1) & 1;
+    double e[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) e[s] = g.E[(size_t)(p * g.Ls + lo + 2 * s) * N + ic];
+    // block layout [m1][m][s]: the rows of one m1 are contiguous with a
+    // compile-time stride S
+    const double *__restrict__ Gb = g.G + g.goff[l * (lmax + 1) + l1] + (size_t)C.m0 * Sfull;
+    const int kb = l1 * (l1 + 1) / 2;
+    const int m1s = (l + 1) * Sfull;
+    double2 xn = load_x<CORR>(a, Kr, nsink, cw, kb, ic);
+    for (int m1i = 0; m1i <= l1; ++m1i) {
+        const double2 xv = xn;
+        if (m1i < l1) xn = load_x<CORR>(a, Kr, nsink, cw, kb + m1i + 1, ic);
+        const double2 xr = cmul(P->ph[lmax - l1 + 2 * m1i], xv);
+        const double *__restrict__ Gr = Gb + (size_t)m1i * m1s;
+#pragma unroll
+        for (int c = 0; c < gmd::CH; ++c) {
+            if (c < C.cnt) {
+                const double *__restrict__ Gc = Gr + c * Sfull;
+                double w = 0.0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) w = fma(Gc[s], e[s], w);
+                acc[c].x = fma(w, xr.x, acc[c].x);
+                acc[c].y = fma(w, xr.y, acc[c].y);
+            }
+        }
+    }
+}
+
+template <int SMAX, bool CORR>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
 {
+    const int MODE = a.mode;
     const Geometry &g = a.g;
     Krylov *Kr = a.Kr;
     if ((MODE == AM_V || MODE == AM_T) && Kr->done) return;
     const StepParams *P = a.P;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    // the wave index is uniform across the wave: keep everything derived from
+    // it in scalar registers (scalar loads of the chunk and the Gaunt table)
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = blockIdx.x;
+    // logical block: consecutive logical blocks run on the same XCD (dispatch
+    // is round-robin over the 8 XCDs). A logical block is one row chunk on 4
+    // adjacent node tiles (the 4 waves share the chunk's Gaunt rows in the
+    // scalar cache); consecutive logical blocks walk the chunks of one tile
+    // group, so its psi tiles stay in that XCD's L2.
     const int lb = (b & 7) * a.per + (b >> 3);
-    const int item = lb * 4 + wv;
+    const int tgroup = lb / g.nchunks;
+    const int chn_b = lb - tgroup * g.nchunks;
+    const int tile_w = tgroup * 4 + wv;
+    const bool item_ok = tgroup < (g.ntiles + 3) / 4 && tile_w < g.ntiles;
     const int N = g.N, lmax = g.lmax;
     const int nsink = P->nsink;
     double2 cw[2], coef[2];
@@ -191,8 +267,8 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
         coef[s] = (MODE == AM_RHS || MODE == AM_DIAG) ? Kr->gq[s] : Kr->coefA[s];
     }
     double red[3] = {0.0, 0.0, 0.0};
-    if (item < a.nitems) {
-        const int tile = item / g.nchunks, chn = item - tile * g.nchunks;
+    if (item_ok) {
+        const int tile = tile_w, chn = chn_b;
         const Chunk C = g.chunks[chn];
         const int l = C.l;
         const int i = tile * gmd::TILE + lane;
@@ -201,33 +277,25 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
         double2 acc[gmd::CH];
 #pragma unroll
         for (int c = 0; c < gmd::CH; ++c) acc[c] = make_double2(0.0, 0.0);
-        for (int l1 = 0; l1 <= lmax; ++l1) {
-            const int S = (l < l1 ? l : l1) + 1;
+        const bool have_cin = a.Cin != nullptr;
+        // multipoles above lcut are below 2^-64 of the tile's leading one
+        const int lc = g.lcut ? g.lcut[tile] : 2 * lmax;
+        for (int l1 = 0; l1 <= (have_cin ? -1 : lmax); ++l1) {
             const int lo = l > l1 ? l - l1 : l1 - l;
-            const int p = (l + l1) & 1;
-            double e[SMAX];
-#pragma unroll
-            for (int s = 0; s < SMAX; ++s)
-                e[s] = s < S ? g.E[(size_t)(p * g.Ls + lo + 2 * s) * N + ic] : 0.0;
-            const double *Gb = g.G + g.goff[l * (lmax + 1) + l1];
-            const int kb = l1 * (l1 + 1) / 2;
-            const int rs = (l1 + 1) * S;
-            for (int m1i = 0; m1i <= l1; ++m1i) {
-                const double2 xv = load_x<CORR>(a, Kr, nsink, cw, kb + m1i, ic);
-                const double2 xr = cmul(P->ph[lmax - l1 + 2 * m1i], xv);
-                const double *Gr = Gb + (size_t)(C.m0 * (l1 + 1) + m1i) * S;
-#pragma unroll
-                for (int c = 0; c < gmd::CH; ++c) {
-                    if (c < C.cnt) {
-                        const double *Gc = Gr + (size_t)c * rs;
-                        double w = 0.0;
-#pragma unroll
-                        for (int s = 0; s < SMAX; ++s)
-                            if (s < S) w = fma(Gc[s], e[s], w);
-                        acc[c].x = fma(w, xr.x, acc[c].x);
-                        acc[c].y = fma(w, xr.y, acc[c].y);
-                    }
-                }
+            const int Sfull = (l < l1 ? l : l1) + 1;
+            const int S = lo > lc ? 0 : min(Sfull, (lc - lo) / 2 + 1);
+            switch (S) {
+#define GM_CASE(n) \
+    case n: \
+        if constexpr (n <= SMAX) coupling_block<n, CORR>(a, Kr, nsink, cw, P, C, l, l1, ic, Sfull, acc); \
+        break;
+                GM_CASE(1) GM_CASE(2) GM_CASE(3) GM_CASE(4) GM_CASE(5) GM_CASE(6) GM_CASE(7) GM_CASE(8)
+                GM_CASE(9) GM_CASE(10) GM_CASE(11) GM_CASE(12) GM_CASE(13) GM_CASE(14) GM_CASE(15) GM_CASE(16)
+                GM_CASE(17) GM_CASE(18) GM_CASE(19) GM_CASE(20) GM_CASE(21) GM_CASE(22) GM_CASE(23) GM_CASE(24)
+                GM_CASE(25) GM_CASE(26) GM_CASE(27) GM_CASE(28) GM_CASE(29) GM_CASE(30) GM_CASE(31) GM_CASE(32)
+                GM_CASE(33) GM_CASE(34) GM_CASE(35) GM_CASE(36) GM_CASE(37) GM_CASE(38) GM_CASE(39) GM_CASE(40)
+#undef GM_CASE
+            default: break;
             }
         }
         // epilogue per row
@@ -243,9 +311,11 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
             const int mi = C.m0 + c;
             const int m = -l + 2 * mi;
             const int j = l * (l + 1) / 2 + mi;
-            const double2 Cc = cdotm(P->ph[m + lmax], acc[c]);
+            double2 Cc = cdotm(P->ph[m + lmax], acc[c]);
             if (valid) {
                 const size_t row = (size_t)j * N + i;
+                if (have_cin) Cc = cscale(Kr->rhs_nrm * (a.mask ? a.mask[i] : 1.0), a.Cin[row]);
+                if (a.Cout) a.Cout[row] = Cc;
                 const double2 xi = load_x<CORR>(a, Kr, nsink, cw, j, i);
                 if (CORR) a.Xc[row] = xi;
                 double2 Hx;
@@ -933,45 +1003,31 @@ __global__ __launch_bounds__(256) void k_diag_final(DiagArgs a)
 
 // ---------------------------------------------------------------- launchers
 
-template <int SMAX, int MODE>
+template <int SMAX>
 static hipError_t launch_apply_s(const ApplyArgs &a, bool corr, int grid, hipStream_t st)
 {
-    if (corr) hipLaunchKernelGGL((k_apply<SMAX, MODE, true>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_apply<SMAX, MODE, false>), dim3(grid), dim3(256), 0, st, a);
+    if (corr) hipLaunchKernelGGL((k_apply<SMAX, true>), dim3(grid), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_apply<SMAX, false>), dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
-template <int MODE>
-static hipError_t launch_apply_m(const ApplyArgs &a, bool corr, int grid, hipStream_t st)
+int apply_grid(int ntiles, int nchunks, int *per)
 {
-    const int smax = a.g.lmax + 1;
-    if (smax <= 8) return launch_apply_s<8, MODE>(a, corr, grid, st);
-    if (smax <= 16) return launch_apply_s<16, MODE>(a, corr, grid, st);
-    if (smax <= 24) return launch_apply_s<24, MODE>(a, corr, grid, st);
-    if (smax <= 32) return launch_apply_s<32, MODE>(a, corr, grid, st);
-    return launch_apply_s<40, MODE>(a, corr, grid, st);
-}
-
-int apply_grid(int nitems, int *per)
-{
-    const int nb = (nitems + 3) / 4;
+    const int nb = (ntiles + 3) / 4 * nchunks;
     *per = (nb + 7) / 8;
     return *per * 8;
 }
 
 hipError_t launch_apply(ApplyArgs a, int mode, bool corr, hipStream_t st)
 {
-    const int grid = apply_grid(a.nitems, &a.per);
-    switch (mode) {
-    case AM_H: return launch_apply_m<AM_H>(a, corr, grid, st);
-    case AM_A: return launch_apply_m<AM_A>(a, corr, grid, st);
-    case AM_V: return launch_apply_m<AM_V>(a, corr, grid, st);
-    case AM_T: return launch_apply_m<AM_T>(a, corr, grid, st);
-    case AM_RHS: return launch_apply_m<AM_RHS>(a, corr, grid, st);
-    case AM_RES: return launch_apply_m<AM_RES>(a, corr, grid, st);
-    case AM_DIAG: return launch_apply_m<AM_DIAG>(a, corr, grid, st);
-    }
-    return hipErrorInvalidValue;
+    const int grid = apply_grid(a.g.ntiles, a.g.nchunks, &a.per);
+    a.mode = mode;
+    const int smax = a.g.lmax + 1;
+    if (smax <= 8) return launch_apply_s<8>(a, corr, grid, st);
+    if (smax <= 16) return launch_apply_s<16>(a, corr, grid, st);
+    if (smax <= 24) return launch_apply_s<24>(a, corr, grid, st);
+    if (smax <= 32) return launch_apply_s<32>(a, corr, grid, st);
+    return launch_apply_s<40>(a, corr, grid, st);
 }
 
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt, double mu,
@@ -979,6 +1035,13 @@ hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, doubl
 {
     const int n = Ls * N;
     hipLaunchKernelGGL(k_build_E, dim3((n + 255) / 256), dim3(256), 0, st, N, Ls, dr, eps, R1, R2, q, Mt, mu, prf, E);
+    return hipGetLastError();
+}
+
+hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hipStream_t st)
+{
+    const int ntiles = (N + gmd::TILE - 1) / gmd::TILE;
+    hipLaunchKernelGGL(k_lcut, dim3(ntiles), dim3(64), 0, st, N, Ls, tau, E, lcut);
     return hipGetLastError();
 }
 

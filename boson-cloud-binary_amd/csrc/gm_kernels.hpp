@@ -22,8 +22,12 @@ struct ApplyArgs {
     double *partial;     // reduction partials (3 * grid doubles)
     double *rowpart;     // AM_DIAG: 6 doubles per (tile, mode)
     double2 *dpsi_ix;    // AM_DIAG: dpsi/dt at the extraction node per mode
+    double2 *Cout;       // AM_RES: coupling V x written per row (or nullptr)
+    const double2 *Cin;  // AM_DIAG: reuse V x_prev; V psi = |rhs| mask V x_prev
+    const double *mask;  // damping mask (AM_DIAG with Cin), or nullptr
     int nitems;          // node tiles x row chunks
     int per;             // set by the launcher
+    int mode;            // ApplyMode, set by the launcher
 };
 
 struct FactorArgs {
@@ -103,10 +107,11 @@ struct DiagArgs {
     double *out;         // flux, energy_flux, norms, energies (K each), then 4 sums
 };
 
-int apply_grid(int nitems, int *per);
+int apply_grid(int ntiles, int nchunks, int *per);
 hipError_t launch_apply(ApplyArgs a, int mode, bool corr, hipStream_t st);
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
                           double mu, double prf, double *E, hipStream_t st);
+hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hipStream_t st);
 hipError_t launch_factor(const FactorArgs &f, hipStream_t st);
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st);
 hipError_t launch_update(const UpdateArgs &a, hipStream_t st);

@@ -129,6 +129,8 @@ struct gm_solver {
     std::vector<gmd::Chunk> chunks_h;
     DevBuf<int> goff;
     DevBuf<double> G, E, dmask;
+    DevBuf<int> lcut;
+    double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
     DevBuf<gmd::Chunk> chunks;
     DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, yc, zc, inv, tmp;
@@ -224,7 +226,7 @@ void prof_collect(gm_solver &S)
 
 // ------------------------------------------------------------------ tables
 
-// Gaunt blocks: for each (l, l1), [m][m1][s] with l* = |l - l1| + 2 s, the
+// Gaunt blocks: for each (l, l1), [m1][m][s] with l* = |l - l1| + 2 s, the
 // angular coefficient of build_coupling_cache (hamiltonian.cpp:77-95).
 void build_gaunt(gm_solver &S)
 {
@@ -235,8 +237,8 @@ void build_gaunt(gm_solver &S)
         for (int l1 = 0; l1 <= S.l_max; ++l1) {
             S.goff_h[l * Lm + l1] = static_cast<int>(S.G_h.size());
             const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
-            for (int mi = 0; mi <= l; ++mi)
-                for (int m1i = 0; m1i <= l1; ++m1i)
+            for (int m1i = 0; m1i <= l1; ++m1i)
+                for (int mi = 0; mi <= l; ++mi)
                     for (int s = 0; s < Sl; ++s) {
                         const int m = -l + 2 * mi, m1 = -l1 + 2 * m1i, ls = lo + 2 * s;
                         const int ms = m1 - m;
@@ -254,7 +256,8 @@ void build_gaunt(gm_solver &S)
                     }
         }
     S.chunks_h.clear();
-    for (int l = 0; l <= S.l_max; ++l)
+    // largest rows first: the tail of the apply grid holds the cheap chunks
+    for (int l = S.l_max; l >= 0; --l)
         for (int m0 = 0; m0 <= l; m0 += gmd::CH)
             S.chunks_h.push_back({l, m0, std::min(gmd::CH, l + 1 - m0), 0});
     S.nchunks = static_cast<int>(S.chunks_h.size());
@@ -268,7 +271,7 @@ double angular(const gm_solver &S, int j, int k, int ls)
     const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
     const int s = (ls - lo) / 2;
     if (ls < lo || s >= Sl || (ls - lo) % 2) return 0.0;
-    return S.G_h[S.goff_h[l * (S.l_max + 1) + l1] + (mi * (l1 + 1) + m1i) * Sl + s];
+    return S.G_h[S.goff_h[l * (S.l_max + 1) + l1] + (m1i * (l + 1) + mi) * Sl + s];
 }
 
 gmd::Geometry geometry(const gm_solver &S)
@@ -288,6 +291,7 @@ gmd::Geometry geometry(const gm_solver &S)
     g.goff = S.goff.p;
     g.G = S.G.p;
     g.E = S.E.p;
+    g.lcut = S.lcut.p;
     g.cap = S.cap ? S.dcap.p : nullptr;
     return g;
 }
@@ -431,6 +435,9 @@ void build_perturbation(gm_solver &S, double time)
         launch(S, GM_PROF_FACTOR, [&] {
             return gmk::launch_build_E(S.Nr, 2 * S.l_max + 1, S.dr, S.epsilon, S.R1, S.R2, S.q, S.M1 + S.M2, S.mu,
                                        S.prf, S.E.p, S.st);
+        });
+        launch(S, GM_PROF_FACTOR, [&] {
+            return gmk::launch_lcut(S.Nr, 2 * S.l_max + 1, S.coupling_tol, S.E.p, S.lcut.p, S.st);
         });
         std::memcpy(S.E_key, key, sizeof(key));
         S.E_valid = true;
@@ -677,6 +684,7 @@ bool solve_cn(gm_solver &S)
     {
         gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
         a.R0 = S.b.p;
+        a.Cout = S.tmp.p;  // V x, reused by the diagnostics of the accepted state
         launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RES, false, S.st); });
     }
     if (wood) sink_proj(S, S.x.p, gmk::SD_QNEW, 1.0, 1);
@@ -700,8 +708,9 @@ void alloc_device(gm_solver &S)
                      &S.inv, &S.tmp})
         if (bf->n != n) bf->alloc(n);
     S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
+    S.lcut.alloc(S.ntiles);
     int per = 0;
-    const int grid = gmk::apply_grid(S.ntiles * S.nchunks, &per);
+    const int grid = gmk::apply_grid(S.ntiles, S.nchunks, &per);
     S.partial.alloc(static_cast<size_t>(3) * std::max(grid, gmd::RED_BLOCKS));
     S.rowpart.alloc(static_cast<size_t>(S.ntiles) * S.K * 6);
     S.dpsi_ix.alloc(S.K);
@@ -885,6 +894,52 @@ void gm_destroy(gm_solver *S)
     for (auto &p : S->pending) S->ev_pool.insert(S->ev_pool.end(), {p.a, p.b});
     for (hipEvent_t e : S->ev_pool) (void)hipEventDestroy(e);
     delete S;
+}
+
+int gm_set_coupling_tolerance(gm_solver *S, double tol)
+{
+    return guarded(S, [&] {
+        if (!(tol >= 0.0) || tol >= 1e-6) throw std::invalid_argument("coupling tolerance must lie in [0, 1e-6)");
+        S->coupling_tol = tol;
+        S->E_valid = false;
+        if (S->initialized) {
+            build_perturbation(*S, S->t_i);
+            HIPCK(hipStreamSynchronize(S->st));
+        }
+    });
+}
+
+int gm_coupling_work(gm_solver *S, double *terms, double *pairs)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        std::vector<int> lc(S->ntiles);
+        HIPCK(hipMemcpyAsync(lc.data(), S->lcut.p, sizeof(int) * S->ntiles, hipMemcpyDeviceToHost, S->st));
+        HIPCK(hipStreamSynchronize(S->st));
+        // per tile: Gaunt terms (|m*| <= l* <= lcut) and mode pairs with a kept term
+        double t = 0.0, pr = 0.0;
+        std::vector<double> tcount(2 * S->l_max + 1, 0.0), pcount(2 * S->l_max + 1, 0.0);
+        for (int j = 0; j < S->K; ++j)
+            for (int k = 0; k < S->K; ++k) {
+                const int l = S->ml[j], l1 = S->ml[k], ms = std::abs(S->mm[k] - S->mm[j]);
+                int lo = std::max(std::abs(l - l1), ms);
+                if ((lo + l + l1) % 2) ++lo;
+                if (lo > l + l1) continue;
+                pcount[lo] += 1.0;
+                for (int ls = lo; ls <= l + l1; ls += 2) tcount[ls] += 1.0;
+            }
+        for (int c = 1; c <= 2 * S->l_max; ++c) {
+            tcount[c] += tcount[c - 1];
+            pcount[c] += pcount[c - 1];
+        }
+        for (int tile = 0; tile < S->ntiles; ++tile) {
+            const int nodes = std::min(gmd::TILE, S->Nr - tile * gmd::TILE);
+            t += nodes * tcount[lc[tile]];
+            pr += nodes * pcount[lc[tile]];
+        }
+        *terms = t;
+        *pairs = pr;
+    });
 }
 
 int gm_profile_enable(gm_solver *S, int32_t on)
@@ -1073,6 +1128,10 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
         if (P.nsink) sink_proj(*S, S->psi.p, gmk::SD_POST, 1.0, 0);
         {
             gmk::ApplyArgs a = apply_args(*S, S->psi.p, nullptr);
+            // V psi = |rhs| mask V x: the coupling is node-local and the mask is
+            // the same for every mode, so only the stencil is recomputed
+            a.Cin = S->tmp.p;
+            a.mask = S->mask ? S->dmask.p : nullptr;
             launch(*S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_DIAG, false, S->st); });
             gmk::DiagArgs da{};
             da.N = S->Nr;

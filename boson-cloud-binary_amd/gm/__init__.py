@@ -24,6 +24,11 @@ def lib():
         capi.bind(_lib, "gm_")
         _lib.gm_version.restype = C.c_char_p
         _lib.gm_device_count.restype = C.c_int
+        vp = C.c_void_p
+        _lib.gm_set_coupling_tolerance.argtypes = [vp, C.c_double]
+        _lib.gm_coupling_work.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib.gm_profile_enable.argtypes = [vp, C.c_int32]
+        _lib.gm_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
     return _lib
 
 
@@ -37,3 +42,22 @@ class CrankNicolsonModes(capi.Solver):
 
     def __init__(self, *args, **kw):
         super().__init__(lib(), "gm_", *args, **kw)
+
+    def set_coupling_tolerance(self, tol):
+        """Multipole truncation of the coupling (0 keeps every term)."""
+        self._check(self._lib.gm_set_coupling_tolerance(self._h, float(tol)))
+
+    def coupling_work(self):
+        """(Gaunt terms, mode pairs) evaluated by one coupling apply."""
+        t, p = C.c_double(), C.c_double()
+        self._check(self._lib.gm_coupling_work(self._h, C.byref(t), C.byref(p)))
+        return t.value, p.value
+
+    def profile(self, on):
+        self._check(self._lib.gm_profile_enable(self._h, int(on)))
+
+    def profile_read(self):
+        ms = (C.c_double * 7)()
+        n = (C.c_int64 * 7)()
+        self._lib.gm_profile_read(self._h, ms, n, 7)
+        return [(ms[c], n[c]) for c in range(7)]

@@ -156,6 +156,16 @@ int gm_apply_sink(gm_solver *solver, const double *in, double *out);
 int gm_schrodinger_rhs(gm_solver *solver, const double *in, double *out);
 double gm_sink_strength(const gm_solver *solver, int32_t body);
 
+/* Multipole truncation of the coupling (no counterpart in the reference, which
+ * evaluates every l*): at each tile of 64 radial nodes, multipoles whose radial
+ * profile is below tol times the tile's leading profile are not evaluated.
+ * Default 2^-64 (their sum lies below the rounding of the kept terms); 0 keeps
+ * every term. Allowed range [0, 1e-6). */
+int gm_set_coupling_tolerance(gm_solver *solver, double tol);
+/* Work of one coupling apply under the current truncation, summed over radial
+ * nodes: Gaunt terms evaluated (terms) and mode pairs coupled (pairs). */
+int gm_coupling_work(gm_solver *solver, double *terms, double *pairs);
+
 /* Measurement hook (no counterpart in the reference; for benchmarks): device
  * time per kernel class, recorded with HIP events on the solver's stream.
  * gm_profile_enable resets the counters; gm_profile_read fills ncat entries. */
