@@ -6,7 +6,7 @@
 //   E table           2 * (2 l_max + 1) * Nr doubles: the radial multipole
 //                     profiles c_l* (f1 + sign prf f2) for even (index 0) and
 //                     odd (index 1) m*-parity pairs;
-//   G table           per (l, l1) block, [m1][m][s] doubles: the Gaunt
+//   G table           per (l, l1) block, [m1][s][m] doubles: the Gaunt
 //                     coefficient of modes (l, m), (l1, m1) at l* = |l-l1| + 2s;
 //   preconditioner    K*Nr double2 inverse pivots of each mode's radial tridiagonal.
 #pragma once
@@ -22,8 +22,11 @@ constexpr int PT = 256;    // threads of a preconditioner workgroup
 constexpr int PE = 8;      // consecutive nodes per preconditioner thread
 constexpr int RED_BLOCKS = 1024;  // max partial-sum slots per reduction
 
+// Eight rows (modes of one l) handled by one wavefront: m-indices
+// [m0, m0 + CH), of which the chunk writes [own_lo, own_hi). Chunks of an l
+// with l + 1 >= CH overlap so that every chunk has CH real rows.
 struct Chunk {
-    int l, m0, cnt, pad;
+    int l, m0, own_lo, own_hi;
 };
 
 // Constant geometry and tables of one solver (device pointers).

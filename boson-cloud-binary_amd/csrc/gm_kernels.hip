@@ -170,39 +170,33 @@ __global__ __launch_bounds__(64) void k_lcut(int N, int Ls, double tau, const do
 
 // ---------------------------------------------------------------- operator apply
 
-template <bool CORR>
-__device__ __forceinline__ double2 load_x(const ApplyArgs &a, const Krylov *Kr, int nsink, const double2 (&c)[2],
-                                          int k, int i)
+__device__ __forceinline__ double2 load_x(const ApplyArgs &a, int k, int i)
 {
-    double2 x = a.X[(size_t)k * a.g.N + i];
-    if (CORR) {
-        for (int s = 0; s < nsink; ++s) {
-            const int wlo = Kr->wrange[2 * s], whi = Kr->wrange[2 * s + 1];
-            if (i >= wlo && i < whi) x = csub(x, cmul(c[s], a.sk[s].W[(size_t)k * a.g.N + i]));
-        }
-    }
-    return x;
+    return a.X[(size_t)k * a.g.N + i];
 }
 
 // Rank-one sink term sum_a coef_a v_a at (mode j, node i).
-__device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, const double2 (&coef)[2], int j, int i)
+__device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, double2 coef0, double2 coef1, int j, int i)
 {
     double2 t = make_double2(0.0, 0.0);
-    for (int s = 0; s < nsink; ++s) {
-        const SinkDev &sk = a.sk[s];
-        const int d = i - sk.lo;
-        if (d >= 0 && d < sk.n) t = cadd(t, cmul(coef[s], sk.v[(size_t)j * sk.n + d]));
+    if (nsink > 0) {
+        const int d = i - a.sk[0].lo;
+        if (d >= 0 && d < a.sk[0].n) t = cadd(t, cmul(coef0, a.sk[0].v[(size_t)j * a.sk[0].n + d]));
+    }
+    if (nsink > 1) {
+        const int d = i - a.sk[1].lo;
+        if (d >= 0 && d < a.sk[1].n) t = cadd(t, cmul(coef1, a.sk[1].v[(size_t)j * a.sk[1].n + d]));
     }
     return t;
 }
 
-// Coupling of the rows of chunk C with the modes of l1 at node ic, for a
-// (l, l1) block with S multipole slots: W = sum_s G[m][m1][s] E_s(r) formed
-// per node, then acc[m] += W * exp(i m1 theta) x_(l1,m1).
-template <int S, bool CORR>
-__device__ __forceinline__ void coupling_block(const ApplyArgs &a, const Krylov *Kr, int nsink, const double2 (&cw)[2],
-                                               const StepParams *P, const Chunk &C, int l, int l1, int ic, int Sfull,
-                                               double2 (&acc)[gmd::CH])
+// Coupling of the CH rows of chunk C with the modes of l1 at node ic, for a
+// (l, l1) block whose first S multipole slots are kept: per column m1,
+// W[c] = sum_s G[m1][s][m0 + c] E_s(r) for all CH rows at once (one scalar
+// load of CH coefficients per slot), then acc[c] += W[c] exp(i m1 theta) x.
+template <int S>
+__device__ __forceinline__ void coupling_block(const ApplyArgs &a, const StepParams *P, const Chunk &C, int l, int l1,
+                                               int ic, int Sfull, double2 (&acc)[gmd::CH])
 {
     const Geometry &g = a.g;
     const int N = g.N, lmax = g.lmax;
@@ -211,32 +205,35 @@ __device__ __forceinline__ void coupling_block(const ApplyArgs &a, const Krylov 
     double e[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) e[s] = g.E[(size_t)(p * g.Ls + lo + 2 * s) * N + ic];
-    // block layout [m1][m][s]: the rows of one m1 are contiguous with a
-    // compile-time stride S
-    const double *__restrict__ Gb = g.G + g.goff[l * (lmax + 1) + l1] + (size_t)C.m0 * Sfull;
+    const int Lr = l + 1;  // rows of the block
+    const double *__restrict__ Gb = g.G + g.goff[l * (lmax + 1) + l1] + C.m0;
     const int kb = l1 * (l1 + 1) / 2;
-    const int m1s = (l + 1) * Sfull;
-    double2 xn = load_x<CORR>(a, Kr, nsink, cw, kb, ic);
+    double2 x0 = load_x(a, kb, ic);
+    double2 x1 = l1 >= 1 ? load_x(a, kb + 1, ic) : x0;
     for (int m1i = 0; m1i <= l1; ++m1i) {
-        const double2 xv = xn;
-        if (m1i < l1) xn = load_x<CORR>(a, Kr, nsink, cw, kb + m1i + 1, ic);
+        const double2 xv = x0;
+        x0 = x1;
+        if (m1i + 2 <= l1) x1 = load_x(a, kb + m1i + 2, ic);
         const double2 xr = cmul(P->ph[lmax - l1 + 2 * m1i], xv);
-        const double *__restrict__ Gr = Gb + (size_t)m1i * m1s;
+        const double *__restrict__ Gm = Gb + (size_t)m1i * Sfull * Lr;
+        double w[gmd::CH];
+#pragma unroll
+        for (int c = 0; c < gmd::CH; ++c) w[c] = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double *__restrict__ Gs = Gm + s * Lr;
+#pragma unroll
+            for (int c = 0; c < gmd::CH; ++c) w[c] = fma(Gs[c], e[s], w[c]);
+        }
 #pragma unroll
         for (int c = 0; c < gmd::CH; ++c) {
-            if (c < C.cnt) {
-                const double *__restrict__ Gc = Gr + c * Sfull;
-                double w = 0.0;
-#pragma unroll
-                for (int s = 0; s < S; ++s) w = fma(Gc[s], e[s], w);
-                acc[c].x = fma(w, xr.x, acc[c].x);
-                acc[c].y = fma(w, xr.y, acc[c].y);
-            }
+            acc[c].x = fma(w[c], xr.x, acc[c].x);
+            acc[c].y = fma(w[c], xr.y, acc[c].y);
         }
     }
 }
 
-template <int SMAX, bool CORR>
+template <int SMAX>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
 {
     const int MODE = a.mode;
@@ -261,11 +258,9 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
     const bool item_ok = tgroup < (g.ntiles + 3) / 4 && tile_w < g.ntiles;
     const int N = g.N, lmax = g.lmax;
     const int nsink = P->nsink;
-    double2 cw[2], coef[2];
-    for (int s = 0; s < 2; ++s) {
-        cw[s] = Kr->c[s];
-        coef[s] = (MODE == AM_RHS || MODE == AM_DIAG) ? Kr->gq[s] : Kr->coefA[s];
-    }
+    const bool gq_mode = MODE == AM_RHS || MODE == AM_DIAG;
+    const double2 coef0 = gq_mode ? Kr->gq[0] : Kr->coefA[0];
+    const double2 coef1 = gq_mode ? Kr->gq[1] : Kr->coefA[1];
     double red[3] = {0.0, 0.0, 0.0};
     if (item_ok) {
         const int tile = tile_w, chn = chn_b;
@@ -287,7 +282,7 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
             switch (S) {
 #define GM_CASE(n) \
     case n: \
-        if constexpr (n <= SMAX) coupling_block<n, CORR>(a, Kr, nsink, cw, P, C, l, l1, ic, Sfull, acc); \
+        if constexpr (n <= SMAX) coupling_block<n>(a, P, C, l, l1, ic, Sfull, acc); \
         break;
                 GM_CASE(1) GM_CASE(2) GM_CASE(3) GM_CASE(4) GM_CASE(5) GM_CASE(6) GM_CASE(7) GM_CASE(8)
                 GM_CASE(9) GM_CASE(10) GM_CASE(11) GM_CASE(12) GM_CASE(13) GM_CASE(14) GM_CASE(15) GM_CASE(16)
@@ -307,8 +302,8 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
         const int ix = P->idx_ex;
 #pragma unroll
         for (int c = 0; c < gmd::CH; ++c) {
-            if (c >= C.cnt) continue;
             const int mi = C.m0 + c;
+            if (mi < C.own_lo || mi >= C.own_hi) continue;
             const int m = -l + 2 * mi;
             const int j = l * (l + 1) / 2 + mi;
             double2 Cc = cdotm(P->ph[m + lmax], acc[c]);
@@ -316,14 +311,13 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
                 const size_t row = (size_t)j * N + i;
                 if (have_cin) Cc = cscale(Kr->rhs_nrm * (a.mask ? a.mask[i] : 1.0), a.Cin[row]);
                 if (a.Cout) a.Cout[row] = Cc;
-                const double2 xi = load_x<CORR>(a, Kr, nsink, cw, j, i);
-                if (CORR) a.Xc[row] = xi;
+                const double2 xi = load_x(a, j, i);
                 double2 Hx;
                 if (interior) {
                     const double ri = g.eps + i * g.dr;
                     const double h0d = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * g.mu);
-                    const double2 xm = load_x<CORR>(a, Kr, nsink, cw, j, i - 1);
-                    const double2 xp = load_x<CORR>(a, Kr, nsink, cw, j, i + 1);
+                    const double2 xm = load_x(a, j, i - 1);
+                    const double2 xp = load_x(a, j, i + 1);
                     Hx = cadd(cscale(g.off, cadd(xm, xp)), cadd(cmul(make_double2(h0d + capv.x, capv.y), xi), Cc));
                 } else {
                     Hx = cmul(capv, xi);
@@ -332,7 +326,7 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
                     a.Y[row] = Hx;
                 } else if (MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_A) {
                     double2 out = cadd(xi, imul(hdt, Hx));
-                    if (nsink) out = cadd(out, sink_term(a, nsink, coef, j, i));
+                    if (nsink) out = cadd(out, sink_term(a, nsink, coef0, coef1, j, i));
                     if (MODE == AM_V) {
                         a.Y[row] = out;
                         const double2 d = cdotm(a.R0[row], out);
@@ -352,14 +346,14 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
                 } else if (MODE == AM_RHS) {
                     const double2 sHx = imul(hdt, Hx);
                     double2 st = make_double2(0.0, 0.0);
-                    if (nsink) st = cscale(0.5 * hdt, sink_term(a, nsink, coef, j, i));  // (dt/4) Gamma psi
+                    if (nsink) st = cscale(0.5 * hdt, sink_term(a, nsink, coef0, coef1, j, i));  // (dt/4) Gamma psi
                     const double2 rhs = csub(csub(xi, sHx), st);
                     a.Y[row] = rhs;
                     a.Y2[row] = cadd(cadd(xi, sHx), st);
                     red[0] += cnorm2(rhs);
                 } else if (MODE == AM_DIAG) {
                     double2 dpsi = make_double2(Hx.y, -Hx.x);  // -i H psi
-                    if (nsink) dpsi = csub(dpsi, cscale(0.5, sink_term(a, nsink, coef, j, i)));
+                    if (nsink) dpsi = csub(dpsi, cscale(0.5, sink_term(a, nsink, coef0, coef1, j, i)));
                     const double2 eh = cdotm(xi, Hx);
                     const double n2 = cnorm2(xi);
                     if (i < ix) {
@@ -423,40 +417,169 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
 
 // ---------------------------------------------------------------- Thomas factor
 
-__global__ void k_factor(FactorArgs f)
+// Diagonal of mode j's CN matrix at node i: 1 + i dt/2 (H0_ii + V_jj(r_i) + CAP_i)
+// (hamiltonian.cpp:224-243, matrices.cpp:27-40); boundary rows keep only the CAP.
+__device__ __forceinline__ double2 cn_diag(const FactorArgs &f, const double *Gjj, int l, int i)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= f.K) return;
+    const int N = f.N;
+    double2 h = make_double2(0.0, 0.0);
+    if (i > 0 && i < N - 1) {
+        const double ri = f.eps + i * f.dr;
+        const double alpha2 = 1.0 / (f.dr * f.dr);
+        double v = 0.0;
+        for (int s = 0; s <= l; ++s) v += Gjj[(size_t)s * (l + 1)] * f.E[(size_t)(2 * s) * N + i];
+        h.x = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * f.mu) + v;
+    }
+    if (f.cap) h = cadd(h, f.cap[i]);
+    return make_double2(1.0 - f.half_dt * h.y, f.half_dt * h.x);
+}
+
+struct Mob {
+    double2 a, b, c, d;  // [[a, b], [c, d]] acting on (u, v), pivot = u / v
+};
+
+// first, then second: second * first, rescaled (the pivot is a ratio)
+__device__ __forceinline__ Mob mob_compose(const Mob &x, const Mob &y)
+{
+    Mob r;
+    r.a = cadd(cmul(y.a, x.a), cmul(y.b, x.c));
+    r.b = cadd(cmul(y.a, x.b), cmul(y.b, x.d));
+    r.c = cadd(cmul(y.c, x.a), cmul(y.d, x.c));
+    r.d = cadd(cmul(y.c, x.b), cmul(y.d, x.d));
+    double m = fmax(fmax(fabs(r.a.x), fabs(r.a.y)), fmax(fabs(r.b.x), fabs(r.b.y)));
+    m = fmax(m, fmax(fmax(fabs(r.c.x), fabs(r.c.y)), fmax(fabs(r.d.x), fabs(r.d.y))));
+    if (m > 0.0 && isfinite(m)) {
+        const double sc = 1.0 / m;
+        r.a = cscale(sc, r.a);
+        r.b = cscale(sc, r.b);
+        r.c = cscale(sc, r.c);
+        r.d = cscale(sc, r.d);
+    }
+    return r;
+}
+
+__device__ __forceinline__ Mob shfl_mob(const Mob &v, int src)
+{
+    Mob r;
+    r.a.x = __shfl(v.a.x, src); r.a.y = __shfl(v.a.y, src);
+    r.b.x = __shfl(v.b.x, src); r.b.y = __shfl(v.b.y, src);
+    r.c.x = __shfl(v.c.x, src); r.c.y = __shfl(v.c.y, src);
+    r.d.x = __shfl(v.d.x, src); r.d.y = __shfl(v.d.y, src);
+    return r;
+}
+
+// Thomas pivots of every mode (include/RadialBlockPreconditioner.hpp:43-76):
+// pivot_i = d_i - lower_i inv_{i-1} upper_{i-1}, with lower = upper = l (the
+// stencil entry of A) on interior rows, so pivot_i = d_i - l^2 / pivot_{i-1}
+// for 2 <= i <= N-2 and pivot_i = d_i otherwise. The recurrence is a Moebius
+// map; one workgroup per mode scans the maps of PE consecutive nodes per
+// thread (2x2 products, rescaled), then every thread replays its nodes with
+// the reference's arithmetic from its exact incoming pivot.
+__global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
+{
+    constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
+    __shared__ Mob wt[T / 64];
+    const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= j) ++l;
     const int mi = j - l * (l + 1) / 2;
-    const double *Gjj = f.G + f.goff[l * (f.lmax + 1) + l] + (size_t)(mi * (l + 1) + mi) * (l + 1);
-    const double alpha2 = 1.0 / (f.dr * f.dr);
+    const double *Gjj = f.G + f.goff[l * (f.lmax + 1) + l] + (size_t)mi * (l + 1) * (l + 1) + mi;
     const double2 ell = make_double2(0.0, f.half_dt * f.off);
-    double2 prev = make_double2(0.0, 0.0);
-    int failed = 0;
+    const double2 ell2 = cmul(ell, ell);
     const int N = f.N;
-    for (int i = 0; i < N; ++i) {
-        double2 h = make_double2(0.0, 0.0);
-        if (i > 0 && i < N - 1) {
-            const double ri = f.eps + i * f.dr;
-            double v = 0.0;
-            for (int s = 0; s <= l; ++s) v += Gjj[s] * f.E[(size_t)(2 * s) * N + i];
-            h.x = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * f.mu) + v;
+    const int ntile = (N + TE - 1) / TE;
+    double2 carry = make_double2(1.0, 0.0);  // pivot before node 0 (unused: node 0 resets)
+    int failed = 0;
+    for (int t = 0; t < ntile; ++t) {
+        const int i0 = t * TE + tid * E;
+        double2 dv[E];
+        Mob loc;
+        loc.a = make_double2(1.0, 0.0);
+        loc.b = make_double2(0.0, 0.0);
+        loc.c = make_double2(0.0, 0.0);
+        loc.d = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = i0 + e;
+            Mob m;
+            if (i >= N) {
+                m = loc;  // identity beyond the grid
+                m.a = make_double2(1.0, 0.0);
+                m.b = make_double2(0.0, 0.0);
+                m.c = make_double2(0.0, 0.0);
+                m.d = make_double2(1.0, 0.0);
+                dv[e] = make_double2(1.0, 0.0);
+            } else {
+                dv[e] = cn_diag(f, Gjj, l, i);
+                if (i >= 2 && i <= N - 2) {
+                    // (u, v) -> (d u - l^2 v, u)
+                    m.a = dv[e];
+                    m.b = make_double2(-ell2.x, -ell2.y);
+                    m.c = make_double2(1.0, 0.0);
+                    m.d = make_double2(0.0, 0.0);
+                } else {
+                    // reset: (u, v) -> (d v, v)
+                    m.a = make_double2(0.0, 0.0);
+                    m.b = dv[e];
+                    m.c = make_double2(0.0, 0.0);
+                    m.d = make_double2(1.0, 0.0);
+                }
+            }
+            loc = mob_compose(loc, m);
         }
-        if (f.cap) h = cadd(h, f.cap[i]);
-        double2 piv = make_double2(1.0 - f.half_dt * h.y, f.half_dt * h.x);
-        if (i >= 2 && i <= N - 2) piv = csub(piv, cmul(cmul(ell, prev), ell));
-        const double ap = sqrt(cnorm2(piv));
-        double2 inv;
-        if (ap == 0.0 || !isfinite(ap)) {
-            failed = 1;
-            inv = make_double2(0.0, 0.0);
-        } else {
-            inv = cdiv(make_double2(1.0, 0.0), piv);
+        // block exclusive scan of the thread maps (increasing node order)
+        Mob inc = loc;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const Mob o = shfl_mob(inc, lane >= d ? lane - d : 0);
+            if (lane >= d) inc = mob_compose(o, inc);
         }
-        f.inv[(size_t)j * N + i] = inv;
-        prev = inv;
+        Mob ex = shfl_mob(inc, lane > 0 ? lane - 1 : 0);
+        if (lane == 0) {
+            ex.a = make_double2(1.0, 0.0);
+            ex.b = make_double2(0.0, 0.0);
+            ex.c = make_double2(0.0, 0.0);
+            ex.d = make_double2(1.0, 0.0);
+        }
+        if (lane == 63) wt[wv] = inc;
+        __syncthreads();
+        Mob pre, tot;
+        pre.a = make_double2(1.0, 0.0);
+        pre.b = make_double2(0.0, 0.0);
+        pre.c = make_double2(0.0, 0.0);
+        pre.d = make_double2(1.0, 0.0);
+        tot = pre;
+        for (int w = 0; w < T / 64; ++w) {
+            if (w == wv) pre = tot;
+            tot = mob_compose(tot, wt[w]);
+        }
+        __syncthreads();
+        const Mob P = mob_compose(pre, ex);
+        // incoming pivot of this thread: P applied to (carry, 1)
+        const double2 u = cadd(cmul(P.a, carry), P.b), v = cadd(cmul(P.c, carry), P.d);
+        double2 prev_inv = cdiv(make_double2(1.0, 0.0), cdiv(u, v));
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = i0 + e;
+            if (i >= N) break;
+            double2 piv = dv[e];
+            if (i >= 2 && i <= N - 2) piv = csub(piv, cmul(cmul(ell, prev_inv), ell));
+            const double ap = sqrt(cnorm2(piv));
+            double2 inv;
+            if (ap == 0.0 || !isfinite(ap)) {
+                failed = 1;
+                inv = make_double2(0.0, 0.0);
+            } else {
+                inv = cdiv(make_double2(1.0, 0.0), piv);
+            }
+            f.inv[(size_t)j * N + i] = inv;
+            prev_inv = inv;
+        }
+        // carry: the last pivot of the tile, from the block total
+        {
+            const double2 uu = cadd(cmul(tot.a, carry), tot.b), vv = cadd(cmul(tot.c, carry), tot.d);
+            carry = cdiv(uu, vv);
+        }
     }
     if (failed) atomicOr(f.failed, 1);
 }
@@ -606,6 +729,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
         __syncthreads();
     }
     // backward sweep
+    int nz_lo = N, nz_hi = 0;
     carry = make_double2(0.0, 0.0);
     for (int t = ntile - 1; t >= 0; --t) {
         const int t0 = t * TE;
@@ -650,9 +774,27 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
         for (int e = 0; e < E; ++e) {
             const int pos = tid + T * e;
             const int idx = t0 + pos;
-            if (idx < N) a.Y[base_j + idx] = sf[lds_slot(pos)];
+            if (idx < N) {
+                const double2 yv = sf[lds_slot(pos)];
+                a.Y[base_j + idx] = yv;
+                if (VAR == PV_PLAIN && a.range && (yv.x != 0.0 || yv.y != 0.0)) {
+                    nz_lo = min(nz_lo, idx);
+                    nz_hi = max(nz_hi, idx + 1);
+                }
+            }
         }
         __syncthreads();
+    }
+    if (VAR == PV_PLAIN && a.range) {
+        // exact non-zero node range of the solution over all modes
+        for (int o = 32; o > 0; o >>= 1) {
+            nz_lo = min(nz_lo, __shfl_xor(nz_lo, o));
+            nz_hi = max(nz_hi, __shfl_xor(nz_hi, o));
+        }
+        if ((tid & 63) == 0) {
+            atomicMin(a.range, nz_lo);
+            atomicMax(a.range + 1, nz_hi);
+        }
     }
     if (!a.wood) return;
     // Woodbury epilogue in the last block (matrices.cpp:146-153):
@@ -933,6 +1075,30 @@ __global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
     }
 }
 
+// Woodbury correction of a preconditioned vector, in place over the node
+// range where the columns W_a can be non-zero (matrices.cpp:151):
+// y -= sum_a c_a W_a.
+__global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a)
+{
+    const Krylov *Kr = a.Kr;
+    if (Kr->done) return;
+    const int ns = a.nsink;
+    const int lo = min(Kr->wrange[0], ns > 1 ? Kr->wrange[2] : a.N);
+    const int hi = max(Kr->wrange[1], ns > 1 ? Kr->wrange[3] : 0);
+    if (hi <= lo) return;
+    const int width = hi - lo;
+    const double2 c0 = Kr->c[0], c1 = Kr->c[1];
+    const size_t n = (size_t)width * a.K;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / width), i = lo + (int)(q - (size_t)k * width);
+        const size_t gi = (size_t)k * a.N + i;
+        double2 y = a.Y[gi];
+        y = csub(y, cmul(c0, a.W0[gi]));
+        if (ns > 1) y = csub(y, cmul(c1, a.W1[gi]));
+        a.Y[gi] = y;
+    }
+}
+
 // Expand a compact sink projector into a full-length vector.
 __global__ void k_expand(int N, int K, SinkDev sk, double2 *out)
 {
@@ -944,70 +1110,41 @@ __global__ void k_expand(int N, int K, SinkDev sk, double2 *out)
     }
 }
 
-// Node range [lo, hi) where W may be non-zero (exact zeros outside).
-__global__ void k_wrange(int N, int K, const double2 *W, int *range)
-{
-    int lo = N, hi = 0;
-    const size_t n = (size_t)N * K;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
-        const double2 w = W[q];
-        if (w.x != 0.0 || w.y != 0.0) {
-            const int i = (int)(q % N);
-            lo = min(lo, i);
-            hi = max(hi, i + 1);
-        }
-    }
-    atomicMin(range, lo);
-    atomicMax(range + 1, hi);
-}
-
 // ---------------------------------------------------------------- diagnostics
 
-__global__ __launch_bounds__(256) void k_diag_final(DiagArgs a)
+// One wavefront per mode: sums of the per-tile partials of the diagnostics
+// apply, the flux at r_ex, and the mode's share of the global sums.
+__global__ __launch_bounds__(64) void k_diag_final(DiagArgs a)
 {
+    const int j = blockIdx.x, lane = threadIdx.x;
     const int K = a.K, N = a.N, ix = a.ix, nt = a.ntiles;
     const double dr = a.dr, rix = a.eps + ix * dr;
-    double v[4] = {0, 0, 0, 0};
-    for (int j = threadIdx.x; j < K; j += blockDim.x) {
-        double n0 = 0, e1 = 0;
-        for (int t = 0; t < nt; ++t) {
-            const double *rp = a.rowpart + ((size_t)t * K + j) * 6;
-            n0 += rp[0];
-            e1 += rp[1];
-            v[0] += rp[2];
-            v[1] += rp[3];
-            v[2] += rp[4];
-            v[3] += rp[5];
-        }
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    // contiguous tile ranges per lane keep the summation order fixed
+    const int per = (nt + 63) / 64;
+    for (int t = lane * per; t < lane * per + per && t < nt; ++t) {
+        const double *rp = a.rowpart + ((size_t)t * K + j) * 6;
+        for (int q = 0; q < 6; ++q) v[q] += rp[q];
+    }
+    for (int q = 0; q < 6; ++q) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
         const double2 um = a.Psi[(size_t)j * N + ix - 1], u0 = a.Psi[(size_t)j * N + ix],
                       up = a.Psi[(size_t)j * N + ix + 1];
         const double2 dU = csub(cscale(1.0 / (2.0 * dr), csub(up, um)), make_double2(u0.x / rix, u0.y / rix));
         a.out[j] = cdotm(u0, dU).y;
         a.out[K + j] = cdotm(a.dpsi_ix[j], dU).x / a.mu;
-        a.out[2 * K + j] = n0 * dr;
-        a.out[3 * K + j] = e1 * dr;
+        a.out[2 * K + j] = v[0] * dr;
+        a.out[3 * K + j] = v[1] * dr;
+        for (int q = 0; q < 4; ++q) a.out[(4 + q) * K + j] = v[2 + q];
     }
-    __shared__ double ws[4][4];
-    for (int q = 0; q < 4; ++q) {
-        const double s = wave_sum(v[q]);
-        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][q] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 4; ++q) {
-            double s = 0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += ws[w][q];
-            a.out[4 * K + q] = s;
-        }
 }
 
 // ---------------------------------------------------------------- launchers
 
 template <int SMAX>
-static hipError_t launch_apply_s(const ApplyArgs &a, bool corr, int grid, hipStream_t st)
+static hipError_t launch_apply_s(const ApplyArgs &a, int grid, hipStream_t st)
 {
-    if (corr) hipLaunchKernelGGL((k_apply<SMAX, true>), dim3(grid), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_apply<SMAX, false>), dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_apply<SMAX>), dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1018,16 +1155,16 @@ int apply_grid(int ntiles, int nchunks, int *per)
     return *per * 8;
 }
 
-hipError_t launch_apply(ApplyArgs a, int mode, bool corr, hipStream_t st)
+hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
     const int grid = apply_grid(a.g.ntiles, a.g.nchunks, &a.per);
     a.mode = mode;
     const int smax = a.g.lmax + 1;
-    if (smax <= 8) return launch_apply_s<8>(a, corr, grid, st);
-    if (smax <= 16) return launch_apply_s<16>(a, corr, grid, st);
-    if (smax <= 24) return launch_apply_s<24>(a, corr, grid, st);
-    if (smax <= 32) return launch_apply_s<32>(a, corr, grid, st);
-    return launch_apply_s<40>(a, corr, grid, st);
+    if (smax <= 8) return launch_apply_s<8>(a, grid, st);
+    if (smax <= 16) return launch_apply_s<16>(a, grid, st);
+    if (smax <= 24) return launch_apply_s<24>(a, grid, st);
+    if (smax <= 32) return launch_apply_s<32>(a, grid, st);
+    return launch_apply_s<40>(a, grid, st);
 }
 
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt, double mu,
@@ -1047,7 +1184,7 @@ hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hi
 
 hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_factor, dim3((f.K + 63) / 64), dim3(64), 0, st, f);
+    hipLaunchKernelGGL(k_factor, dim3(f.K), dim3(gmd::PT), 0, st, f);
     return hipGetLastError();
 }
 
@@ -1096,15 +1233,15 @@ hipError_t launch_expand(int N, int K, const SinkDev &sk, double2 *out, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_wrange(int N, int K, const double2 *W, int *range, hipStream_t st)
+hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_wrange, dim3(stream_grid((size_t)N * K)), dim3(256), 0, st, N, K, W, range);
+    hipLaunchKernelGGL(k_diag_final, dim3(a.K), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st)
+hipError_t launch_wcorr(const WcorrArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_wcorr, dim3(stream_grid((size_t)a.N * a.K)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

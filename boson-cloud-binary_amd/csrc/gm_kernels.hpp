@@ -16,7 +16,6 @@ struct ApplyArgs {
     const double2 *X;    // input vector
     double2 *Y;          // output
     double2 *Y2;         // second output (AM_RHS: A psi)
-    double2 *Xc;         // corrected input (Woodbury), written when corr
     const double2 *R0;   // r0 (AM_V), s (AM_T), b (AM_RES)
     gmd::SinkDev sk[2];
     double *partial;     // reduction partials (3 * grid doubles)
@@ -54,6 +53,14 @@ struct PrecArgs {
     int wood;            // run the Woodbury epilogue
     gmd::SinkDev sk[2];
     const gmd::StepParams *Ps;
+    int *range;          // PV_PLAIN: atomically widened [lo, hi) of non-zero output nodes, or nullptr
+};
+
+struct WcorrArgs {
+    int N, K, nsink;
+    const gmd::Krylov *Kr;
+    const double2 *W0, *W1;
+    double2 *Y;
 };
 
 struct UpdateArgs {
@@ -104,11 +111,11 @@ struct DiagArgs {
     const double2 *Psi;
     const double2 *dpsi_ix;
     const double *rowpart;
-    double *out;         // flux, energy_flux, norms, energies (K each), then 4 sums
+    double *out;         // flux, energy_flux, norms, energies, then E, p_bin, dV, mass shares (K each)
 };
 
 int apply_grid(int ntiles, int nchunks, int *per);
-hipError_t launch_apply(ApplyArgs a, int mode, bool corr, hipStream_t st);
+hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
                           double mu, double prf, double *E, hipStream_t st);
 hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hipStream_t st);
@@ -119,7 +126,7 @@ hipError_t launch_init(const InitArgs &a, hipStream_t st);
 hipError_t launch_accept(const AcceptArgs &a, hipStream_t st);
 hipError_t launch_sink(const SinkArgs &a, hipStream_t st);
 hipError_t launch_expand(int N, int K, const gmd::SinkDev &sk, double2 *out, hipStream_t st);
-hipError_t launch_wrange(int N, int K, const double2 *W, int *range, hipStream_t st);
+hipError_t launch_wcorr(const WcorrArgs &a, hipStream_t st);
 hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st);
 
 }  // namespace gmk

@@ -133,7 +133,7 @@ struct gm_solver {
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
     DevBuf<gmd::Chunk> chunks;
-    DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, yc, zc, inv, tmp;
+    DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, inv, tmp;
     DevBuf<double> partial, rowpart, diagout;
     DevBuf<double2> dpsi_ix;
     DevBuf<gmd::StepParams> dP;
@@ -226,7 +226,7 @@ void prof_collect(gm_solver &S)
 
 // ------------------------------------------------------------------ tables
 
-// Gaunt blocks: for each (l, l1), [m1][m][s] with l* = |l - l1| + 2 s, the
+// Gaunt blocks: for each (l, l1), [m1][s][m] with l* = |l - l1| + 2 s, the
 // angular coefficient of build_coupling_cache (hamiltonian.cpp:77-95).
 void build_gaunt(gm_solver &S)
 {
@@ -238,8 +238,8 @@ void build_gaunt(gm_solver &S)
             S.goff_h[l * Lm + l1] = static_cast<int>(S.G_h.size());
             const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
             for (int m1i = 0; m1i <= l1; ++m1i)
-                for (int mi = 0; mi <= l; ++mi)
-                    for (int s = 0; s < Sl; ++s) {
+                for (int s = 0; s < Sl; ++s)
+                    for (int mi = 0; mi <= l; ++mi) {
                         const int m = -l + 2 * mi, m1 = -l1 + 2 * m1i, ls = lo + 2 * s;
                         const int ms = m1 - m;
                         double ang = 0.0;
@@ -255,11 +255,15 @@ void build_gaunt(gm_solver &S)
                         S.G_h.push_back(ang);
                     }
         }
+    // a chunk reads CH rows from its first row: pad the last block
+    S.G_h.insert(S.G_h.end(), gmd::CH, 0.0);
     S.chunks_h.clear();
     // largest rows first: the tail of the apply grid holds the cheap chunks
     for (int l = S.l_max; l >= 0; --l)
-        for (int m0 = 0; m0 <= l; m0 += gmd::CH)
-            S.chunks_h.push_back({l, m0, std::min(gmd::CH, l + 1 - m0), 0});
+        for (int own = 0; own <= l; own += gmd::CH) {
+            const int m0 = (l + 1 >= gmd::CH) ? std::min(own, l + 1 - gmd::CH) : 0;
+            S.chunks_h.push_back({l, m0, own, std::min(own + gmd::CH, l + 1)});
+        }
     S.nchunks = static_cast<int>(S.chunks_h.size());
     S.ntiles = (S.Nr + gmd::TILE - 1) / gmd::TILE;
 }
@@ -271,7 +275,7 @@ double angular(const gm_solver &S, int j, int k, int ls)
     const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
     const int s = (ls - lo) / 2;
     if (ls < lo || s >= Sl || (ls - lo) % 2) return 0.0;
-    return S.G_h[S.goff_h[l * (S.l_max + 1) + l1] + (m1i * (l + 1) + mi) * Sl + s];
+    return S.G_h[S.goff_h[l * (S.l_max + 1) + l1] + (m1i * Sl + s) * (l + 1) + mi];
 }
 
 gmd::Geometry geometry(const gm_solver &S)
@@ -556,6 +560,11 @@ void woodbury_setup(gm_solver &S)
 {
     const gmd::StepParams &P = *S.hP.p;
     gmk::PrecArgs pa = prec_args(S);
+    // the exact non-zero node range of each column comes out of its solve
+    S.hwr.p[0] = S.hwr.p[2] = S.Nr;
+    S.hwr.p[1] = S.hwr.p[3] = 0;
+    gmd::Krylov *dk = S.dK.p;
+    HIPCK(hipMemcpyAsync(&dk->wrange[0], S.hwr.p, 4 * sizeof(int), hipMemcpyHostToDevice, S.st));
     for (int s = 0; s < P.nsink; ++s) {
         Sink &sk = S.sink[P.sink_body[s]];
         if (sk.W.n != S.nvec()) sk.W.alloc(S.nvec());
@@ -567,15 +576,9 @@ void woodbury_setup(gm_solver &S)
         pa.X = S.tmp.p;
         pa.Y = sk.W.p;
         pa.wood = 0;
+        pa.range = &dk->wrange[2 * s];
         launch(S, GM_PROF_PRECOND_SETUP, [&] { return gmk::launch_precond(pa, gmk::PV_PLAIN, S.st); });
     }
-    // exact non-zero node range of each column
-    S.hwr.p[0] = S.hwr.p[2] = S.Nr;
-    S.hwr.p[1] = S.hwr.p[3] = 0;
-    gmd::Krylov *dk = S.dK.p;
-    HIPCK(hipMemcpyAsync(&dk->wrange[0], S.hwr.p, 4 * sizeof(int), hipMemcpyHostToDevice, S.st));
-    for (int s = 0; s < P.nsink; ++s)
-        launch(S, GM_PROF_SMALL, [&] { return gmk::launch_wrange(S.Nr, S.K, S.sink[P.sink_body[s]].W.p, &dk->wrange[2 * s], S.st); });
     gmk::SinkArgs a{};
     a.N = S.Nr;
     a.K = S.K;
@@ -584,6 +587,21 @@ void woodbury_setup(gm_solver &S)
     a.Kr = S.dK.p;
     a.P = S.dP.p;
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_sink(a, S.st); });
+}
+
+// y -= sum_a c_a W_a over the columns' support (matrices.cpp:151)
+void woodbury_correct(gm_solver &S, double2 *Y)
+{
+    const gmd::StepParams &P = *S.hP.p;
+    gmk::WcorrArgs w{};
+    w.N = S.Nr;
+    w.K = S.K;
+    w.nsink = P.nsink;
+    w.Kr = S.dK.p;
+    w.W0 = S.sink[P.sink_body[0]].W.p;
+    w.W1 = P.nsink > 1 ? S.sink[P.sink_body[1]].W.p : nullptr;
+    w.Y = Y;
+    launch(S, GM_PROF_SMALL, [&] { return gmk::launch_wcorr(w, S.st); });
 }
 
 void enqueue_iteration(gm_solver &S, bool wood)
@@ -596,24 +614,24 @@ void enqueue_iteration(gm_solver &S, bool wood)
     pa.V = S.v.p;
     pa.Y = S.y.p;
     launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_P, S.st); });
+    if (wood) woodbury_correct(S, S.y.p);
     gmk::ApplyArgs a = apply_args(S, S.y.p, S.v.p);
-    a.Xc = S.yc.p;
     a.R0 = S.r0.p;
-    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_V, wood, S.st); });
+    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_V, S.st); });
     // z = M^-1 s with s = r - alpha v
     pa.Pv = S.s.p;
     pa.Y = S.z.p;
     launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_S, S.st); });
+    if (wood) woodbury_correct(S, S.z.p);
     a = apply_args(S, S.z.p, S.t.p);
-    a.Xc = S.zc.p;
     a.R0 = S.s.p;
-    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_T, wood, S.st); });
+    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_T, S.st); });
     gmk::UpdateArgs u{};
     u.n = S.nvec();
     u.Kr = S.dK.p;
     u.X = S.x.p;
-    u.Yc = wood ? S.yc.p : S.y.p;
-    u.Zc = wood ? S.zc.p : S.z.p;
+    u.Yc = S.y.p;
+    u.Zc = S.z.p;
     u.S = S.s.p;
     u.T = S.t.p;
     u.R0 = S.r0.p;
@@ -643,7 +661,7 @@ bool solve_cn(gm_solver &S)
     {
         gmk::ApplyArgs a = apply_args(S, S.psi.p, S.b.p);
         a.Y2 = S.ax.p;
-        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RHS, false, S.st); });
+        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RHS, S.st); });
     }
     // Eigen's BiCGSTAB without sinks, the reference's own recurrence with them
     const int maxit = 500, eigen_mode = wood ? 0 : 1;
@@ -685,7 +703,7 @@ bool solve_cn(gm_solver &S)
         gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
         a.R0 = S.b.p;
         a.Cout = S.tmp.p;  // V x, reused by the diagnostics of the accepted state
-        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RES, false, S.st); });
+        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RES, S.st); });
     }
     if (wood) sink_proj(S, S.x.p, gmk::SD_QNEW, 1.0, 1);
     gmk::AcceptArgs ac{};
@@ -704,8 +722,7 @@ bool solve_cn(gm_solver &S)
 void alloc_device(gm_solver &S)
 {
     const size_t n = S.nvec();
-    for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.ax, &S.yc, &S.zc,
-                     &S.inv, &S.tmp})
+    for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.ax, &S.inv, &S.tmp})
         if (bf->n != n) bf->alloc(n);
     S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.lcut.alloc(S.ntiles);
@@ -714,8 +731,8 @@ void alloc_device(gm_solver &S)
     S.partial.alloc(static_cast<size_t>(3) * std::max(grid, gmd::RED_BLOCKS));
     S.rowpart.alloc(static_cast<size_t>(S.ntiles) * S.K * 6);
     S.dpsi_ix.alloc(S.K);
-    S.diagout.alloc(static_cast<size_t>(4) * S.K + 4);
-    S.hdiag.alloc(static_cast<size_t>(4) * S.K + 4);
+    S.diagout.alloc(static_cast<size_t>(8) * S.K);
+    S.hdiag.alloc(static_cast<size_t>(8) * S.K);
     S.goff.alloc(S.goff_h.size());
     S.G.alloc(std::max<size_t>(S.G_h.size(), 1));
     S.chunks.alloc(S.chunks_h.size());
@@ -796,7 +813,7 @@ void apply_H_host(gm_solver &S, const double *in, double *out)
 {
     to_dev(S, in, S.tmp.p);
     gmk::ApplyArgs a = apply_args(S, S.tmp.p, S.t.p);
-    HIPCK(gmk::launch_apply(a, gmk::AM_H, false, S.st));
+    HIPCK(gmk::launch_apply(a, gmk::AM_H, S.st));
     to_host(S, S.t.p, out);
 }
 
@@ -1132,7 +1149,7 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
             // the same for every mode, so only the stencil is recomputed
             a.Cin = S->tmp.p;
             a.mask = S->mask ? S->dmask.p : nullptr;
-            launch(*S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_DIAG, false, S->st); });
+            launch(*S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_DIAG, S->st); });
             gmk::DiagArgs da{};
             da.N = S->Nr;
             da.K = S->K;
@@ -1147,7 +1164,7 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
             da.out = S->diagout.p;
             launch(*S, GM_PROF_SMALL, [&] { return gmk::launch_diag_final(da, S->st); });
         }
-        HIPCK(hipMemcpyAsync(S->hdiag.p, S->diagout.p, sizeof(double) * (4 * S->K + 4), hipMemcpyDeviceToHost, S->st));
+        HIPCK(hipMemcpyAsync(S->hdiag.p, S->diagout.p, sizeof(double) * (8 * S->K), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipMemcpyAsync(S->hflags.p, S->dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipMemcpyAsync(S->hresid.p, S->dresid.p, sizeof(double), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipMemcpyAsync(S->hK.p, S->dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S->st));
@@ -1198,15 +1215,19 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
         }
         const double *h = S->hdiag.p;
         const int Kn = S->K;
+        // global sums: per-mode shares added in mode order
+        double gsum[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < 4; ++q)
+            for (int j = 0; j < Kn; ++j) gsum[q] += h[(4 + q) * Kn + j];
         const double sign = S->counter ? -1.0 : 1.0;
-        const double p_bin_last = 2.0 * sign * Omega_diag * h[4 * Kn + 1] * S->dr;
+        const double p_bin_last = 2.0 * sign * Omega_diag * gsum[1] * S->dr;
         S->p_bin = S->isolated ? 0.0 : S->avg->update(S->t_i, S->phase, p_bin_last);
         if (d) {
-            d->energy = h[4 * Kn + 0] * S->dr;
+            d->energy = gsum[0] * S->dr;
             d->p_bin = S->p_bin;
             d->p_bin_inst = p_bin_last;
-            d->dV_bin_dt = h[4 * Kn + 2] * S->dr;
-            d->cloud_mass = S->mu * S->dr * h[4 * Kn + 3];
+            d->dV_bin_dt = gsum[2] * S->dr;
+            d->cloud_mass = S->mu * S->dr * gsum[3];
             d->absorbed = absorbed;
             if (d->flux) std::memcpy(d->flux, h, sizeof(double) * Kn);
             if (d->energy_flux) std::memcpy(d->energy_flux, h + Kn, sizeof(double) * Kn);
