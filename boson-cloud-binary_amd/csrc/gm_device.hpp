@@ -6,8 +6,9 @@
 //   E table           2 * (2 l_max + 1) * Nr doubles: the radial multipole
 //                     profiles c_l* (f1 + sign prf f2) for even (index 0) and
 //                     odd (index 1) m*-parity pairs;
-//   G table           per (l, l1) block, [m1][s][m] doubles: the Gaunt
-//                     coefficient of modes (l, m), (l1, m1) at l* = |l-l1| + 2s;
+//   GM table          per (16-row chunk, l1) block, [m1][t][row] doubles: the
+//                     Gaunt coefficient of the row's mode with (l1, m1) at
+//                     l* = 2 (tlo + t) + (l + l1 mod 2), the MFMA A operand;
 //   preconditioner    K*Nr double2 inverse pivots of each mode's radial tridiagonal.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -17,26 +18,36 @@ namespace gmd {
 
 constexpr int TILE = 64;   // radial nodes per wavefront in the operator apply
 constexpr int CH = 8;      // rows (modes of one l) per wavefront in the operator apply
+constexpr int AW = 4;      // wavefronts per operator-apply workgroup
 constexpr int MAXLM = 40;  // largest supported l_max + 1
 constexpr int PT = 256;    // threads of a preconditioner workgroup
 constexpr int PE = 8;      // consecutive nodes per preconditioner thread
 constexpr int RED_BLOCKS = 1024;  // max partial-sum slots per reduction
 
-// Eight rows (modes of one l) handled by one wavefront: m-indices
-// [m0, m0 + CH), of which the chunk writes [own_lo, own_hi). Chunks of an l
-// with l + 1 >= CH overlap so that every chunk has CH real rows.
-struct Chunk {
-    int l, m0, own_lo, own_hi;
+constexpr int MR = 16;     // rows of an MFMA chunk
+
+// Sixteen rows (modes of equal l parity) handled by one wavefront of the
+// operator apply; padding rows have row_j = -1.
+struct MChunk {
+    int row_j[MR], row_l[MR], row_m[MR];
+    int par, pad[3];
+};
+
+// Gaunt coefficients of one (chunk, l1) block: G[goff + (m1 * nT + t) * 16 + row]
+// is the coefficient of the chunk row with mode (l1, m1) at multipole
+// l* = 2 (tlo + t) + p, p = (l + l1) mod 2.
+struct MBlock {
+    int goff, tlo, nT, pad;
 };
 
 // Constant geometry and tables of one solver (device pointers).
 struct Geometry {
     int N, K, lmax, Ls;        // Ls = 2 l_max + 1
-    int ntiles, nchunks;       // node tiles of TILE, row chunks
+    int ntiles, nchunks;       // node tiles of TILE, MFMA row chunks
     double dr, eps, mu, off;   // off = -1/(2 mu dr^2), the stencil off-diagonal of H0
-    const Chunk *chunks;
-    const int *goff;           // (l_max+1)^2 offsets into G
-    const double *G;
+    const MChunk *mchunks;
+    const MBlock *mblocks;     // nchunks * (l_max + 1)
+    const double *GM;          // Gaunt coefficients in MFMA block layout
     const double *E;           // 2 * Ls * N
     const int *lcut;           // per node tile: largest multipole l* evaluated
     const double2 *cap;        // Nr diagonal CAP or nullptr

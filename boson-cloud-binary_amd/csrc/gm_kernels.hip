@@ -24,7 +24,8 @@
 
 namespace gmk {
 
-using gmd::Chunk;
+using gmd::MBlock;
+using gmd::MChunk;
 using gmd::Geometry;
 using gmd::Krylov;
 using gmd::SinkDev;
@@ -170,6 +171,8 @@ __global__ __launch_bounds__(64) void k_lcut(int N, int Ls, double tau, const do
 
 // ---------------------------------------------------------------- operator apply
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ double2 load_x(const ApplyArgs &a, int k, int i)
 {
     return a.X[(size_t)k * a.g.N + i];
@@ -190,72 +193,94 @@ __device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, doub
     return t;
 }
 
-// Coupling of the CH rows of chunk C with the modes of l1 at node ic, for a
-// (l, l1) block whose first S multipole slots are kept: per column m1,
-// W[c] = sum_s G[m1][s][m0 + c] E_s(r) for all CH rows at once (one scalar
-// load of CH coefficients per slot), then acc[c] += W[c] exp(i m1 theta) x.
-template <int S>
-__device__ __forceinline__ void coupling_block(const ApplyArgs &a, const StepParams *P, const Chunk &C, int l, int l1,
-                                               int ic, int Sfull, double2 (&acc)[gmd::CH])
+// Coupling of the 16 rows of a chunk with the modes of l1, at the 64 nodes
+// of a tile, on the matrix cores. For each column m1:
+//   W[row][node] = sum_t G[m1][t][row] E_t(node)   (v_mfma_f64_16x16x4f64, K = 4 slots)
+//   acc[row][node] += W[row][node] exp(i m1 theta) x_(l1,m1)(node)   (VALU)
+// with l* = 2 (tlo + t) + p. Lane layout (gfx950): A[row = lane%16][k = lane/16],
+// B[k = lane/16][node = lane%16], D[row = lane/16 + 4r][node = lane%16].
+template <int SG>
+__device__ __forceinline__ void mfma_block(const ApplyArgs &a, const StepParams *P, int tile, const MBlock &B, int l1,
+                                           int p, int nTe, double2 (&acc)[4][4])
 {
     const Geometry &g = a.g;
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int N = g.N, lmax = g.lmax;
-    const int lo = l > l1 ? l - l1 : l1 - l;
-    const int p = (l + l1) & 1;
-    double e[S];
+    double bf[SG][4];
 #pragma unroll
-    for (int s = 0; s < S; ++s) e[s] = g.E[(size_t)(p * g.Ls + lo + 2 * s) * N + ic];
-    const int Lr = l + 1;  // rows of the block
-    const double *__restrict__ Gb = g.G + g.goff[l * (lmax + 1) + l1] + C.m0;
-    const int kb = l1 * (l1 + 1) / 2;
-    double2 x0 = load_x(a, kb, ic);
-    double2 x1 = l1 >= 1 ? load_x(a, kb + 1, ic) : x0;
-    for (int m1i = 0; m1i <= l1; ++m1i) {
-        const double2 xv = x0;
-        x0 = x1;
-        if (m1i + 2 <= l1) x1 = load_x(a, kb + m1i + 2, ic);
-        const double2 xr = cmul(P->ph[lmax - l1 + 2 * m1i], xv);
-        const double *__restrict__ Gm = Gb + (size_t)m1i * Sfull * Lr;
-        double w[gmd::CH];
+    for (int gg = 0; gg < SG; ++gg) {
+        const int t = 4 * gg + q;
+        const int ls = 2 * (B.tlo + t) + p;
 #pragma unroll
-        for (int c = 0; c < gmd::CH; ++c) w[c] = 0.0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const double *__restrict__ Gs = Gm + s * Lr;
-#pragma unroll
-            for (int c = 0; c < gmd::CH; ++c) w[c] = fma(Gs[c], e[s], w[c]);
+        for (int n = 0; n < 4; ++n) {
+            const int node = tile * gmd::TILE + 16 * n + col;
+            bf[gg][n] = (t < nTe && node < N) ? g.E[(size_t)(p * g.Ls + ls) * N + node] : 0.0;
         }
+    }
+    int nodes[4];
 #pragma unroll
-        for (int c = 0; c < gmd::CH; ++c) {
-            acc[c].x = fma(w[c], xr.x, acc[c].x);
-            acc[c].y = fma(w[c], xr.y, acc[c].y);
+    for (int n = 0; n < 4; ++n) nodes[n] = min(tile * gmd::TILE + 16 * n + col, N - 1);
+    const double *__restrict__ Gb = g.GM + B.goff + col;
+    const int kb = l1 * (l1 + 1) / 2;
+    double2 xa[4];
+    double fa[SG];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) xa[n] = load_x(a, kb, nodes[n]);
+#pragma unroll
+    for (int gg = 0; gg < SG; ++gg) fa[gg] = Gb[(size_t)(4 * gg + q) * 16];
+    for (int m1i = 0; m1i <= l1; ++m1i) {
+        double2 xc[4];
+        double fc[SG];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) xc[n] = xa[n];
+#pragma unroll
+        for (int gg = 0; gg < SG; ++gg) fc[gg] = fa[gg];
+        if (m1i < l1) {
+#pragma unroll
+            for (int n = 0; n < 4; ++n) xa[n] = load_x(a, kb + m1i + 1, nodes[n]);
+#pragma unroll
+            for (int gg = 0; gg < SG; ++gg) fa[gg] = Gb[(size_t)((m1i + 1) * B.nT + 4 * gg + q) * 16];
+        }
+        const double2 ph = P->ph[lmax - l1 + 2 * m1i];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            d4 D = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int gg = 0; gg < SG; ++gg) D = __builtin_amdgcn_mfma_f64_16x16x4f64(fc[gg], bf[gg][n], D, 0, 0, 0);
+            const double2 xr = cmul(ph, xc[n]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[n][r].x = fma(D[r], xr.x, acc[n][r].x);
+                acc[n][r].y = fma(D[r], xr.y, acc[n][r].y);
+            }
         }
     }
 }
 
-template <int SMAX>
-__global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
+// Operator apply. One wavefront: a chunk of 16 rows (modes of equal l parity)
+// at a tile of 64 radial nodes; a workgroup: one chunk on AW adjacent tiles.
+// The epilogue adds the three-point stencil, the centrifugal term and the CAP
+// (hamiltonian.cpp:44-66,224-243), forms A = 1 + i dt/2 H (matrices.cpp:27-40)
+// with the rank-one sinks (matrices.cpp:111-116), and fuses the reductions.
+template <int SGMAX>
+__global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 {
     const int MODE = a.mode;
     const Geometry &g = a.g;
     Krylov *Kr = a.Kr;
     if ((MODE == AM_V || MODE == AM_T) && Kr->done) return;
     const StepParams *P = a.P;
-    const int lane = threadIdx.x & 63;
-    // the wave index is uniform across the wave: keep everything derived from
-    // it in scalar registers (scalar loads of the chunk and the Gaunt table)
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = blockIdx.x;
-    // logical block: consecutive logical blocks run on the same XCD (dispatch
-    // is round-robin over the 8 XCDs). A logical block is one row chunk on 4
-    // adjacent node tiles (the 4 waves share the chunk's Gaunt rows in the
-    // scalar cache); consecutive logical blocks walk the chunks of one tile
-    // group, so its psi tiles stay in that XCD's L2.
+    // consecutive logical blocks run on the same XCD (round-robin dispatch
+    // over the 8 XCDs) and walk the chunks of one tile group, whose psi tiles
+    // then stay in that XCD's L2
     const int lb = (b & 7) * a.per + (b >> 3);
     const int tgroup = lb / g.nchunks;
-    const int chn_b = lb - tgroup * g.nchunks;
-    const int tile_w = tgroup * 4 + wv;
-    const bool item_ok = tgroup < (g.ntiles + 3) / 4 && tile_w < g.ntiles;
+    const int chn = lb - tgroup * g.nchunks;
+    const int tile = tgroup * gmd::AW + wv;
+    const bool item_ok = tgroup < (g.ntiles + gmd::AW - 1) / gmd::AW && tile < g.ntiles;
     const int N = g.N, lmax = g.lmax;
     const int nsink = P->nsink;
     const bool gq_mode = MODE == AM_RHS || MODE == AM_DIAG;
@@ -263,57 +288,57 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
     const double2 coef1 = gq_mode ? Kr->gq[1] : Kr->coefA[1];
     double red[3] = {0.0, 0.0, 0.0};
     if (item_ok) {
-        const int tile = tile_w, chn = chn_b;
-        const Chunk C = g.chunks[chn];
-        const int l = C.l;
-        const int i = tile * gmd::TILE + lane;
-        const bool valid = i < N;
-        const int ic = valid ? i : N - 1;
-        double2 acc[gmd::CH];
+        const MChunk *C = g.mchunks + chn;
+        double2 acc[4][4];
 #pragma unroll
-        for (int c = 0; c < gmd::CH; ++c) acc[c] = make_double2(0.0, 0.0);
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[n][r] = make_double2(0.0, 0.0);
         const bool have_cin = a.Cin != nullptr;
         // multipoles above lcut are below 2^-64 of the tile's leading one
         const int lc = g.lcut ? g.lcut[tile] : 2 * lmax;
+        const int par = C->par;
         for (int l1 = 0; l1 <= (have_cin ? -1 : lmax); ++l1) {
-            const int lo = l > l1 ? l - l1 : l1 - l;
-            const int Sfull = (l < l1 ? l : l1) + 1;
-            const int S = lo > lc ? 0 : min(Sfull, (lc - lo) / 2 + 1);
-            switch (S) {
+            const MBlock B = g.mblocks[chn * (lmax + 1) + l1];
+            const int p = (par + l1) & 1;
+            // slots with l* = 2 (tlo + t) + p <= lc
+            const int tmax = (lc - p) >= 0 ? (lc - p) / 2 : -1;
+            const int nTe = max(0, min(B.nT, tmax - B.tlo + 1));
+            const int SG = (nTe + 3) / 4;
+            switch (SG) {
 #define GM_CASE(n) \
     case n: \
-        if constexpr (n <= SMAX) coupling_block<n>(a, P, C, l, l1, ic, Sfull, acc); \
+        if constexpr (n <= SGMAX) mfma_block<n>(a, P, tile, B, l1, p, nTe, acc); \
         break;
                 GM_CASE(1) GM_CASE(2) GM_CASE(3) GM_CASE(4) GM_CASE(5) GM_CASE(6) GM_CASE(7) GM_CASE(8)
-                GM_CASE(9) GM_CASE(10) GM_CASE(11) GM_CASE(12) GM_CASE(13) GM_CASE(14) GM_CASE(15) GM_CASE(16)
-                GM_CASE(17) GM_CASE(18) GM_CASE(19) GM_CASE(20) GM_CASE(21) GM_CASE(22) GM_CASE(23) GM_CASE(24)
-                GM_CASE(25) GM_CASE(26) GM_CASE(27) GM_CASE(28) GM_CASE(29) GM_CASE(30) GM_CASE(31) GM_CASE(32)
-                GM_CASE(33) GM_CASE(34) GM_CASE(35) GM_CASE(36) GM_CASE(37) GM_CASE(38) GM_CASE(39) GM_CASE(40)
+                GM_CASE(9) GM_CASE(10)
 #undef GM_CASE
             default: break;
             }
         }
-        // epilogue per row
+        // epilogue: lane owns rows q + 4r and nodes 16 n + col
         const double alpha2 = 1.0 / (g.dr * g.dr);
         const double hdt = P->half_dt;
-        const bool interior = i > 0 && i < N - 1;
-        const double2 capv = g.cap && valid ? g.cap[i] : make_double2(0.0, 0.0);
-        double rp[6] = {0, 0, 0, 0, 0, 0};
         const int ix = P->idx_ex;
 #pragma unroll
-        for (int c = 0; c < gmd::CH; ++c) {
-            const int mi = C.m0 + c;
-            if (mi < C.own_lo || mi >= C.own_hi) continue;
-            const int m = -l + 2 * mi;
-            const int j = l * (l + 1) / 2 + mi;
-            double2 Cc = cdotm(P->ph[m + lmax], acc[c]);
-            if (valid) {
+        for (int r = 0; r < 4; ++r) {
+            const int rl = q + 4 * r;
+            const int j = C->row_j[rl];
+            const int l = C->row_l[rl], m = C->row_m[rl];
+            double rp[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const int i = tile * gmd::TILE + 16 * n + col;
+                if (j < 0 || i >= N) continue;
                 const size_t row = (size_t)j * N + i;
+                double2 Cc;
                 if (have_cin) Cc = cscale(Kr->rhs_nrm * (a.mask ? a.mask[i] : 1.0), a.Cin[row]);
+                else Cc = cdotm(P->ph[m + lmax], acc[n][r]);
                 if (a.Cout) a.Cout[row] = Cc;
                 const double2 xi = load_x(a, j, i);
+                const double2 capv = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
                 double2 Hx;
-                if (interior) {
+                if (i > 0 && i < N - 1) {
                     const double ri = g.eps + i * g.dr;
                     const double h0d = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * g.mu);
                     const double2 xm = load_x(a, j, i - 1);
@@ -370,11 +395,13 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
                 }
             }
             if (MODE == AM_DIAG) {
+                // per-row sums over the 16 lanes that share the row
 #pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const double s = wave_sum(rp[q]);
-                    if (lane == 0) a.rowpart[((size_t)tile * g.K + j) * 6 + q] = s;
-                    rp[q] = 0.0;
+                for (int qq = 0; qq < 6; ++qq) {
+                    double s = rp[qq];
+#pragma unroll
+                    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o);
+                    if (col == 0 && j >= 0) a.rowpart[((size_t)tile * g.K + j) * 6 + qq] = s;
                 }
             }
         }
@@ -427,7 +454,7 @@ __device__ __forceinline__ double2 cn_diag(const FactorArgs &f, const double *Gj
         const double ri = f.eps + i * f.dr;
         const double alpha2 = 1.0 / (f.dr * f.dr);
         double v = 0.0;
-        for (int s = 0; s <= l; ++s) v += Gjj[(size_t)s * (l + 1)] * f.E[(size_t)(2 * s) * N + i];
+        for (int s = 0; s <= l; ++s) v += Gjj[s] * f.E[(size_t)(2 * s) * N + i];
         h.x = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * f.mu) + v;
     }
     if (f.cap) h = cadd(h, f.cap[i]);
@@ -483,7 +510,8 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= j) ++l;
     const int mi = j - l * (l + 1) / 2;
-    const double *Gjj = f.G + f.goff[l * (f.lmax + 1) + l] + (size_t)mi * (l + 1) * (l + 1) + mi;
+    (void)mi;
+    const double *Gjj = f.Gdiag + (size_t)j * (f.lmax + 1);
     const double2 ell = make_double2(0.0, f.half_dt * f.off);
     const double2 ell2 = cmul(ell, ell);
     const int N = f.N;
@@ -1141,16 +1169,16 @@ __global__ __launch_bounds__(64) void k_diag_final(DiagArgs a)
 
 // ---------------------------------------------------------------- launchers
 
-template <int SMAX>
+template <int SGMAX>
 static hipError_t launch_apply_s(const ApplyArgs &a, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_apply<SMAX>), dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_apply<SGMAX>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
     return hipGetLastError();
 }
 
 int apply_grid(int ntiles, int nchunks, int *per)
 {
-    const int nb = (ntiles + 3) / 4 * nchunks;
+    const int nb = (ntiles + gmd::AW - 1) / gmd::AW * nchunks;
     *per = (nb + 7) / 8;
     return *per * 8;
 }
@@ -1159,12 +1187,13 @@ hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
     const int grid = apply_grid(a.g.ntiles, a.g.nchunks, &a.per);
     a.mode = mode;
-    const int smax = a.g.lmax + 1;
-    if (smax <= 8) return launch_apply_s<8>(a, grid, st);
-    if (smax <= 16) return launch_apply_s<16>(a, grid, st);
-    if (smax <= 24) return launch_apply_s<24>(a, grid, st);
-    if (smax <= 32) return launch_apply_s<32>(a, grid, st);
-    return launch_apply_s<40>(a, grid, st);
+    // slot groups of 4 multipoles: at most ceil((l_max + 1) / 4)
+    const int sg = (a.g.lmax + 1 + 3) / 4;
+    if (sg <= 2) return launch_apply_s<2>(a, grid, st);
+    if (sg <= 4) return launch_apply_s<4>(a, grid, st);
+    if (sg <= 6) return launch_apply_s<6>(a, grid, st);
+    if (sg <= 8) return launch_apply_s<8>(a, grid, st);
+    return launch_apply_s<10>(a, grid, st);
 }
 
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt, double mu,

@@ -32,8 +32,7 @@ struct ApplyArgs {
 struct FactorArgs {
     int N, K, lmax;
     double dr, eps, mu, off, half_dt;
-    const int *goff;
-    const double *G;
+    const double *Gdiag;  // K * (l_max + 1): coefficient of (j, j) at l* = 2 s
     const double *E;
     const double2 *cap;
     double2 *inv;

@@ -126,13 +126,15 @@ struct gm_solver {
     hipStream_t st = nullptr;
     std::vector<int> goff_h;
     std::vector<double> G_h;
-    std::vector<gmd::Chunk> chunks_h;
-    DevBuf<int> goff;
-    DevBuf<double> G, E, dmask;
+    std::vector<gmd::MChunk> mchunks_h;
+    std::vector<gmd::MBlock> mblocks_h;
+    std::vector<double> GM_h, Gdiag_h;
+    DevBuf<gmd::MChunk> mchunks;
+    DevBuf<gmd::MBlock> mblocks;
+    DevBuf<double> GM, Gdiag, E, dmask;
     DevBuf<int> lcut;
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
-    DevBuf<gmd::Chunk> chunks;
     DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, inv, tmp;
     DevBuf<double> partial, rowpart, diagout;
     DevBuf<double2> dpsi_ix;
@@ -226,6 +228,8 @@ void prof_collect(gm_solver &S)
 
 // ------------------------------------------------------------------ tables
 
+double angular(const gm_solver &S, int j, int k, int ls);
+
 // Gaunt blocks: for each (l, l1), [m1][s][m] with l* = |l - l1| + 2 s, the
 // angular coefficient of build_coupling_cache (hamiltonian.cpp:77-95).
 void build_gaunt(gm_solver &S)
@@ -255,16 +259,84 @@ void build_gaunt(gm_solver &S)
                         S.G_h.push_back(ang);
                     }
         }
-    // a chunk reads CH rows from its first row: pad the last block
-    S.G_h.insert(S.G_h.end(), gmd::CH, 0.0);
-    S.chunks_h.clear();
-    // largest rows first: the tail of the apply grid holds the cheap chunks
-    for (int l = S.l_max; l >= 0; --l)
-        for (int own = 0; own <= l; own += gmd::CH) {
-            const int m0 = (l + 1 >= gmd::CH) ? std::min(own, l + 1 - gmd::CH) : 0;
-            S.chunks_h.push_back({l, m0, own, std::min(own + gmd::CH, l + 1)});
+}
+
+// MFMA tables: 16-row chunks of modes with equal l parity (largest l first),
+// and for every (chunk, l1) the Gaunt block [m1][t][row] with
+// l* = 2 (tlo + t) + p, p = (l + l1) mod 2 (zero where the row has no term).
+void build_mfma_tables(gm_solver &S)
+{
+    const int Lm = S.l_max + 1;
+    S.mchunks_h.clear();
+    for (int par = 0; par < 2; ++par) {
+        std::vector<int> rows;
+        for (int l = S.l_max; l >= 0; --l)
+            if (l % 2 == par)
+                for (int mi = 0; mi <= l; ++mi) rows.push_back(l * (l + 1) / 2 + mi);
+        for (size_t r0 = 0; r0 < rows.size(); r0 += gmd::MR) {
+            gmd::MChunk c{};
+            c.par = par;
+            for (int r = 0; r < gmd::MR; ++r) {
+                const size_t k = r0 + r;
+                c.row_j[r] = k < rows.size() ? rows[k] : -1;
+                c.row_l[r] = k < rows.size() ? S.ml[rows[k]] : 0;
+                c.row_m[r] = k < rows.size() ? S.mm[rows[k]] : 0;
+            }
+            S.mchunks_h.push_back(c);
         }
-    S.nchunks = static_cast<int>(S.chunks_h.size());
+    }
+    // blocks and the chunk work estimate (MFMA slot groups x columns)
+    auto block_range = [&](const gmd::MChunk &c, int l1, int &tlo, int &nT) {
+        const int p = (c.par + l1) & 1;
+        int lo = 1 << 30, hi = -1;
+        for (int r = 0; r < gmd::MR; ++r) {
+            if (c.row_j[r] < 0) continue;
+            const int l = c.row_l[r];
+            lo = std::min(lo, (std::abs(l - l1) - p) / 2);
+            hi = std::max(hi, (l + l1 - p) / 2);
+        }
+        tlo = lo;
+        nT = hi - lo + 1;
+    };
+    std::vector<std::pair<long, int>> order;
+    for (size_t c = 0; c < S.mchunks_h.size(); ++c) {
+        long work = 0;
+        for (int l1 = 0; l1 <= S.l_max; ++l1) {
+            int tlo, nT;
+            block_range(S.mchunks_h[c], l1, tlo, nT);
+            work += static_cast<long>(l1 + 1) * ((nT + 3) / 4);
+        }
+        order.push_back({-work, static_cast<int>(c)});
+    }
+    std::stable_sort(order.begin(), order.end());
+    std::vector<gmd::MChunk> sorted;
+    for (auto &o : order) sorted.push_back(S.mchunks_h[o.second]);
+    S.mchunks_h.swap(sorted);
+    S.mblocks_h.assign(S.mchunks_h.size() * Lm, gmd::MBlock{});
+    S.GM_h.clear();
+    for (size_t c = 0; c < S.mchunks_h.size(); ++c) {
+        const gmd::MChunk &ch = S.mchunks_h[c];
+        for (int l1 = 0; l1 <= S.l_max; ++l1) {
+            gmd::MBlock &B = S.mblocks_h[c * Lm + l1];
+            block_range(ch, l1, B.tlo, B.nT);
+            B.goff = static_cast<int>(S.GM_h.size());
+            const int p = (ch.par + l1) & 1;
+            for (int m1i = 0; m1i <= l1; ++m1i)
+                for (int t = 0; t < B.nT; ++t)
+                    for (int r = 0; r < gmd::MR; ++r) {
+                        double v = 0.0;
+                        if (ch.row_j[r] >= 0) v = angular(S, ch.row_j[r], l1 * (l1 + 1) / 2 + m1i, 2 * (B.tlo + t) + p);
+                        S.GM_h.push_back(v);
+                    }
+        }
+    }
+    // fragments of the last block may read up to 3 slots past its end
+    S.GM_h.insert(S.GM_h.end(), 4 * gmd::MR, 0.0);
+    S.nchunks = static_cast<int>(S.mchunks_h.size());
+    // diagonal coefficients (j, j) at l* = 2 s for the Thomas pivots
+    S.Gdiag_h.assign(static_cast<size_t>(S.K) * Lm, 0.0);
+    for (int j = 0; j < S.K; ++j)
+        for (int s = 0; s <= S.ml[j]; ++s) S.Gdiag_h[static_cast<size_t>(j) * Lm + s] = angular(S, j, j, 2 * s);
     S.ntiles = (S.Nr + gmd::TILE - 1) / gmd::TILE;
 }
 
@@ -291,9 +363,9 @@ gmd::Geometry geometry(const gm_solver &S)
     g.eps = S.epsilon;
     g.mu = S.mu;
     g.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
-    g.chunks = S.chunks.p;
-    g.goff = S.goff.p;
-    g.G = S.G.p;
+    g.mchunks = S.mchunks.p;
+    g.mblocks = S.mblocks.p;
+    g.GM = S.GM.p;
     g.E = S.E.p;
     g.lcut = S.lcut.p;
     g.cap = S.cap ? S.dcap.p : nullptr;
@@ -527,8 +599,7 @@ void factorize(gm_solver &S)
     f.mu = S.mu;
     f.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
     f.half_dt = S.dt * 0.5;
-    f.goff = S.goff.p;
-    f.G = S.G.p;
+    f.Gdiag = S.Gdiag.p;
     f.E = S.E.p;
     f.cap = S.cap ? S.dcap.p : nullptr;
     f.inv = S.inv.p;
@@ -733,14 +804,15 @@ void alloc_device(gm_solver &S)
     S.dpsi_ix.alloc(S.K);
     S.diagout.alloc(static_cast<size_t>(8) * S.K);
     S.hdiag.alloc(static_cast<size_t>(8) * S.K);
-    S.goff.alloc(S.goff_h.size());
-    S.G.alloc(std::max<size_t>(S.G_h.size(), 1));
-    S.chunks.alloc(S.chunks_h.size());
-    HIPCK(hipMemcpyAsync(S.goff.p, S.goff_h.data(), S.goff_h.size() * sizeof(int), hipMemcpyHostToDevice, S.st));
-    if (!S.G_h.empty())
-        HIPCK(hipMemcpyAsync(S.G.p, S.G_h.data(), S.G_h.size() * sizeof(double), hipMemcpyHostToDevice, S.st));
-    HIPCK(hipMemcpyAsync(S.chunks.p, S.chunks_h.data(), S.chunks_h.size() * sizeof(gmd::Chunk), hipMemcpyHostToDevice,
-                         S.st));
+    S.mchunks.alloc(S.mchunks_h.size());
+    S.mblocks.alloc(S.mblocks_h.size());
+    S.GM.alloc(S.GM_h.size());
+    S.Gdiag.alloc(S.Gdiag_h.size());
+    // tables are uploaded from pageable vectors: synchronous copies
+    HIPCK(hipMemcpy(S.mchunks.p, S.mchunks_h.data(), S.mchunks_h.size() * sizeof(gmd::MChunk), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(S.mblocks.p, S.mblocks_h.data(), S.mblocks_h.size() * sizeof(gmd::MBlock), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(S.GM.p, S.GM_h.data(), S.GM_h.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(S.Gdiag.p, S.Gdiag_h.data(), S.Gdiag_h.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCK(hipMemsetAsync(S.dK.p, 0, sizeof(gmd::Krylov), S.st));
     HIPCK(hipMemsetAsync(S.dflags.p, 0, 2 * sizeof(int), S.st));
 }
@@ -894,6 +966,7 @@ int gm_create(const gm_params *p, gm_solver **out)
         S->hwr.alloc(4);
         std::memset(S->hP.p, 0, sizeof(gmd::StepParams));
         build_gaunt(*S);
+        build_mfma_tables(*S);
         alloc_device(*S);
         HIPCK(hipMemsetAsync(S->psi.p, 0, S->nvec() * sizeof(double2), S->st));
         HIPCK(hipStreamSynchronize(S->st));
