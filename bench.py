@@ -37,7 +37,6 @@ FP64_PEAK_TFLOPS = 78.6
 WORKLOAD = dict(Nr=16384, L=4000.0, dt=0.25, epsilon=0.0, l_max=20, mu=0.15, M_c=0.01, q=0.5, a=0.0, R=50.0,
                 counter=False, absorption=True, save_every=50, ramp_periods=0.0, epsilon_smear=2.0)
 MASK = (0.85, 1.0, 6)
-PROF_NAMES = ["apply_krylov", "apply_other", "precond", "precond_setup", "update", "factor", "small"]
 
 
 def initial_state(w, seed=0):
@@ -99,7 +98,7 @@ def profile(s, on):
 
 
 def profile_read(s):
-    return {PROF_NAMES[c]: v for c, v in enumerate(s.profile_read())}
+    return s.profile_read()
 
 
 def run_gpu(w, precision, steps, warmup):
@@ -153,6 +152,83 @@ def cpu_baseline(w, budget_s=20.0):
             "iterations_per_step": o.last_solver_iterations}
 
 
+# Algorithmic HBM bytes per mode-node of the bandwidth-bound kernel classes
+# (complex128 = 16 B per vector element; tables and per-node profiles excluded):
+#   precond  forward: read r, p, v, pivots, write p, y; backward: read pivots,
+#            y, write y (p-update variant 9 vectors; s-update variant 8: r, v,
+#            pivots / s, y / pivots, y / y) -> 8.5 on average
+#   update   read x, y, z, s, t, r0; write x, r -> 8
+#   diag     read psi, V x; per-tile sums only -> 2
+#   apply    read x (+ r0 / b), write y (+ A psi) -> 3 (coupling excluded)
+HBM_PASSES = {"precond": 8.5, "precond_setup": 3.0, "update": 8.0, "diag": 2.0}
+
+
+def coupling_flops(terms, pairs, K, N):
+    """FP64 flops of one full operator apply: one multiply-add per Gaunt term
+    (real W), two per coupled mode pair (real W times complex x), and about
+    20 per mode-node for the stencil, phases and the A = 1 + i dt/2 H form."""
+    return 2.0 * terms + 4.0 * pairs + 20.0 * K * N
+
+
+def rooflines(s, prof, K, N, iters):
+    """The dominant kernel's roofline (the coupling apply k_apply, every working
+    launch of which does the same coupling work) and the achieved rates of the
+    other kernel classes, from HIP-event times on the solver's stream.
+    BiCGSTAB iterations are enqueued in batches, so a few Krylov launches
+    after convergence return at once (about 5 us); the averages count the
+    working launches: two applies per iteration plus the rhs and acceptance
+    applies of every step."""
+    terms, pairs = s.coupling_work()
+    ms = prof["apply_krylov"][0] + prof["apply_other"][0]
+    cnt = min(prof["apply_krylov"][1], 2 * int(sum(iters))) + prof["apply_other"][1]
+    avg_s = ms / max(cnt, 1) * 1e-3
+    flops = coupling_flops(terms, pairs, K, N)
+    roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "kernel": "k_apply", "per_launch": flops, "per_launch_unit": "flop",
+            "avg_launch_ms": avg_s * 1e3, "launches": cnt}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = measured_traffic("k_apply")
+    kernels = {}
+    working = {"precond": min(prof["precond"][1], 2 * int(sum(iters))),
+               "update": min(prof["update"][1], int(sum(iters)))}
+    for name, passes in HBM_PASSES.items():
+        kms, kn = prof[name][0], working.get(name, prof[name][1])
+        if kn == 0:
+            continue
+        byts = passes * 16.0 * K * N
+        a = byts / (kms / kn * 1e-3) / 1e9
+        kernels[name] = {"bound": "hbm", "avg_launch_ms": kms / kn, "bytes_per_launch": byts,
+                         "achieved_GBs": a, "frac": a / HBM_PEAK_GBS}
+    return roof, kernels
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (scaled by a copy-kernel calibration), or None."""
+    path = os.path.join(REPO, "profiles", "k_apply_traffic_l20_n16384.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch " + kernel + " "):
+                    return float(line.split()[2])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(nbytes=1 << 29, reps=5):
+    """A plain device copy of known size for scaling the FETCH_SIZE and
+    WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "fetch_kib_expected": nbytes / 1024,
+                      "write_kib_expected": nbytes / 1024}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,7 +236,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every box here has one MI355X and this build "
               "has no multi-GPU path (ensembles would shard whole members per GPU).", file=sys.stderr)
@@ -170,25 +251,8 @@ def main():
     s, elapsed, prof, iters = run_gpu(w, precision, args.steps, args.warmup)
     K, N = s.n_modes, s.Nr
     value = K * N * args.steps / elapsed
-    # roofline of the dominant kernel class
-    dom = max(prof, key=lambda k: prof[k][0])
-    ms, cnt = prof[dom]
-    avg_s = ms / max(cnt, 1) * 1e-3
+    roof, kernels = rooflines(s, prof, K, N, iters)
     terms, pairs = s.coupling_work()
-    if dom.startswith("apply"):
-        # Gaunt terms (1 FMA each), pair matvec (complex x real, 2 FMA), stencil/CN (~10 FMA per mode.node)
-        flops = 2.0 * terms + 4.0 * pairs + 20.0 * K * N
-        roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "kernel": "k_apply (" + dom + ")", "per_launch": "%.4g flop" % flops}
-        roof["frac"] = roof["achieved"] / roof["peak"]
-    else:
-        passes = {"precond": 6, "precond_setup": 3, "update": 8}.get(dom, 2)
-        byts = passes * 16.0 * K * N
-        roof = {"bound": "hbm", "achieved": byts / avg_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "kernel": dom, "per_launch": "%.4g B" % byts}
-        roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["avg_launch_ms"] = ms / max(cnt, 1)
     out = {
         "metric": METRIC, "value": value, "unit": "mode·node-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
@@ -198,6 +262,7 @@ def main():
                                "dt=0.25, fixed orbit, absorbers on, mask", "K": K, "N": N,
                    "precision": args.precision, "parallelism": "single GPU"},
         "roofline": roof,
+        "kernels": kernels,
         "iterations_per_step": float(np.mean(iters)) if iters else None,
         "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
         "kernel_launches": {k: int(v[1]) for k, v in prof.items()},

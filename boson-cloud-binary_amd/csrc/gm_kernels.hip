@@ -283,7 +283,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     const bool item_ok = tgroup < (g.ntiles + gmd::AW - 1) / gmd::AW && tile < g.ntiles;
     const int N = g.N, lmax = g.lmax;
     const int nsink = P->nsink;
-    const bool gq_mode = MODE == AM_RHS || MODE == AM_DIAG;
+    const bool gq_mode = MODE == AM_RHS;
     const double2 coef0 = gq_mode ? Kr->gq[0] : Kr->coefA[0];
     const double2 coef1 = gq_mode ? Kr->gq[1] : Kr->coefA[1];
     double red[3] = {0.0, 0.0, 0.0};
@@ -294,11 +294,10 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         for (int n = 0; n < 4; ++n)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[n][r] = make_double2(0.0, 0.0);
-        const bool have_cin = a.Cin != nullptr;
         // multipoles above lcut are below 2^-64 of the tile's leading one
         const int lc = g.lcut ? g.lcut[tile] : 2 * lmax;
         const int par = C->par;
-        for (int l1 = 0; l1 <= (have_cin ? -1 : lmax); ++l1) {
+        for (int l1 = 0; l1 <= lmax; ++l1) {
             const MBlock B = g.mblocks[chn * (lmax + 1) + l1];
             const int p = (par + l1) & 1;
             // slots with l* = 2 (tlo + t) + p <= lc
@@ -319,21 +318,17 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         // epilogue: lane owns rows q + 4r and nodes 16 n + col
         const double alpha2 = 1.0 / (g.dr * g.dr);
         const double hdt = P->half_dt;
-        const int ix = P->idx_ex;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int rl = q + 4 * r;
             const int j = C->row_j[rl];
             const int l = C->row_l[rl], m = C->row_m[rl];
-            double rp[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int n = 0; n < 4; ++n) {
                 const int i = tile * gmd::TILE + 16 * n + col;
                 if (j < 0 || i >= N) continue;
                 const size_t row = (size_t)j * N + i;
-                double2 Cc;
-                if (have_cin) Cc = cscale(Kr->rhs_nrm * (a.mask ? a.mask[i] : 1.0), a.Cin[row]);
-                else Cc = cdotm(P->ph[m + lmax], acc[n][r]);
+                const double2 Cc = cdotm(P->ph[m + lmax], acc[n][r]);
                 if (a.Cout) a.Cout[row] = Cc;
                 const double2 xi = load_x(a, j, i);
                 const double2 capv = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
@@ -376,32 +371,6 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                     a.Y[row] = rhs;
                     a.Y2[row] = cadd(cadd(xi, sHx), st);
                     red[0] += cnorm2(rhs);
-                } else if (MODE == AM_DIAG) {
-                    double2 dpsi = make_double2(Hx.y, -Hx.x);  // -i H psi
-                    if (nsink) dpsi = csub(dpsi, cscale(0.5, sink_term(a, nsink, coef0, coef1, j, i)));
-                    const double2 eh = cdotm(xi, Hx);
-                    const double n2 = cnorm2(xi);
-                    if (i < ix) {
-                        rp[0] += n2;
-                        rp[1] += eh.x;
-                    }
-                    if (i <= ix) {
-                        rp[2] += eh.x;
-                        rp[3] += m * cdotm(xi, Cc).y;
-                        rp[4] += cdotm(dpsi, Cc).x;
-                    }
-                    rp[5] += n2;
-                    if (i == ix) a.dpsi_ix[j] = dpsi;
-                }
-            }
-            if (MODE == AM_DIAG) {
-                // per-row sums over the 16 lanes that share the row
-#pragma unroll
-                for (int qq = 0; qq < 6; ++qq) {
-                    double s = rp[qq];
-#pragma unroll
-                    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o);
-                    if (col == 0 && j >= 0) a.rowpart[((size_t)tile * g.K + j) * 6 + qq] = s;
                 }
             }
         }
@@ -439,6 +408,65 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                 Kr->counter = 0;
             }
         }
+    }
+}
+
+// Diagnostics apply on the post-mask state (extraction.cpp:8-129). The
+// coupling is node-local and the mask is one radial profile for every mode, so
+// V psi = |rhs| mask (V x) with V x kept by the acceptance apply; only the
+// stencil is evaluated here. One wavefront per (mode, tile of 64 nodes) writes
+// the tile sums of |U|^2 and Re<U|H U> below idx_ex, Re<U|H U>, Im(m <U|V U>)
+// and Re<dU/dt|V U> up to idx_ex inclusive, and |U|^2 over the grid.
+__global__ __launch_bounds__(256) void k_diag_apply(ApplyArgs a)
+{
+    const Geometry &g = a.g;
+    const StepParams *P = a.P;
+    const Krylov *Kr = a.Kr;
+    const int lane = threadIdx.x & 63;
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= g.K * g.ntiles) return;
+    const int j = item / g.ntiles, tile = item - j * g.ntiles;
+    const int N = g.N, i = tile * gmd::TILE + lane;
+    double rp[6] = {0, 0, 0, 0, 0, 0};
+    if (i < N) {
+        const size_t row = (size_t)j * N + i;
+        const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
+        const int l = (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
+        const int m = 2 * (j - l * (l + 1) / 2) - l;
+        const double2 Cc = cscale(Kr->rhs_nrm * (a.mask ? a.mask[i] : 1.0), a.Cin[row]);
+        const double2 xi = load_x(a, j, i);
+        const double2 capv = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
+        double2 Hx;
+        if (i > 0 && i < N - 1) {
+            const double ri = g.eps + i * g.dr;
+            const double h0d = -(-2.0 / (g.dr * g.dr) - (double)(l * (l + 1)) / (ri * ri)) / (2 * g.mu);
+            const double2 xm = load_x(a, j, i - 1);
+            const double2 xp = load_x(a, j, i + 1);
+            Hx = cadd(cscale(g.off, cadd(xm, xp)), cadd(cmul(make_double2(h0d + capv.x, capv.y), xi), Cc));
+        } else {
+            Hx = cmul(capv, xi);
+        }
+        double2 dpsi = make_double2(Hx.y, -Hx.x);  // -i H psi
+        if (P->nsink) dpsi = csub(dpsi, cscale(0.5, sink_term(a, P->nsink, Kr->gq[0], Kr->gq[1], j, i)));
+        const double2 eh = cdotm(xi, Hx);
+        const double n2 = cnorm2(xi);
+        const int ix = P->idx_ex;
+        if (i < ix) {
+            rp[0] = n2;
+            rp[1] = eh.x;
+        }
+        if (i <= ix) {
+            rp[2] = eh.x;
+            rp[3] = m * cdotm(xi, Cc).y;
+            rp[4] = cdotm(dpsi, Cc).x;
+        }
+        rp[5] = n2;
+        if (i == ix) a.dpsi_ix[j] = dpsi;
+    }
+#pragma unroll
+    for (int qq = 0; qq < 6; ++qq) {
+        const double s = wave_sum(rp[qq]);
+        if (lane == 0) a.rowpart[((size_t)tile * g.K + j) * 6 + qq] = s;
     }
 }
 
@@ -1194,6 +1222,13 @@ hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
     if (sg <= 6) return launch_apply_s<6>(a, grid, st);
     if (sg <= 8) return launch_apply_s<8>(a, grid, st);
     return launch_apply_s<10>(a, grid, st);
+}
+
+hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st)
+{
+    const int items = a.g.K * a.g.ntiles;
+    hipLaunchKernelGGL(k_diag_apply, dim3((items + 3) / 4), dim3(256), 0, st, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt, double mu,

@@ -4,7 +4,7 @@
 
 namespace gmk {
 
-enum ApplyMode { AM_H = 0, AM_A = 1, AM_V = 2, AM_T = 3, AM_RHS = 4, AM_RES = 5, AM_DIAG = 6 };
+enum ApplyMode { AM_H = 0, AM_A = 1, AM_V = 2, AM_T = 3, AM_RHS = 4, AM_RES = 5 };
 enum PrecVar { PV_PLAIN = 0, PV_P = 1, PV_S = 2 };
 enum SinkMode { SK_PROJ = 0, SK_SETUP = 1 };
 enum SinkDest { SD_START = 0, SD_QNEW = 1, SD_POST = 2, SD_COEFA = 3 };
@@ -19,11 +19,11 @@ struct ApplyArgs {
     const double2 *R0;   // r0 (AM_V), s (AM_T), b (AM_RES)
     gmd::SinkDev sk[2];
     double *partial;     // reduction partials (3 * grid doubles)
-    double *rowpart;     // AM_DIAG: 6 doubles per (tile, mode)
-    double2 *dpsi_ix;    // AM_DIAG: dpsi/dt at the extraction node per mode
+    double *rowpart;     // diagnostics apply: 6 doubles per (tile, mode)
+    double2 *dpsi_ix;    // diagnostics apply: dpsi/dt at the extraction node per mode
     double2 *Cout;       // AM_RES: coupling V x written per row (or nullptr)
-    const double2 *Cin;  // AM_DIAG: reuse V x_prev; V psi = |rhs| mask V x_prev
-    const double *mask;  // damping mask (AM_DIAG with Cin), or nullptr
+    const double2 *Cin;  // diagnostics apply: V x of the acceptance apply; V psi = |rhs| mask V x
+    const double *mask;  // diagnostics apply: damping mask, or nullptr
     int nitems;          // node tiles x row chunks
     int per;             // set by the launcher
     int mode;            // ApplyMode, set by the launcher
@@ -126,6 +126,7 @@ hipError_t launch_accept(const AcceptArgs &a, hipStream_t st);
 hipError_t launch_sink(const SinkArgs &a, hipStream_t st);
 hipError_t launch_expand(int N, int K, const gmd::SinkDev &sk, double2 *out, hipStream_t st);
 hipError_t launch_wcorr(const WcorrArgs &a, hipStream_t st);
+hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st);
 hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st);
 
 }  // namespace gmk

@@ -1222,7 +1222,7 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
             // the same for every mode, so only the stencil is recomputed
             a.Cin = S->tmp.p;
             a.mask = S->mask ? S->dmask.p : nullptr;
-            launch(*S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_DIAG, S->st); });
+            launch(*S, GM_PROF_DIAG, [&] { return gmk::launch_diag_apply(a, S->st); });
             gmk::DiagArgs da{};
             da.N = S->Nr;
             da.K = S->K;
@@ -1235,7 +1235,7 @@ int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
             da.dpsi_ix = S->dpsi_ix.p;
             da.rowpart = S->rowpart.p;
             da.out = S->diagout.p;
-            launch(*S, GM_PROF_SMALL, [&] { return gmk::launch_diag_final(da, S->st); });
+            launch(*S, GM_PROF_DIAG, [&] { return gmk::launch_diag_final(da, S->st); });
         }
         HIPCK(hipMemcpyAsync(S->hdiag.p, S->diagout.p, sizeof(double) * (8 * S->K), hipMemcpyDeviceToHost, S->st));
         HIPCK(hipMemcpyAsync(S->hflags.p, S->dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S->st));

@@ -56,8 +56,13 @@ class CrankNicolsonModes(capi.Solver):
     def profile(self, on):
         self._check(self._lib.gm_profile_enable(self._h, int(on)))
 
+    PROFILE_CATEGORIES = ("apply_krylov", "apply_other", "precond", "precond_setup", "update", "factor", "small",
+                          "diag")
+
     def profile_read(self):
-        ms = (C.c_double * 7)()
-        n = (C.c_int64 * 7)()
-        self._lib.gm_profile_read(self._h, ms, n, 7)
-        return [(ms[c], n[c]) for c in range(7)]
+        """{category: (device ms, launches)} since profile(True)."""
+        nc = len(self.PROFILE_CATEGORIES)
+        ms = (C.c_double * nc)()
+        n = (C.c_int64 * nc)()
+        self._lib.gm_profile_read(self._h, ms, n, nc)
+        return {name: (ms[c], n[c]) for c, name in enumerate(self.PROFILE_CATEGORIES)}

@@ -171,13 +171,14 @@ int gm_coupling_work(gm_solver *solver, double *terms, double *pairs);
  * gm_profile_enable resets the counters; gm_profile_read fills ncat entries. */
 enum {
     GM_PROF_APPLY_KRYLOV = 0, /* operator apply inside BiCGSTAB (v = A y, t = A z) */
-    GM_PROF_APPLY_OTHER = 1,  /* rhs, acceptance and diagnostics applies */
+    GM_PROF_APPLY_OTHER = 1,  /* rhs and acceptance applies (same coupling work) */
     GM_PROF_PRECOND = 2,      /* Thomas solves inside BiCGSTAB */
     GM_PROF_PRECOND_SETUP = 3,/* Woodbury columns P^-1 v */
     GM_PROF_UPDATE = 4,       /* x, r update and loop control */
     GM_PROF_FACTOR = 5,       /* Thomas pivots and radial profiles */
-    GM_PROF_SMALL = 6,        /* sink projections, init, accept, diagnostics sums */
-    GM_PROF_NCAT = 7
+    GM_PROF_SMALL = 6,        /* sink projections, init, accept, Woodbury correction */
+    GM_PROF_DIAG = 7,         /* diagnostics stencil and per-mode sums */
+    GM_PROF_NCAT = 8
 };
 int gm_profile_enable(gm_solver *solver, int32_t on);
 int gm_profile_read(const gm_solver *solver, double *ms, int64_t *launches, int32_t ncat);
