@@ -56,7 +56,24 @@ def build(clean=False, verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER_SRC = os.path.join(REPO, "tests", "cpp", "cnm_driver.cpp")
+DRIVER = os.path.join(REPO, "tests", "cpp", "_bin", "cnm_driver")
+
+
+def build_driver(verbose=True):
+    """The C++ test driver of include/gm.hpp, linked against libgm.so."""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"), DRIVER_SRC, "-o", DRIVER,
+           "-L", os.path.dirname(LIB), "-lgm", "-Wl,-rpath,$ORIGIN/../../../boson-cloud-binary_amd/gm"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), res.stdout + res.stderr))
+    if verbose:
+        print("built", DRIVER)
 
 
 if __name__ == "__main__":
