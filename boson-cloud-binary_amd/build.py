@@ -26,7 +26,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I", CSRC, "-I", os.path.join(REPO, "include")]
 # Packed FP32 VALU ops stay off in every device pass (v_pk_* have returned wrong
 # results on this hardware); the host pass warns that it does not know the feature.
-DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          # matrix-core results in ordinary VGPRs (no accumulator copies in the coupling loop)
+          "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def compile_one(src):

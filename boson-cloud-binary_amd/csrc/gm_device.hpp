@@ -16,9 +16,9 @@
 
 namespace gmd {
 
-constexpr int TILE = 64;   // radial nodes per wavefront in the operator apply
-constexpr int CH = 8;      // rows (modes of one l) per wavefront in the operator apply
-constexpr int AW = 4;      // wavefronts per operator-apply workgroup
+constexpr int TILE = 64;   // radial nodes of a multipole-cut / diagnostics tile
+constexpr int AW = 4;      // row chunks (wavefronts) per operator-apply workgroup
+constexpr int APPLY_NG = 2;  // 16-node groups per operator-apply tile
 constexpr int MAXLM = 40;  // largest supported l_max + 1
 constexpr int PT = 256;    // threads of a preconditioner workgroup
 constexpr int PE = 8;      // consecutive nodes per preconditioner thread

@@ -193,78 +193,111 @@ __device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, doub
     return t;
 }
 
-// Coupling of the 16 rows of a chunk with the modes of l1, at the 64 nodes
-// of a tile, on the matrix cores. For each column m1:
-//   W[row][node] = sum_t G[m1][t][row] E_t(node)   (v_mfma_f64_16x16x4f64, K = 4 slots)
-//   acc[row][node] += W[row][node] exp(i m1 theta) x_(l1,m1)(node)   (VALU)
-// with l* = 2 (tlo + t) + p. Lane layout (gfx950): A[row = lane%16][k = lane/16],
-// B[k = lane/16][node = lane%16], D[row = lane/16 + 4r][node = lane%16].
-template <int SG>
-__device__ __forceinline__ void mfma_block(const ApplyArgs &a, const StepParams *P, int tile, const MBlock &B, int l1,
-                                           int p, int nTe, double2 (&acc)[4][4])
+// Coupling of the 16 rows of a chunk with the modes of l1 at the NG x 16 nodes
+// of a tile, for the 4 multipole slots s0 .. s0+3 of the block, on the matrix
+// cores. For each column m1:
+//   W[row][node] = sum_t G[m1][t][row] E_t(node)   (v_mfma_f64_16x16x4f64)
+//   acc[row][node] += W[row][node] xt_(l1,m1)(node)   (VALU)
+// with l* = 2 (tlo + t) + p and xt = exp(i m1 theta) x read from HBM/L2 one
+// column ahead; the profiles E_{l*} of the tile sit in LDS. The sum over slots is
+// linear, so a block with more than 4 slots runs several passes. Lane layout
+// (gfx950): A[row = lane%16][k = lane/16], B[k = lane/16][node = lane%16],
+// D[row = lane/16 + 4r][node = lane%16].
+template <int NG>
+__device__ __forceinline__ void acc_update(double2 (&acc)[NG][4], int n, const d4 &D, double2 xv)
 {
-    const Geometry &g = a.g;
-    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
-    const int N = g.N, lmax = g.lmax;
-    double bf[SG][4];
 #pragma unroll
-    for (int gg = 0; gg < SG; ++gg) {
-        const int t = 4 * gg + q;
-        const int ls = 2 * (B.tlo + t) + p;
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            const int node = tile * gmd::TILE + 16 * n + col;
-            bf[gg][n] = (t < nTe && node < N) ? g.E[(size_t)(p * g.Ls + ls) * N + node] : 0.0;
-        }
-    }
-    int nodes[4];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) nodes[n] = min(tile * gmd::TILE + 16 * n + col, N - 1);
-    const double *__restrict__ Gb = g.GM + B.goff + col;
-    const int kb = l1 * (l1 + 1) / 2;
-    double2 xa[4];
-    double fa[SG];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) xa[n] = load_x(a, kb, nodes[n]);
-#pragma unroll
-    for (int gg = 0; gg < SG; ++gg) fa[gg] = Gb[(size_t)(4 * gg + q) * 16];
-    for (int m1i = 0; m1i <= l1; ++m1i) {
-        double2 xc[4];
-        double fc[SG];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) xc[n] = xa[n];
-#pragma unroll
-        for (int gg = 0; gg < SG; ++gg) fc[gg] = fa[gg];
-        if (m1i < l1) {
-#pragma unroll
-            for (int n = 0; n < 4; ++n) xa[n] = load_x(a, kb + m1i + 1, nodes[n]);
-#pragma unroll
-            for (int gg = 0; gg < SG; ++gg) fa[gg] = Gb[(size_t)((m1i + 1) * B.nT + 4 * gg + q) * 16];
-        }
-        const double2 ph = P->ph[lmax - l1 + 2 * m1i];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            d4 D = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int gg = 0; gg < SG; ++gg) D = __builtin_amdgcn_mfma_f64_16x16x4f64(fc[gg], bf[gg][n], D, 0, 0, 0);
-            const double2 xr = cmul(ph, xc[n]);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                acc[n][r].x = fma(D[r], xr.x, acc[n][r].x);
-                acc[n][r].y = fma(D[r], xr.y, acc[n][r].y);
-            }
-        }
+    for (int r = 0; r < 4; ++r) {
+        acc[n][r].x = fma(D[r], xv.x, acc[n][r].x);
+        acc[n][r].y = fma(D[r], xv.y, acc[n][r].y);
     }
 }
 
-// Operator apply. One wavefront: a chunk of 16 rows (modes of equal l parity)
-// at a tile of 64 radial nodes; a workgroup: one chunk on AW adjacent tiles.
-// The epilogue adds the three-point stencil, the centrifugal term and the CAP
-// (hamiltonian.cpp:44-66,224-243), forms A = 1 + i dt/2 H (matrices.cpp:27-40)
-// with the rank-one sinks (matrices.cpp:111-116), and fuses the reductions.
-template <int SGMAX>
+template <int NG>
+__device__ __forceinline__ void mfma_pass(const Geometry &g, const StepParams *P, const double2 *X, int t0,
+                                          const MBlock &B, int kb, int l1, int p, int s0, int nTe, const double *sE,
+                                          double2 (&acc)[NG][4])
+{
+    constexpr int TN = 16 * NG;
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
+    const int t = s0 + q;
+    // B fragments, constant over m1: slot t at node 16 n + col
+    double bf[NG];
+    const int off = (2 * (B.tlo + t) + p) * TN + col;
+#pragma unroll
+    for (int n = 0; n < NG; ++n) bf[n] = t < nTe ? sE[off + 16 * n] : 0.0;
+    // A fragments G[m1][t][row = col] in a ring of four columns ahead (the
+    // loop is unrolled by four so the ring needs no register moves); slots
+    // past the block read the next block (finite values times a zero B fragment)
+    const double *__restrict__ Gb = g.GM + B.goff + t * 16 + col;
+    const int mstride = B.nT * 16;
+    double fr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fr[k] = k <= l1 ? Gb[k * mstride] : 0.0;
+    // phased column vectors exp(i m1 theta) x_(l1,m1), one column ahead; nodes
+    // past the grid repeat node N-1 (they only feed accumulators never stored)
+    const int N = g.N;
+    const double2 *__restrict__ xg = X + (size_t)kb * N;
+    int xo[NG];
+#pragma unroll
+    for (int n = 0; n < NG; ++n) xo[n] = min(t0 + 16 * n + col, N - 1);
+    double2 xn[NG];
+    auto load_x = [&](int m) {
+        const double2 ph = P->ph[g.lmax - l1 + 2 * m];
+#pragma unroll
+        for (int n = 0; n < NG; ++n) xn[n] = cmul(ph, xg[(size_t)m * N + xo[n]]);
+    };
+    load_x(0);
+    const d4 z = {0.0, 0.0, 0.0, 0.0};
+    // one column: products on the matrix core, accumulation on the VALU; with
+    // pf the ring slot is refilled (unconditionally, clamped index) for m1 + 4
+    auto step = [&](const int k, const int m1i, const bool pf) {
+        const double fc = fr[k];
+        if (pf) fr[k] = Gb[min(m1i + 4, l1) * mstride];
+        double2 xv[NG];
+#pragma unroll
+        for (int n = 0; n < NG; ++n) xv[n] = xn[n];
+        load_x(min(m1i + 1, l1));
+        if constexpr (NG == 2) {
+            // the accumulation of node group 0 runs while group 1 is on the matrix core
+            const d4 D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(fc, bf[0], z, 0, 0, 0);
+            const d4 D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fc, bf[1], z, 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            acc_update<NG>(acc, 0, D0, xv[0]);
+            __builtin_amdgcn_sched_barrier(0);
+            acc_update<NG>(acc, 1, D1, xv[1]);
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            const d4 D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(fc, bf[0], z, 0, 0, 0);
+            acc_update<NG>(acc, 0, D0, xv[0]);
+        }
+    };
+    int m0 = 0;
+#pragma unroll 1
+    for (; m0 + 3 <= l1; m0 += 4) {
+        step(0, m0, true);
+        step(1, m0 + 1, true);
+        step(2, m0 + 2, true);
+        step(3, m0 + 3, true);
+    }
+    if (m0 <= l1) step(0, m0, false);
+    if (m0 + 1 <= l1) step(1, m0 + 1, false);
+    if (m0 + 2 <= l1) step(2, m0 + 2, false);
+}
+
+// Operator apply. A workgroup: one tile of TN = 16 NG radial nodes and a group
+// of AW row chunks (16 modes of equal l parity each), one chunk per wavefront;
+// the tile's profiles E_{l*} are staged in LDS. Many small workgroups per CU
+// let the dispatcher balance tiles near the bodies (many multipoles) against
+// far ones. The epilogue adds the three-point stencil, the centrifugal term
+// and the CAP (hamiltonian.cpp:44-66,224-243), forms A = 1 + i dt/2 H
+// (matrices.cpp:27-40) with the rank-one sinks (matrices.cpp:111-116), and
+// fuses the reductions.
+template <int NG>
 __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 {
+    constexpr int TN = 16 * NG;
+    __shared__ double sE[(2 * gmd::MAXLM - 1) * TN];
     const int MODE = a.mode;
     const Geometry &g = a.g;
     Krylov *Kr = a.Kr;
@@ -272,60 +305,67 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     const StepParams *P = a.P;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int b = blockIdx.x;
-    // consecutive logical blocks run on the same XCD (round-robin dispatch
-    // over the 8 XCDs) and walk the chunks of one tile group, whose psi tiles
-    // then stay in that XCD's L2
-    const int lb = (b & 7) * a.per + (b >> 3);
-    const int tgroup = lb / g.nchunks;
-    const int chn = lb - tgroup * g.nchunks;
-    const int tile = tgroup * gmd::AW + wv;
-    const bool item_ok = tgroup < (g.ntiles + gmd::AW - 1) / gmd::AW && tile < g.ntiles;
     const int N = g.N, lmax = g.lmax;
+    // block b runs on XCD b % 8 (round-robin dispatch): the chunk groups of a
+    // tile share one XCD's L2, tiles are spread over the XCDs and, within an
+    // XCD, dispatched in order of increasing radius (decreasing multipole work)
+    const int b = blockIdx.x;
+    const int ng = a.ngroups;
+    const int local = b >> 3;
+    const int grp = local % ng;
+    const int tile = (local / ng) * 8 + (b & 7);
+    const int ntiles = (N + TN - 1) / TN;
+    const bool item_ok = tile < ntiles;
+    const int chn = grp * gmd::AW + wv;
+    const bool wave_ok = item_ok && chn < g.nchunks;
+    const int t0 = tile * TN;
     const int nsink = P->nsink;
     const bool gq_mode = MODE == AM_RHS;
     const double2 coef0 = gq_mode ? Kr->gq[0] : Kr->coefA[0];
     const double2 coef1 = gq_mode ? Kr->gq[1] : Kr->coefA[1];
     double red[3] = {0.0, 0.0, 0.0};
+    double2 acc[NG][4];
+#pragma unroll
+    for (int n = 0; n < NG; ++n)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[n][r] = make_double2(0.0, 0.0);
     if (item_ok) {
-        const MChunk *C = g.mchunks + chn;
-        double2 acc[4][4];
-#pragma unroll
-        for (int n = 0; n < 4; ++n)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[n][r] = make_double2(0.0, 0.0);
         // multipoles above lcut are below 2^-64 of the tile's leading one
-        const int lc = g.lcut ? g.lcut[tile] : 2 * lmax;
-        const int par = C->par;
-        for (int l1 = 0; l1 <= lmax; ++l1) {
-            const MBlock B = g.mblocks[chn * (lmax + 1) + l1];
-            const int p = (par + l1) & 1;
-            // slots with l* = 2 (tlo + t) + p <= lc
-            const int tmax = (lc - p) >= 0 ? (lc - p) / 2 : -1;
-            const int nTe = max(0, min(B.nT, tmax - B.tlo + 1));
-            const int SG = (nTe + 3) / 4;
-            switch (SG) {
-#define GM_CASE(n) \
-    case n: \
-        if constexpr (n <= SGMAX) mfma_block<n>(a, P, tile, B, l1, p, nTe, acc); \
-        break;
-                GM_CASE(1) GM_CASE(2) GM_CASE(3) GM_CASE(4) GM_CASE(5) GM_CASE(6) GM_CASE(7) GM_CASE(8)
-                GM_CASE(9) GM_CASE(10)
-#undef GM_CASE
-            default: break;
+        const int lc = g.lcut ? g.lcut[t0 / gmd::TILE] : 2 * lmax;
+        for (int e = threadIdx.x; e < (lc + 1) * TN; e += 64 * gmd::AW) {
+            const int ls = e / TN, node = t0 + (e & (TN - 1));
+            sE[e] = node < N ? g.E[(size_t)((ls & 1) * g.Ls + ls) * N + node] : 0.0;
+        }
+        __syncthreads();
+        if (wave_ok) {
+            const int par = g.mchunks[chn].par;
+            const MBlock *Bc = g.mblocks + (size_t)chn * (lmax + 1);
+            for (int l1 = 0; l1 <= lmax; ++l1) {
+                const MBlock B = Bc[l1];
+                const int p = (par + l1) & 1;
+                // slots with l* = 2 (tlo + t) + p <= lc
+                const int tmax = (lc - p) >= 0 ? (lc - p) / 2 : -1;
+                const int nTe = max(0, min(B.nT, tmax - B.tlo + 1));
+                const int kb = l1 * (l1 + 1) / 2;
+                for (int s0 = 0; s0 < nTe; s0 += 4) mfma_pass<NG>(g, P, a.X, t0, B, kb, l1, p, s0, nTe, sE, acc);
             }
         }
+    }
+    if (wave_ok) {
+        const MChunk *C = g.mchunks + chn;
         // epilogue: lane owns rows q + 4r and nodes 16 n + col
         const double alpha2 = 1.0 / (g.dr * g.dr);
         const double hdt = P->half_dt;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+            // one row at a time: bounds the loads the compiler keeps in flight
+            asm volatile("" ::: "memory");
             const int rl = q + 4 * r;
             const int j = C->row_j[rl];
             const int l = C->row_l[rl], m = C->row_m[rl];
 #pragma unroll
-            for (int n = 0; n < 4; ++n) {
-                const int i = tile * gmd::TILE + 16 * n + col;
+            for (int n = 0; n < NG; ++n) {
+                const int i = t0 + 16 * n + col;
                 if (j < 0 || i >= N) continue;
                 const size_t row = (size_t)j * N + i;
                 const double2 Cc = cdotm(P->ph[m + lmax], acc[n][r]);
@@ -1197,31 +1237,20 @@ __global__ __launch_bounds__(64) void k_diag_final(DiagArgs a)
 
 // ---------------------------------------------------------------- launchers
 
-template <int SGMAX>
-static hipError_t launch_apply_s(const ApplyArgs &a, int grid, hipStream_t st)
+int apply_grid(int N, int nchunks, int *ngroups)
 {
-    hipLaunchKernelGGL((k_apply<SGMAX>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
-    return hipGetLastError();
-}
-
-int apply_grid(int ntiles, int nchunks, int *per)
-{
-    const int nb = (ntiles + gmd::AW - 1) / gmd::AW * nchunks;
-    *per = (nb + 7) / 8;
-    return *per * 8;
+    constexpr int tn = 16 * gmd::APPLY_NG;
+    const int ntiles = (N + tn - 1) / tn;
+    *ngroups = (nchunks + gmd::AW - 1) / gmd::AW;
+    return (ntiles + 7) / 8 * 8 * *ngroups;
 }
 
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
-    const int grid = apply_grid(a.g.ntiles, a.g.nchunks, &a.per);
+    const int grid = apply_grid(a.g.N, a.g.nchunks, &a.ngroups);
     a.mode = mode;
-    // slot groups of 4 multipoles: at most ceil((l_max + 1) / 4)
-    const int sg = (a.g.lmax + 1 + 3) / 4;
-    if (sg <= 2) return launch_apply_s<2>(a, grid, st);
-    if (sg <= 4) return launch_apply_s<4>(a, grid, st);
-    if (sg <= 6) return launch_apply_s<6>(a, grid, st);
-    if (sg <= 8) return launch_apply_s<8>(a, grid, st);
-    return launch_apply_s<10>(a, grid, st);
+    hipLaunchKernelGGL(k_apply<gmd::APPLY_NG>, dim3(grid), dim3(64 * gmd::AW), 0, st, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st)

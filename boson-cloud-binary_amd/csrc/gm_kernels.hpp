@@ -24,8 +24,7 @@ struct ApplyArgs {
     double2 *Cout;       // AM_RES: coupling V x written per row (or nullptr)
     const double2 *Cin;  // diagnostics apply: V x of the acceptance apply; V psi = |rhs| mask V x
     const double *mask;  // diagnostics apply: damping mask, or nullptr
-    int nitems;          // node tiles x row chunks
-    int per;             // set by the launcher
+    int ngroups;         // chunk groups per tile, set by the launcher
     int mode;            // ApplyMode, set by the launcher
 };
 
@@ -113,7 +112,7 @@ struct DiagArgs {
     double *out;         // flux, energy_flux, norms, energies, then E, p_bin, dV, mass shares (K each)
 };
 
-int apply_grid(int ntiles, int nchunks, int *per);
+int apply_grid(int N, int nchunks, int *ngroups);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
                           double mu, double prf, double *E, hipStream_t st);

@@ -567,7 +567,6 @@ gmk::ApplyArgs apply_args(gm_solver &S, const double2 *X, double2 *Y)
     a.partial = S.partial.p;
     a.rowpart = S.rowpart.p;
     a.dpsi_ix = S.dpsi_ix.p;
-    a.nitems = S.ntiles * S.nchunks;
     return a;
 }
 
@@ -798,7 +797,7 @@ void alloc_device(gm_solver &S)
     S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.lcut.alloc(S.ntiles);
     int per = 0;
-    const int grid = gmk::apply_grid(S.ntiles, S.nchunks, &per);
+    const int grid = gmk::apply_grid(S.Nr, S.nchunks, &per);
     S.partial.alloc(static_cast<size_t>(3) * std::max(grid, gmd::RED_BLOCKS));
     S.rowpart.alloc(static_cast<size_t>(S.ntiles) * S.K * 6);
     S.dpsi_ix.alloc(S.K);
