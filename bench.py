@@ -1,18 +1,21 @@
 """Benchmark of the MI355X Crank-Nicolson step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--accuracy-steps A]
 
-Workload (one GPU): the plain FP64 drop-in at scale -- binary q=0.5, R=50 on a
-fixed orbit with both s-wave absorbers on, boson mass mu=0.15, N=16384 radial
-nodes over L=4000, l_max=20 (231 modes), dt=0.25, damping mask (0.85, 1, 6),
-r_ex=0.8 L; initial state the 2p atom on the primary translated to the centre
-of mass plus a seeded 1e-3 complex Gaussian perturbation (synthetic inputs).
+Workload (one GPU): binary q=0.5, R=50 on a fixed orbit with both s-wave
+absorbers on, boson mass mu=0.15, N=16384 radial nodes over L=4000, l_max=20
+(231 modes), dt=0.25, damping mask (0.85, 1, 6), r_ex=0.8 L; initial state the
+2p atom on the primary translated to the centre of mass plus a seeded 1e-3
+complex Gaussian perturbation (synthetic inputs).
 
 A step is one CrankNicolsonModes::step: the CN solve with its BiCGSTAB, the
-mask and every diagnostic. The metric is mode.node-steps/s = K*N*steps/time.
-The JSON line also carries the dominant kernel's roofline (HIP-event kernel
-times on the solver's stream) and a CPU baseline timed on the host: the FP64
-oracle on a bounded sample of the same workload.
+mask and every diagnostic. The metric is mode.node-steps/s = K*N*steps/time,
+reported for the mixed-precision step (FP32 coupling in the Krylov applies,
+everything else FP64) -- the JSON value -- and for the plain FP64 step, with
+the relative L2 difference of the two states after A untimed steps from the
+same initial state. The line also carries the dominant kernels' rooflines
+(HIP-event kernel times on the solver's stream) and a CPU baseline timed on
+the host: the FP64 oracle on a bounded sample of the same workload.
 """
 import argparse
 import ctypes as C
@@ -32,7 +35,8 @@ from gm import initial  # noqa: E402
 
 METRIC = "CN mode·node-steps/s, mixed vs FP64 (≤1e-8 rel err); p99 served step latency"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6   # dense FP64 matrix (and vector) peak of one MI355X
+FP32_PEAK_TFLOPS = 157.3  # dense FP32 matrix peak
 
 WORKLOAD = dict(Nr=16384, L=4000.0, dt=0.25, epsilon=0.0, l_max=20, mu=0.15, M_c=0.01, q=0.5, a=0.0, R=50.0,
                 counter=False, absorption=True, save_every=50, ramp_periods=0.0, epsilon_smear=2.0)
@@ -170,36 +174,63 @@ def coupling_flops(terms, pairs, K, N):
     return 2.0 * terms + 4.0 * pairs + 20.0 * K * N
 
 
-def rooflines(s, prof, K, N, iters):
-    """The dominant kernel's roofline (the coupling apply k_apply, every working
-    launch of which does the same coupling work) and the achieved rates of the
-    other kernel classes, from HIP-event times on the solver's stream.
-    BiCGSTAB iterations are enqueued in batches, so a few Krylov launches
-    after convergence return at once (about 5 us); the averages count the
-    working launches: two applies per iteration plus the rhs and acceptance
-    applies of every step."""
+def rooflines(s, prof, K, N, iters, mixed):
+    """The dominant kernel's roofline and the achieved rates of the other
+    kernel classes, from HIP-event times on the solver's stream.
+
+    Plain FP64: the coupling apply k_apply, whose working launches (two per
+    BiCGSTAB iteration plus the rhs and acceptance applies of every step) all
+    do the same FP64 coupling work. Mixed: the FP32 Krylov applies (two per
+    iteration) against the FP32 matrix peak. BiCGSTAB iterations are enqueued
+    in batches, so a few Krylov launches after convergence return at once
+    (about 5 us); they are not counted."""
     terms, pairs = s.coupling_work()
-    ms = prof["apply_krylov"][0] + prof["apply_other"][0]
-    cnt = min(prof["apply_krylov"][1], 2 * int(sum(iters))) + prof["apply_other"][1]
+    krylov = min(prof["apply_krylov"][1], 2 * int(sum(iters)))
+    if mixed:
+        ms, cnt = prof["apply_krylov"][0], krylov
+        peak, name, dt = FP32_PEAK_TFLOPS, "k_apply<fp32> (Krylov applies)", "f32"
+    else:
+        ms = prof["apply_krylov"][0] + prof["apply_other"][0]
+        cnt = krylov + prof["apply_other"][1]
+        peak, name, dt = FP64_PEAK_TFLOPS, "k_apply", "f64"
     avg_s = ms / max(cnt, 1) * 1e-3
     flops = coupling_flops(terms, pairs, K, N)
-    roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "kernel": "k_apply", "per_launch": flops, "per_launch_unit": "flop",
+    roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": peak, "unit": "TFLOP/s",
+            "kernel": name, "dtype": dt, "per_launch": flops, "per_launch_unit": "flop",
             "avg_launch_ms": avg_s * 1e3, "launches": cnt}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = measured_traffic("k_apply")
+    roof["traffic"] = measured_traffic("k_apply") if not mixed else None
     kernels = {}
     working = {"precond": min(prof["precond"][1], 2 * int(sum(iters))),
                "update": min(prof["update"][1], int(sum(iters)))}
-    for name, passes in HBM_PASSES.items():
-        kms, kn = prof[name][0], working.get(name, prof[name][1])
+    for kname, passes in HBM_PASSES.items():
+        kms, kn = prof[kname][0], working.get(kname, prof[kname][1])
         if kn == 0:
             continue
         byts = passes * 16.0 * K * N
         a = byts / (kms / kn * 1e-3) / 1e9
-        kernels[name] = {"bound": "hbm", "avg_launch_ms": kms / kn, "bytes_per_launch": byts,
-                         "achieved_GBs": a, "frac": a / HBM_PEAK_GBS}
+        kernels[kname] = {"bound": "hbm", "avg_launch_ms": kms / kn, "bytes_per_launch": byts,
+                          "achieved_GBs": a, "frac": a / HBM_PEAK_GBS}
     return roof, kernels
+
+
+def accuracy(w, steps):
+    """Relative L2 difference between the mixed-precision and the FP64 states
+    after `steps` steps from the same initial state (and the same for the
+    cloud mass and energy)."""
+    runs = []
+    for precision in (gm.GM_PRECISION_FP64, gm.GM_PRECISION_MIXED):
+        s = make_solver(w, precision)
+        d = None
+        for k in range(1, steps + 1):
+            d = s.step(k, diag=(k == steps))
+        runs.append((s.psi, d))
+        del s
+    (p64, d64), (pmx, dmx) = runs
+    return {"steps": steps, "psi_rel_l2": float(np.linalg.norm(pmx - p64) / np.linalg.norm(p64)),
+            "cloud_mass_rel": abs(dmx["cloud_mass"] - d64["cloud_mass"]) / abs(d64["cloud_mass"]),
+            "energy_rel": abs(dmx["energy"] - d64["energy"]) / abs(d64["energy"]),
+            "reference": "plain FP64 GPU step (itself within 1e-10 of the FP64 oracle per step)"}
 
 
 def measured_traffic(kernel):
@@ -234,7 +265,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="untimed steps of the mixed-vs-FP64 comparison (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
@@ -247,28 +279,36 @@ def main():
               "has no multi-GPU path (ensembles would shard whole members per GPU).", file=sys.stderr)
         sys.exit(2)
     w = dict(WORKLOAD)
-    precision = gm.GM_PRECISION_MIXED if args.precision == "mixed" else gm.GM_PRECISION_FP64
-    s, elapsed, prof, iters = run_gpu(w, precision, args.steps, args.warmup)
-    K, N = s.n_modes, s.Nr
-    value = K * N * args.steps / elapsed
-    roof, kernels = rooflines(s, prof, K, N, iters)
-    terms, pairs = s.coupling_work()
+    runs = {}
+    for name, precision in (("fp64", gm.GM_PRECISION_FP64), ("mixed", gm.GM_PRECISION_MIXED)):
+        s, elapsed, prof, iters = run_gpu(w, precision, args.steps, args.warmup)
+        K, N = s.n_modes, s.Nr
+        roof, kernels = rooflines(s, prof, K, N, iters, name == "mixed")
+        runs[name] = {"value": K * N * args.steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
+                      "iterations_per_step": float(np.mean(iters)) if iters else None, "roofline": roof,
+                      "kernels": kernels, "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
+                      "kernel_launches": {k: int(v[1]) for k, v in prof.items()}}
+        terms, pairs = s.coupling_work()
+        del s
+    mx, f64 = runs["mixed"], runs["fp64"]
     out = {
-        "metric": METRIC, "value": value, "unit": "mode·node-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if precision == gm.GM_PRECISION_FP64 else "f64+f32",
-        "data": "synthetic",
-        "config": {"workload": "plain FP64 CN step, l_max=20 (K=231), N=16384, L=4000, q=0.5, R=50, mu=0.15, "
-                               "dt=0.25, fixed orbit, absorbers on, mask", "K": K, "N": N,
-                   "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": roof,
-        "kernels": kernels,
-        "iterations_per_step": float(np.mean(iters)) if iters else None,
-        "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
-        "kernel_launches": {k: int(v[1]) for k, v in prof.items()},
+        "metric": METRIC, "value": mx["value"], "unit": "mode·node-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": mx["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "CN step, l_max=20 (K=231), N=16384, L=4000, q=0.5, R=50, mu=0.15, dt=0.25, "
+                               "fixed orbit, absorbers on, mask; value = mixed precision (FP32 coupling in the "
+                               "Krylov applies), fp64 = plain FP64 drop-in", "K": K, "N": N,
+                   "parallelism": "single GPU"},
+        "roofline": mx["roofline"],
+        "fp64": f64,
+        "mixed": {k: v for k, v in mx.items() if k != "roofline"},
+        "mixed_over_fp64": mx["value"] / f64["value"],
+        "p99_served_step_latency_ms": None,
         "coupling_work": {"terms_per_apply": terms, "pairs_per_apply": pairs,
                           "untruncated_terms_per_apply": float(coupling_terms(w["l_max"])) * (N - 2)},
     }
+    if args.accuracy_steps > 0:
+        out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w)

@@ -49,6 +49,8 @@ struct Geometry {
     const MBlock *mblocks;     // nchunks * (l_max + 1)
     const double *GM;          // Gaunt coefficients in MFMA block layout
     const double *E;           // 2 * Ls * N
+    const float *GM32;         // FP32 copies for the mixed-precision Krylov applies
+    const float *E32;
     const int *lcut;           // per node tile: largest multipole l* evaluated
     const double2 *cap;        // Nr diagonal CAP or nullptr
 };

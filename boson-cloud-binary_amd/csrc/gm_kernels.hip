@@ -21,6 +21,7 @@
 #include "gm_kernels.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace gmk {
 
@@ -124,7 +125,7 @@ __device__ double sum_partials(const double *partial, int q)
 // ---------------------------------------------------------------- E table
 
 __global__ void k_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
-                          double mu, double prf, double *E)
+                          double mu, double prf, double *E, float *E32)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= Ls * N) return;
@@ -143,6 +144,8 @@ __global__ void k_build_E(int N, int Ls, double dr, double eps, double R1, doubl
     }
     E[(size_t)ls * N + i] = e0;
     E[(size_t)(Ls + ls) * N + i] = e1;
+    E32[(size_t)ls * N + i] = (float)e0;
+    E32[(size_t)(Ls + ls) * N + i] = (float)e1;
 }
 
 // Largest multipole l* of each node tile whose profile exceeds tau times the
@@ -203,8 +206,38 @@ __device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, doub
 // linear, so a block with more than 4 slots runs several passes. Lane layout
 // (gfx950): A[row = lane%16][k = lane/16], B[k = lane/16][node = lane%16],
 // D[row = lane/16 + 4r][node = lane%16].
-template <int NG>
-__device__ __forceinline__ void acc_update(double2 (&acc)[NG][4], int n, const d4 &D, double2 xv)
+// Arithmetic of the coupling: FP64 (matrix-core products and accumulation in
+// double) or FP32 (products on v_mfma_f32_16x16x4f32 and accumulation in
+// float, for the mixed-precision Krylov applies).
+typedef float f4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Arith;
+template <> struct Arith<double> {
+    typedef double2 C;
+    typedef d4 V4;
+    static __device__ __forceinline__ V4 mfma(double a, double b, const V4 &c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ C from(double2 v) { return v; }
+    static __device__ __forceinline__ double2 to_d(C v) { return v; }
+    // D row held in accumulator register r by lane group q (measured layout)
+    static __device__ __forceinline__ int row(int q, int r) { return q + 4 * r; }
+};
+template <> struct Arith<float> {
+    typedef float2 C;
+    typedef f4 V4;
+    static __device__ __forceinline__ V4 mfma(float a, float b, const V4 &c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ C from(double2 v) { return make_float2((float)v.x, (float)v.y); }
+    static __device__ __forceinline__ double2 to_d(C v) { return make_double2(v.x, v.y); }
+    static __device__ __forceinline__ int row(int q, int r) { return 4 * q + r; }
+};
+
+template <int NG, typename T>
+__device__ __forceinline__ void acc_update(typename Arith<T>::C (&acc)[NG][4], int n, const typename Arith<T>::V4 &D,
+                                           typename Arith<T>::C xv)
 {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -213,27 +246,29 @@ __device__ __forceinline__ void acc_update(double2 (&acc)[NG][4], int n, const d
     }
 }
 
-template <int NG>
-__device__ __forceinline__ void mfma_pass(const Geometry &g, const StepParams *P, const double2 *X, int t0,
-                                          const MBlock &B, int kb, int l1, int p, int s0, int nTe, const double *sE,
-                                          double2 (&acc)[NG][4])
+template <int NG, typename T>
+__device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const StepParams *P, const double2 *X, int t0,
+                                          const MBlock &B, int kb, int l1, int p, int s0, int nTe, const T *sE,
+                                          typename Arith<T>::C (&acc)[NG][4])
 {
+    typedef Arith<T> AR;
+    typedef typename AR::C C;
     constexpr int TN = 16 * NG;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int t = s0 + q;
     // B fragments, constant over m1: slot t at node 16 n + col
-    double bf[NG];
+    T bf[NG];
     const int off = (2 * (B.tlo + t) + p) * TN + col;
 #pragma unroll
-    for (int n = 0; n < NG; ++n) bf[n] = t < nTe ? sE[off + 16 * n] : 0.0;
+    for (int n = 0; n < NG; ++n) bf[n] = t < nTe ? sE[off + 16 * n] : T(0);
     // A fragments G[m1][t][row = col] in a ring of four columns ahead (the
     // loop is unrolled by four so the ring needs no register moves); slots
     // past the block read the next block (finite values times a zero B fragment)
-    const double *__restrict__ Gb = g.GM + B.goff + t * 16 + col;
+    const T *__restrict__ Gb = GM + B.goff + t * 16 + col;
     const int mstride = B.nT * 16;
-    double fr[4];
+    T fr[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) fr[k] = k <= l1 ? Gb[k * mstride] : 0.0;
+    for (int k = 0; k < 4; ++k) fr[k] = k <= l1 ? Gb[k * mstride] : T(0);
     // phased column vectors exp(i m1 theta) x_(l1,m1), one column ahead; nodes
     // past the grid repeat node N-1 (they only feed accumulators never stored)
     const int N = g.N;
@@ -241,35 +276,35 @@ __device__ __forceinline__ void mfma_pass(const Geometry &g, const StepParams *P
     int xo[NG];
 #pragma unroll
     for (int n = 0; n < NG; ++n) xo[n] = min(t0 + 16 * n + col, N - 1);
-    double2 xn[NG];
+    C xn[NG];
     auto load_x = [&](int m) {
         const double2 ph = P->ph[g.lmax - l1 + 2 * m];
 #pragma unroll
-        for (int n = 0; n < NG; ++n) xn[n] = cmul(ph, xg[(size_t)m * N + xo[n]]);
+        for (int n = 0; n < NG; ++n) xn[n] = AR::from(cmul(ph, xg[(size_t)m * N + xo[n]]));
     };
     load_x(0);
-    const d4 z = {0.0, 0.0, 0.0, 0.0};
+    const typename AR::V4 z = {T(0), T(0), T(0), T(0)};
     // one column: products on the matrix core, accumulation on the VALU; with
     // pf the ring slot is refilled (unconditionally, clamped index) for m1 + 4
     auto step = [&](const int k, const int m1i, const bool pf) {
-        const double fc = fr[k];
+        const T fc = fr[k];
         if (pf) fr[k] = Gb[min(m1i + 4, l1) * mstride];
-        double2 xv[NG];
+        C xv[NG];
 #pragma unroll
         for (int n = 0; n < NG; ++n) xv[n] = xn[n];
         load_x(min(m1i + 1, l1));
         if constexpr (NG == 2) {
             // the accumulation of node group 0 runs while group 1 is on the matrix core
-            const d4 D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(fc, bf[0], z, 0, 0, 0);
-            const d4 D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(fc, bf[1], z, 0, 0, 0);
+            const typename AR::V4 D0 = AR::mfma(fc, bf[0], z);
+            const typename AR::V4 D1 = AR::mfma(fc, bf[1], z);
             __builtin_amdgcn_sched_barrier(0);
-            acc_update<NG>(acc, 0, D0, xv[0]);
+            acc_update<NG, T>(acc, 0, D0, xv[0]);
             __builtin_amdgcn_sched_barrier(0);
-            acc_update<NG>(acc, 1, D1, xv[1]);
+            acc_update<NG, T>(acc, 1, D1, xv[1]);
             __builtin_amdgcn_sched_barrier(0);
         } else {
-            const d4 D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(fc, bf[0], z, 0, 0, 0);
-            acc_update<NG>(acc, 0, D0, xv[0]);
+            const typename AR::V4 D0 = AR::mfma(fc, bf[0], z);
+            acc_update<NG, T>(acc, 0, D0, xv[0]);
         }
     };
     int m0 = 0;
@@ -293,11 +328,12 @@ __device__ __forceinline__ void mfma_pass(const Geometry &g, const StepParams *P
 // and the CAP (hamiltonian.cpp:44-66,224-243), forms A = 1 + i dt/2 H
 // (matrices.cpp:27-40) with the rank-one sinks (matrices.cpp:111-116), and
 // fuses the reductions.
-template <int NG>
+template <int NG, typename T>
 __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 {
+    typedef Arith<T> AR;
     constexpr int TN = 16 * NG;
-    __shared__ double sE[(2 * gmd::MAXLM - 1) * TN];
+    __shared__ T sE[(2 * gmd::MAXLM - 1) * TN];
     const int MODE = a.mode;
     const Geometry &g = a.g;
     Krylov *Kr = a.Kr;
@@ -324,17 +360,19 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     const double2 coef0 = gq_mode ? Kr->gq[0] : Kr->coefA[0];
     const double2 coef1 = gq_mode ? Kr->gq[1] : Kr->coefA[1];
     double red[3] = {0.0, 0.0, 0.0};
-    double2 acc[NG][4];
+    typename AR::C acc[NG][4];
 #pragma unroll
     for (int n = 0; n < NG; ++n)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[n][r] = make_double2(0.0, 0.0);
+        for (int r = 0; r < 4; ++r) acc[n][r] = AR::from(make_double2(0.0, 0.0));
+    const T *E = std::is_same<T, double>::value ? (const T *)g.E : (const T *)g.E32;
+    const T *GM = std::is_same<T, double>::value ? (const T *)g.GM : (const T *)g.GM32;
     if (item_ok) {
         // multipoles above lcut are below 2^-64 of the tile's leading one
         const int lc = g.lcut ? g.lcut[t0 / gmd::TILE] : 2 * lmax;
         for (int e = threadIdx.x; e < (lc + 1) * TN; e += 64 * gmd::AW) {
             const int ls = e / TN, node = t0 + (e & (TN - 1));
-            sE[e] = node < N ? g.E[(size_t)((ls & 1) * g.Ls + ls) * N + node] : 0.0;
+            sE[e] = node < N ? E[(size_t)((ls & 1) * g.Ls + ls) * N + node] : T(0);
         }
         __syncthreads();
         if (wave_ok) {
@@ -347,7 +385,8 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                 const int tmax = (lc - p) >= 0 ? (lc - p) / 2 : -1;
                 const int nTe = max(0, min(B.nT, tmax - B.tlo + 1));
                 const int kb = l1 * (l1 + 1) / 2;
-                for (int s0 = 0; s0 < nTe; s0 += 4) mfma_pass<NG>(g, P, a.X, t0, B, kb, l1, p, s0, nTe, sE, acc);
+                for (int s0 = 0; s0 < nTe; s0 += 4)
+                    mfma_pass<NG, T>(g, GM, P, a.X, t0, B, kb, l1, p, s0, nTe, sE, acc);
             }
         }
     }
@@ -360,7 +399,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         for (int r = 0; r < 4; ++r) {
             // one row at a time: bounds the loads the compiler keeps in flight
             asm volatile("" ::: "memory");
-            const int rl = q + 4 * r;
+            const int rl = AR::row(q, r);
             const int j = C->row_j[rl];
             const int l = C->row_l[rl], m = C->row_m[rl];
 #pragma unroll
@@ -368,7 +407,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                 const int i = t0 + 16 * n + col;
                 if (j < 0 || i >= N) continue;
                 const size_t row = (size_t)j * N + i;
-                const double2 Cc = cdotm(P->ph[m + lmax], acc[n][r]);
+                const double2 Cc = cdotm(P->ph[m + lmax], AR::to_d(acc[n][r]));
                 if (a.Cout) a.Cout[row] = Cc;
                 const double2 xi = load_x(a, j, i);
                 const double2 capv = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
@@ -1249,7 +1288,10 @@ hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
     const int grid = apply_grid(a.g.N, a.g.nchunks, &a.ngroups);
     a.mode = mode;
-    hipLaunchKernelGGL(k_apply<gmd::APPLY_NG>, dim3(grid), dim3(64 * gmd::AW), 0, st, a);
+    if (a.mixed)
+        hipLaunchKernelGGL((k_apply<gmd::APPLY_NG, float>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_apply<gmd::APPLY_NG, double>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1261,10 +1303,11 @@ hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st)
 }
 
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt, double mu,
-                          double prf, double *E, hipStream_t st)
+                          double prf, double *E, float *E32, hipStream_t st)
 {
     const int n = Ls * N;
-    hipLaunchKernelGGL(k_build_E, dim3((n + 255) / 256), dim3(256), 0, st, N, Ls, dr, eps, R1, R2, q, Mt, mu, prf, E);
+    hipLaunchKernelGGL(k_build_E, dim3((n + 255) / 256), dim3(256), 0, st, N, Ls, dr, eps, R1, R2, q, Mt, mu, prf, E,
+                       E32);
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ struct ApplyArgs {
     const double2 *Cin;  // diagnostics apply: V x of the acceptance apply; V psi = |rhs| mask V x
     const double *mask;  // diagnostics apply: damping mask, or nullptr
     int ngroups;         // chunk groups per tile, set by the launcher
+    int mixed;           // coupling in FP32 (E32, GM32) with FP32 accumulation
     int mode;            // ApplyMode, set by the launcher
 };
 
@@ -115,7 +116,7 @@ struct DiagArgs {
 int apply_grid(int N, int nchunks, int *ngroups);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
-                          double mu, double prf, double *E, hipStream_t st);
+                          double mu, double prf, double *E, float *E32, hipStream_t st);
 hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hipStream_t st);
 hipError_t launch_factor(const FactorArgs &f, hipStream_t st);
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st);

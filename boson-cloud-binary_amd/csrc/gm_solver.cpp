@@ -132,6 +132,7 @@ struct gm_solver {
     DevBuf<gmd::MChunk> mchunks;
     DevBuf<gmd::MBlock> mblocks;
     DevBuf<double> GM, Gdiag, E, dmask;
+    DevBuf<float> GM32, E32;  // FP32 copies for the mixed-precision Krylov applies
     DevBuf<int> lcut;
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
@@ -366,6 +367,8 @@ gmd::Geometry geometry(const gm_solver &S)
     g.mchunks = S.mchunks.p;
     g.mblocks = S.mblocks.p;
     g.GM = S.GM.p;
+    g.GM32 = S.GM32.p;
+    g.E32 = S.E32.p;
     g.E = S.E.p;
     g.lcut = S.lcut.p;
     g.cap = S.cap ? S.dcap.p : nullptr;
@@ -510,7 +513,7 @@ void build_perturbation(gm_solver &S, double time)
     if (!S.E_valid || std::memcmp(key, S.E_key, sizeof(key)) != 0) {
         launch(S, GM_PROF_FACTOR, [&] {
             return gmk::launch_build_E(S.Nr, 2 * S.l_max + 1, S.dr, S.epsilon, S.R1, S.R2, S.q, S.M1 + S.M2, S.mu,
-                                       S.prf, S.E.p, S.st);
+                                       S.prf, S.E.p, S.E32.p, S.st);
         });
         launch(S, GM_PROF_FACTOR, [&] {
             return gmk::launch_lcut(S.Nr, 2 * S.l_max + 1, S.coupling_tol, S.E.p, S.lcut.p, S.st);
@@ -685,8 +688,12 @@ void enqueue_iteration(gm_solver &S, bool wood)
     pa.Y = S.y.p;
     launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_P, S.st); });
     if (wood) woodbury_correct(S, S.y.p);
+    // mixed precision: the coupling of the two Krylov applies runs in FP32;
+    // rhs, residual check and diagnostics keep the FP64 operator
+    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
     gmk::ApplyArgs a = apply_args(S, S.y.p, S.v.p);
     a.R0 = S.r0.p;
+    a.mixed = mixed;
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_V, S.st); });
     // z = M^-1 s with s = r - alpha v
     pa.Pv = S.s.p;
@@ -695,6 +702,7 @@ void enqueue_iteration(gm_solver &S, bool wood)
     if (wood) woodbury_correct(S, S.z.p);
     a = apply_args(S, S.z.p, S.t.p);
     a.R0 = S.s.p;
+    a.mixed = mixed;
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_T, S.st); });
     gmk::UpdateArgs u{};
     u.n = S.nvec();
@@ -795,6 +803,7 @@ void alloc_device(gm_solver &S)
     for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.ax, &S.inv, &S.tmp})
         if (bf->n != n) bf->alloc(n);
     S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
+    S.E32.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.lcut.alloc(S.ntiles);
     int per = 0;
     const int grid = gmk::apply_grid(S.Nr, S.nchunks, &per);
@@ -806,11 +815,16 @@ void alloc_device(gm_solver &S)
     S.mchunks.alloc(S.mchunks_h.size());
     S.mblocks.alloc(S.mblocks_h.size());
     S.GM.alloc(S.GM_h.size());
+    S.GM32.alloc(S.GM_h.size());
     S.Gdiag.alloc(S.Gdiag_h.size());
     // tables are uploaded from pageable vectors: synchronous copies
     HIPCK(hipMemcpy(S.mchunks.p, S.mchunks_h.data(), S.mchunks_h.size() * sizeof(gmd::MChunk), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(S.mblocks.p, S.mblocks_h.data(), S.mblocks_h.size() * sizeof(gmd::MBlock), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(S.GM.p, S.GM_h.data(), S.GM_h.size() * sizeof(double), hipMemcpyHostToDevice));
+    {
+        const std::vector<float> gm32(S.GM_h.begin(), S.GM_h.end());
+        HIPCK(hipMemcpy(S.GM32.p, gm32.data(), gm32.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     HIPCK(hipMemcpy(S.Gdiag.p, S.Gdiag_h.data(), S.Gdiag_h.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCK(hipMemsetAsync(S.dK.p, 0, sizeof(gmd::Krylov), S.st));
     HIPCK(hipMemsetAsync(S.dflags.p, 0, 2 * sizeof(int), S.st));

@@ -215,3 +215,31 @@ def test_multipole_truncation_is_below_rounding():
     assert rel(cut.psi, full.psi) < 1e-14
     assert rel(cut.psi, o.psi) < 1e-10
     assert rel(full.psi, o.psi) < 1e-10
+
+
+def test_mixed_precision_tracks_the_oracle():
+    """FP32 coupling in the Krylov applies (rhs, residual check and every
+    diagnostic stay FP64): after 40 steps psi is within 1e-8 of the FP64
+    oracle, the FP64 path within 1e-10, and iteration counts agree to +-1."""
+    announce("mixed precision 40 steps")
+    args = dict(Nr=513, L=256.0, dt=0.5, epsilon=0.0, l_max=6, mu=0.2, M_c=0.01, q=0.1, a=0.0, R=30.0,
+                counter=False, absorption=True, save_every=1000, ramp_periods=0.0, epsilon_smear=2.0)
+    f64, mix = gm.CrankNicolsonModes(**args), gm.CrankNicolsonModes(**args)
+    o = oracle.Oracle(**args)
+    psi = generic_state(o)
+    for s, prec in ((f64, gm.GM_PRECISION_FP64), (mix, gm.GM_PRECISION_MIXED), (o, None)):
+        if prec is None:
+            s.set_flags(evolve_orbit=0)
+        else:
+            s.set_flags(evolve_orbit=0, precision=prec)
+        s.set_initial_state_full(psi)
+        s.initialize()
+        s.build_damping_mask(0.85, 1.0, 6)
+        s.begin_run(200.0, 41)
+    for n in range(1, 41):
+        df, dm, do = f64.step(n), mix.step(n), o.step(n)
+        assert abs(mix.last_solver_iterations - f64.last_solver_iterations) <= 1
+    assert rel(f64.psi, o.psi) < 1e-10
+    assert rel(mix.psi, o.psi) < 1e-8
+    for k in ("energy", "cloud_mass", "p_bin_inst"):
+        assert dm[k] == pytest.approx(do[k], rel=1e-7, abs=1e-14), k
