@@ -20,8 +20,8 @@ constexpr int TILE = 64;   // radial nodes of a multipole-cut / diagnostics tile
 constexpr int AW = 4;      // row chunks (wavefronts) per operator-apply workgroup
 constexpr int APPLY_NG = 2;  // 16-node groups per operator-apply tile
 constexpr int MAXLM = 40;  // largest supported l_max + 1
-constexpr int PT = 256;    // threads of a preconditioner workgroup
-constexpr int PE = 8;      // consecutive nodes per preconditioner thread
+constexpr int PT = 512;    // threads of a preconditioner or factor workgroup
+constexpr int PE = 4;      // consecutive nodes per preconditioner thread
 constexpr int RED_BLOCKS = 1024;  // max partial-sum slots per reduction
 
 constexpr int MR = 16;     // rows of an MFMA chunk

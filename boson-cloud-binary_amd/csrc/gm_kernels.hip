@@ -201,8 +201,8 @@ __device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, doub
 // cores. For each column m1:
 //   W[row][node] = sum_t G[m1][t][row] E_t(node)   (v_mfma_f64_16x16x4f64)
 //   acc[row][node] += W[row][node] xt_(l1,m1)(node)   (VALU)
-// with l* = 2 (tlo + t) + p and xt = exp(i m1 theta) x read from HBM/L2 one
-// column ahead; the profiles E_{l*} of the tile sit in LDS. The sum over slots is
+// with l* = 2 (tlo + t) + p and xt = exp(i m1 theta) x read from HBM/L2 two
+// columns ahead; the profiles E_{l*} of the tile sit in LDS. The sum over slots is
 // linear, so a block with more than 4 slots runs several passes. Lane layout
 // (gfx950): A[row = lane%16][k = lane/16], B[k = lane/16][node = lane%16],
 // D[row = lane/16 + 4r][node = lane%16].
@@ -269,20 +269,19 @@ __device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const 
     T fr[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) fr[k] = k <= l1 ? Gb[k * mstride] : T(0);
-    // phased column vectors exp(i m1 theta) x_(l1,m1), one column ahead; nodes
-    // past the grid repeat node N-1 (they only feed accumulators never stored)
+    // column vectors x_(l1,m1) in a ring of two columns ahead, phased by
+    // exp(i m1 theta) at use; nodes past the grid repeat node N-1 (they only
+    // feed accumulators never stored)
     const int N = g.N;
     const double2 *__restrict__ xg = X + (size_t)kb * N;
     int xo[NG];
 #pragma unroll
     for (int n = 0; n < NG; ++n) xo[n] = min(t0 + 16 * n + col, N - 1);
-    C xn[NG];
-    auto load_x = [&](int m) {
-        const double2 ph = P->ph[g.lmax - l1 + 2 * m];
+    double2 xr[2][NG];
 #pragma unroll
-        for (int n = 0; n < NG; ++n) xn[n] = AR::from(cmul(ph, xg[(size_t)m * N + xo[n]]));
-    };
-    load_x(0);
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int n = 0; n < NG; ++n) xr[k][n] = xg[(size_t)min(k, l1) * N + xo[n]];
     const typename AR::V4 z = {T(0), T(0), T(0), T(0)};
     // one column: products on the matrix core, accumulation on the VALU; with
     // pf the ring slot is refilled (unconditionally, clamped index) for m1 + 4
@@ -290,9 +289,13 @@ __device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const 
         const T fc = fr[k];
         if (pf) fr[k] = Gb[min(m1i + 4, l1) * mstride];
         C xv[NG];
+        const double2 ph = P->ph[g.lmax - l1 + 2 * m1i];
 #pragma unroll
-        for (int n = 0; n < NG; ++n) xv[n] = xn[n];
-        load_x(min(m1i + 1, l1));
+        for (int n = 0; n < NG; ++n) xv[n] = AR::from(cmul(ph, xr[k & 1][n]));
+        if (pf || m1i + 2 <= l1) {
+#pragma unroll
+            for (int n = 0; n < NG; ++n) xr[k & 1][n] = xg[(size_t)min(m1i + 2, l1) * N + xo[n]];
+        }
         if constexpr (NG == 2) {
             // the accumulation of node group 0 runs while group 1 is on the matrix core
             const typename AR::V4 D0 = AR::mfma(fc, bf[0], z);
