@@ -105,22 +105,33 @@ def profile_read(s):
     return s.profile_read()
 
 
-def run_gpu(w, precision, steps, warmup):
+def run_gpu(w, precision, steps, warmup, prof_steps=5):
+    """Warmup, then `steps` timed steps (graph replay on, no per-kernel events),
+    then `prof_steps` steps with HIP-event kernel timing (graphs off) for the
+    kernel table and the roofline."""
     s = make_solver(w, precision)
-    for k in range(1, warmup + 1):
+    k = 0
+    for _ in range(warmup):
+        k += 1
         s.step(k, diag=True)
     iters = []
-    profile(s, True)
     device_sync()
     t0 = time.perf_counter()
-    for k in range(warmup + 1, warmup + steps + 1):
+    for _ in range(steps):
+        k += 1
         s.step(k, diag=True)
         iters.append(s.last_solver_iterations)
     device_sync()
     t1 = time.perf_counter()
+    profile(s, True)
+    piters = []
+    for _ in range(prof_steps):
+        k += 1
+        s.step(k, diag=True)
+        piters.append(s.last_solver_iterations)
     prof = profile_read(s)
     profile(s, False)
-    return s, t1 - t0, prof, iters
+    return s, t1 - t0, prof, iters, piters
 
 
 def cpu_baseline(w, budget_s=20.0):
@@ -281,12 +292,13 @@ def main():
     w = dict(WORKLOAD)
     runs = {}
     for name, precision in (("fp64", gm.GM_PRECISION_FP64), ("mixed", gm.GM_PRECISION_MIXED)):
-        s, elapsed, prof, iters = run_gpu(w, precision, args.steps, args.warmup)
+        s, elapsed, prof, iters, piters = run_gpu(w, precision, args.steps, args.warmup)
         K, N = s.n_modes, s.Nr
-        roof, kernels = rooflines(s, prof, K, N, iters, name == "mixed")
+        roof, kernels = rooflines(s, prof, K, N, piters, name == "mixed")
         runs[name] = {"value": K * N * args.steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
                       "iterations_per_step": float(np.mean(iters)) if iters else None, "roofline": roof,
-                      "kernels": kernels, "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
+                      "kernels": kernels, "profiled_steps": len(piters),
+                      "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
                       "kernel_launches": {k: int(v[1]) for k, v in prof.items()}}
         terms, pairs = s.coupling_work()
         del s

@@ -157,6 +157,18 @@ struct gm_solver {
     double fac_key[3] = {0, 0, 0};
     int fac_cap_version = -1;
     int prev_iters = 4;
+    // progress of the step in flight (step_begin / solve_advance / step_finish)
+    double step_start = 0.0;
+    bool step_wood = false;
+    int step_enqueued = 0;
+    // captured parts of the step, keyed by every kernel argument that can change
+    struct GraphEntry {
+        int kind = 0;
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<GraphEntry> graphs;
+    std::vector<uint64_t> last_key[3];  // last key requested per kind (capture once it repeats)
     std::string err;
     // profiling (gm_profile_enable)
     bool prof_on = false;
@@ -718,20 +730,128 @@ void enqueue_iteration(gm_solver &S, bool wood)
     launch(S, GM_PROF_UPDATE, [&] { return gmk::launch_update(u, S.st); });
 }
 
-void sync_krylov(gm_solver &S)
+
+constexpr int kMaxIt = 500;  // matrices.cpp:104
+
+// Graph-captured parts of a step: the start of the solve (Woodbury columns,
+// rhs, init, first iterations), a batch of iterations, and the tail (residual
+// check, mask, diagnostics, readbacks). Per-step values (theta, phases, sink
+// strengths, Krylov scalars) live in device memory, so a graph stays valid
+// while the kernel arguments below do not change; it is captured once the same
+// arguments were requested on two consecutive steps (an evolving orbit moves
+// the absorber supports every step and runs without graphs).
+enum GraphKind { GK_BEGIN = 0, GK_BATCH = 1, GK_TAIL = 2 };
+
+std::vector<uint64_t> graph_key(const gm_solver &S, int batch)
 {
-    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
-    HIPCK(hipStreamSynchronize(S.st));
-    prof_collect(S);
+    std::vector<uint64_t> k;
+    auto put_d = [&](double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, 8);
+        k.push_back(u);
+    };
+    k.push_back(static_cast<uint64_t>(batch));
+    k.push_back(S.step_wood);
+    k.push_back(static_cast<uint64_t>(S.flags.precision));
+    k.push_back(S.mask);
+    put_d(S.dt);
+    gmd::SinkDev sk[2];
+    fill_sinks(S, sk);
+    for (const auto &d : sk) {
+        k.push_back((static_cast<uint64_t>(static_cast<uint32_t>(d.lo)) << 32) | static_cast<uint32_t>(d.n));
+        k.push_back(reinterpret_cast<uint64_t>(d.v));
+        k.push_back(reinterpret_cast<uint64_t>(d.W));
+    }
+    const gmd::StepParams &P = *S.hP.p;
+    k.push_back(static_cast<uint64_t>(P.nsink) | (static_cast<uint64_t>(P.sink_body[0]) << 8) |
+                (static_cast<uint64_t>(P.sink_body[1]) << 16));
+    return k;
 }
 
-// solve_CN (matrices.cpp:94-215) with the block preconditioner. Returns
-// false when the residual check fails.
-bool solve_cn(gm_solver &S)
+bool graphs_enabled(const gm_solver &S)
+{
+    return S.flags.use_graphs && !S.prof_on;
+}
+
+// Enqueue `work` on the solver's stream, through a cached graph when enabled.
+template <class F>
+void run_captured(gm_solver &S, int kind, const std::vector<uint64_t> &key, F &&work)
+{
+    if (!graphs_enabled(S)) {
+        work();
+        return;
+    }
+    for (auto &e : S.graphs)
+        if (e.kind == kind && e.key == key) {
+            HIPCK(hipGraphLaunch(e.exec, S.st));
+            return;
+        }
+    const bool repeat = S.last_key[kind] == key;
+    S.last_key[kind] = key;
+    if (!repeat) {
+        work();
+        return;
+    }
+    hipGraph_t g = nullptr;
+    HIPCK(hipStreamBeginCapture(S.st, hipStreamCaptureModeThreadLocal));
+    try {
+        work();
+    } catch (...) {
+        (void)hipStreamEndCapture(S.st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCK(hipStreamEndCapture(S.st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCK(e);
+    // a handful of shapes per solver: keep the newest
+    if (S.graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(S.graphs.front().exec);
+        S.graphs.erase(S.graphs.begin());
+    }
+    S.graphs.push_back({kind, key, ex});
+    HIPCK(hipGraphLaunch(ex, S.st));
+}
+
+void drop_graphs(gm_solver &S)
+{
+    for (auto &e : S.graphs) (void)hipGraphExecDestroy(e.exec);
+    S.graphs.clear();
+    for (auto &k : S.last_key) k.clear();
+}
+
+// A batch of BiCGSTAB iterations followed by the readback of the Krylov state.
+void enqueue_batch(gm_solver &S, int batch)
+{
+    run_captured(S, GK_BATCH, graph_key(S, batch), [&] {
+        for (int k = 0; k < batch; ++k) enqueue_iteration(S, S.step_wood);
+        HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
+    });
+    S.step_enqueued += batch;
+}
+
+void solve_begin_work(gm_solver &S, int batch);
+
+// solve_CN (matrices.cpp:94-215) with the block preconditioner, up to the
+// first convergence check: pivots, Woodbury columns, rhs, BiCGSTAB init and a
+// first batch of iterations (as many as the previous step needed).
+void solve_begin(gm_solver &S)
 {
     const gmd::StepParams &P = *S.hP.p;
     const bool wood = P.nsink > 0;
+    S.step_wood = wood;
+    S.step_enqueued = 0;
     factorize(S);
+    const int batch = std::max(1, S.prev_iters);
+    run_captured(S, GK_BEGIN, graph_key(S, batch), [&] { solve_begin_work(S, batch); });
+    S.step_enqueued = batch;
+}
+
+void solve_begin_work(gm_solver &S, int batch)
+{
+    const bool wood = S.step_wood;
     if (wood) woodbury_setup(S);
     gmd::Krylov *dk = S.dK.p;
     // rhs = B psi - (dt/4) Gamma psi and A psi in one apply
@@ -742,7 +862,6 @@ bool solve_cn(gm_solver &S)
         launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RHS, S.st); });
     }
     // Eigen's BiCGSTAB without sinks, the reference's own recurrence with them
-    const int maxit = 500, eigen_mode = wood ? 0 : 1;
     {
         gmk::InitArgs ia{};
         ia.n = S.nvec();
@@ -756,25 +875,40 @@ bool solve_cn(gm_solver &S)
         ia.Pv = S.p.p;
         ia.V = S.v.p;
         ia.tol = 1e-10;
-        ia.maxit = maxit;
-        ia.eigen_mode = eigen_mode;
+        ia.maxit = kMaxIt;
+        ia.eigen_mode = wood ? 0 : 1;
         ia.partial = S.partial.p;
         launch(S, GM_PROF_SMALL, [&] { return gmk::launch_init(ia, S.st); });
     }
-    int batch = std::max(1, S.prev_iters);
-    int enqueued = 0;
-    for (;;) {
-        for (int k = 0; k < batch; ++k) enqueue_iteration(S, wood);
-        enqueued += batch;
-        sync_krylov(S);
-        const gmd::Krylov &K = *S.hK.p;
-        if (K.restart) throw std::runtime_error("BiCGSTAB lost bi-orthogonality; the restart path is not available");
-        if (K.done) break;
-        if (enqueued > 2 * maxit) throw std::runtime_error("BiCGSTAB did not report convergence");
-        batch = 2;
+    for (int k = 0; k < batch; ++k) enqueue_iteration(S, wood);
+    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
+}
+
+void solve_tail_work(gm_solver &S);
+
+// After a convergence check (host copy of the Krylov state is current): more
+// iterations (false) or the acceptance check, mask and diagnostics with their
+// readback (true).
+bool solve_advance(gm_solver &S)
+{
+    const gmd::Krylov &K = *S.hK.p;
+    if (K.restart) throw std::runtime_error("BiCGSTAB lost bi-orthogonality; the restart path is not available");
+    if (!K.done) {
+        if (S.step_enqueued > 2 * kMaxIt) throw std::runtime_error("BiCGSTAB did not report convergence");
+        enqueue_batch(S, 2);
+        return false;
     }
-    S.last_iterations = S.hK.p->iter;
+    S.last_iterations = K.iter;
     S.prev_iters = std::max(1, S.last_iterations);
+    run_captured(S, GK_TAIL, graph_key(S, 0), [&] { solve_tail_work(S); });
+    return true;
+}
+
+void solve_tail_work(gm_solver &S)
+{
+    const gmd::StepParams &P = *S.hP.p;
+    const bool wood = S.step_wood;
+    gmd::Krylov *dk = S.dK.p;
     // acceptance: |A_full x - b| with the rank-one sinks of x
     if (wood) sink_proj(S, S.x.p, gmk::SD_COEFA, 1.0, 0);
     {
@@ -794,7 +928,33 @@ bool solve_cn(gm_solver &S)
     ac.status = S.dflags.p + 1;
     ac.resid = S.dresid.p;
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_accept(ac, S.st); });
-    return true;
+    // diagnostics on the post-mask state (extraction.cpp), with Gamma psi
+    if (P.nsink) sink_proj(S, S.psi.p, gmk::SD_POST, 1.0, 0);
+    {
+        gmk::ApplyArgs a = apply_args(S, S.psi.p, nullptr);
+        // V psi = |rhs| mask V x: the coupling is node-local and the mask is
+        // the same for every mode, so only the stencil is recomputed
+        a.Cin = S.tmp.p;
+        a.mask = S.mask ? S.dmask.p : nullptr;
+        launch(S, GM_PROF_DIAG, [&] { return gmk::launch_diag_apply(a, S.st); });
+        gmk::DiagArgs da{};
+        da.N = S.Nr;
+        da.K = S.K;
+        da.ix = S.idx_ex;
+        da.ntiles = S.ntiles;
+        da.dr = S.dr;
+        da.eps = S.epsilon;
+        da.mu = S.mu;
+        da.Psi = S.psi.p;
+        da.dpsi_ix = S.dpsi_ix.p;
+        da.rowpart = S.rowpart.p;
+        da.out = S.diagout.p;
+        launch(S, GM_PROF_DIAG, [&] { return gmk::launch_diag_final(da, S.st); });
+    }
+    HIPCK(hipMemcpyAsync(S.hdiag.p, S.diagout.p, sizeof(double) * (8 * S.K), hipMemcpyDeviceToHost, S.st));
+    HIPCK(hipMemcpyAsync(S.hflags.p, S.dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S.st));
+    HIPCK(hipMemcpyAsync(S.hresid.p, S.dresid.p, sizeof(double), hipMemcpyDeviceToHost, S.st));
+    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
 }
 
 void alloc_device(gm_solver &S)
@@ -956,7 +1116,7 @@ int gm_create(const gm_params *p, gm_solver **out)
         S->R2 = 1 / (1 + p->q) * p->R;
         S->M1 = 1.0;
         S->M2 = p->q;
-        S->flags = gm_flags{1, 1, 50.0, 1, 2.0, GM_PRECISION_FP64};
+        S->flags = gm_flags{1, 1, 50.0, 1, 2.0, GM_PRECISION_FP64, 1};
         S->avg = std::make_unique<gmh::RollingPhaseAverage>(2.0);
         S->r.resize(S->Nr);
         for (int i = 0; i < S->Nr; ++i) S->r[i] = S->epsilon + i * S->dr;
@@ -992,6 +1152,7 @@ void gm_destroy(gm_solver *S)
     if (!S) return;
     if (S->st) {
         (void)hipStreamSynchronize(S->st);
+        drop_graphs(*S);
         (void)hipStreamDestroy(S->st);
     }
     for (auto &p : S->pending) S->ev_pool.insert(S->ev_pool.end(), {p.a, p.b});
@@ -1213,114 +1374,154 @@ int gm_begin_run(gm_solver *S, double r_ex, int32_t n_steps)
     });
 }
 
-int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
+// step (src/CrankNicolsonModes.cpp:145-234), phase 1: advance t, theta and the
+// phase, rebuild the perturbation and enqueue the solve up to its first
+// convergence check.
+void step_begin(gm_solver &S)
 {
-    (void)step_id;
-    return guarded(S, [&] {
-        require_initialized(*S);
-        if (!S->flags.use_block_preconditioner)
-            throw std::runtime_error("ILUT preconditioner is not available in this build");
-        const double step_start = S->t_i;
-        S->t_i += S->dt;
-        S->theta += (S->counter ? -1.0 : 1.0) * S->Omega * S->dt;
-        S->phase += S->Omega * S->dt;
-        build_perturbation(*S, S->t_i);
-        const gmd::StepParams &P = *S->hP.p;
-        solve_cn(*S);
-        // diagnostics on the post-mask state (extraction.cpp), with Gamma psi
-        if (P.nsink) sink_proj(*S, S->psi.p, gmk::SD_POST, 1.0, 0);
-        {
-            gmk::ApplyArgs a = apply_args(*S, S->psi.p, nullptr);
-            // V psi = |rhs| mask V x: the coupling is node-local and the mask is
-            // the same for every mode, so only the stencil is recomputed
-            a.Cin = S->tmp.p;
-            a.mask = S->mask ? S->dmask.p : nullptr;
-            launch(*S, GM_PROF_DIAG, [&] { return gmk::launch_diag_apply(a, S->st); });
-            gmk::DiagArgs da{};
-            da.N = S->Nr;
-            da.K = S->K;
-            da.ix = S->idx_ex;
-            da.ntiles = S->ntiles;
-            da.dr = S->dr;
-            da.eps = S->epsilon;
-            da.mu = S->mu;
-            da.Psi = S->psi.p;
-            da.dpsi_ix = S->dpsi_ix.p;
-            da.rowpart = S->rowpart.p;
-            da.out = S->diagout.p;
-            launch(*S, GM_PROF_DIAG, [&] { return gmk::launch_diag_final(da, S->st); });
-        }
-        HIPCK(hipMemcpyAsync(S->hdiag.p, S->diagout.p, sizeof(double) * (8 * S->K), hipMemcpyDeviceToHost, S->st));
-        HIPCK(hipMemcpyAsync(S->hflags.p, S->dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S->st));
-        HIPCK(hipMemcpyAsync(S->hresid.p, S->dresid.p, sizeof(double), hipMemcpyDeviceToHost, S->st));
-        HIPCK(hipMemcpyAsync(S->hK.p, S->dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S->st));
-        HIPCK(hipStreamSynchronize(S->st));
-        prof_collect(*S);
-        if (S->hflags.p[0]) {
-            S->fac_valid = false;
+    require_initialized(S);
+    if (!S.flags.use_block_preconditioner)
+        throw std::runtime_error("ILUT preconditioner is not available in this build");
+    S.step_start = S.t_i;
+    S.t_i += S.dt;
+    S.theta += (S.counter ? -1.0 : 1.0) * S.Omega * S.dt;
+    S.phase += S.Omega * S.dt;
+    build_perturbation(S, S.t_i);
+    solve_begin(S);
+}
+
+// Phase 3, once the stream has drained after the diagnostics readback: the
+// residual check, absorbed mass, orbit update and diagnostics.
+void step_finish(gm_solver &S, gm_diag *d)
+{
+    const double step_start = S.step_start;
+    const gmd::StepParams &P = *S.hP.p;
+    {
+        if (S.hflags.p[0]) {
+            S.fac_valid = false;
             int zero = 0;
-            HIPCK(hipMemcpy(S->dflags.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+            HIPCK(hipMemcpy(S.dflags.p, &zero, sizeof(int), hipMemcpyHostToDevice));
             throw std::runtime_error("Radial preconditioner found a zero pivot; the ILUT fallback is not available");
         }
-        S->last_residual = S->hresid.p[0];
-        if (!S->hflags.p[1]) throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
-        const gmd::Krylov &K = *S->hK.p;
+        S.last_residual = S.hresid.p[0];
+        if (!S.hflags.p[1]) throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
+        const gmd::Krylov &K = *S.hK.p;
         // absorbed mass: mu dr dt sum_a g_a |v_a^H psi_mid|^2 (CN balance)
         double absorbed = 0.0, Mdot[2] = {0.0, 0.0};
         for (int s = 0; s < P.nsink; ++s) {
             const double mr = 0.5 * (K.qold[s].x + K.qnew[s].x), mi = 0.5 * (K.qold[s].y + K.qnew[s].y);
-            absorbed += S->mu * S->dr * S->dt * P.g[s] * (mr * mr + mi * mi);
+            absorbed += S.mu * S.dr * S.dt * P.g[s] * (mr * mr + mi * mi);
             // update_time_dependent_increment uses the post-mask midpoint (evolution.cpp:20-30)
             const double pr = 0.5 * (K.qpost[s].x + K.qold[s].x), pi = 0.5 * (K.qpost[s].y + K.qold[s].y);
-            Mdot[P.sink_body[s]] = S->mu * S->dr * P.g[s] * (pr * pr + pi * pi);
+            Mdot[P.sink_body[s]] = S.mu * S.dr * P.g[s] * (pr * pr + pi * pi);
         }
-        const double Omega_diag = S->Omega_V;
-        if (!S->isolated && S->flags.evolve_orbit && S->t_i > S->ramp_end_time()) {
+        const double Omega_diag = S.Omega_V;
+        if (!S.isolated && S.flags.evolve_orbit && S.t_i > S.ramp_end_time()) {
             // update_time_dependent_increment (evolution.cpp:17-72)
-            const double orbit_dt = S->t_i - std::max(step_start, S->ramp_end_time());
-            const double Mdot1 = S->absorption ? Mdot[0] : 0.0, Mdot2 = S->absorption ? Mdot[1] : 0.0;
+            const double orbit_dt = S.t_i - std::max(step_start, S.ramp_end_time());
+            const double Mdot1 = S.absorption ? Mdot[0] : 0.0, Mdot2 = S.absorption ? Mdot[1] : 0.0;
             double ramp = 1.0;
-            if (S->t_i < S->ramp_end_time()) ramp = 0.0;
-            else if (S->t_i < S->ramp_end_time() + S->initial_period) {
-                const double xx = (S->t_i - S->ramp_end_time()) / S->initial_period;
+            if (S.t_i < S.ramp_end_time()) ramp = 0.0;
+            else if (S.t_i < S.ramp_end_time() + S.initial_period) {
+                const double xx = (S.t_i - S.ramp_end_time()) / S.initial_period;
                 ramp = 3.0 * xx * xx - 2.0 * xx * xx * xx;
             }
-            const double dR_gw = -(64.0 / 5.0) * S->M1 * S->M2 * (S->M1 + S->M2) / std::pow(S->R, 3);
-            const double dR_cloud = -2.0 * S->R * S->R * S->p_bin / (S->M1 * S->M2);
+            const double dR_gw = -(64.0 / 5.0) * S.M1 * S.M2 * (S.M1 + S.M2) / std::pow(S.R, 3);
+            const double dR_cloud = -2.0 * S.R * S.R * S.p_bin / (S.M1 * S.M2);
             const double dRdt = (dR_gw + dR_cloud) * ramp;
-            S->R += dRdt * orbit_dt;
-            S->M1 += Mdot1 * orbit_dt;
-            S->M2 += Mdot2 * orbit_dt;
-            S->q = S->M2 / S->M1;
-            const double Mt = S->M1 + S->M2;
-            S->R1 = S->M2 / Mt * S->R;
-            S->R2 = S->M1 / Mt * S->R;
-            S->Omega = std::sqrt(Mt / std::pow(S->R, 3));
-            S->P = 2.0 * M_PI / S->Omega;
-            if (S->ramp_factor == 1.0) S->dt = S->P / S->flags.steps_per_period;
+            S.R += dRdt * orbit_dt;
+            S.M1 += Mdot1 * orbit_dt;
+            S.M2 += Mdot2 * orbit_dt;
+            S.q = S.M2 / S.M1;
+            const double Mt = S.M1 + S.M2;
+            S.R1 = S.M2 / Mt * S.R;
+            S.R2 = S.M1 / Mt * S.R;
+            S.Omega = std::sqrt(Mt / std::pow(S.R, 3));
+            S.P = 2.0 * M_PI / S.Omega;
+            if (S.ramp_factor == 1.0) S.dt = S.P / S.flags.steps_per_period;
         }
-        const double *h = S->hdiag.p;
-        const int Kn = S->K;
+        const double *h = S.hdiag.p;
+        const int Kn = S.K;
         // global sums: per-mode shares added in mode order
         double gsum[4] = {0.0, 0.0, 0.0, 0.0};
         for (int q = 0; q < 4; ++q)
             for (int j = 0; j < Kn; ++j) gsum[q] += h[(4 + q) * Kn + j];
-        const double sign = S->counter ? -1.0 : 1.0;
-        const double p_bin_last = 2.0 * sign * Omega_diag * gsum[1] * S->dr;
-        S->p_bin = S->isolated ? 0.0 : S->avg->update(S->t_i, S->phase, p_bin_last);
+        const double sign = S.counter ? -1.0 : 1.0;
+        const double p_bin_last = 2.0 * sign * Omega_diag * gsum[1] * S.dr;
+        S.p_bin = S.isolated ? 0.0 : S.avg->update(S.t_i, S.phase, p_bin_last);
         if (d) {
-            d->energy = gsum[0] * S->dr;
-            d->p_bin = S->p_bin;
+            d->energy = gsum[0] * S.dr;
+            d->p_bin = S.p_bin;
             d->p_bin_inst = p_bin_last;
-            d->dV_bin_dt = gsum[2] * S->dr;
-            d->cloud_mass = S->mu * S->dr * gsum[3];
+            d->dV_bin_dt = gsum[2] * S.dr;
+            d->cloud_mass = S.mu * S.dr * gsum[3];
             d->absorbed = absorbed;
             if (d->flux) std::memcpy(d->flux, h, sizeof(double) * Kn);
             if (d->energy_flux) std::memcpy(d->energy_flux, h + Kn, sizeof(double) * Kn);
             if (d->norms) std::memcpy(d->norms, h + 2 * Kn, sizeof(double) * Kn);
             if (d->energies) std::memcpy(d->energies, h + 3 * Kn, sizeof(double) * Kn);
         }
+    }
+}
+
+int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
+{
+    (void)step_id;
+    return guarded(S, [&] {
+        step_begin(*S);
+        for (;;) {
+            HIPCK(hipStreamSynchronize(S->st));
+            prof_collect(*S);
+            if (solve_advance(*S)) break;
+        }
+        HIPCK(hipStreamSynchronize(S->st));
+        prof_collect(*S);
+        step_finish(*S, d);
     });
+}
+
+int gm_ensemble_step(gm_solver *const *members, int32_t n, int32_t step_id, gm_diag *diags)
+{
+    (void)step_id;
+    if (n < 0 || (n > 0 && !members)) return GM_E_INVALID;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < i; ++k)
+            if (!members[i] || members[i] == members[k]) return GM_E_INVALID;
+    // every member runs on its own stream: enqueue all, then walk the members
+    // through their convergence checks; a member that fails keeps its error
+    // (gm_last_error) and drops out of the step
+    std::vector<int> st(n, GM_OK), phase(n, 0);
+    for (int i = 0; i < n; ++i) {
+        st[i] = guarded(members[i], [&] { step_begin(*members[i]); });
+        phase[i] = st[i] == GM_OK ? 1 : -1;
+    }
+    for (bool busy = true; busy;) {
+        busy = false;
+        for (int i = 0; i < n; ++i) {
+            if (phase[i] != 1) continue;
+            gm_solver *S = members[i];
+            st[i] = guarded(S, [&] {
+                HIPCK(hipStreamSynchronize(S->st));
+                prof_collect(*S);
+                if (solve_advance(*S)) phase[i] = 2;
+            });
+            if (st[i] != GM_OK) phase[i] = -1;
+            busy = busy || phase[i] == 1;
+        }
+    }
+    int first = GM_OK;
+    for (int i = 0; i < n; ++i) {
+        if (phase[i] == 2) {
+            gm_solver *S = members[i];
+            st[i] = guarded(S, [&] {
+                HIPCK(hipStreamSynchronize(S->st));
+                prof_collect(*S);
+                step_finish(*S, diags ? diags + i : nullptr);
+            });
+        }
+        if (first == GM_OK && st[i] != GM_OK) first = st[i];
+    }
+    return first;
 }
 
 int gm_get_scalars(const gm_solver *S, gm_scalars *o)

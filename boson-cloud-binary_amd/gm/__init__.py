@@ -29,6 +29,7 @@ def lib():
         _lib.gm_coupling_work.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib.gm_profile_enable.argtypes = [vp, C.c_int32]
         _lib.gm_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
+        _lib.gm_ensemble_step.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(capi.Diag)]
     return _lib
 
 
@@ -66,3 +67,33 @@ class CrankNicolsonModes(capi.Solver):
         n = (C.c_int64 * nc)()
         self._lib.gm_profile_read(self._h, ms, n, nc)
         return {name: (ms[c], n[c]) for c, name in enumerate(self.PROFILE_CATEGORIES)}
+
+
+def ensemble_step(members, step_id=1, diag=True):
+    """One step of every solver in `members` (gm_ensemble_step): the solvers run
+    on their own streams and overlap on the device. Returns one diagnostics
+    dict per member (None when diag is False). Raises the first member's error."""
+    import numpy as np
+    n = len(members)
+    handles = (C.c_void_p * n)(*[m._h for m in members])
+    arrs, diags = [], None
+    if diag:
+        diags = (capi.Diag * n)()
+        for i, m in enumerate(members):
+            a = {k: np.zeros(m.n_modes) for k in ("flux", "energy_flux", "norms", "energies")}
+            for k, v in a.items():
+                setattr(diags[i], k, capi._ptr(v))
+            arrs.append(a)
+    st = lib().gm_ensemble_step(handles, n, int(step_id), diags)
+    if st != 0:
+        bad = next((m for m in members if lib().gm_last_error(m._h)), members[0])
+        bad._check(st)
+    if not diag:
+        return [None] * n
+    out = []
+    for i in range(n):
+        d = {k: getattr(diags[i], k) for k in ("energy", "p_bin", "p_bin_inst", "dV_bin_dt", "cloud_mass",
+                                                "absorbed")}
+        d.update(arrs[i])
+        out.append(d)
+    return out

@@ -73,6 +73,8 @@ typedef struct gm_flags {
     int32_t use_block_preconditioner; /* default 1 */
     double power_average_orbits;      /* default 2 */
     int32_t precision;                /* GM_PRECISION_FP64 (default) or _MIXED */
+    int32_t use_graphs;               /* replay captured parts of the step (default 1);
+                                         profiling (gm_profile_enable) runs without graphs */
 } gm_flags;
 
 /* Orbit state: the 13 doubles of the checkpoint (restart.cpp:153-163) in the
@@ -140,6 +142,15 @@ int gm_begin_run(gm_solver *solver, double r_ex, int32_t n_steps);
 /* step (src/CrankNicolsonModes.cpp:145-234) without the text output; the
  * diagnostics that step() writes are returned in diag (may be NULL). */
 int gm_step(gm_solver *solver, int32_t step_id, gm_diag *diag);
+
+/* Ensemble step (no counterpart in the reference, whose drivers run one
+ * binary per process): one step() of each of n independent solvers, which may
+ * differ in every parameter (q, R, mu, N, l_max, precision). Each solver runs
+ * on its own HIP stream; their kernels overlap on the device. diags may be
+ * NULL or hold n entries (filled as by gm_step). Returns GM_OK or the status of
+ * the first member that failed; a failed member keeps its message in
+ * gm_last_error and the others complete their step. A solver may appear once. */
+int gm_ensemble_step(gm_solver *const *members, int32_t n, int32_t step_id, gm_diag *diags);
 
 /* psi (include/CrankNicolsonModes.hpp:162): read, or overwrite without scaling. */
 int gm_get_state(const gm_solver *solver, double *psi);

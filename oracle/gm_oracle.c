@@ -353,6 +353,7 @@ int om_create(const gm_params *p, gm_solver **out)
     s->flags.use_block_preconditioner = 1;
     s->flags.power_average_orbits = 2.0;
     s->flags.precision = GM_PRECISION_FP64;
+    s->flags.use_graphs = 1;
     rolling_init(&s->avg, 2.0);
     s->r = (double *)malloc(sizeof(double) * s->Nr);
     for (int i = 0; i < s->Nr; i++) s->r[i] = s->epsilon + i * s->dr;
