@@ -300,6 +300,8 @@ def main():
                       "kernels": kernels, "profiled_steps": len(piters),
                       "kernel_ms": {k: round(v[0], 4) for k, v in prof.items()},
                       "kernel_launches": {k: int(v[1]) for k, v in prof.items()}}
+        tile, tms = s.apply_config(precision)
+        runs[name]["apply_tile"] = {"nodes": tile, "tuning_ms": tms}
         terms, pairs = s.coupling_work()
         del s
     mx, f64 = runs["mixed"], runs["fp64"]

@@ -18,7 +18,6 @@ namespace gmd {
 
 constexpr int TILE = 64;   // radial nodes of a multipole-cut / diagnostics tile
 constexpr int AW = 4;      // row chunks (wavefronts) per operator-apply workgroup
-constexpr int APPLY_NG = 2;  // 16-node groups per operator-apply tile
 constexpr int MAXLM = 40;  // largest supported l_max + 1
 constexpr int PT = 512;    // threads of a preconditioner or factor workgroup
 constexpr int PE = 4;      // consecutive nodes per preconditioner thread

@@ -296,18 +296,22 @@ __device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const 
 #pragma unroll
             for (int n = 0; n < NG; ++n) xr[k & 1][n] = xg[(size_t)min(m1i + 2, l1) * N + xo[n]];
         }
-        if constexpr (NG == 2) {
-            // the accumulation of node group 0 runs while group 1 is on the matrix core
+        if constexpr (NG == 1) {
             const typename AR::V4 D0 = AR::mfma(fc, bf[0], z);
-            const typename AR::V4 D1 = AR::mfma(fc, bf[1], z);
-            __builtin_amdgcn_sched_barrier(0);
             acc_update<NG, T>(acc, 0, D0, xv[0]);
-            __builtin_amdgcn_sched_barrier(0);
-            acc_update<NG, T>(acc, 1, D1, xv[1]);
-            __builtin_amdgcn_sched_barrier(0);
         } else {
-            const typename AR::V4 D0 = AR::mfma(fc, bf[0], z);
-            acc_update<NG, T>(acc, 0, D0, xv[0]);
+            // node groups in pairs: the accumulation of group n runs while
+            // group n + 1 is on the matrix core
+#pragma unroll
+            for (int n = 0; n < NG; n += 2) {
+                const typename AR::V4 D0 = AR::mfma(fc, bf[n], z);
+                const typename AR::V4 D1 = AR::mfma(fc, bf[n + 1], z);
+                __builtin_amdgcn_sched_barrier(0);
+                acc_update<NG, T>(acc, n, D0, xv[n]);
+                __builtin_amdgcn_sched_barrier(0);
+                acc_update<NG, T>(acc, n + 1, D1, xv[n + 1]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     };
     int m0 = 0;
@@ -1279,23 +1283,32 @@ __global__ __launch_bounds__(64) void k_diag_final(DiagArgs a)
 
 // ---------------------------------------------------------------- launchers
 
-int apply_grid(int N, int nchunks, int *ngroups)
+int apply_grid(int N, int nchunks, int ng, int *ngroups)
 {
-    constexpr int tn = 16 * gmd::APPLY_NG;
+    const int tn = 16 * ng;
     const int ntiles = (N + tn - 1) / tn;
     *ngroups = (nchunks + gmd::AW - 1) / gmd::AW;
     return (ntiles + 7) / 8 * 8 * *ngroups;
 }
 
+template <int NG>
+static hipError_t launch_apply_ng(const ApplyArgs &a, int grid, hipStream_t st)
+{
+    if (a.mixed)
+        hipLaunchKernelGGL((k_apply<NG, float>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_apply<NG, double>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
-    const int grid = apply_grid(a.g.N, a.g.nchunks, &a.ngroups);
+    if (a.ng != 1 && a.ng != 2 && a.ng != 4) return hipErrorInvalidValue;
+    const int grid = apply_grid(a.g.N, a.g.nchunks, a.ng, &a.ngroups);
     a.mode = mode;
-    if (a.mixed)
-        hipLaunchKernelGGL((k_apply<gmd::APPLY_NG, float>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_apply<gmd::APPLY_NG, double>), dim3(grid), dim3(64 * gmd::AW), 0, st, a);
-    return hipGetLastError();
+    if (a.ng == 1) return launch_apply_ng<1>(a, grid, st);
+    if (a.ng == 2) return launch_apply_ng<2>(a, grid, st);
+    return launch_apply_ng<4>(a, grid, st);
 }
 
 hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st)

@@ -26,6 +26,7 @@ struct ApplyArgs {
     const double *mask;  // diagnostics apply: damping mask, or nullptr
     int ngroups;         // chunk groups per tile, set by the launcher
     int mixed;           // coupling in FP32 (E32, GM32) with FP32 accumulation
+    int ng;              // 16-node groups per tile (1, 2 or 4), chosen by the autotuner
     int mode;            // ApplyMode, set by the launcher
 };
 
@@ -113,7 +114,8 @@ struct DiagArgs {
     double *out;         // flux, energy_flux, norms, energies, then E, p_bin, dV, mass shares (K each)
 };
 
-int apply_grid(int N, int nchunks, int *ngroups);
+// grid of the operator apply for tiles of 16 ng nodes (ng = 1, 2 or 4)
+int apply_grid(int N, int nchunks, int ng, int *ngroups);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
                           double mu, double prf, double *E, float *E32, hipStream_t st);

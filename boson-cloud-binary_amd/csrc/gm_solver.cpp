@@ -11,11 +11,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "gm_kernels.hpp"
@@ -169,6 +173,10 @@ struct gm_solver {
     };
     std::vector<GraphEntry> graphs;
     std::vector<uint64_t> last_key[3];  // last key requested per kind (capture once it repeats)
+    // operator-apply tile (16-node groups) per precision, 0 until tuned, and
+    // the measured time per candidate (ms; index ng = 1, 2, 4 -> 0, 1, 2)
+    int apply_ng[2] = {0, 0};
+    double apply_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
     std::string err;
     // profiling (gm_profile_enable)
     bool prof_on = false;
@@ -582,6 +590,7 @@ gmk::ApplyArgs apply_args(gm_solver &S, const double2 *X, double2 *Y)
     a.partial = S.partial.p;
     a.rowpart = S.rowpart.p;
     a.dpsi_ix = S.dpsi_ix.p;
+    a.ng = S.apply_ng[0] ? S.apply_ng[0] : 2;
     return a;
 }
 
@@ -706,6 +715,7 @@ void enqueue_iteration(gm_solver &S, bool wood)
     gmk::ApplyArgs a = apply_args(S, S.y.p, S.v.p);
     a.R0 = S.r0.p;
     a.mixed = mixed;
+    if (mixed && S.apply_ng[1]) a.ng = S.apply_ng[1];
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_V, S.st); });
     // z = M^-1 s with s = r - alpha v
     pa.Pv = S.s.p;
@@ -715,6 +725,7 @@ void enqueue_iteration(gm_solver &S, bool wood)
     a = apply_args(S, S.z.p, S.t.p);
     a.R0 = S.s.p;
     a.mixed = mixed;
+    if (mixed && S.apply_ng[1]) a.ng = S.apply_ng[1];
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_T, S.st); });
     gmk::UpdateArgs u{};
     u.n = S.nvec();
@@ -754,6 +765,7 @@ std::vector<uint64_t> graph_key(const gm_solver &S, int batch)
     k.push_back(S.step_wood);
     k.push_back(static_cast<uint64_t>(S.flags.precision));
     k.push_back(S.mask);
+    k.push_back(static_cast<uint64_t>(S.apply_ng[0] | (S.apply_ng[1] << 8)));
     put_d(S.dt);
     gmd::SinkDev sk[2];
     fill_sinks(S, sk);
@@ -966,7 +978,7 @@ void alloc_device(gm_solver &S)
     S.E32.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.lcut.alloc(S.ntiles);
     int per = 0;
-    const int grid = gmk::apply_grid(S.Nr, S.nchunks, &per);
+    const int grid = gmk::apply_grid(S.Nr, S.nchunks, 1, &per);
     S.partial.alloc(static_cast<size_t>(3) * std::max(grid, gmd::RED_BLOCKS));
     S.rowpart.alloc(static_cast<size_t>(S.ntiles) * S.K * 6);
     S.dpsi_ix.alloc(S.K);
@@ -1116,7 +1128,7 @@ int gm_create(const gm_params *p, gm_solver **out)
         S->R2 = 1 / (1 + p->q) * p->R;
         S->M1 = 1.0;
         S->M2 = p->q;
-        S->flags = gm_flags{1, 1, 50.0, 1, 2.0, GM_PRECISION_FP64, 1};
+        S->flags = gm_flags{1, 1, 50.0, 1, 2.0, GM_PRECISION_FP64, 1, 0};
         S->avg = std::make_unique<gmh::RollingPhaseAverage>(2.0);
         S->r.resize(S->Nr);
         for (int i = 0; i < S->Nr; ++i) S->r[i] = S->epsilon + i * S->dr;
@@ -1160,12 +1172,23 @@ void gm_destroy(gm_solver *S)
     delete S;
 }
 
+int gm_apply_config(const gm_solver *S, int32_t precision, int32_t *tile_nodes, double *ms)
+{
+    if (!S || (precision != GM_PRECISION_FP64 && precision != GM_PRECISION_MIXED) || !tile_nodes) return GM_E_INVALID;
+    const int m = precision == GM_PRECISION_MIXED;
+    *tile_nodes = 16 * S->apply_ng[m];
+    if (ms)
+        for (int c = 0; c < 3; ++c) ms[c] = S->apply_ms[m][c];
+    return GM_OK;
+}
+
 int gm_set_coupling_tolerance(gm_solver *S, double tol)
 {
     return guarded(S, [&] {
         if (!(tol >= 0.0) || tol >= 1e-6) throw std::invalid_argument("coupling tolerance must lie in [0, 1e-6)");
         S->coupling_tol = tol;
         S->E_valid = false;
+        S->apply_ng[0] = S->apply_ng[1] = 0;  // the cut changes the work per tile: tune again
         if (S->initialized) {
             build_perturbation(*S, S->t_i);
             HIPCK(hipStreamSynchronize(S->st));
@@ -1251,9 +1274,12 @@ int gm_set_flags(gm_solver *S, const gm_flags *f)
         if (!(f->power_average_orbits > 0.0)) throw std::invalid_argument("RollingPhaseAverage: n_orbits must be positive");
         if (f->precision != GM_PRECISION_FP64 && f->precision != GM_PRECISION_MIXED)
             throw std::invalid_argument("precision must be GM_PRECISION_FP64 or GM_PRECISION_MIXED");
+        if (f->apply_tile != 0 && f->apply_tile != 16 && f->apply_tile != 32 && f->apply_tile != 64)
+            throw std::invalid_argument("apply_tile must be 0 (autotune), 16, 32 or 64");
         if (f->power_average_orbits != S->flags.power_average_orbits)
             S->avg = std::make_unique<gmh::RollingPhaseAverage>(f->power_average_orbits);
         if (f->ramp_secondary_potential != S->flags.ramp_secondary_potential) S->E_valid = false;
+        if (f->apply_tile != S->flags.apply_tile) S->apply_ng[0] = S->apply_ng[1] = 0;
         S->flags = *f;
     });
 }
@@ -1374,6 +1400,85 @@ int gm_begin_run(gm_solver *S, double r_ex, int32_t n_steps)
     });
 }
 
+// Autotuner of the operator-apply tile: 16, 32 or 64 radial nodes per
+// workgroup tile (register pressure and occupancy against reuse of the Gaunt
+// fragments and load balance). Each candidate is timed with HIP events on the
+// solver's stream over a few applies of A to psi (into the Krylov vector t,
+// which every iteration overwrites); the fastest is kept for this shape in a
+// process-wide table, so later solvers of the same shape skip the timing.
+struct TuneKey {
+    int N, lmax, mixed;
+    double tol;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(N, lmax, mixed, tol) < std::tie(o.N, o.lmax, o.mixed, o.tol);
+    }
+};
+std::mutex tune_mu;
+std::map<TuneKey, std::pair<int, std::array<double, 3>>> tune_table;
+
+void tune_apply(gm_solver &S, int mixed)
+{
+    if (S.flags.apply_tile) {
+        S.apply_ng[mixed] = S.flags.apply_tile / 16;
+        return;
+    }
+    if (S.apply_ng[mixed]) return;
+    const TuneKey key{S.Nr, S.l_max, mixed, S.coupling_tol};
+    {
+        std::lock_guard<std::mutex> lk(tune_mu);
+        auto it = tune_table.find(key);
+        if (it != tune_table.end()) {
+            S.apply_ng[mixed] = it->second.first;
+            for (int c = 0; c < 3; ++c) S.apply_ms[mixed][c] = it->second.second[c];
+            return;
+        }
+    }
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    const int cand[3] = {1, 2, 4};
+    std::array<double, 3> ms{};
+    int best = 2;
+    double best_ms = 1e300;
+    try {
+        // the residual form: full coupling plus a fused reduction, like the
+        // Krylov applies (its result in the Krylov state is overwritten by the
+        // step's own residual check); two interleaved rounds, best of each
+        for (int c = 0; c < 3; ++c) ms[c] = 1e300;
+        for (int round = 0; round < 2; ++round)
+            for (int c = 0; c < 3; ++c) {
+                gmk::ApplyArgs a = apply_args(S, S.psi.p, nullptr);
+                a.R0 = S.psi.p;
+                a.mixed = mixed;
+                a.ng = cand[c];
+                HIPCK(gmk::launch_apply(a, gmk::AM_RES, S.st));  // warm
+                HIPCK(hipEventRecord(e0, S.st));
+                for (int r = 0; r < 5; ++r) HIPCK(gmk::launch_apply(a, gmk::AM_RES, S.st));
+                HIPCK(hipEventRecord(e1, S.st));
+                HIPCK(hipEventSynchronize(e1));
+                float t = 0.0f;
+                HIPCK(hipEventElapsedTime(&t, e0, e1));
+                ms[c] = std::min(ms[c], t / 5.0);
+            }
+        for (int c = 0; c < 3; ++c)
+            if (ms[c] < best_ms) {
+                best_ms = ms[c];
+                best = cand[c];
+            }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    S.apply_ng[mixed] = best;
+    for (int c = 0; c < 3; ++c) S.apply_ms[mixed][c] = ms[c];
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_table[key] = {best, ms};
+}
+
 // step (src/CrankNicolsonModes.cpp:145-234), phase 1: advance t, theta and the
 // phase, rebuild the perturbation and enqueue the solve up to its first
 // convergence check.
@@ -1387,6 +1492,8 @@ void step_begin(gm_solver &S)
     S.theta += (S.counter ? -1.0 : 1.0) * S.Omega * S.dt;
     S.phase += S.Omega * S.dt;
     build_perturbation(S, S.t_i);
+    tune_apply(S, 0);
+    if (S.flags.precision == GM_PRECISION_MIXED) tune_apply(S, 1);
     solve_begin(S);
 }
 

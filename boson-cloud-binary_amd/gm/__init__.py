@@ -30,6 +30,7 @@ def lib():
         _lib.gm_profile_enable.argtypes = [vp, C.c_int32]
         _lib.gm_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
         _lib.gm_ensemble_step.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(capi.Diag)]
+        _lib.gm_apply_config.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     return _lib
 
 
@@ -53,6 +54,13 @@ class CrankNicolsonModes(capi.Solver):
         t, p = C.c_double(), C.c_double()
         self._check(self._lib.gm_coupling_work(self._h, C.byref(t), C.byref(p)))
         return t.value, p.value
+
+    def apply_config(self, precision=GM_PRECISION_FP64):
+        """(tile nodes chosen by the autotuner, {16: ms, 32: ms, 64: ms})."""
+        t = C.c_int32()
+        ms = (C.c_double * 3)()
+        self._check(self._lib.gm_apply_config(self._h, int(precision), C.byref(t), ms))
+        return t.value, {16: ms[0], 32: ms[1], 64: ms[2]}
 
     def profile(self, on):
         self._check(self._lib.gm_profile_enable(self._h, int(on)))

@@ -27,7 +27,7 @@ class Flags(C.Structure):
     _fields_ = [("evolve_orbit", C.c_int32), ("ramp_secondary_potential", C.c_int32),
                 ("steps_per_period", C.c_double), ("use_block_preconditioner", C.c_int32),
                 ("power_average_orbits", C.c_double), ("precision", C.c_int32),
-                ("use_graphs", C.c_int32)]
+                ("use_graphs", C.c_int32), ("apply_tile", C.c_int32)]
 
 
 SCALAR_NAMES = ("t", "theta", "phase", "R", "M1", "M2", "q", "R1", "R2", "Omega", "P",

@@ -75,6 +75,7 @@ typedef struct gm_flags {
     int32_t precision;                /* GM_PRECISION_FP64 (default) or _MIXED */
     int32_t use_graphs;               /* replay captured parts of the step (default 1);
                                          profiling (gm_profile_enable) runs without graphs */
+    int32_t apply_tile;               /* operator-apply tile: 0 = autotune (default), or 16, 32, 64 nodes */
 } gm_flags;
 
 /* Orbit state: the 13 doubles of the checkpoint (restart.cpp:153-163) in the
@@ -176,6 +177,13 @@ int gm_set_coupling_tolerance(gm_solver *solver, double tol);
 /* Work of one coupling apply under the current truncation, summed over radial
  * nodes: Gaunt terms evaluated (terms) and mode pairs coupled (pairs). */
 int gm_coupling_work(gm_solver *solver, double *terms, double *pairs);
+
+/* Operator-apply tile chosen by the autotuner (no counterpart in the
+ * reference): on a solver's first step each tile of 16, 32 or 64 radial nodes
+ * is timed on its stream and the fastest kept, per (N, l_max, precision,
+ * coupling tolerance) for the whole process. tile_nodes receives the choice (0
+ * before the first step); ms, if not NULL, the 3 measured times in ms. */
+int gm_apply_config(const gm_solver *solver, int32_t precision, int32_t *tile_nodes, double *ms);
 
 /* Measurement hook (no counterpart in the reference; for benchmarks): device
  * time per kernel class, recorded with HIP events on the solver's stream.

@@ -354,6 +354,7 @@ int om_create(const gm_params *p, gm_solver **out)
     s->flags.power_average_orbits = 2.0;
     s->flags.precision = GM_PRECISION_FP64;
     s->flags.use_graphs = 1;
+    s->flags.apply_tile = 0;
     rolling_init(&s->avg, 2.0);
     s->r = (double *)malloc(sizeof(double) * s->Nr);
     for (int i = 0; i < s->Nr; i++) s->r[i] = s->epsilon + i * s->dr;

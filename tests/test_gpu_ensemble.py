@@ -52,3 +52,32 @@ def test_ensemble_rejects_duplicates():
     s = make(MEMBERS[2])
     with pytest.raises(gm.InvalidArgument):
         gm.ensemble_step([s, s], 1)
+
+
+@pytest.mark.parametrize("precision", [gm.GM_PRECISION_FP64, gm.GM_PRECISION_MIXED])
+def test_apply_tiles_agree(precision):
+    """Every operator-apply tile (16, 32, 64 nodes) and the autotuned choice
+    give the same steps: per-node arithmetic is identical, only the order of
+    the fused reductions differs."""
+    print(f"apply tiles, precision {precision}", flush=True)
+    spec = dict(MEMBERS[1], precision=precision)
+    runs = {}
+    for tile in (0, 16, 32, 64):
+        s = make(spec)
+        s.set_flags(apply_tile=tile)
+        for n in range(1, 11):
+            s.step(n, diag=False)
+        runs[tile] = s.psi
+        chosen, ms = s.apply_config(precision)
+        assert chosen == (tile or chosen) and chosen in (16, 32, 64)
+        if tile == 0:
+            assert all(v > 0 for v in ms.values())
+    ref = runs[32]
+    for tile, psi in runs.items():
+        assert np.linalg.norm(psi - ref) / np.linalg.norm(ref) < 1e-11, tile
+
+
+def test_apply_tile_flag_is_validated():
+    s = make(MEMBERS[2])
+    with pytest.raises(gm.InvalidArgument):
+        s.set_flags(apply_tile=48)
