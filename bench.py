@@ -278,6 +278,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="untimed steps of the mixed-vs-FP64 comparison (0 skips it)")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="length of the served Poisson stream for p50/p99 step latency (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
@@ -323,6 +325,16 @@ def main():
     }
     if args.accuracy_steps > 0:
         out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
+    if args.serve_seconds > 0:
+        from gm import serve
+        st = serve.served_stream(duration=args.serve_seconds, rate=20.0, seed=0, hbm_budget=160e9, max_active=16)
+        out["p99_served_step_latency_ms"] = st["p99_ms"]
+        out["serving"] = {"arrivals": "Poisson 20/s", "seconds": args.serve_seconds, "submitted": st["submitted"],
+                          "completed": st["completed"], "steps": st["steps"], "p50_ms": st["p50_ms"],
+                          "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+                          "mode_node_steps_per_s": st["mode_node_steps_per_s"], "active_at_end": st["active"],
+                          "queued_at_end": st["queued"], "hbm_budget_GB": 160, "max_active": 16,
+                          "precision": "mixed", "back_reaction": True}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w)

@@ -13,12 +13,15 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -255,31 +258,47 @@ double angular(const gm_solver &S, int j, int k, int ls);
 // angular coefficient of build_coupling_cache (hamiltonian.cpp:77-95).
 void build_gaunt(gm_solver &S)
 {
+    // blocks (l, l1) of [m1][s][m] coefficients; offsets first, then the
+    // blocks in parallel on the host cores (binary128 3j symbols dominate)
     const int Lm = S.l_max + 1;
     S.goff_h.assign(Lm * Lm, 0);
-    S.G_h.clear();
+    size_t total = 0;
     for (int l = 0; l <= S.l_max; ++l)
         for (int l1 = 0; l1 <= S.l_max; ++l1) {
-            S.goff_h[l * Lm + l1] = static_cast<int>(S.G_h.size());
-            const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
-            for (int m1i = 0; m1i <= l1; ++m1i)
-                for (int s = 0; s < Sl; ++s)
-                    for (int mi = 0; mi <= l; ++mi) {
-                        const int m = -l + 2 * mi, m1 = -l1 + 2 * m1i, ls = lo + 2 * s;
-                        const int ms = m1 - m;
-                        double ang = 0.0;
-                        if (std::abs(ms) <= ls) {
-                            const double w1 = gmh::wigner3j(ls, l1, l, 0, 0, 0);
-                            const double w2 = gmh::wigner3j(ls, l1, l, -ms, m1, -m);
-                            double yv = gmh::sph_plm0(ls, std::abs(ms));
-                            if (ms < 0 && std::abs(ms) % 2 != 0) yv = -yv;
-                            const double pref = (m1 % 2 == 0 ? 1.0 : -1.0) *
-                                                std::sqrt((2.0 * l + 1) * (2.0 * l1 + 1) * (2.0 * ls + 1) / (4.0 * M_PI));
-                            ang = pref * w1 * w2 * yv;
-                        }
-                        S.G_h.push_back(ang);
-                    }
+            S.goff_h[l * Lm + l1] = static_cast<int>(total);
+            total += static_cast<size_t>(l1 + 1) * (std::min(l, l1) + 1) * (l + 1);
         }
+    S.G_h.assign(total, 0.0);
+    auto block = [&](int l, int l1) {
+        double *G = S.G_h.data() + S.goff_h[l * Lm + l1];
+        const int Sl = std::min(l, l1) + 1, lo = std::abs(l - l1);
+        for (int s = 0; s < Sl; ++s) {
+            const int ls = lo + 2 * s;
+            const double w1 = gmh::wigner3j(ls, l1, l, 0, 0, 0);
+            const double norm = std::sqrt((2.0 * l + 1) * (2.0 * l1 + 1) * (2.0 * ls + 1) / (4.0 * M_PI));
+            for (int m1i = 0; m1i <= l1; ++m1i)
+                for (int mi = 0; mi <= l; ++mi) {
+                    const int m = -l + 2 * mi, m1 = -l1 + 2 * m1i;
+                    const int ms = m1 - m;
+                    double ang = 0.0;
+                    if (std::abs(ms) <= ls && w1 != 0.0) {
+                        const double w2 = gmh::wigner3j(ls, l1, l, -ms, m1, -m);
+                        double yv = gmh::sph_plm0(ls, std::abs(ms));
+                        if (ms < 0 && std::abs(ms) % 2 != 0) yv = -yv;
+                        ang = (m1 % 2 == 0 ? 1.0 : -1.0) * norm * w1 * w2 * yv;
+                    }
+                    G[(m1i * Sl + s) * (l + 1) + mi] = ang;
+                }
+        }
+    };
+    const int nb = Lm * Lm;
+    const int nt = std::max(1, std::min(nb, static_cast<int>(std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int b = t; b < nb; b += nt) block(b / Lm, b % Lm);
+        });
+    for (auto &th : pool) th.join();
 }
 
 // MFMA tables: 16-row chunks of modes with equal l parity (largest l first),
@@ -358,6 +377,51 @@ void build_mfma_tables(gm_solver &S)
     S.Gdiag_h.assign(static_cast<size_t>(S.K) * Lm, 0.0);
     for (int j = 0; j < S.K; ++j)
         for (int s = 0; s <= S.ml[j]; ++s) S.Gdiag_h[static_cast<size_t>(j) * Lm + s] = angular(S, j, j, 2 * s);
+    S.ntiles = (S.Nr + gmd::TILE - 1) / gmd::TILE;
+}
+
+// The Gaunt and MFMA tables depend on l_max only: built once per process and
+// l_max, then copied into every solver of that l_max.
+struct HostTables {
+    std::vector<int> goff;
+    std::vector<double> G, GM, Gdiag;
+    std::vector<gmd::MChunk> mchunks;
+    std::vector<gmd::MBlock> mblocks;
+    int nchunks = 0;
+};
+std::mutex tables_mu;
+std::map<int, std::shared_ptr<const HostTables>> tables_cache;
+
+void host_tables(gm_solver &S)
+{
+    std::shared_ptr<const HostTables> T;
+    {
+        std::lock_guard<std::mutex> lk(tables_mu);
+        auto it = tables_cache.find(S.l_max);
+        if (it != tables_cache.end()) T = it->second;
+    }
+    if (!T) {
+        build_gaunt(S);
+        build_mfma_tables(S);
+        auto H = std::make_shared<HostTables>();
+        H->goff = S.goff_h;
+        H->G = S.G_h;
+        H->GM = S.GM_h;
+        H->Gdiag = S.Gdiag_h;
+        H->mchunks = S.mchunks_h;
+        H->mblocks = S.mblocks_h;
+        H->nchunks = S.nchunks;
+        std::lock_guard<std::mutex> lk(tables_mu);
+        tables_cache.emplace(S.l_max, H);
+        return;
+    }
+    S.goff_h = T->goff;
+    S.G_h = T->G;
+    S.GM_h = T->GM;
+    S.Gdiag_h = T->Gdiag;
+    S.mchunks_h = T->mchunks;
+    S.mblocks_h = T->mblocks;
+    S.nchunks = T->nchunks;
     S.ntiles = (S.Nr + gmd::TILE - 1) / gmd::TILE;
 }
 
@@ -1150,8 +1214,7 @@ int gm_create(const gm_params *p, gm_solver **out)
         S->hresid.alloc(1);
         S->hwr.alloc(4);
         std::memset(S->hP.p, 0, sizeof(gmd::StepParams));
-        build_gaunt(*S);
-        build_mfma_tables(*S);
+        host_tables(*S);
         alloc_device(*S);
         HIPCK(hipMemsetAsync(S->psi.p, 0, S->nvec() * sizeof(double2), S->st));
         HIPCK(hipStreamSynchronize(S->st));
@@ -1400,6 +1463,10 @@ int gm_begin_run(gm_solver *S, double r_ex, int32_t n_steps)
     });
 }
 
+}  // extern "C"
+
+namespace {
+
 // Autotuner of the operator-apply tile: 16, 32 or 64 radial nodes per
 // workgroup tile (register pressure and occupancy against reuse of the Gaunt
 // fragments and load balance). Each candidate is timed with HIP events on the
@@ -1571,6 +1638,10 @@ void step_finish(gm_solver &S, gm_diag *d)
     }
 }
 
+}  // namespace
+
+extern "C" {
+
 int gm_step(gm_solver *S, int32_t step_id, gm_diag *d)
 {
     (void)step_id;
@@ -1653,6 +1724,89 @@ int gm_get_scalars(const gm_solver *S, gm_scalars *o)
     return GM_OK;
 }
 
+// checkpoint.bin (src/CrankNicolsonModes_restart.cpp:139-233): the magic
+// "GMCKPT01", int32 {step_id, Nr, n_modes}, the 13 doubles t, theta, phase, R,
+// M1, M2, q, R1, R2, Omega, P, dt, p_bin, uint64 psi size, psi as complex128,
+// then the rolling phase average block; written to <path>.tmp and renamed, so
+// a crash mid-write keeps the previous checkpoint.
+constexpr char kCheckpointMagic[8] = {'G', 'M', 'C', 'K', 'P', 'T', '0', '1'};
+
+int gm_save_checkpoint(gm_solver *S, const char *path, int32_t step_id)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        if (!path) throw std::invalid_argument("checkpoint path is NULL");
+        std::vector<double> psi(2 * S->nvec());
+        to_host(*S, S->psi.p, psi.data());
+        const std::string final_path(path), tmp = final_path + ".tmp";
+        {
+            std::ofstream out(tmp, std::ios::binary | std::ios::trunc);
+            if (!out) throw std::runtime_error("Could not write checkpoint " + tmp);
+            out.write(kCheckpointMagic, sizeof(kCheckpointMagic));
+            const int32_t header[3] = {step_id, S->Nr, S->K};
+            out.write(reinterpret_cast<const char *>(header), sizeof(header));
+            const double v[13] = {S->t_i, S->theta, S->phase, S->R, S->M1, S->M2, S->q,
+                                  S->R1, S->R2, S->Omega, S->P, S->dt, S->p_bin};
+            out.write(reinterpret_cast<const char *>(v), sizeof(v));
+            const uint64_t size = S->nvec();
+            out.write(reinterpret_cast<const char *>(&size), sizeof(size));
+            out.write(reinterpret_cast<const char *>(psi.data()), static_cast<std::streamsize>(psi.size() * 8));
+            S->avg->save(out);
+            if (!out) throw std::runtime_error("Could not write checkpoint " + tmp);
+        }
+        if (std::rename(tmp.c_str(), final_path.c_str()) != 0)
+            throw std::runtime_error("Could not replace checkpoint " + final_path);
+    });
+}
+
+int gm_load_checkpoint(gm_solver *S, const char *path, int32_t step_id, int32_t *loaded)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        if (!path || !loaded) throw std::invalid_argument("checkpoint path or result is NULL");
+        *loaded = 0;
+        std::ifstream in(path, std::ios::binary);
+        if (!in) return;
+        char magic[8];
+        int32_t header[3] = {-1, -1, -1};
+        in.read(magic, sizeof(magic));
+        in.read(reinterpret_cast<char *>(header), sizeof(header));
+        if (!in || std::memcmp(magic, kCheckpointMagic, 8) != 0 || header[0] != step_id || header[1] != S->Nr ||
+            header[2] != S->K)
+            return;
+        double v[13];
+        in.read(reinterpret_cast<char *>(v), sizeof(v));
+        uint64_t size = 0;
+        in.read(reinterpret_cast<char *>(&size), sizeof(size));
+        if (!in || size != S->nvec()) return;
+        std::vector<double> psi(2 * size);
+        in.read(reinterpret_cast<char *>(psi.data()), static_cast<std::streamsize>(psi.size() * 8));
+        if (!in) return;
+        auto avg = std::make_unique<gmh::RollingPhaseAverage>(S->flags.power_average_orbits);
+        try {
+            avg->load(in);
+        } catch (const std::exception &) {
+            return;
+        }
+        S->t_i = v[0];
+        S->theta = v[1];
+        S->phase = v[2];
+        S->R = v[3];
+        S->M1 = v[4];
+        S->M2 = v[5];
+        S->q = v[6];
+        S->R1 = v[7];
+        S->R2 = v[8];
+        S->Omega = v[9];
+        S->P = v[10];
+        S->dt = v[11];
+        S->p_bin = v[12];
+        to_dev(*S, psi.data(), S->psi.p);
+        S->avg = std::move(avg);
+        *loaded = 1;
+    });
+}
+
 int gm_set_scalars(gm_solver *S, const gm_scalars *o)
 {
     S->t_i = o->t;
@@ -1709,3 +1863,6 @@ double gm_sink_strength(const gm_solver *S, int32_t body)
 }
 
 }  // extern "C"
+
+// serving loop (gm_server_*): same translation unit, it drives the step phases
+#include "gm_server.inl"

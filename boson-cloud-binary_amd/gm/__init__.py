@@ -31,6 +31,8 @@ def lib():
         _lib.gm_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
         _lib.gm_ensemble_step.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(capi.Diag)]
         _lib.gm_apply_config.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        _lib.gm_save_checkpoint.argtypes = [vp, C.c_char_p, C.c_int32]
+        _lib.gm_load_checkpoint.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]
     return _lib
 
 

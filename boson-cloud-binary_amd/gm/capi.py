@@ -207,6 +207,17 @@ class Solver:
         out.update(arrs)
         return out
 
+    # checkpoint -------------------------------------------------------------
+    def save_checkpoint(self, path, step_id):
+        """save_checkpoint (restart.cpp:139-176) to `path`."""
+        self._check(self._fn("save_checkpoint")(self._h, str(path).encode(), int(step_id)))
+
+    def load_checkpoint(self, path, step_id):
+        """load_checkpoint (restart.cpp:178-233): True when restored."""
+        ok = C.c_int32(0)
+        self._check(self._fn("load_checkpoint")(self._h, str(path).encode(), int(step_id), C.byref(ok)))
+        return bool(ok.value)
+
     # state -----------------------------------------------------------------
     @property
     def psi(self):

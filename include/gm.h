@@ -160,6 +160,15 @@ int gm_set_state(gm_solver *solver, const double *psi);
 int gm_get_scalars(const gm_solver *solver, gm_scalars *out);
 int gm_set_scalars(gm_solver *solver, const gm_scalars *in);
 
+/* save_checkpoint / load_checkpoint (src/CrankNicolsonModes_restart.cpp:139-233):
+ * the reference's checkpoint.bin byte layout at `path` (written to path.tmp,
+ * then renamed). gm_load_checkpoint sets *loaded to 0 and changes nothing when
+ * the file is missing, truncated, or its magic, step_id, Nr or n_modes differ
+ * (load_checkpoint returning false); otherwise restores psi, the 13 orbit
+ * scalars and the rolling power average, and sets *loaded to 1. */
+int gm_save_checkpoint(gm_solver *solver, const char *path, int32_t step_id);
+int gm_load_checkpoint(gm_solver *solver, const char *path, int32_t step_id, int32_t *loaded);
+
 /* Operators on host buffers of Nr*n_modes complex values, with the current H:
  * H*psi (include/CrankNicolsonModes.hpp:197), apply_sink (:210),
  * schrodinger_rhs (:213) and sink_strength (:216). */
@@ -201,6 +210,59 @@ enum {
 };
 int gm_profile_enable(gm_solver *solver, int32_t on);
 int gm_profile_read(const gm_solver *solver, double *ms, int64_t *launches, int32_t ncat);
+
+/* ---------------------------------------------------------------------------
+ * Serving (no counterpart in the reference, whose drivers run one binary per
+ * process): a request loop over device-resident solvers.
+ *
+ * Requests queue in arrival order. gm_server_run admits them while their HBM
+ * footprint (gm_footprint) fits the budget and fewer than max_active run,
+ * creates their solvers (state resident until the request completes), and
+ * advances every active solver through its step as soon as its own stream is
+ * idle: steps of different sizes overlap on the device and requests that
+ * arrive while others run join at once. Results are bitwise those of the same
+ * request run alone. Per-step latency (start of a step to the end of its
+ * host-side finish) is recorded for every step.
+ * ------------------------------------------------------------------------- */
+typedef struct gm_server gm_server;
+
+enum { GM_SERVE_QUEUED = 0, GM_SERVE_RUNNING = 1, GM_SERVE_DONE = 2, GM_SERVE_FAILED = 3 };
+
+typedef struct gm_request {
+    gm_params params;
+    gm_flags flags;
+    const double *psi0;    /* initial state, Nr*n_modes complex (copied at submit) */
+    double mask_L_factor;  /* damping mask (build_damping_mask), used when mask_n > 0 */
+    double mask_W;
+    int32_t mask_n;
+    double r_ex;           /* extraction radius */
+    int32_t n_steps;       /* steps to run */
+} gm_request;
+
+typedef struct gm_serve_stats {
+    int64_t steps;           /* steps completed */
+    double mode_node_steps;  /* sum of K*N over completed steps */
+    double busy_seconds;     /* wall time spent inside gm_server_run */
+    double p50_ms, p99_ms, max_ms;  /* per-step latency */
+    int32_t queued, active;
+    double hbm_in_use;       /* footprint of the active solvers, bytes */
+} gm_serve_stats;
+
+/* Estimated device bytes of one solver with these constructor arguments. */
+int gm_footprint(const gm_params *params, double *bytes);
+int gm_server_create(double hbm_budget_bytes, int32_t max_active, gm_server **out);
+void gm_server_destroy(gm_server *server);
+/* Queue a request; GM_E_INVALID if its footprint exceeds the whole budget. */
+int gm_server_submit(gm_server *server, const gm_request *req, int64_t *ticket);
+/* Run the loop for up to `seconds` of wall time, or until nothing is queued or
+ * active; completed (may be NULL) receives the requests finished in this call. */
+int gm_server_run(gm_server *server, double seconds, int32_t *completed);
+int gm_server_status(const gm_server *server, int64_t ticket, int32_t *state, int32_t *steps_done);
+/* Final state and scalars of a finished request (psi: Nr*n_modes complex; any
+ * pointer may be NULL), then forget it. GM_E_RUNTIME with the message in
+ * error for a failed request; GM_E_LOGIC while it is queued or running. */
+int gm_server_take(gm_server *server, int64_t ticket, double *psi, gm_scalars *sc, char *error, int32_t error_len);
+int gm_server_stats(const gm_server *server, gm_serve_stats *out);
 
 #ifdef __cplusplus
 }
