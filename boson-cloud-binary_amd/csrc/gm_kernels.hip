@@ -52,6 +52,17 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b)
     return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
 __device__ __forceinline__ double cnorm2(double2 a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ __forceinline__ C cone();
+template <> __device__ __forceinline__ double2 cone<double2>() { return make_double2(1.0, 0.0); }
+template <> __device__ __forceinline__ float2 cone<float2>() { return make_float2(1.0f, 0.0f); }
+template <typename C> __device__ __forceinline__ C czero();
+template <> __device__ __forceinline__ double2 czero<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ __forceinline__ float2 czero<float2>() { return make_float2(0.0f, 0.0f); }
 // i * h * a for real h
 __device__ __forceinline__ double2 imul(double h, double2 a) { return make_double2(-h * a.y, h * a.x); }
 
@@ -715,6 +726,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
                 inv = cdiv(make_double2(1.0, 0.0), piv);
             }
             f.inv[(size_t)j * N + i] = inv;
+            if (f.inv32) f.inv32[(size_t)j * N + i] = make_float2((float)inv.x, (float)inv.y);
             prev_inv = inv;
         }
         // carry: the last pivot of the tile, from the block total
@@ -728,22 +740,25 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
 
 // ---------------------------------------------------------------- Thomas solve
 
+template <typename C>
 struct Aff {
-    double2 A, F;
+    C A, F;
 };
 
 // first, then second
-__device__ __forceinline__ Aff compose(const Aff &first, const Aff &second)
+template <typename C>
+__device__ __forceinline__ Aff<C> compose(const Aff<C> &first, const Aff<C> &second)
 {
-    Aff r;
+    Aff<C> r;
     r.A = cmul(second.A, first.A);
     r.F = cadd(cmul(second.A, first.F), second.F);
     return r;
 }
 
-__device__ __forceinline__ Aff shfl_aff(const Aff &v, int src)
+template <typename C>
+__device__ __forceinline__ Aff<C> shfl_aff(const Aff<C> &v, int src)
 {
-    Aff r;
+    Aff<C> r;
     r.A.x = __shfl(v.A.x, src);
     r.A.y = __shfl(v.A.y, src);
     r.F.x = __shfl(v.F.x, src);
@@ -754,31 +769,31 @@ __device__ __forceinline__ Aff shfl_aff(const Aff &v, int src)
 // Exclusive scan of affine maps over the PT threads of a block in the order
 // of increasing (REV = false) or decreasing (REV = true) thread index.
 // Returns the thread's exclusive prefix; `total` receives the block total.
-template <bool REV>
-__device__ Aff block_scan(Aff v, Aff &total)
+template <bool REV, typename C>
+__device__ Aff<C> block_scan(Aff<C> v, Aff<C> &total)
 {
-    __shared__ Aff wt[gmd::PT / 64];
+    __shared__ Aff<C> wt[gmd::PT / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int vl = REV ? 63 - lane : lane;  // position in scan order within the wave
-    Aff inc = v;
+    Aff<C> inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const int src = REV ? lane + d : lane - d;
-        const Aff o = shfl_aff(inc, src < 0 ? 0 : (src > 63 ? 63 : src));
+        const Aff<C> o = shfl_aff(inc, src < 0 ? 0 : (src > 63 ? 63 : src));
         if (vl >= d) inc = compose(o, inc);
     }
-    Aff ex = shfl_aff(inc, REV ? (lane < 63 ? lane + 1 : 63) : (lane > 0 ? lane - 1 : 0));
+    Aff<C> ex = shfl_aff(inc, REV ? (lane < 63 ? lane + 1 : 63) : (lane > 0 ? lane - 1 : 0));
     if (vl == 0) {
-        ex.A = make_double2(1.0, 0.0);
-        ex.F = make_double2(0.0, 0.0);
+        ex.A = cone<C>();
+        ex.F = czero<C>();
     }
     if (vl == 63) wt[wv] = inc;
     __syncthreads();
     const int nw = gmd::PT / 64;
-    Aff pre;
-    pre.A = make_double2(1.0, 0.0);
-    pre.F = make_double2(0.0, 0.0);
-    Aff tot = pre;
+    Aff<C> pre;
+    pre.A = cone<C>();
+    pre.F = czero<C>();
+    Aff<C> tot = pre;
     for (int k = 0; k < nw; ++k) {
         const int w = REV ? nw - 1 - k : k;
         if (w == wv) pre = tot;
@@ -791,19 +806,27 @@ __device__ Aff block_scan(Aff v, Aff &total)
 
 __device__ __forceinline__ int lds_slot(int pos) { return pos + pos / gmd::PE; }
 
-template <int VAR>
+template <int VAR, typename RT>
 __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
 {
+    // RT = float: the two sweeps in FP32 on FP32 pivots (the forward result in
+    // an FP32 scratch vector); the p / s updates and the output stay FP64
+    typedef typename Arith<RT>::C C;
+    constexpr bool F32 = std::is_same<RT, float>::value;
     Krylov *Kr = a.Kr;
     if (VAR != PV_PLAIN && Kr->done) return;
     constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
-    __shared__ double2 sa[TE + TE / E];
-    __shared__ double2 sf[TE + TE / E];
+    __shared__ C sa[TE + TE / E];
+    __shared__ C sf[TE + TE / E];
     const int j = blockIdx.x;
     const int N = a.N;
     const int tid = threadIdx.x;
     const size_t base_j = (size_t)j * N;
-    const double2 ell = make_double2(0.0, a.half_dt * a.off);
+    const C ell = Arith<RT>::from(make_double2(0.0, a.half_dt * a.off));
+    auto pivot = [&](size_t q) -> C {
+        if constexpr (F32) return a.inv32[q];
+        else return a.inv[q];
+    };
     const int ntile = (N + TE - 1) / TE;
     double2 beta = make_double2(0, 0), wv = make_double2(0, 0), alpha = make_double2(0, 0);
     if (VAR == PV_P) {
@@ -812,14 +835,14 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     }
     if (VAR == PV_S) alpha = Kr->alpha;
     // forward sweep
-    double2 carry = make_double2(0.0, 0.0);
+    C carry = czero<C>();
     for (int t = 0; t < ntile; ++t) {
         const int t0 = t * TE;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int pos = tid + T * e;
             const int idx = t0 + pos;
-            double2 av = make_double2(1.0, 0.0), fv = make_double2(0.0, 0.0);
+            C av = cone<C>(), fv = czero<C>();
             if (idx < N) {
                 const size_t q = base_j + idx;
                 double2 in;
@@ -832,19 +855,19 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
                 } else {
                     in = a.X[q];
                 }
-                const double2 iv = a.inv[q];
-                fv = cmul(iv, in);
-                av = (idx >= 1 && idx <= N - 2) ? make_double2(-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x))
-                                                : make_double2(0.0, 0.0);
+                const C iv = pivot(q);
+                fv = cmul(iv, Arith<RT>::from(in));
+                av = (idx >= 1 && idx <= N - 2) ? C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)}
+                                                : czero<C>();
             }
             sa[lds_slot(pos)] = av;
             sf[lds_slot(pos)] = fv;
         }
         __syncthreads();
-        Aff loc;
-        loc.A = make_double2(1.0, 0.0);
-        loc.F = make_double2(0.0, 0.0);
-        double2 ae[E], fe[E];
+        Aff<C> loc;
+        loc.A = cone<C>();
+        loc.F = czero<C>();
+        C ae[E], fe[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             ae[e] = sa[lds_slot(tid * E + e)];
@@ -852,9 +875,9 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             loc.A = cmul(ae[e], loc.A);
             loc.F = cadd(cmul(ae[e], loc.F), fe[e]);
         }
-        Aff tot;
-        const Aff ex = block_scan<false>(loc, tot);
-        double2 y = cadd(cmul(ex.A, carry), ex.F);
+        Aff<C> tot;
+        const Aff<C> ex = block_scan<false>(loc, tot);
+        C y = cadd(cmul(ex.A, carry), ex.F);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             y = cadd(cmul(ae[e], y), fe[e]);
@@ -866,35 +889,39 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
         for (int e = 0; e < E; ++e) {
             const int pos = tid + T * e;
             const int idx = t0 + pos;
-            if (idx < N) a.Y[base_j + idx] = sf[lds_slot(pos)];
+            if (idx < N) {
+                if constexpr (F32) a.Y32[base_j + idx] = sf[lds_slot(pos)];
+                else a.Y[base_j + idx] = sf[lds_slot(pos)];
+            }
         }
         __syncthreads();
     }
     // backward sweep
     int nz_lo = N, nz_hi = 0;
-    carry = make_double2(0.0, 0.0);
+    carry = czero<C>();
     for (int t = ntile - 1; t >= 0; --t) {
         const int t0 = t * TE;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int pos = tid + T * e;
             const int idx = t0 + pos;
-            double2 cv = make_double2(1.0, 0.0), fv = make_double2(0.0, 0.0);
+            C cv = cone<C>(), fv = czero<C>();
             if (idx < N) {
                 const size_t q = base_j + idx;
-                const double2 iv = a.inv[q];
-                fv = a.Y[q];
-                cv = (idx >= 1 && idx <= N - 2) ? make_double2(-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x))
-                                                : make_double2(0.0, 0.0);
+                const C iv = pivot(q);
+                if constexpr (F32) fv = a.Y32[q];
+                else fv = a.Y[q];
+                cv = (idx >= 1 && idx <= N - 2) ? C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)}
+                                                : czero<C>();
             }
             sa[lds_slot(pos)] = cv;
             sf[lds_slot(pos)] = fv;
         }
         __syncthreads();
-        Aff loc;
-        loc.A = make_double2(1.0, 0.0);
-        loc.F = make_double2(0.0, 0.0);
-        double2 ce[E], fe[E];
+        Aff<C> loc;
+        loc.A = cone<C>();
+        loc.F = czero<C>();
+        C ce[E], fe[E];
 #pragma unroll
         for (int e = E - 1; e >= 0; --e) {
             ce[e] = sa[lds_slot(tid * E + e)];
@@ -902,9 +929,9 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             loc.A = cmul(ce[e], loc.A);
             loc.F = cadd(cmul(ce[e], loc.F), fe[e]);
         }
-        Aff tot;
-        const Aff ex = block_scan<true>(loc, tot);
-        double2 y = cadd(cmul(ex.A, carry), ex.F);
+        Aff<C> tot;
+        const Aff<C> ex = block_scan<true>(loc, tot);
+        C y = cadd(cmul(ex.A, carry), ex.F);
 #pragma unroll
         for (int e = E - 1; e >= 0; --e) {
             y = cadd(cmul(ce[e], y), fe[e]);
@@ -917,7 +944,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             const int pos = tid + T * e;
             const int idx = t0 + pos;
             if (idx < N) {
-                const double2 yv = sf[lds_slot(pos)];
+                const double2 yv = Arith<RT>::to_d(sf[lds_slot(pos)]);
                 a.Y[base_j + idx] = yv;
                 if (VAR == PV_PLAIN && a.range && (yv.x != 0.0 || yv.y != 0.0)) {
                     nz_lo = min(nz_lo, idx);
@@ -1342,9 +1369,12 @@ hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st)
 {
-    if (var == PV_P) hipLaunchKernelGGL(k_precond<PV_P>, dim3(a.K), dim3(gmd::PT), 0, st, a);
-    else if (var == PV_S) hipLaunchKernelGGL(k_precond<PV_S>, dim3(a.K), dim3(gmd::PT), 0, st, a);
-    else hipLaunchKernelGGL(k_precond<PV_PLAIN>, dim3(a.K), dim3(gmd::PT), 0, st, a);
+    const bool f32 = a.inv32 != nullptr && a.Y32 != nullptr && var != PV_PLAIN;
+    if (var == PV_P && f32) hipLaunchKernelGGL((k_precond<PV_P, float>), dim3(a.K), dim3(gmd::PT), 0, st, a);
+    else if (var == PV_S && f32) hipLaunchKernelGGL((k_precond<PV_S, float>), dim3(a.K), dim3(gmd::PT), 0, st, a);
+    else if (var == PV_P) hipLaunchKernelGGL((k_precond<PV_P, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
+    else if (var == PV_S) hipLaunchKernelGGL((k_precond<PV_S, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
+    else hipLaunchKernelGGL((k_precond<PV_PLAIN, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
     return hipGetLastError();
 }
 

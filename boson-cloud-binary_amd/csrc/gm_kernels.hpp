@@ -37,6 +37,7 @@ struct FactorArgs {
     const double *E;
     const double2 *cap;
     double2 *inv;
+    float2 *inv32;        // FP32 copy of the pivots, or nullptr
     int *failed;
 };
 
@@ -54,6 +55,8 @@ struct PrecArgs {
     gmd::SinkDev sk[2];
     const gmd::StepParams *Ps;
     int *range;          // PV_PLAIN: atomically widened [lo, hi) of non-zero output nodes, or nullptr
+    const float2 *inv32; // FP32 pivots: with Y32, the p / s variants sweep in FP32 (mixed precision)
+    float2 *Y32;         // FP32 forward-sweep scratch
 };
 
 struct WcorrArgs {

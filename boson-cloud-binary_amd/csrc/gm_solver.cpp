@@ -144,6 +144,7 @@ struct gm_solver {
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
     DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, inv, tmp;
+    DevBuf<float2> inv32, y32;  // FP32 pivots and sweep scratch (mixed-precision preconditioner)
     DevBuf<double> partial, rowpart, diagout;
     DevBuf<double2> dpsi_ix;
     DevBuf<gmd::StepParams> dP;
@@ -690,6 +691,7 @@ void factorize(gm_solver &S)
     f.E = S.E.p;
     f.cap = S.cap ? S.dcap.p : nullptr;
     f.inv = S.inv.p;
+    f.inv32 = S.inv32.p;
     f.failed = S.dflags.p;
     launch(S, GM_PROF_FACTOR, [&] { return gmk::launch_factor(f, S.st); });
     std::memcpy(S.fac_key, key, sizeof(key));
@@ -766,6 +768,11 @@ void enqueue_iteration(gm_solver &S, bool wood)
 {
     gmk::PrecArgs pa = prec_args(S);
     pa.wood = wood;
+    if (S.flags.precision == GM_PRECISION_MIXED) {
+        // mixed precision: the Thomas sweeps in FP32 on FP32 pivots
+        pa.inv32 = S.inv32.p;
+        pa.Y32 = S.y32.p;
+    }
     // y = M^-1 p with p = r + beta (p - w v)
     pa.R = S.rr.p;
     pa.Pv = S.p.p;
@@ -1037,6 +1044,8 @@ void alloc_device(gm_solver &S)
 {
     const size_t n = S.nvec();
     for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.ax, &S.inv, &S.tmp})
+        if (bf->n != n) bf->alloc(n);
+    for (auto *bf : {&S.inv32, &S.y32})
         if (bf->n != n) bf->alloc(n);
     S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.E32.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
