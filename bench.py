@@ -210,7 +210,7 @@ def rooflines(s, prof, K, N, iters, mixed):
             "kernel": name, "dtype": dt, "per_launch": flops, "per_launch_unit": "flop",
             "avg_launch_ms": avg_s * 1e3, "launches": cnt}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = measured_traffic("k_apply") if not mixed else None
+    roof["traffic"] = measured_traffic("k_apply_fp32" if mixed else "k_apply")
     kernels = {}
     working = {"precond": min(prof["precond"][1], 2 * int(sum(iters))),
                "update": min(prof["update"][1], int(sum(iters)))}
