@@ -258,7 +258,7 @@ __device__ __forceinline__ void acc_update(typename Arith<T>::C (&acc)[NG][4], i
 }
 
 template <int NG, typename T>
-__device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const StepParams *P, const double2 *X, int t0,
+__device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const typename Arith<T>::C *Xt, int t0,
                                           const MBlock &B, int kb, int l1, int p, int s0, int nTe, const T *sE,
                                           typename Arith<T>::C (&acc)[NG][4])
 {
@@ -280,15 +280,15 @@ __device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const 
     T fr[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) fr[k] = k <= l1 ? Gb[k * mstride] : T(0);
-    // column vectors x_(l1,m1) in a ring of two columns ahead, phased by
-    // exp(i m1 theta) at use; nodes past the grid repeat node N-1 (they only
-    // feed accumulators never stored)
+    // phased column vectors exp(i m1 theta) x_(l1,m1) in a ring of two columns
+    // ahead; nodes past the grid repeat node N-1 (they only feed accumulators
+    // never stored)
     const int N = g.N;
-    const double2 *__restrict__ xg = X + (size_t)kb * N;
+    const C *__restrict__ xg = Xt + (size_t)kb * N;
     int xo[NG];
 #pragma unroll
     for (int n = 0; n < NG; ++n) xo[n] = min(t0 + 16 * n + col, N - 1);
-    double2 xr[2][NG];
+    C xr[2][NG];
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -300,9 +300,8 @@ __device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const 
         const T fc = fr[k];
         if (pf) fr[k] = Gb[min(m1i + 4, l1) * mstride];
         C xv[NG];
-        const double2 ph = P->ph[g.lmax - l1 + 2 * m1i];
 #pragma unroll
-        for (int n = 0; n < NG; ++n) xv[n] = AR::from(cmul(ph, xr[k & 1][n]));
+        for (int n = 0; n < NG; ++n) xv[n] = xr[k & 1][n];
         if (pf || m1i + 2 <= l1) {
 #pragma unroll
             for (int n = 0; n < NG; ++n) xr[k & 1][n] = xg[(size_t)min(m1i + 2, l1) * N + xo[n]];
@@ -404,7 +403,8 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                 const int nTe = max(0, min(B.nT, tmax - B.tlo + 1));
                 const int kb = l1 * (l1 + 1) / 2;
                 for (int s0 = 0; s0 < nTe; s0 += 4)
-                    mfma_pass<NG, T>(g, GM, P, a.X, t0, B, kb, l1, p, s0, nTe, sE, acc);
+                    mfma_pass<NG, T>(g, GM, static_cast<const typename AR::C *>(a.Xt), t0, B, kb, l1, p, s0,
+                                     nTe, sE, acc);
             }
         }
     }
@@ -1318,6 +1318,24 @@ int apply_grid(int N, int nchunks, int ng, int *ngroups)
     return (ntiles + 7) / 8 * 8 * *ngroups;
 }
 
+// Column vectors of the coupling, exp(i m theta) x in the arithmetic of the
+// apply (FP64, or FP32 for the mixed Krylov applies): one row of blocks per mode.
+template <typename C>
+__global__ __launch_bounds__(256) void k_phase(const double2 *X, C *Xt, int N, int lmax, const StepParams *P)
+{
+    const int j = blockIdx.y;
+    const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
+    const int l = (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
+    const int m = 2 * (j - l * (l + 1) / 2) - l;
+    const double2 ph = P->ph[m + lmax];
+    const size_t base = (size_t)j * N;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double2 v = cmul(ph, X[base + i]);
+        if constexpr (std::is_same<C, float2>::value) Xt[base + i] = make_float2((float)v.x, (float)v.y);
+        else Xt[base + i] = v;
+    }
+}
+
 template <int NG>
 static hipError_t launch_apply_ng(const ApplyArgs &a, int grid, hipStream_t st)
 {
@@ -1331,8 +1349,18 @@ static hipError_t launch_apply_ng(const ApplyArgs &a, int grid, hipStream_t st)
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
     if (a.ng != 1 && a.ng != 2 && a.ng != 4) return hipErrorInvalidValue;
+    if (!a.Xt) return hipErrorInvalidValue;
     const int grid = apply_grid(a.g.N, a.g.nchunks, a.ng, &a.ngroups);
     a.mode = mode;
+    {
+        const dim3 pg((a.g.N + 255) / 256 < 16 ? (a.g.N + 255) / 256 : 16, a.g.K);
+        if (a.mixed)
+            hipLaunchKernelGGL(k_phase<float2>, pg, dim3(256), 0, st, a.X, (float2 *)a.Xt, a.g.N, a.g.lmax, a.P);
+        else
+            hipLaunchKernelGGL(k_phase<double2>, pg, dim3(256), 0, st, a.X, (double2 *)a.Xt, a.g.N, a.g.lmax, a.P);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     if (a.ng == 1) return launch_apply_ng<1>(a, grid, st);
     if (a.ng == 2) return launch_apply_ng<2>(a, grid, st);
     return launch_apply_ng<4>(a, grid, st);

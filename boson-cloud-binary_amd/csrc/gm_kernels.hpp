@@ -27,6 +27,7 @@ struct ApplyArgs {
     int ngroups;         // chunk groups per tile, set by the launcher
     int mixed;           // coupling in FP32 (E32, GM32) with FP32 accumulation
     int ng;              // 16-node groups per tile (1, 2 or 4), chosen by the autotuner
+    void *Xt;            // scratch: exp(i m theta) X, double2 (FP64) or float2 (mixed), K * N
     int mode;            // ApplyMode, set by the launcher
 };
 

@@ -145,6 +145,7 @@ struct gm_solver {
     DevBuf<double2> dcap;
     DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, inv, tmp;
     DevBuf<float2> inv32, y32;  // FP32 pivots and sweep scratch (mixed-precision preconditioner)
+    DevBuf<double2> xt;         // phased column vectors of the coupling (FP64; FP32 uses the first half)
     DevBuf<double> partial, rowpart, diagout;
     DevBuf<double2> dpsi_ix;
     DevBuf<gmd::StepParams> dP;
@@ -656,6 +657,7 @@ gmk::ApplyArgs apply_args(gm_solver &S, const double2 *X, double2 *Y)
     a.rowpart = S.rowpart.p;
     a.dpsi_ix = S.dpsi_ix.p;
     a.ng = S.apply_ng[0] ? S.apply_ng[0] : 2;
+    a.Xt = S.xt.p;
     return a;
 }
 
@@ -1047,6 +1049,7 @@ void alloc_device(gm_solver &S)
         if (bf->n != n) bf->alloc(n);
     for (auto *bf : {&S.inv32, &S.y32})
         if (bf->n != n) bf->alloc(n);
+    if (S.xt.n != n) S.xt.alloc(n);
     S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.E32.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.lcut.alloc(S.ntiles);
