@@ -66,6 +66,7 @@ struct SinkDev {
 // replay a step with new values.
 struct StepParams {
     double2 ph[2 * MAXLM + 1];  // exp(i m theta), index m + l_max
+    double2 wrot[2 * MAXLM + 1];  // exp(-i m (theta - theta_W)): Woodbury columns computed at theta_W
     double half_dt;             // dt / 2
     double dt;
     double g[2];                // sink strengths g_a (0 when inactive)

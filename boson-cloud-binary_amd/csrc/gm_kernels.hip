@@ -81,6 +81,14 @@ __device__ __forceinline__ double2 vload2(const double2 *p)
     return make_double2(q[0], q[1]);
 }
 
+// Azimuthal number m of mode j (modes ordered by l, then m = -l, -l+2, ..., l).
+__device__ __forceinline__ int mode_m(int j)
+{
+    const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
+    const int l = (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
+    return 2 * (j - l * (l + 1) / 2) - l;
+}
+
 // Sum NQ per-thread values over the block; every block stores its totals in
 // partial[q * nblocks + block]. Returns true in the last block to finish, whose
 // threads then see every partial. blockDim.x must be a multiple of 64, <= 1024.
@@ -1017,7 +1025,8 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             const int k = q / sk.n, d = q - k * sk.n;
             const size_t gi = (size_t)k * N + sk.lo + d;
             double2 y = vload2(a.Y + gi);
-            for (int u = 0; u < ns; ++u) y = csub(y, cmul(c[u], a.sk[u].W[gi]));
+            const double2 rot = P->wrot[mode_m(k) + a.lmax];
+            for (int u = 0; u < ns; ++u) y = csub(y, cmul(cmul(c[u], rot), a.sk[u].W[gi]));
             const double2 d2 = cdotm(sk.v[q], y);
             v[2 * s] += d2.x;
             v[2 * s + 1] += d2.y;
@@ -1186,8 +1195,9 @@ __global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
                 v[2 * s] += d2.x;
                 v[2 * s + 1] += d2.y;
             } else {
+                const double2 rot = P->wrot[mode_m(k) + a.lmax];
                 for (int u = 0; u < ns; ++u) {
-                    const double2 d2 = cdotm(sk.v[q], a.sk[u].W[gi]);
+                    const double2 d2 = cdotm(sk.v[q], cmul(rot, a.sk[u].W[gi]));
                     v[4 * s + 2 * u] += d2.x;
                     v[4 * s + 2 * u + 1] += d2.y;
                 }
@@ -1261,10 +1271,10 @@ __global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a)
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         const int k = (int)(q / width), i = lo + (int)(q - (size_t)k * width);
         const size_t gi = (size_t)k * a.N + i;
-        double2 y = a.Y[gi];
-        y = csub(y, cmul(c0, a.W0[gi]));
-        if (ns > 1) y = csub(y, cmul(c1, a.W1[gi]));
-        a.Y[gi] = y;
+        // stored columns W(theta_W); W(theta) = exp(-i m (theta - theta_W)) W(theta_W)
+        double2 w = cmul(c0, a.W0[gi]);
+        if (ns > 1) w = cadd(w, cmul(c1, a.W1[gi]));
+        a.Y[gi] = csub(a.Y[gi], cmul(a.P->wrot[mode_m(k) + a.lmax], w));
     }
 }
 

@@ -56,12 +56,14 @@ struct PrecArgs {
     gmd::SinkDev sk[2];
     const gmd::StepParams *Ps;
     int *range;          // PV_PLAIN: atomically widened [lo, hi) of non-zero output nodes, or nullptr
+    int lmax;            // Woodbury epilogue: mode -> m for the column rotation
     const float2 *inv32; // FP32 pivots: with Y32, the p / s variants sweep in FP32 (mixed precision)
     float2 *Y32;         // FP32 forward-sweep scratch
 };
 
 struct WcorrArgs {
-    int N, K, nsink;
+    int N, K, nsink, lmax;
+    const gmd::StepParams *P;  // wrot: rotation of the stored Woodbury columns
     const gmd::Krylov *Kr;
     const double2 *W0, *W1;
     double2 *Y;
@@ -99,7 +101,7 @@ struct AcceptArgs {
 };
 
 struct SinkArgs {
-    int N, K;
+    int N, K, lmax;
     int mode, dest;
     double scale;
     int scale_by_nrm;    // multiply by |rhs| (device value)

@@ -165,6 +165,10 @@ struct gm_solver {
     bool fac_valid = false;
     double fac_key[3] = {0, 0, 0};
     int fac_cap_version = -1;
+    // Woodbury columns W_a = P^-1 v_a stored at angle w_theta (see build_perturbation)
+    bool w_valid = false, w_dirty = true;
+    double w_key[14] = {};
+    double w_theta = 0.0;
     int prev_iters = 4;
     // progress of the step in flight (step_begin / solve_advance / step_finish)
     double step_start = 0.0;
@@ -626,6 +630,28 @@ void build_perturbation(gm_solver &S, double time)
                 P.g[P.nsink] = S.sink[bdy].strength;
                 P.nsink++;
             }
+    // Woodbury columns: P is block-diagonal in the modes and v_a(theta) =
+    // D(theta) v_a(0) with D = diag(exp(-i m_j theta)) (absorption.cpp:127-151),
+    // so W_a(theta) = D(theta - theta_W) W_a(theta_W) and V^H W does not depend
+    // on theta. The columns are solved again only when P (dt, profiles, CAP) or
+    // an absorber support changes; otherwise the kernels rotate the stored ones.
+    {
+        double wk[14] = {S.dt, S.E_key[0], S.E_key[1], S.E_key[2], S.E_key[3], S.E_key[4], S.E_key[5],
+                         static_cast<double>(S.cap_version), static_cast<double>(P.nsink),
+                         static_cast<double>(P.sink_body[0] + 2 * P.sink_body[1]), 0, 0, 0, 0};
+        for (int s = 0; s < P.nsink; ++s) {
+            const Sink &sk = S.sink[P.sink_body[s]];
+            wk[10 + 2 * s] = sk.lo;
+            wk[11 + 2 * s] = sk.n;
+        }
+        S.w_dirty = P.nsink > 0 && (!S.w_valid || std::memcmp(wk, S.w_key, sizeof(wk)) != 0);
+        if (S.w_dirty) {
+            std::memcpy(S.w_key, wk, sizeof(wk));
+            S.w_theta = S.theta;
+        }
+        for (int m = -S.l_max; m <= S.l_max; ++m)
+            P.wrot[m + S.l_max] = make_double2(std::cos(m * (S.theta - S.w_theta)), -std::sin(m * (S.theta - S.w_theta)));
+    }
     HIPCK(hipMemcpyAsync(S.dP.p, S.hP.p, sizeof(gmd::StepParams), hipMemcpyHostToDevice, S.st));
 }
 
@@ -672,6 +698,7 @@ gmk::PrecArgs prec_args(gm_solver &S)
     a.Kr = S.dK.p;
     fill_sinks(S, a.sk);
     a.Ps = S.dP.p;
+    a.lmax = S.l_max;
     return a;
 }
 
@@ -726,25 +753,29 @@ void woodbury_setup(gm_solver &S)
     S.hwr.p[0] = S.hwr.p[2] = S.Nr;
     S.hwr.p[1] = S.hwr.p[3] = 0;
     gmd::Krylov *dk = S.dK.p;
-    HIPCK(hipMemcpyAsync(&dk->wrange[0], S.hwr.p, 4 * sizeof(int), hipMemcpyHostToDevice, S.st));
-    for (int s = 0; s < P.nsink; ++s) {
-        Sink &sk = S.sink[P.sink_body[s]];
-        if (sk.W.n != S.nvec()) sk.W.alloc(S.nvec());
-        gmd::SinkDev sd{};
-        sd.lo = sk.lo;
-        sd.n = sk.n;
-        sd.v = sk.dv.p;
-        launch(S, GM_PROF_SMALL, [&] { return gmk::launch_expand(S.Nr, S.K, sd, S.tmp.p, S.st); });
-        pa.X = S.tmp.p;
-        pa.Y = sk.W.p;
-        pa.wood = 0;
-        pa.range = &dk->wrange[2 * s];
-        launch(S, GM_PROF_PRECOND_SETUP, [&] { return gmk::launch_precond(pa, gmk::PV_PLAIN, S.st); });
+    if (S.w_dirty) {
+        HIPCK(hipMemcpyAsync(&dk->wrange[0], S.hwr.p, 4 * sizeof(int), hipMemcpyHostToDevice, S.st));
+        for (int s = 0; s < P.nsink; ++s) {
+            Sink &sk = S.sink[P.sink_body[s]];
+            if (sk.W.n != S.nvec()) sk.W.alloc(S.nvec());
+            gmd::SinkDev sd{};
+            sd.lo = sk.lo;
+            sd.n = sk.n;
+            sd.v = sk.dv.p;
+            launch(S, GM_PROF_SMALL, [&] { return gmk::launch_expand(S.Nr, S.K, sd, S.tmp.p, S.st); });
+            pa.X = S.tmp.p;
+            pa.Y = sk.W.p;
+            pa.wood = 0;
+            pa.range = &dk->wrange[2 * s];
+            launch(S, GM_PROF_PRECOND_SETUP, [&] { return gmk::launch_precond(pa, gmk::PV_PLAIN, S.st); });
+        }
+        S.w_valid = true;
     }
     gmk::SinkArgs a{};
     a.N = S.Nr;
     a.K = S.K;
     a.mode = gmk::SK_SETUP;
+    a.lmax = S.l_max;
     fill_sinks(S, a.sk);
     a.Kr = S.dK.p;
     a.P = S.dP.p;
@@ -763,6 +794,8 @@ void woodbury_correct(gm_solver &S, double2 *Y)
     w.W0 = S.sink[P.sink_body[0]].W.p;
     w.W1 = P.nsink > 1 ? S.sink[P.sink_body[1]].W.p : nullptr;
     w.Y = Y;
+    w.P = S.dP.p;
+    w.lmax = S.l_max;
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_wcorr(w, S.st); });
 }
 
@@ -839,6 +872,7 @@ std::vector<uint64_t> graph_key(const gm_solver &S, int batch)
     k.push_back(static_cast<uint64_t>(S.flags.precision));
     k.push_back(S.mask);
     k.push_back(static_cast<uint64_t>(S.apply_ng[0] | (S.apply_ng[1] << 8)));
+    k.push_back(S.w_dirty);
     put_d(S.dt);
     gmd::SinkDev sk[2];
     fill_sinks(S, sk);
