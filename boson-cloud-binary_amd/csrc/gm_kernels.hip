@@ -837,6 +837,8 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     };
     const int ntile = (N + TE - 1) / TE;
     double2 beta = make_double2(0, 0), wv = make_double2(0, 0), alpha = make_double2(0, 0);
+    // first iteration: p = r + beta (p - w v) with p = v = 0 is r
+    const bool first = VAR == PV_P && Kr->iter == 0;
     if (VAR == PV_P) {
         beta = Kr->beta;
         wv = Kr->w;
@@ -855,7 +857,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
                 const size_t q = base_j + idx;
                 double2 in;
                 if (VAR == PV_P) {
-                    in = cadd(a.R[q], cmul(beta, csub(a.Pv[q], cmul(wv, a.V[q]))));
+                    in = first ? a.R[q] : cadd(a.R[q], cmul(beta, csub(a.Pv[q], cmul(wv, a.V[q]))));
                     a.Pv[q] = in;
                 } else if (VAR == PV_S) {
                     in = csub(a.R[q], cmul(alpha, a.V[q]));
@@ -1116,8 +1118,7 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a)
             const double2 r = make_double2((rhs.x - ax.x) / nrm, (rhs.y - ax.y) / nrm);
             a.R[q] = r;
             a.R0[q] = r;
-            a.Pv[q] = make_double2(0.0, 0.0);
-            a.V[q] = make_double2(0.0, 0.0);
+            // p = v = 0 (Eigen BiCGSTAB): the first p-update reads neither
             v[0] += cnorm2(r);
             v[1] += cnorm2(b);
         }
