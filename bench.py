@@ -278,6 +278,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="untimed steps of the mixed-vs-FP64 comparison (0 skips it)")
+    ap.add_argument("--ensemble-members", type=int, default=64,
+                    help="members of the batched ragged ensemble measurement (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the served Poisson stream for p50/p99 step latency (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -325,6 +327,13 @@ def main():
     }
     if args.accuracy_steps > 0:
         out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
+    if args.ensemble_members > 0:
+        from gm import ensemble
+        ens = ensemble.run(n=args.ensemble_members, steps=10, warmup=3, seed=0)
+        ens["config"] = ("ragged fixed-orbit binaries from seed 0: q log-uniform [0.05, 1], R [20, 60], "
+                         "mu [0.1, 0.3], N in {4096..32768} (L = N/4), l_max in {8..24}; mixed precision")
+        ens["over_fp64_single_step"] = ens["mode_node_steps_per_s"] / f64["value"]
+        out["ensemble"] = ens
     if args.serve_seconds > 0:
         from gm import serve
         st = serve.served_stream(duration=args.serve_seconds, rate=20.0, seed=0, hbm_budget=160e9, max_active=16)
