@@ -1853,6 +1853,44 @@ int gm_load_checkpoint(gm_solver *S, const char *path, int32_t step_id, int32_t 
     });
 }
 
+// reconstruct_state_from_diagnostics (src/CrankNicolsonModes_restart.cpp:238-285).
+int gm_restore_from_diagnostics(gm_solver *S, int32_t n_rows, const double *rows)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        if (n_rows < 0 || (n_rows > 0 && !rows)) throw std::invalid_argument("diagnostic rows are missing");
+        // before step 1 Omega is what the perturbation at t = 0 set
+        double Omega_previous = S->isolated ? 0.0 : (S->ramp_end_time() > 0.0 ? 0.0 : S->Omega_0);
+        double t_previous = 0.0;
+        S->phase = 0.0;
+        S->avg->reset();
+        for (int k = 0; k < n_rows; ++k) {
+            const double *row = rows + 8 * static_cast<size_t>(k);
+            S->phase += Omega_previous * (row[0] - t_previous);
+            if (!S->isolated) S->avg->update(row[0], S->phase, row[7]);
+            Omega_previous = row[3];
+            t_previous = row[0];
+        }
+        if (n_rows == 0) return;
+        const double *last = rows + 8 * static_cast<size_t>(n_rows - 1);
+        S->t_i = last[0];
+        S->R = last[1];
+        S->theta = last[2];
+        S->M1 = last[4];
+        S->q = last[5];
+        S->M2 = S->q * S->M1;
+        S->R1 = S->M2 / (S->M1 + S->M2) * S->R;
+        S->R2 = S->M1 / (S->M1 + S->M2) * S->R;
+        S->Omega = last[3];
+        S->p_bin = last[6];
+        // past the ramp an evolving orbit sets P and dt = P / steps_per_period
+        if (!S->isolated && S->flags.evolve_orbit && S->t_i > S->ramp_end_time()) {
+            S->P = 2.0 * M_PI / S->Omega;
+            S->dt = S->P / S->flags.steps_per_period;
+        }
+    });
+}
+
 int gm_set_scalars(gm_solver *S, const gm_scalars *o)
 {
     S->t_i = o->t;

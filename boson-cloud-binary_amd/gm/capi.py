@@ -101,6 +101,9 @@ def bind(lib, prefix):
     f("apply_sink", C.c_int, vp, _DP, _DP)
     f("schrodinger_rhs", C.c_int, vp, _DP, _DP)
     f("sink_strength", C.c_double, vp, C.c_int32)
+    f("save_checkpoint", C.c_int, vp, C.c_char_p, C.c_int32)
+    f("load_checkpoint", C.c_int, vp, C.c_char_p, C.c_int32, C.POINTER(C.c_int32))
+    f("restore_from_diagnostics", C.c_int, vp, C.c_int32, _DP)
     return lib
 
 
@@ -135,6 +138,7 @@ class Solver:
         self.r = np.zeros(self.Nr)
         self._fn("grid")(h, _ptr(self.r))
         self.dr = float(L) / (int(Nr) - 1)
+        self.isolated = False
 
     def _fn(self, name):
         return getattr(self._lib, self._p + name)
@@ -174,6 +178,7 @@ class Solver:
     # lifecycle -------------------------------------------------------------
     def configure_isolated_black_hole(self, x):
         self._check(self._fn("configure_isolated_black_hole")(self._h, float(x)))
+        self.isolated = True
 
     def set_initial_state_full(self, psi):
         a = np.ascontiguousarray(np.asarray(psi, dtype=np.complex128).ravel())
@@ -217,6 +222,12 @@ class Solver:
         ok = C.c_int32(0)
         self._check(self._fn("load_checkpoint")(self._h, str(path).encode(), int(step_id), C.byref(ok)))
         return bool(ok.value)
+
+    def restore_from_diagnostics(self, rows):
+        """reconstruct_state_from_diagnostics (restart.cpp:238-285): rows is
+        (n, 8) = t, R, theta, Omega, M1, q, p_bin, p_bin_inst per step."""
+        a = np.ascontiguousarray(np.asarray(rows, dtype=np.float64).reshape(-1, 8))
+        self._check(self._fn("restore_from_diagnostics")(self._h, a.shape[0], _ptr(a)))
 
     # state -----------------------------------------------------------------
     @property

@@ -168,6 +168,14 @@ int gm_set_scalars(gm_solver *solver, const gm_scalars *in);
  * scalars and the rolling power average, and sets *loaded to 1. */
 int gm_save_checkpoint(gm_solver *solver, const char *path, int32_t step_id);
 int gm_load_checkpoint(gm_solver *solver, const char *path, int32_t step_id, int32_t *loaded);
+/* reconstruct_state_from_diagnostics (src/CrankNicolsonModes_restart.cpp:238-285),
+ * for a resume without a matching checkpoint: rebuilds the orbital phase and
+ * the rolling power average from the first n_rows diagnostic rows, and the
+ * orbit scalars from the last of them. rows holds n_rows x 8 doubles, row k
+ * being what step k+1 wrote to t.txt, R_orb.txt, theta_orb.txt,
+ * Omega_orb.txt, M_1.txt, q.txt, p_bin.txt and p_bin_inst.txt, in that order.
+ * n_rows = 0 resets the phase and the average only. psi is not touched. */
+int gm_restore_from_diagnostics(gm_solver *solver, int32_t n_rows, const double *rows);
 
 /* Operators on host buffers of Nr*n_modes complex values, with the current H:
  * H*psi (include/CrankNicolsonModes.hpp:197), apply_sink (:210),

@@ -774,25 +774,36 @@ static void apply_op(const gm_solver *s, int op, const zc *x, zc *y)
     }
 }
 
-/* Eigen-style reductions. */
+/* Eigen-style reductions. Partial sums over fixed chunks of OM_CHUNK
+ * elements, added in chunk order, so results do not depend on the thread
+ * count or on the order in which threads finish. */
+#define OM_CHUNK 4096
+
 static zc vdot(const zc *a, const zc *b, size_t n) /* sum conj(a) b */
 {
-    double re = 0, im = 0;
-#pragma omp parallel for reduction(+:re,im) schedule(static)
-    for (size_t k = 0; k < n; ++k) {
-        zc p = conj(a[k]) * b[k];
-        re += creal(p);
-        im += cimag(p);
+    const size_t nc = (n + OM_CHUNK - 1) / OM_CHUNK;
+    double *part = (double *)calloc(2 * (nc ? nc : 1), sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < nc; ++c) {
+        double re = 0, im = 0;
+        const size_t e = (c + 1) * OM_CHUNK < n ? (c + 1) * OM_CHUNK : n;
+        for (size_t k = c * OM_CHUNK; k < e; ++k) {
+            zc p = conj(a[k]) * b[k];
+            re += creal(p);
+            im += cimag(p);
+        }
+        part[2 * c] = re;
+        part[2 * c + 1] = im;
     }
+    double re = 0, im = 0;
+    for (size_t c = 0; c < nc; ++c) { re += part[2 * c]; im += part[2 * c + 1]; }
+    free(part);
     return re + I * im;
 }
 
 static double vnorm2(const zc *a, size_t n)
 {
-    double acc = 0;
-#pragma omp parallel for reduction(+:acc) schedule(static)
-    for (size_t k = 0; k < n; ++k) acc += creal(a[k]) * creal(a[k]) + cimag(a[k]) * cimag(a[k]);
-    return acc;
+    return creal(vdot(a, a, n));
 }
 
 /* apply_sink (src/CrankNicolsonModes_absorption.cpp:100-109). */
@@ -1354,6 +1365,146 @@ int om_thomas_probe(int32_t N, int32_t K, const double *diag, const double *lowe
     free(b); free(y);
     precond_free(&pc);
     return failed ? GM_E_RUNTIME : GM_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Checkpoints and resume (src/CrankNicolsonModes_restart.cpp:139-285).      */
+/* ------------------------------------------------------------------------ */
+
+static const char om_magic[8] = {'G', 'M', 'C', 'K', 'P', 'T', '0', '1'};
+
+/* RollingPhaseAverage::save / load (src/RollingPhase.cpp:169-209): window
+ * phase, previous (time, phase, value), has_previous and ready as one byte
+ * each, initial phase, integral, duration, average, then the uint64 segment
+ * count and the segments as 7 doubles each. */
+static int rolling_save(const om_rolling *a, FILE *f)
+{
+    const double prev[3] = {a->prev_time, a->prev_phase, a->prev_value};
+    const unsigned char flags[2] = {(unsigned char)(a->has_previous != 0), (unsigned char)(a->ready != 0)};
+    const double tail[4] = {a->initial_phase, a->integral, a->duration, a->average};
+    const uint64_t n = a->count;
+    int ok = fwrite(&a->window_phase, 8, 1, f) == 1 && fwrite(prev, 8, 3, f) == 3 && fwrite(flags, 1, 2, f) == 2 &&
+             fwrite(tail, 8, 4, f) == 4 && fwrite(&n, 8, 1, f) == 1;
+    if (ok && n) ok = fwrite(a->seg + a->head, sizeof(om_segment), n, f) == n;
+    return ok;
+}
+
+static int rolling_load(om_rolling *a, FILE *f)
+{
+    double prev[3], tail[4];
+    unsigned char flags[2];
+    uint64_t n = 0;
+    if (fread(&a->window_phase, 8, 1, f) != 1 || fread(prev, 8, 3, f) != 3 || fread(flags, 1, 2, f) != 2 ||
+        fread(tail, 8, 4, f) != 4 || fread(&n, 8, 1, f) != 1)
+        return 0;
+    a->prev_time = prev[0]; a->prev_phase = prev[1]; a->prev_value = prev[2];
+    a->has_previous = flags[0]; a->ready = flags[1];
+    a->initial_phase = tail[0]; a->integral = tail[1]; a->duration = tail[2]; a->average = tail[3];
+    free(a->seg);
+    a->seg = NULL;
+    a->head = a->count = a->cap = 0;
+    if (n > ((uint64_t)1 << 32)) return 0;
+    if (n) {
+        a->seg = (om_segment *)malloc(sizeof(om_segment) * n);
+        a->cap = a->count = n;
+        if (fread(a->seg, sizeof(om_segment), n, f) != n) return 0;
+    }
+    return 1;
+}
+
+int om_save_checkpoint(gm_solver *s, const char *path, int32_t step_id)
+{
+    if (!s->initialized) return fail(s, GM_E_LOGIC, "initialize() has not been called");
+    if (!path) return fail(s, GM_E_INVALID, "checkpoint path is NULL");
+    char tmp[4096];
+    snprintf(tmp, sizeof(tmp), "%s.tmp", path);
+    FILE *f = fopen(tmp, "wb");
+    if (!f) return fail(s, GM_E_RUNTIME, "Could not write checkpoint");
+    const int32_t header[3] = {step_id, s->Nr, s->K};
+    const double v[13] = {s->t_i, s->theta, s->phase, s->R, s->M1, s->M2, s->q,
+                          s->R1, s->R2, s->Omega, s->P, s->dt, s->p_bin};
+    const uint64_t n = nvec(s);
+    int ok = fwrite(om_magic, 1, 8, f) == 8 && fwrite(header, 4, 3, f) == 3 && fwrite(v, 8, 13, f) == 13 &&
+             fwrite(&n, 8, 1, f) == 1 && fwrite(s->psi, sizeof(zc), n, f) == n && rolling_save(&s->avg, f);
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(s, GM_E_RUNTIME, "Could not write checkpoint");
+    if (rename(tmp, path) != 0) return fail(s, GM_E_RUNTIME, "Could not replace checkpoint");
+    return GM_OK;
+}
+
+int om_load_checkpoint(gm_solver *s, const char *path, int32_t step_id, int32_t *loaded)
+{
+    if (!s->initialized) return fail(s, GM_E_LOGIC, "initialize() has not been called");
+    if (!path || !loaded) return fail(s, GM_E_INVALID, "checkpoint path or result is NULL");
+    *loaded = 0;
+    FILE *f = fopen(path, "rb");
+    if (!f) return GM_OK;
+    char magic[8];
+    int32_t header[3] = {-1, -1, -1};
+    double v[13];
+    uint64_t n = 0;
+    int ok = fread(magic, 1, 8, f) == 8 && fread(header, 4, 3, f) == 3 && memcmp(magic, om_magic, 8) == 0 &&
+             header[0] == step_id && header[1] == s->Nr && header[2] == s->K && fread(v, 8, 13, f) == 13 &&
+             fread(&n, 8, 1, f) == 1 && n == nvec(s);
+    zc *psi = ok ? (zc *)malloc(sizeof(zc) * n) : NULL;
+    ok = ok && fread(psi, sizeof(zc), n, f) == n;
+    om_rolling avg;
+    memset(&avg, 0, sizeof(avg));
+    ok = ok && rolling_load(&avg, f);
+    fclose(f);
+    if (!ok) {
+        free(psi);
+        free(avg.seg);
+        return GM_OK;
+    }
+    s->t_i = v[0]; s->theta = v[1]; s->phase = v[2]; s->R = v[3]; s->M1 = v[4]; s->M2 = v[5]; s->q = v[6];
+    s->R1 = v[7]; s->R2 = v[8]; s->Omega = v[9]; s->P = v[10]; s->dt = v[11]; s->p_bin = v[12];
+    memcpy(s->psi, psi, sizeof(zc) * n);
+    free(psi);
+    free(s->avg.seg);
+    s->avg = avg;
+    *loaded = 1;
+    return GM_OK;
+}
+
+/* reconstruct_state_from_diagnostics (src/CrankNicolsonModes_restart.cpp:238-285).
+ * rows holds n_rows rows of 8 doubles: t, R, theta, Omega, M1, q, p_bin,
+ * p_bin_inst (row k is the one step k+1 wrote). */
+int om_restore_from_diagnostics(gm_solver *s, int32_t n_rows, const double *rows)
+{
+    if (!s->initialized) return fail(s, GM_E_LOGIC, "initialize() has not been called");
+    if (n_rows < 0 || (n_rows > 0 && !rows)) return fail(s, GM_E_INVALID, "diagnostic rows are missing");
+    double Omega_previous = s->isolated ? 0.0 : (ramp_end_time(s) > 0.0 ? 0.0 : s->Omega_0);
+    double t_previous = 0.0, out = 0.0;
+    const double window = s->avg.window_phase;
+    s->phase = 0.0;
+    rolling_init(&s->avg, 1.0);
+    s->avg.window_phase = window;
+    for (int k = 0; k < n_rows; ++k) {
+        const double *row = rows + 8 * (size_t)k;
+        s->phase += Omega_previous * (row[0] - t_previous);
+        if (!s->isolated && rolling_update(&s->avg, row[0], s->phase, row[7], &out) != 0)
+            return fail(s, GM_E_RUNTIME, "RollingPhaseAverage: time and phase must increase");
+        Omega_previous = row[3];
+        t_previous = row[0];
+    }
+    if (n_rows == 0) return GM_OK;
+    const double *last = rows + 8 * (size_t)(n_rows - 1);
+    s->t_i = last[0];
+    s->R = last[1];
+    s->theta = last[2];
+    s->M1 = last[4];
+    s->q = last[5];
+    s->M2 = s->q * s->M1;
+    s->R1 = s->M2 / (s->M1 + s->M2) * s->R;
+    s->R2 = s->M1 / (s->M1 + s->M2) * s->R;
+    s->Omega = last[3];
+    s->p_bin = last[6];
+    if (!s->isolated && s->flags.evolve_orbit && s->t_i > ramp_end_time(s)) {
+        s->P = 2.0 * M_PI / s->Omega;
+        s->dt = s->P / s->flags.steps_per_period;
+    }
+    return GM_OK;
 }
 
 /* Rolling average probe (tests/rolling_average.cpp). */
