@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "gm", "libgm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["gm_kernels.hip", "gm_math.cpp", "gm_solver.cpp"]
+SOURCES = ["gm_kernels.hip", "gm_math.cpp", "gm_ilut.cpp", "gm_solver.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I", CSRC, "-I", os.path.join(REPO, "include")]
 # Packed FP32 VALU ops stay off in every device pass (v_pk_* have returned wrong
@@ -36,6 +36,8 @@ def compile_one(src):
     cmd = [HIPCC] + COMMON
     if src.endswith(".hip"):
         cmd += DEVICE + ["-x", "hip"]
+    else:
+        cmd += ["--offload-host-only"]  # host code: no device pass
     cmd += ["-c", os.path.join(CSRC, src), "-o", out]
     res = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [ln for ln in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in ln]

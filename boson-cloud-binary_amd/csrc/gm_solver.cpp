@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "gm_kernels.hpp"
+#include "gm_ilut.hpp"
 #include "gm_math.hpp"
 
 namespace {
@@ -173,6 +174,7 @@ struct gm_solver {
     // progress of the step in flight (step_begin / solve_advance / step_finish)
     double step_start = 0.0;
     bool step_wood = false;
+    bool step_ilut = false;  // this step's solve is the host ILUT one
     int step_enqueued = 0;
     // captured parts of the step, keyed by every kernel argument that can change
     struct GraphEntry {
@@ -968,10 +970,21 @@ void solve_begin(gm_solver &S)
     S.step_enqueued = batch;
 }
 
+void rhs_and_init(gm_solver &S);
+
 void solve_begin_work(gm_solver &S, int batch)
 {
     const bool wood = S.step_wood;
     if (wood) woodbury_setup(S);
+    rhs_and_init(S);
+    for (int k = 0; k < batch; ++k) enqueue_iteration(S, wood);
+    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
+}
+
+// Right-hand side, its norm and the BiCGSTAB start (b, x, r, r0 normalised).
+void rhs_and_init(gm_solver &S)
+{
+    const bool wood = S.step_wood;
     gmd::Krylov *dk = S.dK.p;
     // rhs = B psi - (dt/4) Gamma psi and A psi in one apply
     if (wood) sink_proj(S, S.psi.p, gmk::SD_START, 1.0, 0);
@@ -999,19 +1012,25 @@ void solve_begin_work(gm_solver &S, int batch)
         ia.partial = S.partial.p;
         launch(S, GM_PROF_SMALL, [&] { return gmk::launch_init(ia, S.st); });
     }
-    for (int k = 0; k < batch; ++k) enqueue_iteration(S, wood);
-    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
 }
 
 void solve_tail_work(gm_solver &S);
+void ilut_host_solve(gm_solver &S);
 
 // After a convergence check (host copy of the Krylov state is current): more
 // iterations (false) or the acceptance check, mask and diagnostics with their
 // readback (true).
 bool solve_advance(gm_solver &S)
 {
-    const gmd::Krylov &K = *S.hK.p;
-    if (K.restart) throw std::runtime_error("BiCGSTAB lost bi-orthogonality; the restart path is not available");
+    gmd::Krylov &K = *S.hK.p;
+    if (K.restart && !K.done) {
+        // Bi-orthogonality lost on the radial path: BiCGSTAB is not restarted
+        // on the device; the step is solved by the full ILUT instead.
+        S.step_ilut = true;
+        ilut_host_solve(S);
+        K.done = 1;
+        K.iter = S.last_iterations;
+    }
     if (!K.done) {
         if (S.step_enqueued > 2 * kMaxIt) throw std::runtime_error("BiCGSTAB did not report convergence");
         enqueue_batch(S, 2);
@@ -1187,6 +1206,204 @@ void apply_H_host(gm_solver &S, const double *in, double *out)
 void require_initialized(const gm_solver &S)
 {
     if (!S.initialized) throw std::logic_error("initialize() must be called first");
+}
+
+// ------------------------------------------------------------------ full-ILUT solve
+
+// Host memory the ILUT solve may use: the factor of the node-major CN operator
+// keeps about 3K entries per row (24 bytes each with its column), plus the
+// extracted operator and ten work vectors. Larger systems are refused.
+constexpr double kIlutHostBytes = 32e9;
+
+double ilut_bytes(const gm_solver &S)
+{
+    const double n = static_cast<double>(S.nvec());
+    return n * (3.0 * S.K + 3.0) * 24.0 + n * (S.K + 3.0) * 24.0 + 12.0 * 16.0 * n;
+}
+
+// H as node-major CSR (row i*K + j for mode j at node i), read off the device
+// operator with 3K applies. The probe for (c, j) has ones on mode j at every
+// node i = c (mod 3); H couples modes only within a node and nodes only within
+// a mode and to their neighbours, so row (k, i') of the result is the entry of
+// the single probed column (j, i) with |i - i'| <= 1.
+gmh::CsrZ extract_H(gm_solver &S)
+{
+    const int K = S.K, N = S.Nr;
+    const size_t n = S.nvec();
+    std::vector<std::vector<std::pair<int, gmh::cplx>>> rows(n);
+    std::vector<double> probe(2 * n), out(2 * n);
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < K; ++j) {
+            std::fill(probe.begin(), probe.end(), 0.0);
+            for (int i = c; i < N; i += 3) probe[2 * (static_cast<size_t>(j) * N + i)] = 1.0;
+            to_dev(S, probe.data(), S.s.p);
+            gmk::ApplyArgs a = apply_args(S, S.s.p, S.t.p);
+            launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_H, S.st); });
+            to_host(S, S.t.p, out.data());
+            for (int k = 0; k < K; ++k)
+                for (int i2 = 0; i2 < N; ++i2) {
+                    const size_t q = static_cast<size_t>(k) * N + i2;
+                    const gmh::cplx v(out[2 * q], out[2 * q + 1]);
+                    if (v == gmh::cplx(0.0)) continue;
+                    const int d = ((c - i2) % 3 + 3) % 3;
+                    const int i = i2 + (d == 2 ? -1 : d);
+                    if (i < 0 || i >= N) throw std::logic_error("operator probe: entry outside the stencil");
+                    rows[static_cast<size_t>(i2) * K + k].emplace_back(i * K + j, v);
+                }
+        }
+    gmh::CsrZ H;
+    H.n = static_cast<int>(n);
+    H.ptr.assign(n + 1, 0);
+    for (size_t r = 0; r < n; ++r) {
+        auto &row = rows[r];
+        std::sort(row.begin(), row.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (const auto &e : row) {
+            H.col.push_back(e.first);
+            H.val.push_back(e.second);
+        }
+        H.ptr[r + 1] = static_cast<int>(H.val.size());
+        std::vector<std::pair<int, gmh::cplx>>().swap(row);
+    }
+    return H;
+}
+
+// A = I + i (dt/2) H (update_CN_values, matrices.cpp:27-42), with every
+// diagonal entry present.
+gmh::CsrZ cn_operator(const gmh::CsrZ &H, double half_dt)
+{
+    gmh::CsrZ A;
+    A.n = H.n;
+    A.ptr.assign(static_cast<size_t>(H.n) + 1, 0);
+    A.col.reserve(H.nnz() + H.n);
+    A.val.reserve(H.nnz() + H.n);
+    const gmh::cplx s(0.0, half_dt);
+    for (int r = 0; r < H.n; ++r) {
+        bool diag = false;
+        for (int q = H.ptr[r]; q < H.ptr[r + 1]; ++q) {
+            const int c = H.col[q];
+            if (!diag && c > r) {
+                A.col.push_back(r);
+                A.val.push_back(1.0);
+                diag = true;
+            }
+            A.col.push_back(c);
+            A.val.push_back((c == r ? 1.0 : 0.0) + s * H.val[q]);
+            diag = diag || c == r;
+        }
+        if (!diag) {
+            A.col.push_back(r);
+            A.val.push_back(1.0);
+        }
+        A.ptr[r + 1] = static_cast<int>(A.val.size());
+    }
+    return A;
+}
+
+// The full-ILUT solve of solve_CN (src/CrankNicolsonModes_matrices.cpp:173-185,
+// 203-214) on the host: ILUT (fill factor 50, drop tolerance 1e-9) of the
+// operator the device applies, under Eigen's BiCGSTAB without sinks or the
+// reference's BiCGSTAB with the Woodbury sink correction, tolerance 1e-10, at
+// most 5000 iterations, from the guess psi/|rhs|. Reads b = rhs/|rhs| and
+// |rhs| from the device (after rhs_and_init) and leaves x = solution/|rhs| in
+// S.x, where the tail's acceptance check, mask and diagnostics take it over.
+void ilut_host_solve(gm_solver &S)
+{
+    HIPCK(hipStreamSynchronize(S.st));
+    if (ilut_bytes(S) > kIlutHostBytes)
+        throw std::runtime_error("ILUT preconditioner: the factor of " + std::to_string(S.nvec()) +
+                                 " unknowns would exceed the host memory budget of the ILUT solve");
+    gmd::Krylov kr;
+    HIPCK(hipMemcpy(&kr, S.dK.p, sizeof(kr), hipMemcpyDeviceToHost));
+    S.last_iterations = 0;
+    if (kr.zero_rhs) return;  // the acceptance writes psi = 0
+    const gmd::StepParams &P = *S.hP.p;
+    const int K = S.K, N = S.Nr;
+    const size_t n = S.nvec();
+    const gmh::CsrZ A = cn_operator(extract_H(S), P.half_dt);
+    gmh::Ilut M;
+    std::string why;
+    if (!M.compute(A, 1e-9, 50, &why)) throw std::runtime_error("Crank-Nicolson ILUT factorization failed");
+    std::vector<double> hb(2 * n), hpsi(2 * n);
+    to_host(S, S.b.p, hb.data());
+    to_host(S, S.psi.p, hpsi.data());
+    auto node_major = [&](size_t q) { return (q % N) * K + q / N; };
+    std::vector<gmh::cplx> b(n), x(n);
+    for (size_t q = 0; q < n; ++q) {
+        b[node_major(q)] = gmh::cplx(hb[2 * q], hb[2 * q + 1]);
+        x[node_major(q)] = gmh::cplx(hpsi[2 * q], hpsi[2 * q + 1]) / kr.rhs_nrm;
+    }
+    // active sinks: A_full = A + (dt/4) sum_a g_a v_a v_a^H (matrices.cpp:108-116)
+    const int ns = P.nsink;
+    std::vector<std::vector<std::pair<size_t, gmh::cplx>>> sv(ns);
+    std::vector<double> beta(ns);
+    for (int a = 0; a < ns; ++a) {
+        const Sink &sk = S.sink[P.sink_body[a]];
+        beta[a] = 0.5 * P.half_dt * P.g[a];
+        for (int k = 0; k < K; ++k)
+            for (int d = 0; d < sk.n; ++d) {
+                const double2 v = sk.v[static_cast<size_t>(k) * sk.n + d];
+                if (v.x != 0.0 || v.y != 0.0)
+                    sv[a].emplace_back(static_cast<size_t>(sk.lo + d) * K + k, gmh::cplx(v.x, v.y));
+            }
+    }
+    auto vdot = [&](int a, const gmh::cplx *y) {
+        gmh::cplx s = 0.0;
+        for (const auto &e : sv[a]) s += std::conj(e.second) * y[e.first];
+        return s;
+    };
+    const gmh::LinOp apply_A = [&](const gmh::cplx *in, gmh::cplx *out) {
+        A.multiply(in, out);
+        for (int a = 0; a < ns; ++a) {
+            const gmh::cplx c = beta[a] * vdot(a, in);
+            for (const auto &e : sv[a]) out[e.first] += c * e.second;
+        }
+    };
+    // Woodbury (matrices.cpp:133-159): W_a = M^-1 v_a, G = diag(1/beta) + V^H W
+    std::vector<std::vector<gmh::cplx>> W(ns, std::vector<gmh::cplx>(n));
+    gmh::cplx Ginv[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    if (ns > 0) {
+        for (int a = 0; a < ns; ++a) {
+            std::vector<gmh::cplx> va(n, 0.0);
+            for (const auto &e : sv[a]) va[e.first] = e.second;
+            M.solve(va.data(), W[a].data());
+        }
+        gmh::cplx G[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        for (int a = 0; a < ns; ++a) {
+            G[a][a] += 1.0 / beta[a];
+            for (int c = 0; c < ns; ++c) G[a][c] += vdot(a, W[c].data());
+        }
+        if (ns == 1) {
+            Ginv[0][0] = 1.0 / G[0][0];
+        } else {
+            const gmh::cplx det = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+            Ginv[0][0] = G[1][1] / det;
+            Ginv[0][1] = -G[0][1] / det;
+            Ginv[1][0] = -G[1][0] / det;
+            Ginv[1][1] = G[0][0] / det;
+        }
+    }
+    const gmh::LinOp apply_M = [&](const gmh::cplx *in, gmh::cplx *out) {
+        M.solve(in, out);
+        if (ns == 0) return;
+        gmh::cplx proj[2] = {0.0, 0.0}, corr[2] = {0.0, 0.0};
+        for (int a = 0; a < ns; ++a) proj[a] = vdot(a, out);
+        for (int a = 0; a < ns; ++a)
+            for (int c = 0; c < ns; ++c) corr[a] += Ginv[a][c] * proj[c];
+        for (size_t q = 0; q < n; ++q) {
+            gmh::cplx t = 0.0;
+            for (int a = 0; a < ns; ++a) t += W[a][q] * corr[a];
+            out[q] -= t;
+        }
+    };
+    int its = 0;
+    gmh::bicgstab(apply_A, apply_M, b.data(), x.data(), n, 1e-10, 5000, ns == 0, &its);
+    S.last_iterations = its;
+    for (size_t q = 0; q < n; ++q) {
+        const gmh::cplx v = x[node_major(q)];
+        hb[2 * q] = v.real();
+        hb[2 * q + 1] = v.imag();
+    }
+    to_dev(S, hb.data(), S.x.p);
 }
 
 }  // namespace
@@ -1598,13 +1815,26 @@ void tune_apply(gm_solver &S, int mixed)
 void step_begin(gm_solver &S)
 {
     require_initialized(S);
-    if (!S.flags.use_block_preconditioner)
-        throw std::runtime_error("ILUT preconditioner is not available in this build");
     S.step_start = S.t_i;
     S.t_i += S.dt;
     S.theta += (S.counter ? -1.0 : 1.0) * S.Omega * S.dt;
     S.phase += S.Omega * S.dt;
     build_perturbation(S, S.t_i);
+    S.step_ilut = false;
+    if (!S.flags.use_block_preconditioner) {
+        // the "ilut" preconditioner key: right-hand side on the device, solve
+        // on the host, then the tail as after a converged BiCGSTAB
+        S.step_wood = S.hP.p->nsink > 0;
+        S.step_enqueued = 0;
+        S.step_ilut = true;
+        rhs_and_init(S);
+        ilut_host_solve(S);
+        gmd::Krylov &K = *S.hK.p;
+        K.done = 1;
+        K.restart = 0;
+        K.iter = S.last_iterations;
+        return;
+    }
     tune_apply(S, 0);
     if (S.flags.precision == GM_PRECISION_MIXED) tune_apply(S, 1);
     solve_begin(S);
@@ -1617,14 +1847,25 @@ void step_finish(gm_solver &S, gm_diag *d)
     const double step_start = S.step_start;
     const gmd::StepParams &P = *S.hP.p;
     {
-        if (S.hflags.p[0]) {
-            S.fac_valid = false;
-            int zero = 0;
-            HIPCK(hipMemcpy(S.dflags.p, &zero, sizeof(int), hipMemcpyHostToDevice));
-            throw std::runtime_error("Radial preconditioner found a zero pivot; the ILUT fallback is not available");
+        for (;;) {
+            const bool bad_factor = S.hflags.p[0] != 0;
+            if (bad_factor) {
+                S.fac_valid = false;
+                S.hflags.p[0] = 0;
+                int zero = 0;
+                HIPCK(hipMemcpy(S.dflags.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+            }
+            if (S.hflags.p[1] && !bad_factor) break;
+            if (S.step_ilut) throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
+            // "Radial preconditioner did not converge; retrying with full ILUT."
+            // (matrices.cpp:173,203): psi is still the step's start state
+            S.step_ilut = true;
+            ilut_host_solve(S);
+            run_captured(S, GK_TAIL, graph_key(S, 0), [&] { solve_tail_work(S); });
+            HIPCK(hipStreamSynchronize(S.st));
+            prof_collect(S);
         }
         S.last_residual = S.hresid.p[0];
-        if (!S.hflags.p[1]) throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
         const gmd::Krylov &K = *S.hK.p;
         // absorbed mass: mu dr dt sum_a g_a |v_a^H psi_mid|^2 (CN balance)
         double absorbed = 0.0, Mdot[2] = {0.0, 0.0};
@@ -1851,6 +2092,65 @@ int gm_load_checkpoint(gm_solver *S, const char *path, int32_t step_id, int32_t 
         S->avg = std::move(avg);
         *loaded = 1;
     });
+}
+
+int gm_ilut_bicgstab(int32_t n, const int32_t *ptr, const int32_t *col, const double *val, const double *b,
+                     double *x, double droptol, int32_t fill_factor, double tol, int32_t max_iterations,
+                     int32_t eigen_mode, int32_t *iterations)
+{
+    if (iterations) *iterations = 0;
+    if (n < 1 || !ptr || !col || !val || !b || !x || !iterations || fill_factor < 1 || !(droptol >= 0.0) ||
+        !(tol > 0.0) || max_iterations < 0) {
+        g_create_err = "gm_ilut_bicgstab: invalid argument";
+        return GM_E_INVALID;
+    }
+    try {
+        gmh::CsrZ A;
+        A.n = n;
+        A.ptr.assign(ptr, ptr + n + 1);
+        if (A.ptr[0] != 0) throw std::invalid_argument("gm_ilut_bicgstab: ptr[0] must be 0");
+        for (int i = 0; i < n; ++i)
+            if (A.ptr[i + 1] < A.ptr[i]) throw std::invalid_argument("gm_ilut_bicgstab: ptr must not decrease");
+        const size_t nnz = static_cast<size_t>(A.ptr[n]);
+        A.col.assign(col, col + nnz);
+        for (int i = 0; i < n; ++i)
+            for (int q = A.ptr[i]; q < A.ptr[i + 1]; ++q)
+                if (A.col[q] < 0 || A.col[q] >= n || (q > A.ptr[i] && A.col[q] <= A.col[q - 1]))
+                    throw std::invalid_argument("gm_ilut_bicgstab: columns must be in range and increasing per row");
+        A.val.resize(nnz);
+        for (size_t q = 0; q < nnz; ++q) A.val[q] = gmh::cplx(val[2 * q], val[2 * q + 1]);
+        gmh::Ilut M;
+        std::string why;
+        if (!M.compute(A, droptol, fill_factor, &why)) {
+            g_create_err = "Crank-Nicolson ILUT factorization failed: " + why;
+            return GM_E_RUNTIME;
+        }
+        std::vector<gmh::cplx> bb(static_cast<size_t>(n)), xx(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            bb[i] = gmh::cplx(b[2 * i], b[2 * i + 1]);
+            xx[i] = gmh::cplx(x[2 * i], x[2 * i + 1]);
+        }
+        int its = 0;
+        const bool ok = gmh::bicgstab([&](const gmh::cplx *in, gmh::cplx *out) { A.multiply(in, out); },
+                                      [&](const gmh::cplx *in, gmh::cplx *out) { M.solve(in, out); }, bb.data(),
+                                      xx.data(), static_cast<size_t>(n), tol, max_iterations, eigen_mode != 0, &its);
+        *iterations = its;
+        for (int i = 0; i < n; ++i) {
+            x[2 * i] = xx[i].real();
+            x[2 * i + 1] = xx[i].imag();
+        }
+        if (!ok) {
+            g_create_err = "gm_ilut_bicgstab: BiCGSTAB stopped above the tolerance";
+            return GM_E_RUNTIME;
+        }
+        return GM_OK;
+    } catch (const std::invalid_argument &e) {
+        g_create_err = e.what();
+        return GM_E_INVALID;
+    } catch (const std::exception &e) {
+        g_create_err = e.what();
+        return GM_E_RUNTIME;
+    }
 }
 
 // reconstruct_state_from_diagnostics (src/CrankNicolsonModes_restart.cpp:238-285).

@@ -31,9 +31,36 @@ def lib():
         _lib.gm_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
         _lib.gm_ensemble_step.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(capi.Diag)]
         _lib.gm_apply_config.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
-        _lib.gm_save_checkpoint.argtypes = [vp, C.c_char_p, C.c_int32]
-        _lib.gm_load_checkpoint.argtypes = [vp, C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]
+        ip, dp = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        _lib.gm_ilut_bicgstab.argtypes = [C.c_int32, ip, ip, dp, dp, dp, C.c_double, C.c_int32, C.c_double,
+                                          C.c_int32, C.c_int32, ip]
+        _lib.gm_last_error.restype = C.c_char_p
     return _lib
+
+
+def ilut_bicgstab(ptr, col, val, b, x0=None, droptol=1e-9, fill_factor=50, tol=1e-10, max_iterations=5000,
+                  eigen_mode=True):
+    """gm_ilut_bicgstab: solve the complex CSR system (ptr, col, val) for b with
+    ILUT-preconditioned BiCGSTAB on the host. Returns (x, iterations); raises
+    GMError when the factorization fails or the tolerance is not reached."""
+    import numpy as np
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    n = ptr.size - 1
+    x = np.zeros(n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+    its = C.c_int32()
+    ip, dp = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    L = lib()
+    rc = L.gm_ilut_bicgstab(n, ptr.ctypes.data_as(ip), col.ctypes.data_as(ip), val.view(np.float64).ctypes.data_as(dp),
+                            b.view(np.float64).ctypes.data_as(dp), x.view(np.float64).ctypes.data_as(dp),
+                            float(droptol), int(fill_factor), float(tol), int(max_iterations), int(bool(eigen_mode)),
+                            C.byref(its))
+    if rc != 0:
+        err = capi._ERRORS.get(rc, GMError)
+        raise err(rc, (L.gm_last_error(None) or b"").decode())
+    return x, its.value
 
 
 def device_count():

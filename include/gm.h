@@ -185,6 +185,21 @@ int gm_apply_sink(gm_solver *solver, const double *in, double *out);
 int gm_schrodinger_rhs(gm_solver *solver, const double *in, double *out);
 double gm_sink_strength(const gm_solver *solver, int32_t body);
 
+/* The reference's ilut_solver (include/CrankNicolsonModes.hpp:264, configured at
+ * src/CrankNicolsonModes_matrices.cpp:21-24) on a caller's system, host only:
+ * ILUT(fill_factor, droptol) of the n x n complex CSR matrix (ptr[n+1],
+ * col[nnz] sorted per row, val[2*nnz] interleaved re/im) under right-
+ * preconditioned BiCGSTAB from the guess in x (2n doubles, overwritten with the
+ * solution) for b (2n doubles), to |A x - b| <= tol |b| in at most
+ * max_iterations. eigen_mode 1 is Eigen's recurrence, 0 the reference's
+ * (src/CrankNicolsonModes_matrices.cpp:47-91). The step uses the same code
+ * with fill 50, droptol 1e-9, tol 1e-10, 5000 iterations, on the operator in
+ * node-major order. GM_E_RUNTIME when the factorization fails (an empty row)
+ * or BiCGSTAB stops above the tolerance; *iterations is set in both cases. */
+int gm_ilut_bicgstab(int32_t n, const int32_t *ptr, const int32_t *col, const double *val, const double *b,
+                     double *x, double droptol, int32_t fill_factor, double tol, int32_t max_iterations,
+                     int32_t eigen_mode, int32_t *iterations);
+
 /* Multipole truncation of the coupling (no counterpart in the reference, which
  * evaluates every l*): at each tile of 64 radial nodes, multipoles whose radial
  * profile is below tol times the tile's leading profile are not evaluated.

@@ -96,6 +96,37 @@ def test_one_step_matches_oracle(scenario):
         assert rel(dg[k], do[k]) < 1e-9 or np.linalg.norm(do[k]) < 1e-300, k
 
 
+@pytest.mark.parametrize("scenario", range(5))
+def test_ilut_key_matches_radial_path(scenario):
+    """tests/parallel_correctness.cpp:189-193: with use_block_preconditioner
+    off (the "ilut" key) the step equals the radial path's to 1e-8 and passes
+    the 1e-9 acceptance check. Here the GPU's ILUT step (operator read off the
+    device, factor and BiCGSTAB on the host, acceptance, mask and diagnostics
+    on the device) against the oracle's radial step, over five steps."""
+    announce(f"ilut key scenario {scenario}")
+    g, o = correctness_pair(scenario)
+    g.set_flags(use_block_preconditioner=0)
+    for n in range(1, 6):
+        dg, do = g.step(n), o.step(n)
+        assert rel(g.psi, o.psi) < 1e-8, n
+        assert g.scalars()["last_solver_residual"] <= 1e-9
+        assert dg["energy"] == pytest.approx(do["energy"], rel=1e-7, abs=1e-14), n
+    assert 1 <= g.scalars()["last_solver_iterations"] <= 10
+
+
+def test_ilut_key_then_radial_path():
+    """Switching the key between steps: an ILUT step followed by radial steps
+    tracks the oracle's radial run."""
+    announce("ilut then radial")
+    g, o = correctness_pair(1)
+    g.set_flags(use_block_preconditioner=0)
+    g.step(1), o.step(1)
+    g.set_flags(use_block_preconditioner=1)
+    for n in range(2, 6):
+        g.step(n), o.step(n)
+    assert rel(g.psi, o.psi) < 1e-8
+
+
 def run_both(objs, steps, r_ex):
     g, o = objs
     both(objs, "begin_run", r_ex, steps + 1)
