@@ -215,16 +215,14 @@ __device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, doub
     return t;
 }
 
-// Coupling of the 16 rows of a chunk with the modes of l1 at the NG x 16 nodes
-// of a tile, for the 4 multipole slots s0 .. s0+3 of the block, on the matrix
-// cores. For each column m1:
-//   W[row][node] = sum_t G[m1][t][row] E_t(node)   (v_mfma_f64_16x16x4f64)
+// Coupling of the 16 rows of a chunk at the NG x 16 nodes of a tile, on the
+// matrix cores. The chunk's pass table lists its (l1, slot group) blocks; each
+// pass runs the columns m1 = 0 .. l1 of degree l1, and each (pass, m1) step is
+//   W[row][node] = sum_t G[m1][t][row] E_t(node)   (one MFMA per 16 nodes)
 //   acc[row][node] += W[row][node] xt_(l1,m1)(node)   (VALU)
-// with l* = 2 (tlo + t) + p and xt = exp(i m1 theta) x read from HBM/L2 two
-// columns ahead; the profiles E_{l*} of the tile sit in LDS. The sum over slots is
-// linear, so a block with more than 4 slots runs several passes. Lane layout
-// (gfx950): A[row = lane%16][k = lane/16], B[k = lane/16][node = lane%16],
-// D[row = lane/16 + 4r][node = lane%16].
+// for the slots t = 0 .. 3 of the pass, l* = ls0 + 2 t, with xt = exp(i m1
+// theta) x. Lane layout (gfx950): A[row = lane%16][k = lane/16],
+// B[k = lane/16][node = lane%16], D rows per Arith::row, node = lane%16.
 // Arithmetic of the coupling: FP64 (matrix-core products and accumulation in
 // double) or FP32 (products on v_mfma_f32_16x16x4f32 and accumulation in
 // float, for the mixed-precision Krylov applies).
@@ -265,84 +263,139 @@ __device__ __forceinline__ void acc_update(typename Arith<T>::C (&acc)[NG][4], i
     }
 }
 
+// Raw buffer loads: a scalar resource over a whole table, a fixed per-lane
+// byte offset and a per-step scalar byte offset (reads past num_records
+// return zeros). Word 3 of the resource is the gfx9 default format.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename V> __device__ __forceinline__ V buffer_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <> __device__ __forceinline__ float buffer_ld<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <> __device__ __forceinline__ double buffer_ld<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+template <> __device__ __forceinline__ float2 buffer_ld<float2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+template <> __device__ __forceinline__ double2 buffer_ld<double2>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+
+// The coupling of one chunk at one tile as a single stream of steps from the
+// chunk's step table: a tile with multipole cut lc runs the first
+// step_count[lc] steps (the table is sorted by the first slot's l*). The A
+// fragment G[m1][slot][row] and the phased column vectors of the next RING
+// steps are in flight, as buffer loads with fixed per-lane offsets and the
+// step's scalar offsets; the B fragment E_{l*}(node) is read from LDS one step
+// ahead. No lane masking is needed: G holds zeros for the padding slots of a
+// block and the LDS rows above lc hold zeros. After the last step the ring
+// repeats the last step's offsets (unused loads), so the loads stay
+// unconditional.
 template <int NG, typename T>
-__device__ __forceinline__ void mfma_pass(const Geometry &g, const T *GM, const typename Arith<T>::C *Xt, int t0,
-                                          const MBlock &B, int kb, int l1, int p, int s0, int nTe, const T *sE,
-                                          typename Arith<T>::C (&acc)[NG][4])
+__device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, const typename Arith<T>::C *Xt, int t0,
+                                            int chn, int lc, const T *sE, typename Arith<T>::C (&acc)[NG][4])
 {
     typedef Arith<T> AR;
     typedef typename AR::C C;
     constexpr int TN = 16 * NG;
+    constexpr int RING = gmd::RING;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
-    const int t = s0 + q;
-    // B fragments, constant over m1: slot t at node 16 n + col
-    T bf[NG];
-    const int off = (2 * (B.tlo + t) + p) * TN + col;
-#pragma unroll
-    for (int n = 0; n < NG; ++n) bf[n] = t < nTe ? sE[off + 16 * n] : T(0);
-    // A fragments G[m1][t][row = col] in a ring of four columns ahead (the
-    // loop is unrolled by four so the ring needs no register moves); slots
-    // past the block read the next block (finite values times a zero B fragment)
-    const T *__restrict__ Gb = GM + B.goff + t * 16 + col;
-    const int mstride = B.nT * 16;
-    T fr[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) fr[k] = k <= l1 ? Gb[k * mstride] : T(0);
-    // phased column vectors exp(i m1 theta) x_(l1,m1) in a ring of two columns
-    // ahead; nodes past the grid repeat node N-1 (they only feed accumulators
-    // never stored)
     const int N = g.N;
-    const C *__restrict__ xg = Xt + (size_t)kb * N;
-    int xo[NG];
+    const int total = g.pass_steps[chn * (2 * g.lmax + 1) + lc];
+    if (total == 0) return;
+    const gmd::MStep *st = g.msteps + g.step_start[chn];
+    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(GM, g.gm_n * (unsigned)sizeof(T));
+    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(Xt, (unsigned)g.K * (unsigned)N * (unsigned)sizeof(C));
+    // per-lane byte offsets; nodes past the grid repeat node N-1 (they only
+    // feed accumulators never stored)
+    const uint32_t g_lane = (uint32_t)(q * 16 + col) * (uint32_t)sizeof(T);
+    uint32_t x_lane[NG];
 #pragma unroll
-    for (int n = 0; n < NG; ++n) xo[n] = min(t0 + 16 * n + col, N - 1);
-    C xr[2][NG];
+    for (int n = 0; n < NG; ++n) x_lane[n] = (uint32_t)min(t0 + 16 * n + col, N - 1) * (uint32_t)sizeof(C);
+    const uint32_t e_lane = (uint32_t)(2 * q * TN + col) * (uint32_t)sizeof(T);
+    const uint32_t xcol = (uint32_t)N * (uint32_t)sizeof(C);
+    // The stream runs a multiple of RING steps (one loop exit per ring
+    // round). The steps past the tile's `total` are the chunk's steps whose
+    // first slot lies above lc, or its padding steps: both read LDS rows above
+    // lc, which hold zeros, so they add nothing. Step entries are read RING
+    // steps before their loads are issued.
+    const int padded = (total + RING - 1) / RING * RING;
+    auto entry = [&](const int step) { return st[min(step, padded - 1)]; };
+    gmd::MStep ent[RING];
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < RING; ++k) ent[k] = entry(k);
+    T fr[RING];
+    C xv[RING][NG];
+    uint32_t eoff[RING];
+    // issue the loads of `step` into slot k (ent[k] holds its entry) and read
+    // the entry of the step RING later
+    auto issue = [&](const int k, const int step) {
+        const gmd::MStep e = ent[k];
+        fr[k] = buffer_ld<T>(rg, g_lane, (uint32_t)e.g * (uint32_t)sizeof(T));
 #pragma unroll
-        for (int n = 0; n < NG; ++n) xr[k][n] = xg[(size_t)min(k, l1) * N + xo[n]];
+        for (int n = 0; n < NG; ++n) xv[k][n] = buffer_ld<C>(rx, x_lane[n], (uint32_t)e.x * xcol);
+        eoff[k] = (uint32_t)e.e * (uint32_t)(TN * sizeof(T));
+        ent[k] = entry(step + RING);
+    };
+    auto bf_load = [&](const int k, T (&bf)[NG]) {
+        const T *eb = (const T *)((const char *)sE + eoff[k] + e_lane);
+#pragma unroll
+        for (int n = 0; n < NG; ++n) bf[n] = eb[16 * n];
+    };
     const typename AR::V4 z = {T(0), T(0), T(0), T(0)};
-    // one column: products on the matrix core, accumulation on the VALU; with
-    // pf the ring slot is refilled (unconditionally, clamped index) for m1 + 4
-    auto step = [&](const int k, const int m1i, const bool pf) {
-        const T fc = fr[k];
-        if (pf) fr[k] = Gb[min(m1i + 4, l1) * mstride];
-        C xv[NG];
-#pragma unroll
-        for (int n = 0; n < NG; ++n) xv[n] = xr[k & 1][n];
-        if (pf || m1i + 2 <= l1) {
-#pragma unroll
-            for (int n = 0; n < NG; ++n) xr[k & 1][n] = xg[(size_t)min(m1i + 2, l1) * N + xo[n]];
-        }
+    auto consume = [&](const int k, const T (&bf)[NG]) {
         if constexpr (NG == 1) {
-            const typename AR::V4 D0 = AR::mfma(fc, bf[0], z);
-            acc_update<NG, T>(acc, 0, D0, xv[0]);
+            const typename AR::V4 D0 = AR::mfma(fr[k], bf[0], z);
+            acc_update<NG, T>(acc, 0, D0, xv[k][0]);
         } else {
             // node groups in pairs: the accumulation of group n runs while
             // group n + 1 is on the matrix core
 #pragma unroll
             for (int n = 0; n < NG; n += 2) {
-                const typename AR::V4 D0 = AR::mfma(fc, bf[n], z);
-                const typename AR::V4 D1 = AR::mfma(fc, bf[n + 1], z);
+                const typename AR::V4 D0 = AR::mfma(fr[k], bf[n], z);
+                const typename AR::V4 D1 = AR::mfma(fr[k], bf[n + 1], z);
                 __builtin_amdgcn_sched_barrier(0);
-                acc_update<NG, T>(acc, n, D0, xv[n]);
+                acc_update<NG, T>(acc, n, D0, xv[k][n]);
                 __builtin_amdgcn_sched_barrier(0);
-                acc_update<NG, T>(acc, n + 1, D1, xv[n + 1]);
+                acc_update<NG, T>(acc, n + 1, D1, xv[k][n + 1]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
     };
-    int m0 = 0;
-#pragma unroll 1
-    for (; m0 + 3 <= l1; m0 += 4) {
-        step(0, m0, true);
-        step(1, m0 + 1, true);
-        step(2, m0 + 2, true);
-        step(3, m0 + 3, true);
+    // prologue in ring order (kept by the barriers), so that the loop is
+    // entered with the same loads in flight as at its back edge
+#pragma unroll
+    for (int k = 0; k < RING; ++k) {
+        issue(k, k);
+        __builtin_amdgcn_sched_barrier(0);
     }
-    if (m0 <= l1) step(0, m0, false);
-    if (m0 + 1 <= l1) step(1, m0 + 1, false);
-    if (m0 + 2 <= l1) step(2, m0 + 2, false);
+    T bfa[NG], bfb[NG];
+    bf_load(0, bfa);
+#pragma unroll 1
+    for (int s = 0; s < padded; s += RING) {
+#pragma unroll
+        for (int k = 0; k < RING; ++k) {
+            // slot k + 1 was issued before slot k is consumed: its B fragment
+            // is read now and used at the next step
+            if (k & 1) {
+                bf_load((k + 1) % RING, bfa);
+                consume(k, bfb);
+            } else {
+                bf_load((k + 1) % RING, bfb);
+                consume(k, bfa);
+            }
+            // unconditional refill (a conditional one would make the
+            // compiler copy registers with loads in flight)
+            issue(k, s + k + RING);
+        }
+    }
 }
 
 // Operator apply. A workgroup: one tile of TN = 16 NG radial nodes and a group
@@ -358,7 +411,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 {
     typedef Arith<T> AR;
     constexpr int TN = 16 * NG;
-    __shared__ T sE[(2 * gmd::MAXLM - 1) * TN];
+    __shared__ T sE[(gmd::ZROW + 7) * TN];  // rows l* = 0 .. lc, then zeros
     const int MODE = a.mode;
     const Geometry &g = a.g;
     Krylov *Kr = a.Kr;
@@ -399,22 +452,10 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
             const int ls = e / TN, node = t0 + (e & (TN - 1));
             sE[e] = node < N ? E[(size_t)((ls & 1) * g.Ls + ls) * N + node] : T(0);
         }
+        for (int e = (lc + 1) * TN + threadIdx.x; e < (gmd::ZROW + 7) * TN; e += 64 * gmd::AW) sE[e] = T(0);
         __syncthreads();
-        if (wave_ok) {
-            const int par = g.mchunks[chn].par;
-            const MBlock *Bc = g.mblocks + (size_t)chn * (lmax + 1);
-            for (int l1 = 0; l1 <= lmax; ++l1) {
-                const MBlock B = Bc[l1];
-                const int p = (par + l1) & 1;
-                // slots with l* = 2 (tlo + t) + p <= lc
-                const int tmax = (lc - p) >= 0 ? (lc - p) / 2 : -1;
-                const int nTe = max(0, min(B.nT, tmax - B.tlo + 1));
-                const int kb = l1 * (l1 + 1) / 2;
-                for (int s0 = 0; s0 < nTe; s0 += 4)
-                    mfma_pass<NG, T>(g, GM, static_cast<const typename AR::C *>(a.Xt), t0, B, kb, l1, p, s0,
-                                     nTe, sE, acc);
-            }
-        }
+        if (wave_ok)
+            mfma_stream<NG, T>(g, GM, static_cast<const typename AR::C *>(a.Xt), t0, chn, lc, sE, acc);
     }
     if (wave_ok) {
         const MChunk *C = g.mchunks + chn;

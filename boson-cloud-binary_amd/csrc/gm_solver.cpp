@@ -139,6 +139,11 @@ struct gm_solver {
     std::vector<double> GM_h, Gdiag_h;
     DevBuf<gmd::MChunk> mchunks;
     DevBuf<gmd::MBlock> mblocks;
+    DevBuf<gmd::MPass> mpass;
+    DevBuf<int> pass_start;
+    DevBuf<int> pass_steps;
+    DevBuf<gmd::MStep> msteps;
+    DevBuf<int> mstep_start;
     DevBuf<double> GM, Gdiag, E, dmask;
     DevBuf<float> GM32, E32;  // FP32 copies for the mixed-precision Krylov applies
     DevBuf<int> lcut;
@@ -369,16 +374,19 @@ void build_mfma_tables(gm_solver &S)
             block_range(ch, l1, B.tlo, B.nT);
             B.goff = static_cast<int>(S.GM_h.size());
             const int p = (ch.par + l1) & 1;
+            // slots padded with zeros to a multiple of four (one MFMA k-group)
+            const int nTp = (B.nT + 3) / 4 * 4;
             for (int m1i = 0; m1i <= l1; ++m1i)
-                for (int t = 0; t < B.nT; ++t)
+                for (int t = 0; t < nTp; ++t)
                     for (int r = 0; r < gmd::MR; ++r) {
                         double v = 0.0;
-                        if (ch.row_j[r] >= 0) v = angular(S, ch.row_j[r], l1 * (l1 + 1) / 2 + m1i, 2 * (B.tlo + t) + p);
+                        if (ch.row_j[r] >= 0 && t < B.nT)
+                            v = angular(S, ch.row_j[r], l1 * (l1 + 1) / 2 + m1i, 2 * (B.tlo + t) + p);
                         S.GM_h.push_back(v);
                     }
         }
     }
-    // fragments of the last block may read up to 3 slots past its end
+    // fragments read past the last block stay in bounds
     S.GM_h.insert(S.GM_h.end(), 4 * gmd::MR, 0.0);
     S.nchunks = static_cast<int>(S.mchunks_h.size());
     // diagonal coefficients (j, j) at l* = 2 s for the Thomas pivots
@@ -458,6 +466,12 @@ gmd::Geometry geometry(const gm_solver &S)
     g.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
     g.mchunks = S.mchunks.p;
     g.mblocks = S.mblocks.p;
+    g.mpass = S.mpass.p;
+    g.pass_start = S.pass_start.p;
+    g.pass_steps = S.pass_steps.p;
+    g.msteps = S.msteps.p;
+    g.step_start = S.mstep_start.p;
+    g.gm_n = static_cast<unsigned>(S.GM_h.size());
     g.GM = S.GM.p;
     g.GM32 = S.GM32.p;
     g.E32 = S.E32.p;
@@ -1121,6 +1135,68 @@ void alloc_device(gm_solver &S)
     // tables are uploaded from pageable vectors: synchronous copies
     HIPCK(hipMemcpy(S.mchunks.p, S.mchunks_h.data(), S.mchunks_h.size() * sizeof(gmd::MChunk), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(S.mblocks.p, S.mblocks_h.data(), S.mblocks_h.size() * sizeof(gmd::MBlock), hipMemcpyHostToDevice));
+    {
+        // pass table: per chunk, l1 from l_max down to 0, slot groups of four
+        const int Lm = S.l_max + 1;
+        std::vector<gmd::MPass> passes;
+        std::vector<int> start(static_cast<size_t>(S.nchunks) + 1, 0);
+        for (int c = 0; c < S.nchunks; ++c) {
+            start[c] = static_cast<int>(passes.size());
+            for (int l1 = S.l_max; l1 >= 0; --l1) {
+                const gmd::MBlock &B = S.mblocks_h[static_cast<size_t>(c) * Lm + l1];
+                const int p = (S.mchunks_h[c].par + l1) & 1;
+                const int next = static_cast<int>(passes.size()) + (B.nT + 3) / 4;
+                for (int s0 = 0; s0 < B.nT; s0 += 4) {
+                    gmd::MPass e{};
+                    e.goff = B.goff + s0 * gmd::MR;
+                    e.mstride = (B.nT + 3) / 4 * 4 * gmd::MR;
+                    e.kb = l1 * (l1 + 1) / 2;
+                    e.ls0 = 2 * (B.tlo + s0) + p;
+                    e.nv = std::min(4, B.nT - s0);
+                    e.l1 = l1;
+                    e.next = next;
+                    passes.push_back(e);
+                }
+            }
+        }
+        start[S.nchunks] = static_cast<int>(passes.size());
+        passes.push_back(gmd::MPass{});  // sentinel
+        // steps of each chunk's stream at every multipole cut lc: the passes
+        // with ls0 <= lc, l1 + 1 steps each
+        const int Lc = 2 * S.l_max + 1;
+        std::vector<int> steps(static_cast<size_t>(S.nchunks) * Lc, 0);
+        for (int c = 0; c < S.nchunks; ++c)
+            for (int lc = 0; lc < Lc; ++lc)
+                for (int i = start[c]; i < start[c + 1]; ++i)
+                    if (passes[i].ls0 <= lc) steps[static_cast<size_t>(c) * Lc + lc] += passes[i].l1 + 1;
+        // the step list of each chunk, sorted by the first slot's l*
+        std::vector<gmd::MStep> msteps;
+        std::vector<int> sstart(static_cast<size_t>(S.nchunks) + 1, 0);
+        for (int c = 0; c < S.nchunks; ++c) {
+            sstart[c] = static_cast<int>(msteps.size());
+            std::vector<gmd::MStep> v;
+            for (int i = start[c]; i < start[c + 1]; ++i)
+                for (int m = 0; m <= passes[i].l1; ++m)
+                    v.push_back(gmd::MStep{passes[i].goff + m * passes[i].mstride, passes[i].kb + m, passes[i].ls0, 0});
+            std::stable_sort(v.begin(), v.end(), [](const gmd::MStep &a, const gmd::MStep &b) { return a.e < b.e; });
+            msteps.insert(msteps.end(), v.begin(), v.end());
+            // padding steps: valid G and column offsets, the zero LDS row
+            const gmd::MStep last = v.empty() ? gmd::MStep{} : v.back();
+            for (int k = 0; k + 1 < gmd::RING; ++k) msteps.push_back(gmd::MStep{last.g, last.x, gmd::ZROW, 0});
+        }
+        sstart[S.nchunks] = static_cast<int>(msteps.size());
+        msteps.push_back(gmd::MStep{});
+        S.msteps.alloc(msteps.size());
+        S.mstep_start.alloc(sstart.size());
+        HIPCK(hipMemcpy(S.msteps.p, msteps.data(), msteps.size() * sizeof(gmd::MStep), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.mstep_start.p, sstart.data(), sstart.size() * sizeof(int), hipMemcpyHostToDevice));
+        S.mpass.alloc(passes.size());
+        S.pass_start.alloc(start.size());
+        S.pass_steps.alloc(steps.size());
+        HIPCK(hipMemcpy(S.mpass.p, passes.data(), passes.size() * sizeof(gmd::MPass), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.pass_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.pass_steps.p, steps.data(), steps.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     HIPCK(hipMemcpy(S.GM.p, S.GM_h.data(), S.GM_h.size() * sizeof(double), hipMemcpyHostToDevice));
     {
         const std::vector<float> gm32(S.GM_h.begin(), S.GM_h.end());
