@@ -40,29 +40,16 @@ struct MBlock {
     int goff, tlo, nT, pad;
 };
 
-// One pass of a chunk's coupling stream: the multipole slots s0 .. s0+nv-1 of
-// the (chunk, l1) block against the columns m1 = 0 .. l1 of degree l1. A
-// chunk's passes are stored in order of decreasing l1, then increasing s0.
-struct MPass {
-    int goff;     // G offset of slot s0, row 0, at m1 = 0
-    int mstride;  // G offset from one m1 to the next (slots padded to a multiple of 4, times 16)
-    int kb;       // first column mode of degree l1
-    int ls0;      // l* of slot s0 (slot t has l* = ls0 + 2 t)
-    int nv;       // slots of the pass (1..4)
-    int l1;
-    int next;     // index of the first pass of the next l1 (skips the rest of this l1)
-    int pad;
-};
-
 // Depth of the operand ring of the coupling stream; each chunk's step list
 // ends with RING - 1 padding steps that add nothing.
 constexpr int RING = 4;
 // LDS row read by padding steps (every row above a tile's cut holds zeros).
 constexpr int ZROW = 2 * MAXLM - 1;
 
-// One step of a chunk's coupling stream: the slot group of a pass at one
-// column m1. A chunk's steps are sorted by the l* of their first slot, so the
-// steps a tile with multipole cut lc needs are the first step_count[lc].
+// One step of a chunk's coupling stream: four consecutive multipole slots of
+// the (chunk, l1) block at one column m1. A chunk's steps are sorted by the l*
+// of their first slot, so the steps a tile with multipole cut lc needs are
+// the first step_count[lc].
 struct MStep {
     int g;    // G element of (slot group, row 0) at this m1
     int x;    // column mode kb + m1
@@ -77,9 +64,7 @@ struct Geometry {
     double dr, eps, mu, off;   // off = -1/(2 mu dr^2), the stencil off-diagonal of H0
     const MChunk *mchunks;
     const MBlock *mblocks;     // nchunks * (l_max + 1)
-    const MPass *mpass;        // passes of every chunk
-    const int *pass_start;     // nchunks + 1 offsets into mpass
-    const int *pass_steps;     // nchunks x (2 l_max + 1): steps of a chunk's stream at cut lc
+    const int *step_count;     // nchunks x (2 l_max + 1): steps of a chunk's stream at cut lc
     const MStep *msteps;       // steps of every chunk
     const int *step_start;     // nchunks + 1 offsets into msteps
     unsigned gm_n;             // elements of GM / GM32

@@ -308,7 +308,7 @@ __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, cons
     constexpr int RING = gmd::RING;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int N = g.N;
-    const int total = g.pass_steps[chn * (2 * g.lmax + 1) + lc];
+    const int total = g.step_count[chn * (2 * g.lmax + 1) + lc];
     if (total == 0) return;
     const gmd::MStep *st = g.msteps + g.step_start[chn];
     const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(GM, g.gm_n * (unsigned)sizeof(T));
@@ -412,6 +412,10 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     typedef Arith<T> AR;
     constexpr int TN = 16 * NG;
     __shared__ T sE[(gmd::ZROW + 7) * TN];  // rows l* = 0 .. lc, then zeros
+    // the workgroup's chunk rows (mode, l, m) and the phases exp(i m theta),
+    // staged with the profiles for the epilogue
+    __shared__ int sJ[gmd::AW][gmd::MR], sL[gmd::AW][gmd::MR], sM[gmd::AW][gmd::MR];
+    __shared__ double2 sPh[2 * gmd::MAXLM + 1];
     const int MODE = a.mode;
     const Geometry &g = a.g;
     Krylov *Kr = a.Kr;
@@ -453,28 +457,35 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
             sE[e] = node < N ? E[(size_t)((ls & 1) * g.Ls + ls) * N + node] : T(0);
         }
         for (int e = (lc + 1) * TN + threadIdx.x; e < (gmd::ZROW + 7) * TN; e += 64 * gmd::AW) sE[e] = T(0);
+        for (int e = threadIdx.x; e < gmd::AW * gmd::MR; e += 64 * gmd::AW) {
+            const int w = e / gmd::MR, r = e % gmd::MR, c = grp * gmd::AW + w;
+            const bool ok = c < g.nchunks;
+            sJ[w][r] = ok ? g.mchunks[c].row_j[r] : -1;
+            sL[w][r] = ok ? g.mchunks[c].row_l[r] : 0;
+            sM[w][r] = ok ? g.mchunks[c].row_m[r] : 0;
+        }
+        for (int e = threadIdx.x; e < 2 * lmax + 1; e += 64 * gmd::AW) sPh[e] = P->ph[e];
         __syncthreads();
         if (wave_ok)
             mfma_stream<NG, T>(g, GM, static_cast<const typename AR::C *>(a.Xt), t0, chn, lc, sE, acc);
     }
     if (wave_ok) {
-        const MChunk *C = g.mchunks + chn;
         // epilogue: lane owns rows q + 4r and nodes 16 n + col
         const double alpha2 = 1.0 / (g.dr * g.dr);
         const double hdt = P->half_dt;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            // one row at a time: bounds the loads the compiler keeps in flight
-            asm volatile("" ::: "memory");
+            // two rows at a time: bounds the loads the compiler keeps in flight
+            if ((r & 1) == 0) asm volatile("" ::: "memory");
             const int rl = AR::row(q, r);
-            const int j = C->row_j[rl];
-            const int l = C->row_l[rl], m = C->row_m[rl];
+            const int j = sJ[wv][rl];
+            const int l = sL[wv][rl], m = sM[wv][rl];
 #pragma unroll
             for (int n = 0; n < NG; ++n) {
                 const int i = t0 + 16 * n + col;
                 if (j < 0 || i >= N) continue;
                 const size_t row = (size_t)j * N + i;
-                const double2 Cc = cdotm(P->ph[m + lmax], AR::to_d(acc[n][r]));
+                const double2 Cc = cdotm(sPh[m + lmax], AR::to_d(acc[n][r]));
                 if (a.Cout) a.Cout[row] = Cc;
                 const double2 xi = load_x(a, j, i);
                 const double2 capv = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
@@ -789,6 +800,16 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
 
 // ---------------------------------------------------------------- Thomas solve
 
+// Workgroup barrier that orders LDS accesses only: global loads and stores in
+// flight stay in flight (__syncthreads also fences global memory, which waits
+// for every outstanding load and store).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename C>
 struct Aff {
     C A, F;
@@ -837,7 +858,7 @@ __device__ Aff<C> block_scan(Aff<C> v, Aff<C> &total)
         ex.F = czero<C>();
     }
     if (vl == 63) wt[wv] = inc;
-    __syncthreads();
+    lds_barrier();
     const int nw = gmd::PT / 64;
     Aff<C> pre;
     pre.A = cone<C>();
@@ -848,7 +869,7 @@ __device__ Aff<C> block_scan(Aff<C> v, Aff<C> &total)
         if (w == wv) pre = tot;
         tot = compose(tot, wt[w]);
     }
-    __syncthreads();
+    lds_barrier();
     total = tot;
     return compose(pre, ex);
 }
@@ -885,9 +906,32 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
         wv = Kr->w;
     }
     if (VAR == PV_S) alpha = Kr->alpha;
+    // Both sweeps walk the row in tiles of TE nodes; the global inputs of the
+    // next tile are loaded (at clamped, always valid indices) while the
+    // current one is scanned, two tiles per loop turn so that the buffers
+    // need no copies.
+    struct FIn {
+        double2 r, p, v;
+        C iv;
+    };
+    auto fload = [&](const int t, FIn (&b)[E]) __attribute__((always_inline)) {
+        const int tc = min(t, ntile - 1);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const size_t q = base_j + min(tc * TE + tid + T * e, N - 1);
+            if (VAR == PV_PLAIN) {
+                b[e].r = a.X[q];
+            } else {
+                b[e].r = a.R[q];
+                b[e].v = a.V[q];
+                if (VAR == PV_P) b[e].p = a.Pv[q];  // unused at the first iteration
+            }
+            b[e].iv = pivot(q);
+        }
+    };
     // forward sweep
     C carry = czero<C>();
-    for (int t = 0; t < ntile; ++t) {
+    auto fproc = [&](const int t, const FIn (&b)[E]) __attribute__((always_inline)) {
         const int t0 = t * TE;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -898,15 +942,15 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
                 const size_t q = base_j + idx;
                 double2 in;
                 if (VAR == PV_P) {
-                    in = first ? a.R[q] : cadd(a.R[q], cmul(beta, csub(a.Pv[q], cmul(wv, a.V[q]))));
+                    in = first ? b[e].r : cadd(b[e].r, cmul(beta, csub(b[e].p, cmul(wv, b[e].v))));
                     a.Pv[q] = in;
                 } else if (VAR == PV_S) {
-                    in = csub(a.R[q], cmul(alpha, a.V[q]));
+                    in = csub(b[e].r, cmul(alpha, b[e].v));
                     a.Pv[q] = in;
                 } else {
-                    in = a.X[q];
+                    in = b[e].r;
                 }
-                const C iv = pivot(q);
+                const C iv = b[e].iv;
                 fv = cmul(iv, Arith<RT>::from(in));
                 av = (idx >= 1 && idx <= N - 2) ? C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)}
                                                 : czero<C>();
@@ -914,7 +958,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             sa[lds_slot(pos)] = av;
             sf[lds_slot(pos)] = fv;
         }
-        __syncthreads();
+        lds_barrier();
         Aff<C> loc;
         loc.A = cone<C>();
         loc.F = czero<C>();
@@ -935,7 +979,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             sf[lds_slot(tid * E + e)] = y;
         }
         carry = cadd(cmul(tot.A, carry), tot.F);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int pos = tid + T * e;
@@ -945,12 +989,37 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
                 else a.Y[base_j + idx] = sf[lds_slot(pos)];
             }
         }
-        __syncthreads();
+        lds_barrier();
+    };
+    {
+        FIn fa[E], fb[E];
+        fload(0, fa);
+        for (int t = 0; t < ntile; t += 2) {
+            fload(t + 1, fb);
+            fproc(t, fa);
+            fload(t + 2, fa);
+            if (t + 1 < ntile) fproc(t + 1, fb);
+        }
     }
-    // backward sweep
+    // backward sweep; the forward results written by the other threads of the
+    // block are read back after this barrier (with its global fence)
+    __syncthreads();
+    struct BIn {
+        C y, iv;
+    };
+    auto bload = [&](const int t, BIn (&b)[E]) __attribute__((always_inline)) {
+        const int tc = max(t, 0);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const size_t q = base_j + min(tc * TE + tid + T * e, N - 1);
+            if constexpr (F32) b[e].y = a.Y32[q];
+            else b[e].y = a.Y[q];
+            b[e].iv = pivot(q);
+        }
+    };
     int nz_lo = N, nz_hi = 0;
     carry = czero<C>();
-    for (int t = ntile - 1; t >= 0; --t) {
+    auto bproc = [&](const int t, const BIn (&b)[E]) __attribute__((always_inline)) {
         const int t0 = t * TE;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -958,17 +1027,15 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             const int idx = t0 + pos;
             C cv = cone<C>(), fv = czero<C>();
             if (idx < N) {
-                const size_t q = base_j + idx;
-                const C iv = pivot(q);
-                if constexpr (F32) fv = a.Y32[q];
-                else fv = a.Y[q];
+                const C iv = b[e].iv;
+                fv = b[e].y;
                 cv = (idx >= 1 && idx <= N - 2) ? C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)}
                                                 : czero<C>();
             }
             sa[lds_slot(pos)] = cv;
             sf[lds_slot(pos)] = fv;
         }
-        __syncthreads();
+        lds_barrier();
         Aff<C> loc;
         loc.A = cone<C>();
         loc.F = czero<C>();
@@ -989,7 +1056,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             sf[lds_slot(tid * E + e)] = y;
         }
         carry = cadd(cmul(tot.A, carry), tot.F);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int pos = tid + T * e;
@@ -1003,7 +1070,17 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
+    };
+    {
+        BIn ba[E], bb[E];
+        bload(ntile - 1, ba);
+        for (int t = ntile - 1; t >= 0; t -= 2) {
+            bload(t - 1, bb);
+            bproc(t, ba);
+            bload(t - 2, ba);
+            if (t - 1 >= 0) bproc(t - 1, bb);
+        }
     }
     if (VAR == PV_PLAIN && a.range) {
         // exact non-zero node range of the solution over all modes

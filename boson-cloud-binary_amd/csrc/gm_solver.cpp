@@ -139,9 +139,7 @@ struct gm_solver {
     std::vector<double> GM_h, Gdiag_h;
     DevBuf<gmd::MChunk> mchunks;
     DevBuf<gmd::MBlock> mblocks;
-    DevBuf<gmd::MPass> mpass;
-    DevBuf<int> pass_start;
-    DevBuf<int> pass_steps;
+    DevBuf<int> step_count;
     DevBuf<gmd::MStep> msteps;
     DevBuf<int> mstep_start;
     DevBuf<double> GM, Gdiag, E, dmask;
@@ -322,10 +320,16 @@ void build_mfma_tables(gm_solver &S)
     const int Lm = S.l_max + 1;
     S.mchunks_h.clear();
     for (int par = 0; par < 2; ++par) {
+        // rows of one l parity ordered by m, then by decreasing l: a chunk of
+        // 16 rows spans a narrow band of m, so columns far from it in m1 (and
+        // the low multipoles of the columns near it) carry no coupling
         std::vector<int> rows;
         for (int l = S.l_max; l >= 0; --l)
             if (l % 2 == par)
                 for (int mi = 0; mi <= l; ++mi) rows.push_back(l * (l + 1) / 2 + mi);
+        std::stable_sort(rows.begin(), rows.end(), [&](int x, int y) {
+            return S.mm[x] != S.mm[y] ? S.mm[x] < S.mm[y] : S.ml[x] > S.ml[y];
+        });
         for (size_t r0 = 0; r0 < rows.size(); r0 += gmd::MR) {
             gmd::MChunk c{};
             c.par = par;
@@ -466,9 +470,7 @@ gmd::Geometry geometry(const gm_solver &S)
     g.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
     g.mchunks = S.mchunks.p;
     g.mblocks = S.mblocks.p;
-    g.mpass = S.mpass.p;
-    g.pass_start = S.pass_start.p;
-    g.pass_steps = S.pass_steps.p;
+    g.step_count = S.step_count.p;
     g.msteps = S.msteps.p;
     g.step_start = S.mstep_start.p;
     g.gm_n = static_cast<unsigned>(S.GM_h.size());
@@ -1136,49 +1138,44 @@ void alloc_device(gm_solver &S)
     HIPCK(hipMemcpy(S.mchunks.p, S.mchunks_h.data(), S.mchunks_h.size() * sizeof(gmd::MChunk), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(S.mblocks.p, S.mblocks_h.data(), S.mblocks_h.size() * sizeof(gmd::MBlock), hipMemcpyHostToDevice));
     {
-        // pass table: per chunk, l1 from l_max down to 0, slot groups of four
-        const int Lm = S.l_max + 1;
-        std::vector<gmd::MPass> passes;
-        std::vector<int> start(static_cast<size_t>(S.nchunks) + 1, 0);
-        for (int c = 0; c < S.nchunks; ++c) {
-            start[c] = static_cast<int>(passes.size());
-            for (int l1 = S.l_max; l1 >= 0; --l1) {
-                const gmd::MBlock &B = S.mblocks_h[static_cast<size_t>(c) * Lm + l1];
-                const int p = (S.mchunks_h[c].par + l1) & 1;
-                const int next = static_cast<int>(passes.size()) + (B.nT + 3) / 4;
-                for (int s0 = 0; s0 < B.nT; s0 += 4) {
-                    gmd::MPass e{};
-                    e.goff = B.goff + s0 * gmd::MR;
-                    e.mstride = (B.nT + 3) / 4 * 4 * gmd::MR;
-                    e.kb = l1 * (l1 + 1) / 2;
-                    e.ls0 = 2 * (B.tlo + s0) + p;
-                    e.nv = std::min(4, B.nT - s0);
-                    e.l1 = l1;
-                    e.next = next;
-                    passes.push_back(e);
-                }
-            }
-        }
-        start[S.nchunks] = static_cast<int>(passes.size());
-        passes.push_back(gmd::MPass{});  // sentinel
-        // steps of each chunk's stream at every multipole cut lc: the passes
-        // with ls0 <= lc, l1 + 1 steps each
-        const int Lc = 2 * S.l_max + 1;
-        std::vector<int> steps(static_cast<size_t>(S.nchunks) * Lc, 0);
-        for (int c = 0; c < S.nchunks; ++c)
-            for (int lc = 0; lc < Lc; ++lc)
-                for (int i = start[c]; i < start[c + 1]; ++i)
-                    if (passes[i].ls0 <= lc) steps[static_cast<size_t>(c) * Lc + lc] += passes[i].l1 + 1;
-        // the step list of each chunk, sorted by the first slot's l*
+        // The coupling stream of each chunk: for every column (l1, m1), the
+        // slot groups of four covering the slots where some row of the chunk
+        // has a non-zero Gaunt coefficient (the groups stay inside the
+        // block's nTp slots; slots outside that range hold zeros for this
+        // column, so a group may extend over them). Steps are sorted by the
+        // l* of their first slot, so a tile with cut lc runs a prefix.
+        const int Lm = S.l_max + 1, Lc = 2 * S.l_max + 1;
         std::vector<gmd::MStep> msteps;
         std::vector<int> sstart(static_cast<size_t>(S.nchunks) + 1, 0);
+        std::vector<int> count(static_cast<size_t>(S.nchunks) * Lc, 0);
         for (int c = 0; c < S.nchunks; ++c) {
             sstart[c] = static_cast<int>(msteps.size());
             std::vector<gmd::MStep> v;
-            for (int i = start[c]; i < start[c + 1]; ++i)
-                for (int m = 0; m <= passes[i].l1; ++m)
-                    v.push_back(gmd::MStep{passes[i].goff + m * passes[i].mstride, passes[i].kb + m, passes[i].ls0, 0});
-            std::stable_sort(v.begin(), v.end(), [](const gmd::MStep &a, const gmd::MStep &b) { return a.e < b.e; });
+            for (int l1 = S.l_max; l1 >= 0; --l1) {
+                const gmd::MBlock &B = S.mblocks_h[static_cast<size_t>(c) * Lm + l1];
+                const int p = (S.mchunks_h[c].par + l1) & 1;
+                const int nTp = (B.nT + 3) / 4 * 4;
+                for (int m1i = 0; m1i <= l1; ++m1i) {
+                    int tl = nTp, th = -1;
+                    for (int t = 0; t < B.nT; ++t)
+                        for (int r = 0; r < gmd::MR; ++r)
+                            if (S.GM_h[static_cast<size_t>(B.goff) + (static_cast<size_t>(m1i) * nTp + t) * gmd::MR + r] != 0.0) {
+                                tl = std::min(tl, t);
+                                th = std::max(th, t);
+                            }
+                    if (th < 0) continue;
+                    const int ng = (th - tl + 4) / 4;
+                    const int t0 = std::min(tl, nTp - 4 * ng);
+                    for (int k = 0; k < ng; ++k) {
+                        const int t = t0 + 4 * k;
+                        v.push_back(gmd::MStep{B.goff + (m1i * nTp + t) * gmd::MR, l1 * (l1 + 1) / 2 + m1i,
+                                               2 * (B.tlo + t) + p, 0});
+                    }
+                }
+            }
+            std::stable_sort(v.begin(), v.end(), [](const gmd::MStep &x, const gmd::MStep &y) { return x.e < y.e; });
+            for (int lc = 0; lc < Lc; ++lc)
+                for (const gmd::MStep &e : v) count[static_cast<size_t>(c) * Lc + lc] += e.e <= lc;
             msteps.insert(msteps.end(), v.begin(), v.end());
             // padding steps: valid G and column offsets, the zero LDS row
             const gmd::MStep last = v.empty() ? gmd::MStep{} : v.back();
@@ -1188,14 +1185,10 @@ void alloc_device(gm_solver &S)
         msteps.push_back(gmd::MStep{});
         S.msteps.alloc(msteps.size());
         S.mstep_start.alloc(sstart.size());
+        S.step_count.alloc(count.size());
         HIPCK(hipMemcpy(S.msteps.p, msteps.data(), msteps.size() * sizeof(gmd::MStep), hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(S.mstep_start.p, sstart.data(), sstart.size() * sizeof(int), hipMemcpyHostToDevice));
-        S.mpass.alloc(passes.size());
-        S.pass_start.alloc(start.size());
-        S.pass_steps.alloc(steps.size());
-        HIPCK(hipMemcpy(S.mpass.p, passes.data(), passes.size() * sizeof(gmd::MPass), hipMemcpyHostToDevice));
-        HIPCK(hipMemcpy(S.pass_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
-        HIPCK(hipMemcpy(S.pass_steps.p, steps.data(), steps.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.step_count.p, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     HIPCK(hipMemcpy(S.GM.p, S.GM_h.data(), S.GM_h.size() * sizeof(double), hipMemcpyHostToDevice));
     {
