@@ -876,6 +876,14 @@ __device__ Aff<C> block_scan(Aff<C> v, Aff<C> &total)
 
 __device__ __forceinline__ int lds_slot(int pos) { return pos + pos / gmd::PE; }
 
+// exp(i m theta) y into the coupling scratch of the next apply, in its arithmetic
+__device__ __forceinline__ void store_phased(void *Xt, int f32, size_t q, double2 ph, double2 y)
+{
+    const double2 v = cmul(ph, y);
+    if (f32) ((float2 *)Xt)[q] = make_float2((float)v.x, (float)v.y);
+    else ((double2 *)Xt)[q] = v;
+}
+
 template <int VAR, typename RT>
 __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
 {
@@ -1019,6 +1027,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     };
     int nz_lo = N, nz_hi = 0;
     carry = czero<C>();
+    const double2 phj = a.Xt ? a.Ps->ph[mode_m(j) + a.lmax] : make_double2(1.0, 0.0);
     auto bproc = [&](const int t, const BIn (&b)[E]) __attribute__((always_inline)) {
         const int t0 = t * TE;
 #pragma unroll
@@ -1064,6 +1073,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             if (idx < N) {
                 const double2 yv = Arith<RT>::to_d(sf[lds_slot(pos)]);
                 a.Y[base_j + idx] = yv;
+                if (a.Xt) store_phased(a.Xt, a.xt_f32, base_j + idx, phj, yv);
                 if (VAR == PV_PLAIN && a.range && (yv.x != 0.0 || yv.y != 0.0)) {
                     nz_lo = min(nz_lo, idx);
                     nz_hi = max(nz_hi, idx + 1);
@@ -1393,7 +1403,10 @@ __global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a)
         // stored columns W(theta_W); W(theta) = exp(-i m (theta - theta_W)) W(theta_W)
         double2 w = cmul(c0, a.W0[gi]);
         if (ns > 1) w = cadd(w, cmul(c1, a.W1[gi]));
-        a.Y[gi] = csub(a.Y[gi], cmul(a.P->wrot[mode_m(k) + a.lmax], w));
+        const int mk = mode_m(k);
+        const double2 y = csub(a.Y[gi], cmul(a.P->wrot[mk + a.lmax], w));
+        a.Y[gi] = y;
+        if (a.Xt) store_phased(a.Xt, a.xt_f32, gi, a.P->ph[mk + a.lmax], y);
     }
 }
 
@@ -1481,7 +1494,7 @@ hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
     if (!a.Xt) return hipErrorInvalidValue;
     const int grid = apply_grid(a.g.N, a.g.nchunks, a.ng, &a.ngroups);
     a.mode = mode;
-    {
+    if (!a.xt_ready) {
         const dim3 pg((a.g.N + 255) / 256 < 16 ? (a.g.N + 255) / 256 : 16, a.g.K);
         if (a.mixed)
             hipLaunchKernelGGL(k_phase<float2>, pg, dim3(256), 0, st, a.X, (float2 *)a.Xt, a.g.N, a.g.lmax, a.P);

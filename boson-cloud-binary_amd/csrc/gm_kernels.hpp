@@ -28,6 +28,7 @@ struct ApplyArgs {
     int mixed;           // coupling in FP32 (E32, GM32) with FP32 accumulation
     int ng;              // 16-node groups per tile (1, 2 or 4), chosen by the autotuner
     void *Xt;            // scratch: exp(i m theta) X, double2 (FP64) or float2 (mixed), K * N
+    int xt_ready;        // Xt already holds the phased X (written by its producer): no phase pass
     int mode;            // ApplyMode, set by the launcher
 };
 
@@ -59,6 +60,8 @@ struct PrecArgs {
     int lmax;            // Woodbury epilogue: mode -> m for the column rotation
     const float2 *inv32; // FP32 pivots: with Y32, the p / s variants sweep in FP32 (mixed precision)
     float2 *Y32;         // FP32 forward-sweep scratch
+    void *Xt;            // also write exp(i m theta) y here for the next apply (or nullptr)
+    int xt_f32;          // Xt holds float2 (FP32 coupling) instead of double2
 };
 
 struct WcorrArgs {
@@ -67,6 +70,8 @@ struct WcorrArgs {
     const gmd::Krylov *Kr;
     const double2 *W0, *W1;
     double2 *Y;
+    void *Xt;            // phased copy of Y to correct as well (or nullptr)
+    int xt_f32;
 };
 
 struct UpdateArgs {

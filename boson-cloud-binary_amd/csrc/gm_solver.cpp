@@ -801,7 +801,7 @@ void woodbury_setup(gm_solver &S)
 }
 
 // y -= sum_a c_a W_a over the columns' support (matrices.cpp:151)
-void woodbury_correct(gm_solver &S, double2 *Y)
+void woodbury_correct(gm_solver &S, double2 *Y, void *Xt = nullptr, int xt_f32 = 0)
 {
     const gmd::StepParams &P = *S.hP.p;
     gmk::WcorrArgs w{};
@@ -812,6 +812,8 @@ void woodbury_correct(gm_solver &S, double2 *Y)
     w.W0 = S.sink[P.sink_body[0]].W.p;
     w.W1 = P.nsink > 1 ? S.sink[P.sink_body[1]].W.p : nullptr;
     w.Y = Y;
+    w.Xt = Xt;
+    w.xt_f32 = xt_f32;
     w.P = S.dP.p;
     w.lmax = S.l_max;
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_wcorr(w, S.st); });
@@ -826,29 +828,35 @@ void enqueue_iteration(gm_solver &S, bool wood)
         pa.inv32 = S.inv32.p;
         pa.Y32 = S.y32.p;
     }
+    // mixed precision: the coupling of the two Krylov applies runs in FP32;
+    // rhs, residual check and diagnostics keep the FP64 operator
+    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
+    // the preconditioner (and the Woodbury correction) also write the phased
+    // copy exp(i m theta) y that the next apply's coupling reads
+    pa.Xt = S.xt.p;
+    pa.xt_f32 = mixed;
     // y = M^-1 p with p = r + beta (p - w v)
     pa.R = S.rr.p;
     pa.Pv = S.p.p;
     pa.V = S.v.p;
     pa.Y = S.y.p;
     launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_P, S.st); });
-    if (wood) woodbury_correct(S, S.y.p);
-    // mixed precision: the coupling of the two Krylov applies runs in FP32;
-    // rhs, residual check and diagnostics keep the FP64 operator
-    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
+    if (wood) woodbury_correct(S, S.y.p, S.xt.p, mixed);
     gmk::ApplyArgs a = apply_args(S, S.y.p, S.v.p);
     a.R0 = S.r0.p;
     a.mixed = mixed;
+    a.xt_ready = 1;
     if (mixed && S.apply_ng[1]) a.ng = S.apply_ng[1];
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_V, S.st); });
     // z = M^-1 s with s = r - alpha v
     pa.Pv = S.s.p;
     pa.Y = S.z.p;
     launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_S, S.st); });
-    if (wood) woodbury_correct(S, S.z.p);
+    if (wood) woodbury_correct(S, S.z.p, S.xt.p, mixed);
     a = apply_args(S, S.z.p, S.t.p);
     a.R0 = S.s.p;
     a.mixed = mixed;
+    a.xt_ready = 1;
     if (mixed && S.apply_ng[1]) a.ng = S.apply_ng[1];
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_T, S.st); });
     gmk::UpdateArgs u{};
