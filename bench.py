@@ -105,11 +105,13 @@ def profile_read(s):
     return s.profile_read()
 
 
-def run_gpu(w, precision, steps, warmup, prof_steps=5):
-    """Warmup, then `steps` timed steps (graph replay on, no per-kernel events),
-    then `prof_steps` steps with HIP-event kernel timing (graphs off) for the
-    kernel table and the roofline."""
+def run_gpu(w, precision, steps, warmup, prof_steps=5, graphs=True):
+    """Warmup, then `steps` timed steps (graph replay on unless `graphs` is
+    false, no per-kernel events), then `prof_steps` steps with HIP-event kernel
+    timing (graphs off) for the kernel table and the roofline."""
     s = make_solver(w, precision)
+    if not graphs:
+        s.set_flags(use_graphs=0)
     k = 0
     for _ in range(warmup):
         k += 1
@@ -283,6 +285,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the served Poisson stream for p50/p99 step latency (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="timed steps without graph replay (for kernel-trace profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -296,7 +300,7 @@ def main():
     w = dict(WORKLOAD)
     runs = {}
     for name, precision in (("fp64", gm.GM_PRECISION_FP64), ("mixed", gm.GM_PRECISION_MIXED)):
-        s, elapsed, prof, iters, piters = run_gpu(w, precision, args.steps, args.warmup)
+        s, elapsed, prof, iters, piters = run_gpu(w, precision, args.steps, args.warmup, graphs=not args.no_graphs)
         K, N = s.n_modes, s.Nr
         roof, kernels = rooflines(s, prof, K, N, piters, name == "mixed")
         runs[name] = {"value": K * N * args.steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3,

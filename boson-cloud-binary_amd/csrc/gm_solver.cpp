@@ -140,6 +140,7 @@ struct gm_solver {
     DevBuf<gmd::MChunk> mchunks;
     DevBuf<gmd::MBlock> mblocks;
     DevBuf<int> step_count;
+    unsigned gm_n = 0;  // elements of GM / GM32 (Gaunt fragments in stream order)
     DevBuf<gmd::MStep> msteps;
     DevBuf<int> mstep_start;
     DevBuf<double> GM, Gdiag, E, dmask;
@@ -473,7 +474,7 @@ gmd::Geometry geometry(const gm_solver &S)
     g.step_count = S.step_count.p;
     g.msteps = S.msteps.p;
     g.step_start = S.mstep_start.p;
-    g.gm_n = static_cast<unsigned>(S.GM_h.size());
+    g.gm_n = S.gm_n;
     g.GM = S.GM.p;
     g.GM32 = S.GM32.p;
     g.E32 = S.E32.p;
@@ -1139,8 +1140,6 @@ void alloc_device(gm_solver &S)
     S.hdiag.alloc(static_cast<size_t>(8) * S.K);
     S.mchunks.alloc(S.mchunks_h.size());
     S.mblocks.alloc(S.mblocks_h.size());
-    S.GM.alloc(S.GM_h.size());
-    S.GM32.alloc(S.GM_h.size());
     S.Gdiag.alloc(S.Gdiag_h.size());
     // tables are uploaded from pageable vectors: synchronous copies
     HIPCK(hipMemcpy(S.mchunks.p, S.mchunks_h.data(), S.mchunks_h.size() * sizeof(gmd::MChunk), hipMemcpyHostToDevice));
@@ -1191,17 +1190,32 @@ void alloc_device(gm_solver &S)
         }
         sstart[S.nchunks] = static_cast<int>(msteps.size());
         msteps.push_back(gmd::MStep{});
+        // Gaunt fragments in stream order: the 4 slots x 16 rows of every step
+        // stored contiguously (a tile streams a prefix of its chunk's table)
+        std::vector<double> gmc;
+        gmc.reserve(msteps.size() * 4 * gmd::MR);
+        for (size_t k = 0; k < msteps.size(); ++k) {
+            gmd::MStep &e = msteps[k];
+            if (e.e == gmd::ZROW && k > 0) {  // padding step: the previous step's fragment
+                e.g = msteps[k - 1].g;
+                continue;
+            }
+            const size_t src = static_cast<size_t>(e.g);
+            e.g = static_cast<int>(gmc.size());
+            for (int v = 0; v < 4 * gmd::MR; ++v) gmc.push_back(src + v < S.GM_h.size() ? S.GM_h[src + v] : 0.0);
+        }
+        S.gm_n = static_cast<unsigned>(gmc.size());
+        S.GM.alloc(gmc.size());
+        S.GM32.alloc(gmc.size());
+        HIPCK(hipMemcpy(S.GM.p, gmc.data(), gmc.size() * sizeof(double), hipMemcpyHostToDevice));
+        const std::vector<float> gm32(gmc.begin(), gmc.end());
+        HIPCK(hipMemcpy(S.GM32.p, gm32.data(), gm32.size() * sizeof(float), hipMemcpyHostToDevice));
         S.msteps.alloc(msteps.size());
         S.mstep_start.alloc(sstart.size());
         S.step_count.alloc(count.size());
         HIPCK(hipMemcpy(S.msteps.p, msteps.data(), msteps.size() * sizeof(gmd::MStep), hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(S.mstep_start.p, sstart.data(), sstart.size() * sizeof(int), hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(S.step_count.p, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice));
-    }
-    HIPCK(hipMemcpy(S.GM.p, S.GM_h.data(), S.GM_h.size() * sizeof(double), hipMemcpyHostToDevice));
-    {
-        const std::vector<float> gm32(S.GM_h.begin(), S.GM_h.end());
-        HIPCK(hipMemcpy(S.GM32.p, gm32.data(), gm32.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     HIPCK(hipMemcpy(S.Gdiag.p, S.Gdiag_h.data(), S.Gdiag_h.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCK(hipMemsetAsync(S.dK.p, 0, sizeof(gmd::Krylov), S.st));
