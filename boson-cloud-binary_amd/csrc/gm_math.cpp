@@ -15,9 +15,22 @@ using f128 = __float128;
 f128 sqrt128(f128 v)
 {
     if (v <= 0) return 0;
+    // into the double range by exact powers of 4 (3j prefactors pass 1e308
+    // from degree ~25 on), then Newton from the double root
+    f128 scale = 1;
+    const f128 big = static_cast<f128>(1e300), small = static_cast<f128>(1e-300);
+    const f128 down = static_cast<f128>(std::ldexp(1.0, -200)), up = static_cast<f128>(std::ldexp(1.0, 200));
+    while (v > big) {
+        v *= down;
+        scale *= static_cast<f128>(std::ldexp(1.0, 100));
+    }
+    while (v < small) {
+        v *= up;
+        scale *= static_cast<f128>(std::ldexp(1.0, -100));
+    }
     f128 x = static_cast<f128>(std::sqrt(static_cast<double>(v)));
     for (int k = 0; k < 3; ++k) x = (x + v / x) / 2;
-    return x;
+    return x * scale;
 }
 
 const std::vector<f128> &factorials()

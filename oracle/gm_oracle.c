@@ -46,9 +46,20 @@ typedef __float128 q128;
 static q128 q_sqrt(q128 v)
 {
     if (v <= 0) return 0;
+    /* bring v into the double range by exact powers of 4 (the Racah
+     * prefactors pass 1e308 from l ~ 25 on), then Newton from the double root */
+    q128 scale = 1;
+    while (v > 1e300Q) {
+        v *= 0x1p-200Q;
+        scale *= 0x1p100Q;
+    }
+    while (v < 1e-300Q) {
+        v *= 0x1p200Q;
+        scale *= 0x1p-100Q;
+    }
     q128 x = (q128)sqrt((double)v);
     for (int it = 0; it < 3; ++it) x = 0.5Q * (x + v / x);
-    return x;
+    return x * scale;
 }
 
 static q128 q_fact(int n)

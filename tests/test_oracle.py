@@ -22,6 +22,19 @@ def test_wigner3j_matches_sympy(args):
     assert oracle.wigner3j(*args) == pytest.approx(rd.w3j(*args), rel=1e-14, abs=1e-300)
 
 
+@pytest.mark.parametrize("j1,j2,m1,m2", [(25, 30, 3, -7), (30, 30, 0, 0), (40, 35, -12, 20), (60, 60, 5, 5)])
+def test_wigner3j_orthogonality_at_high_degree(j1, j2, m1, m2):
+    """sum_j3 (2 j3 + 1) (j1 j2 j3; m1 m2 -m1-m2)^2 = 1 (orthogonality of the
+    3j symbols) up to the degrees of l_max = 30..60, where the Racah sums and
+    prefactors leave the double range; plus the closed form
+    (j j 0; m -m 0) = (-1)^(j-m) / sqrt(2 j + 1)."""
+    m3 = -m1 - m2
+    s = sum((2 * j3 + 1) * oracle.wigner3j(j1, j2, j3, m1, m2, m3) ** 2 for j3 in range(abs(j1 - j2), j1 + j2 + 1))
+    assert s == pytest.approx(1.0, rel=1e-13)
+    assert oracle.wigner3j(j1, j1, 0, m1, -m1, 0) == pytest.approx((-1) ** (j1 - m1) / math.sqrt(2 * j1 + 1),
+                                                                   rel=1e-14)
+
+
 def test_wigner3j_selection_rules():
     assert oracle.wigner3j(1, 1, 1, 0, 0, 0) == 0.0      # odd sum with m = 0
     assert oracle.wigner3j(2, 2, 5, 0, 0, 0) == 0.0      # triangle violated
