@@ -228,22 +228,62 @@ def rooflines(s, prof, K, N, iters, mixed):
 
 
 def accuracy(w, steps):
-    """Relative L2 difference between the mixed-precision and the FP64 states
-    after `steps` steps from the same initial state (and the same for the
-    cloud mass and energy)."""
-    runs = []
-    for precision in (gm.GM_PRECISION_FP64, gm.GM_PRECISION_MIXED):
-        s = make_solver(w, precision)
-        d = None
-        for k in range(1, steps + 1):
-            d = s.step(k, diag=(k == steps))
-        runs.append((s.psi, d))
-        del s
-    (p64, d64), (pmx, dmx) = runs
-    return {"steps": steps, "psi_rel_l2": float(np.linalg.norm(pmx - p64) / np.linalg.norm(p64)),
-            "cloud_mass_rel": abs(dmx["cloud_mass"] - d64["cloud_mass"]) / abs(d64["cloud_mass"]),
-            "energy_rel": abs(dmx["energy"] - d64["energy"]) / abs(d64["energy"]),
+    """Mixed precision against the plain FP64 GPU step after `steps` steps
+    from the same initial state, both advanced in lockstep: relative L2
+    difference of psi and of each U_lm, the drift of the cloud mass, the flux
+    and energy-flux error at r_ex, the energy, the binary power, the absorbed
+    mass, and the BiCGSTAB iteration counts of both."""
+    f64, mix = make_solver(w, gm.GM_PRECISION_FP64), make_solver(w, gm.GM_PRECISION_MIXED)
+    it64, itmx = [], []
+    d64 = dmx = None
+    for k in range(1, steps + 1):
+        last = k == steps
+        d64 = f64.step(k, diag=last)
+        dmx = mix.step(k, diag=last)
+        it64.append(f64.last_solver_iterations)
+        itmx.append(mix.last_solver_iterations)
+    p64, pmx = f64.psi.reshape(f64.n_modes, -1), mix.psi.reshape(mix.n_modes, -1)
+    del f64, mix
+    n64 = np.linalg.norm(p64, axis=1)
+    big = n64 > 1e-6 * n64.max()
+    per_mode = np.linalg.norm(pmx - p64, axis=1)[big] / n64[big]
+
+    def vec_err(key):
+        a, b = np.asarray(dmx[key]), np.asarray(d64[key])
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+    def rel_err(key):
+        return float(abs(dmx[key] - d64[key]) / max(abs(d64[key]), 1e-300))
+
+    return {"steps": steps,
+            "psi_rel_l2": float(np.linalg.norm(pmx - p64) / np.linalg.norm(p64)),
+            "U_lm_rel_l2_max": float(per_mode.max()), "U_lm_rel_l2_median": float(np.median(per_mode)),
+            "modes_compared": int(big.sum()),
+            "cloud_mass_rel": rel_err("cloud_mass"), "energy_rel": rel_err("energy"),
+            "p_bin_inst_rel": rel_err("p_bin_inst"), "absorbed_rel": rel_err("absorbed"),
+            "flux_err_rel_max": vec_err("flux"), "energy_flux_err_rel_max": vec_err("energy_flux"),
+            "norms_err_rel_max": vec_err("norms"),
+            "iterations_fp64_mean": float(np.mean(it64)), "iterations_mixed_mean": float(np.mean(itmx)),
+            "iterations_max_abs_diff": int(np.max(np.abs(np.array(it64) - np.array(itmx)))),
             "reference": "plain FP64 GPU step (itself within 1e-10 of the FP64 oracle per step)"}
+
+
+def large_case(steps, warmup, accuracy_steps):
+    """The second size of the mixed-precision study: l_max 30 (496 modes),
+    N 32768, L 8000, otherwise the l_max 20 workload (same perturbation
+    recipe). Timed steps of both precisions, then the lockstep comparison."""
+    w = dict(WORKLOAD, Nr=32768, L=8000.0, l_max=30)
+    out = {"config": "l_max=30 (K=496), N=32768, L=8000, q=0.5, R=50, mu=0.15, dt=0.25, fixed orbit, "
+                     "absorbers on, mask"}
+    for name, precision in (("fp64", gm.GM_PRECISION_FP64), ("mixed", gm.GM_PRECISION_MIXED)):
+        s, elapsed, _, iters, _ = run_gpu(w, precision, steps, warmup, prof_steps=0)
+        out[name] = {"value": s.n_modes * s.Nr * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+                     "iterations_per_step": float(np.mean(iters)), "apply_tile": s.apply_config(precision)[0]}
+        del s
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if accuracy_steps > 0:
+        out["mixed_vs_fp64"] = accuracy(w, accuracy_steps)
+    return out
 
 
 def measured_traffic(kernel):
@@ -284,6 +324,10 @@ def main():
                     help="members of the batched ragged ensemble measurement (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="length of the served Poisson stream for p50/p99 step latency (0 skips it)")
+    ap.add_argument("--large-steps", type=int, default=10,
+                    help="timed steps per precision at l_max 30, N 32768 (0 skips the second size)")
+    ap.add_argument("--large-accuracy-steps", type=int, default=100,
+                    help="lockstep mixed-vs-FP64 steps at l_max 30, N 32768 (0 skips them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="timed steps without graph replay (for kernel-trace profiling runs)")
@@ -331,6 +375,8 @@ def main():
     }
     if args.accuracy_steps > 0:
         out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
+    if args.large_steps > 0:
+        out["second_size"] = large_case(args.large_steps, 2, args.large_accuracy_steps)
     if args.ensemble_members > 0:
         from gm import ensemble
         ens = ensemble.run(n=args.ensemble_members, steps=10, warmup=3, seed=0)
