@@ -103,6 +103,7 @@ struct Krylov {
     double2 rho, rho_old, alpha, w, beta;
     double r2, tol2, r0_sq, restart_thr;
     double rhs_nrm;             // |rhs| of the CN system
+    double r0_raw2;             // |rhs - A psi|^2 (the unnormalised shadow residual)
     double res2;                // |A x - b|^2 of the acceptance check
     double2 c[2];               // Woodbury coefficients of the last preconditioner apply
     double2 coefA[2];           // rank-one coefficients (dt/4) g_a (v_a^H x) for the next apply

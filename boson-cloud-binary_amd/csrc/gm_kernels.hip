@@ -441,6 +441,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     const bool gq_mode = MODE == AM_RHS;
     const double2 coef0 = gq_mode ? Kr->gq[0] : Kr->coefA[0];
     const double2 coef1 = gq_mode ? Kr->gq[1] : Kr->coefA[1];
+    const double rhs_nrm = MODE == AM_RES ? Kr->rhs_nrm : 0.0;
     double red[3] = {0.0, 0.0, 0.0};
     typename AR::C acc[NG][4];
 #pragma unroll
@@ -518,16 +519,21 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                     } else if (MODE == AM_A) {
                         a.Y[row] = out;
                     } else {
-                        red[0] += cnorm2(csub(out, a.R0[row]));
+                        double2 bq = a.R0[row];
+                        if (a.r0_raw && rhs_nrm != 0.0) bq = make_double2(bq.x / rhs_nrm, bq.y / rhs_nrm);
+                        red[0] += cnorm2(csub(out, bq));
                     }
                 } else if (MODE == AM_RHS) {
                     const double2 sHx = imul(hdt, Hx);
                     double2 st = make_double2(0.0, 0.0);
                     if (nsink) st = cscale(0.5 * hdt, sink_term(a, nsink, coef0, coef1, j, i));  // (dt/4) Gamma psi
                     const double2 rhs = csub(csub(xi, sHx), st);
+                    const double2 ax = cadd(cadd(xi, sHx), st);
+                    const double2 r0 = make_double2(rhs.x - ax.x, rhs.y - ax.y);
                     a.Y[row] = rhs;
-                    a.Y2[row] = cadd(cadd(xi, sHx), st);
+                    a.Y2[row] = r0;
                     red[0] += cnorm2(rhs);
+                    red[1] += cnorm2(r0);
                 }
             }
         }
@@ -550,10 +556,11 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
             }
         }
     } else if (MODE == AM_RHS) {
-        if (block_partials<1>({red[0]}, a.partial, &Kr->counter)) {
-            const double s = sum_partials(a.partial, 0);
+        if (block_partials<2>({red[0], red[1]}, a.partial, &Kr->counter)) {
+            const double s = sum_partials(a.partial, 0), s0 = sum_partials(a.partial, 1);
             if (threadIdx.x == 0) {
                 Kr->rhs_nrm = sqrt(s);
+                Kr->r0_raw2 = s0;
                 Kr->counter = 0;
             }
         }
@@ -909,6 +916,10 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     double2 beta = make_double2(0, 0), wv = make_double2(0, 0), alpha = make_double2(0, 0);
     // first iteration: p = r + beta (p - w v) with p = v = 0 is r
     const bool first = VAR == PV_P && Kr->iter == 0;
+    // the first iteration reads r = r0' / |rhs| from the unnormalised shadow residual
+    const bool it0 = VAR != PV_PLAIN && Kr->iter == 0;
+    const double nrm0 = it0 ? Kr->rhs_nrm : 1.0;
+    const double2 *Rin = it0 ? a.R0 : a.R;
     if (VAR == PV_P) {
         beta = Kr->beta;
         wv = Kr->w;
@@ -930,7 +941,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             if (VAR == PV_PLAIN) {
                 b[e].r = a.X[q];
             } else {
-                b[e].r = a.R[q];
+                b[e].r = Rin[q];
                 b[e].v = a.V[q];
                 if (VAR == PV_P) b[e].p = a.Pv[q];  // unused at the first iteration
             }
@@ -949,11 +960,12 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             if (idx < N) {
                 const size_t q = base_j + idx;
                 double2 in;
+                const double2 rq = it0 ? make_double2(b[e].r.x / nrm0, b[e].r.y / nrm0) : b[e].r;
                 if (VAR == PV_P) {
-                    in = first ? b[e].r : cadd(b[e].r, cmul(beta, csub(b[e].p, cmul(wv, b[e].v))));
+                    in = first ? rq : cadd(rq, cmul(beta, csub(b[e].p, cmul(wv, b[e].v))));
                     a.Pv[q] = in;
                 } else if (VAR == PV_S) {
-                    in = csub(b[e].r, cmul(alpha, b[e].v));
+                    in = csub(rq, cmul(alpha, b[e].v));
                     a.Pv[q] = in;
                 } else {
                     in = b[e].r;
@@ -1229,53 +1241,47 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a)
     }
 }
 
-// b = rhs/|rhs|, x = psi/|rhs|, r = (rhs - A psi)/|rhs|, r0 = r, p = v = 0.
+// Start of the normalised solve (Eigen BiCGSTAB on A x = b, b = rhs/|rhs|,
+// guess x = psi/|rhs|): the rhs apply left |rhs|, the raw shadow residual
+// r0' = rhs - A psi in R0 and |r0'|^2. The shadow residual enters BiCGSTAB
+// only through ratios of inner products, so it stays unnormalised (r0' =
+// |rhs| r0): rho and r0.v carry the factor |rhs|, which cancels in alpha and
+// beta, and the restart thresholds are scaled by it. r = r0'/|rhs| is formed
+// where the first iteration reads it; p = v = 0 are never read.
 __global__ __launch_bounds__(256) void k_init(InitArgs a)
 {
     Krylov *Kr = a.Kr;
     const double nrm = Kr->rhs_nrm;
-    double v[2] = {0, 0};
     if (nrm != 0.0) {
         for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (size_t)gridDim.x * blockDim.x) {
-            const double2 rhs = a.B[q];
-            const double2 b = make_double2(rhs.x / nrm, rhs.y / nrm);
-            a.B[q] = b;
             const double2 x = a.Psi[q];
             a.X[q] = make_double2(x.x / nrm, x.y / nrm);
-            const double2 ax = a.AX[q];
-            const double2 r = make_double2((rhs.x - ax.x) / nrm, (rhs.y - ax.y) / nrm);
-            a.R[q] = r;
-            a.R0[q] = r;
-            // p = v = 0 (Eigen BiCGSTAB): the first p-update reads neither
-            v[0] += cnorm2(r);
-            v[1] += cnorm2(b);
         }
     }
-    if (block_partials<2>(v, a.partial, &Kr->counter)) {
-        const double r2 = sum_partials(a.partial, 0), b2 = sum_partials(a.partial, 1);
-        if (threadIdx.x == 0) {
-            Kr->counter = 0;
-            Kr->iter = 0;
-            Kr->maxit = a.maxit;
-            Kr->eigen_mode = a.eigen_mode;
-            Kr->restarts = 0;
-            Kr->restart = 0;
-            Kr->accepted = 0;
-            Kr->done = 0;
-            Kr->zero_rhs = nrm == 0.0;
-            Kr->rho = make_double2(1.0, 0.0);
-            Kr->alpha = make_double2(1.0, 0.0);
-            Kr->w = make_double2(1.0, 0.0);
-            Kr->r0_sq = r2;
-            Kr->restart_thr = r2 * 1e-32;
-            Kr->tol2 = a.tol * a.tol * b2;
-            if (nrm == 0.0) {
-                Kr->done = 1;
-            } else if (b2 == 0.0) {
-                Kr->done = 1;
-            } else {
-                loop_top(Kr, r2, make_double2(r2, 0.0));
-            }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // |r|^2 and |b|^2 of the normalised system
+        const double r2 = nrm != 0.0 ? Kr->r0_raw2 / (nrm * nrm) : 0.0;
+        const double b2 = nrm != 0.0 ? 1.0 : 0.0;
+        const double rho0 = nrm != 0.0 ? Kr->r0_raw2 / nrm : 0.0;  // r0'.r = |rhs| |r|^2
+        Kr->counter = 0;
+        Kr->iter = 0;
+        Kr->maxit = a.maxit;
+        Kr->eigen_mode = a.eigen_mode;
+        Kr->restarts = 0;
+        Kr->restart = 0;
+        Kr->accepted = 0;
+        Kr->done = 0;
+        Kr->zero_rhs = nrm == 0.0;
+        Kr->rho = make_double2(1.0, 0.0);
+        Kr->alpha = make_double2(1.0, 0.0);
+        Kr->w = make_double2(1.0, 0.0);
+        Kr->r0_sq = rho0;               // |rhs| |r0|^2: Eigen's eps^2 |r0|^2 test on the scaled rho
+        Kr->restart_thr = rho0 * 1e-32;
+        Kr->tol2 = a.tol * a.tol * b2;
+        if (nrm == 0.0) {
+            Kr->done = 1;
+        } else {
+            loop_top(Kr, r2, make_double2(rho0, 0.0));
         }
     }
 }

@@ -15,8 +15,9 @@ struct ApplyArgs {
     gmd::Krylov *Kr;
     const double2 *X;    // input vector
     double2 *Y;          // output
-    double2 *Y2;         // second output (AM_RHS: A psi)
+    double2 *Y2;         // second output (AM_RHS: rhs - A psi, the unnormalised shadow residual)
     const double2 *R0;   // r0 (AM_V), s (AM_T), b (AM_RES)
+    int r0_raw;          // AM_RES: R0 holds the raw rhs; b = rhs / |rhs| is formed here
     gmd::SinkDev sk[2];
     double *partial;     // reduction partials (3 * grid doubles)
     double *rowpart;     // diagnostics apply: 6 doubles per (tile, mode)
@@ -49,6 +50,7 @@ struct PrecArgs {
     const double2 *inv;
     gmd::Krylov *Kr;
     const double2 *R;
+    const double2 *R0;   // unnormalised shadow residual: r = R0 / |rhs| at the first iteration
     double2 *Pv;         // p (PV_P, updated in place) or s (PV_S, written)
     const double2 *V;
     const double2 *X;    // PV_PLAIN input
@@ -86,12 +88,10 @@ struct UpdateArgs {
 struct InitArgs {
     size_t n;
     gmd::Krylov *Kr;
-    double2 *B;          // rhs in, b out
-    const double2 *Psi, *AX;
-    double2 *X, *R, *R0, *Pv, *V;
+    const double2 *Psi;
+    double2 *X;
     double tol;
     int maxit, eigen_mode;
-    double *partial;
 };
 
 struct AcceptArgs {

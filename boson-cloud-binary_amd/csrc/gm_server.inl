@@ -53,7 +53,7 @@ double footprint_bytes(const gm_params &p)
 {
     const double K = (p.l_max + 1.0) * (p.l_max + 2.0) / 2.0, N = p.Nr;
     const double Ls = 2.0 * p.l_max + 1.0;
-    const double vectors = (14.0 + 2.0) * K * N * 16.0;
+    const double vectors = (13.0 + 2.0) * K * N * 16.0;
     const double profiles = 2.0 * Ls * N * (8.0 + 4.0);
     const double tables = 12.0 * K * K * (p.l_max + 1.0);
     const double rows = (N / gmd::TILE + 1.0) * K * 6.0 * 8.0;

@@ -148,7 +148,7 @@ struct gm_solver {
     DevBuf<int> lcut;
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
-    DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, ax, inv, tmp;
+    DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, inv, tmp;
     DevBuf<float2> inv32, y32;  // FP32 pivots and sweep scratch (mixed-precision preconditioner)
     DevBuf<double2> xt;         // phased column vectors of the coupling (FP64; FP32 uses the first half)
     DevBuf<double> partial, rowpart, diagout;
@@ -838,6 +838,7 @@ void enqueue_iteration(gm_solver &S, bool wood)
     pa.xt_f32 = mixed;
     // y = M^-1 p with p = r + beta (p - w v)
     pa.R = S.rr.p;
+    pa.R0 = S.r0.p;
     pa.Pv = S.p.p;
     pa.V = S.v.p;
     pa.Y = S.y.p;
@@ -1011,11 +1012,12 @@ void rhs_and_init(gm_solver &S)
 {
     const bool wood = S.step_wood;
     gmd::Krylov *dk = S.dK.p;
-    // rhs = B psi - (dt/4) Gamma psi and A psi in one apply
+    // rhs = B psi - (dt/4) Gamma psi and the raw shadow residual rhs - A psi
+    // in one apply (b stays unnormalised in S.b; the acceptance divides)
     if (wood) sink_proj(S, S.psi.p, gmk::SD_START, 1.0, 0);
     {
         gmk::ApplyArgs a = apply_args(S, S.psi.p, S.b.p);
-        a.Y2 = S.ax.p;
+        a.Y2 = S.r0.p;
         launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RHS, S.st); });
     }
     // Eigen's BiCGSTAB without sinks, the reference's own recurrence with them
@@ -1023,18 +1025,11 @@ void rhs_and_init(gm_solver &S)
         gmk::InitArgs ia{};
         ia.n = S.nvec();
         ia.Kr = dk;
-        ia.B = S.b.p;
         ia.Psi = S.psi.p;
-        ia.AX = S.ax.p;
         ia.X = S.x.p;
-        ia.R = S.rr.p;
-        ia.R0 = S.r0.p;
-        ia.Pv = S.p.p;
-        ia.V = S.v.p;
         ia.tol = 1e-10;
         ia.maxit = kMaxIt;
         ia.eigen_mode = wood ? 0 : 1;
-        ia.partial = S.partial.p;
         launch(S, GM_PROF_SMALL, [&] { return gmk::launch_init(ia, S.st); });
     }
 }
@@ -1077,6 +1072,7 @@ void solve_tail_work(gm_solver &S)
     {
         gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
         a.R0 = S.b.p;
+        a.r0_raw = 1;  // b = rhs / |rhs|
         a.Cout = S.tmp.p;  // V x, reused by the diagnostics of the accepted state
         launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RES, S.st); });
     }
@@ -1123,7 +1119,7 @@ void solve_tail_work(gm_solver &S)
 void alloc_device(gm_solver &S)
 {
     const size_t n = S.nvec();
-    for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.ax, &S.inv, &S.tmp})
+    for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.inv, &S.tmp})
         if (bf->n != n) bf->alloc(n);
     for (auto *bf : {&S.inv32, &S.y32})
         if (bf->n != n) bf->alloc(n);
@@ -1420,7 +1416,7 @@ void ilut_host_solve(gm_solver &S)
     auto node_major = [&](size_t q) { return (q % N) * K + q / N; };
     std::vector<gmh::cplx> b(n), x(n);
     for (size_t q = 0; q < n; ++q) {
-        b[node_major(q)] = gmh::cplx(hb[2 * q], hb[2 * q + 1]);
+        b[node_major(q)] = gmh::cplx(hb[2 * q] / kr.rhs_nrm, hb[2 * q + 1] / kr.rhs_nrm);
         x[node_major(q)] = gmh::cplx(hpsi[2 * q], hpsi[2 * q + 1]) / kr.rhs_nrm;
     }
     // active sinks: A_full = A + (dt/4) sum_a g_a v_a v_a^H (matrices.cpp:108-116)
