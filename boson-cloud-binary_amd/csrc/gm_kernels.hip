@@ -108,13 +108,20 @@ __device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned 
         for (int q = 0; q < NQ; ++q) {
             double s = 0.0;
             for (int w = 0; w < nw; ++w) s += ws[w][q];
-            partial[(size_t)q * gridDim.x + blockIdx.x] = s;
+            // agent-scope store: written through to the device coherence
+            // point, so no release fence (a write-back of this XCD's whole
+            // L2, which every block of a streaming kernel would wait on) is
+            // needed for the partials
+            __hip_atomic_store(partial + (size_t)q * gridDim.x + blockIdx.x, s, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
+        // the partial stores have completed before this block is counted
+        __builtin_amdgcn_s_waitcnt(0);
+        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = (t == gridDim.x - 1);
     }
     __syncthreads();
+    // acquire in the one block that reads every partial (coherent loads)
     if (last) __threadfence();
     return last;
 }
