@@ -1047,6 +1047,13 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     int nz_lo = N, nz_hi = 0;
     carry = czero<C>();
     const double2 phj = a.Xt ? a.Ps->ph[mode_m(j) + a.lmax] : make_double2(1.0, 0.0);
+    // Woodbury epilogue: the last block reads y on the sink supports, so those
+    // values are stored at agent scope (written through to the device
+    // coherence point) instead of fencing the whole of each XCD's L2
+    const bool wood_out = VAR != PV_PLAIN && a.wood;
+    const int ns_w = wood_out ? a.Ps->nsink : 0;
+    const int s0lo = ns_w > 0 ? a.sk[0].lo : 0, s0hi = ns_w > 0 ? a.sk[0].lo + a.sk[0].n : 0;
+    const int s1lo = ns_w > 1 ? a.sk[1].lo : 0, s1hi = ns_w > 1 ? a.sk[1].lo + a.sk[1].n : 0;
     auto bproc = [&](const int t, const BIn (&b)[E]) __attribute__((always_inline)) {
         const int t0 = t * TE;
 #pragma unroll
@@ -1091,7 +1098,13 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             const int idx = t0 + pos;
             if (idx < N) {
                 const double2 yv = Arith<RT>::to_d(sf[lds_slot(pos)]);
-                a.Y[base_j + idx] = yv;
+                if ((idx >= s0lo && idx < s0hi) || (idx >= s1lo && idx < s1hi)) {
+                    double *yp = reinterpret_cast<double *>(a.Y + base_j + idx);
+                    __hip_atomic_store(yp, yv.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(yp + 1, yv.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    a.Y[base_j + idx] = yv;
+                }
                 if (a.Xt) store_phased(a.Xt, a.xt_f32, base_j + idx, phj, yv);
                 if (VAR == PV_PLAIN && a.range && (yv.x != 0.0 || yv.y != 0.0)) {
                     nz_lo = min(nz_lo, idx);
@@ -1127,10 +1140,10 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     // c = Ginv (V^H y), then the rank-one coefficients of A applied to
     // y - W c: coefA_a = (dt/4) g_a v_a^H (y - W c).
     __shared__ bool last;
-    if (tid == 0) {
-        __threadfence();
-        last = atomicAdd(&Kr->counter, 1u) == gridDim.x - 1;
-    }
+    // every thread's support stores have completed before the block is counted
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) last = __hip_atomic_fetch_add(&Kr->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
     __syncthreads();
     if (!last) return;
     __threadfence();
