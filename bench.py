@@ -316,8 +316,8 @@ def copy_calibration(nbytes=1 << 29, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="untimed steps of the mixed-vs-FP64 comparison (0 skips it)")
     ap.add_argument("--ensemble-members", type=int, default=64,

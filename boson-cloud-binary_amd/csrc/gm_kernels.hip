@@ -1384,6 +1384,12 @@ __global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
             } else if (a.dest == SD_POST) {
                 Kr->qpost[s] = pr;
                 Kr->gq[s] = cscale(P->g[s], pr);
+            } else if (a.dest == SD_COEFA_QNEW) {
+                // one projection of the accepted iterate x: the rank-one
+                // coefficients of the residual apply and |rhs| v^H x
+                const double2 raw = make_double2(red[2 * s], red[2 * s + 1]);
+                Kr->coefA[s] = cscale(0.5 * P->half_dt * P->g[s], raw);
+                Kr->qnew[s] = cscale(Kr->rhs_nrm, raw);
             } else {
                 Kr->coefA[s] = cscale(0.5 * P->half_dt * P->g[s], pr);
             }

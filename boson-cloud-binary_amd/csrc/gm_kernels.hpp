@@ -7,7 +7,7 @@ namespace gmk {
 enum ApplyMode { AM_H = 0, AM_A = 1, AM_V = 2, AM_T = 3, AM_RHS = 4, AM_RES = 5 };
 enum PrecVar { PV_PLAIN = 0, PV_P = 1, PV_S = 2 };
 enum SinkMode { SK_PROJ = 0, SK_SETUP = 1 };
-enum SinkDest { SD_START = 0, SD_QNEW = 1, SD_POST = 2, SD_COEFA = 3 };
+enum SinkDest { SD_START = 0, SD_QNEW = 1, SD_POST = 2, SD_COEFA = 3, SD_COEFA_QNEW = 4 };
 
 struct ApplyArgs {
     gmd::Geometry g;

@@ -1067,8 +1067,9 @@ void solve_tail_work(gm_solver &S)
     const gmd::StepParams &P = *S.hP.p;
     const bool wood = S.step_wood;
     gmd::Krylov *dk = S.dK.p;
-    // acceptance: |A_full x - b| with the rank-one sinks of x
-    if (wood) sink_proj(S, S.x.p, gmk::SD_COEFA, 1.0, 0);
+    // acceptance: |A_full x - b| with the rank-one sinks of x; one projection
+    // of x gives their coefficients and |rhs| v^H x for the absorbed mass
+    if (wood) sink_proj(S, S.x.p, gmk::SD_COEFA_QNEW, 1.0, 0);
     {
         gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
         a.R0 = S.b.p;
@@ -1076,7 +1077,6 @@ void solve_tail_work(gm_solver &S)
         a.Cout = S.tmp.p;  // V x, reused by the diagnostics of the accepted state
         launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RES, S.st); });
     }
-    if (wood) sink_proj(S, S.x.p, gmk::SD_QNEW, 1.0, 1);
     gmk::AcceptArgs ac{};
     ac.n = S.nvec();
     ac.N = S.Nr;
