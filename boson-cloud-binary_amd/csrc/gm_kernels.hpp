@@ -16,7 +16,7 @@ struct ApplyArgs {
     const double2 *X;    // input vector
     double2 *Y;          // output
     double2 *Y2;         // second output (AM_RHS: rhs - A psi, the unnormalised shadow residual)
-    const double2 *R0;   // r0 (AM_V), s (AM_T), b (AM_RES)
+    const double2 *R0;   // unnormalised shadow residual (AM_V), s (AM_T), b or the raw rhs (AM_RES)
     int r0_raw;          // AM_RES: R0 holds the raw rhs; b = rhs / |rhs| is formed here
     gmd::SinkDev sk[2];
     double *partial;     // reduction partials (3 * grid doubles)

@@ -1390,9 +1390,10 @@ gmh::CsrZ cn_operator(const gmh::CsrZ &H, double half_dt)
 // 203-214) on the host: ILUT (fill factor 50, drop tolerance 1e-9) of the
 // operator the device applies, under Eigen's BiCGSTAB without sinks or the
 // reference's BiCGSTAB with the Woodbury sink correction, tolerance 1e-10, at
-// most 5000 iterations, from the guess psi/|rhs|. Reads b = rhs/|rhs| and
-// |rhs| from the device (after rhs_and_init) and leaves x = solution/|rhs| in
-// S.x, where the tail's acceptance check, mask and diagnostics take it over.
+// most 5000 iterations, from the guess psi/|rhs|. Reads the raw rhs and |rhs|
+// from the device (after rhs_and_init), solves for b = rhs/|rhs| and leaves
+// x = solution/|rhs| in S.x, where the tail's acceptance check, mask and
+// diagnostics take it over.
 void ilut_host_solve(gm_solver &S)
 {
     HIPCK(hipStreamSynchronize(S.st));
