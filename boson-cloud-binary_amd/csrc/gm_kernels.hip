@@ -1149,8 +1149,12 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     __threadfence();
     const StepParams *P = a.Ps;
     const int ns = P->nsink;
+    // (loops over the at most two sinks have constant bounds, so the small
+    // arrays stay in registers)
     double v[4] = {0, 0, 0, 0};
-    for (int s = 0; s < ns; ++s) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (s >= ns) break;
         const SinkDev &sk = a.sk[s];
         const int tot = a.K * sk.n;
         for (int q = tid; q < tot; q += T) {
@@ -1176,11 +1180,16 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     __syncthreads();
     double2 proj[2] = {make_double2(red[0], red[1]), make_double2(red[2], red[3])};
     double2 c[2] = {make_double2(0, 0), make_double2(0, 0)};
-    for (int s = 0; s < ns; ++s)
-        for (int u = 0; u < ns; ++u) c[s] = cadd(c[s], cmul(Kr->Ginv[s * 2 + u], proj[u]));
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (s < ns && u < ns) c[s] = cadd(c[s], cmul(Kr->Ginv[s * 2 + u], proj[u]));
     __syncthreads();
     for (int q = 0; q < 4; ++q) v[q] = 0.0;
-    for (int s = 0; s < ns; ++s) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (s >= ns) break;
         const SinkDev &sk = a.sk[s];
         const int tot = a.K * sk.n;
         for (int q = tid; q < tot; q += T) {
@@ -1188,7 +1197,9 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             const size_t gi = (size_t)k * N + sk.lo + d;
             double2 y = vload2(a.Y + gi);
             const double2 rot = P->wrot[mode_m(k) + a.lmax];
-            for (int u = 0; u < ns; ++u) y = csub(y, cmul(cmul(c[u], rot), a.sk[u].W[gi]));
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if (u < ns) y = csub(y, cmul(cmul(c[u], rot), a.sk[u].W[gi]));
             const double2 d2 = cdotm(sk.v[q], y);
             v[2 * s] += d2.x;
             v[2 * s + 1] += d2.y;
@@ -1205,10 +1216,12 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
             for (int w = 0; w < T / 64; ++w) s += ws[w][q];
             red[q] = s;
         }
-        for (int s = 0; s < ns; ++s) {
-            Kr->c[s] = c[s];
-            Kr->coefA[s] = cscale(0.5 * P->half_dt * P->g[s], make_double2(red[2 * s], red[2 * s + 1]));
-        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            if (s < ns) {
+                Kr->c[s] = c[s];
+                Kr->coefA[s] = cscale(0.5 * P->half_dt * P->g[s], make_double2(red[2 * s], red[2 * s + 1]));
+            }
         Kr->counter = 0;
     }
 }
@@ -1337,9 +1350,13 @@ __global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
     const StepParams *P = a.P;
     const int ns = P->nsink;
     const int tid = threadIdx.x;
+    // loops over the at most two sinks have constant bounds, so the
+    // accumulators stay in registers
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int N = a.N;
-    for (int s = 0; s < ns; ++s) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (s >= ns) break;
         const SinkDev &sk = a.sk[s];
         const int tot = a.K * sk.n;
         for (int q = tid; q < tot; q += blockDim.x) {
@@ -1351,7 +1368,9 @@ __global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
                 v[2 * s + 1] += d2.y;
             } else {
                 const double2 rot = P->wrot[mode_m(k) + a.lmax];
-                for (int u = 0; u < ns; ++u) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    if (u >= ns) break;
                     const double2 d2 = cdotm(sk.v[q], cmul(rot, a.sk[u].W[gi]));
                     v[4 * s + 2 * u] += d2.x;
                     v[4 * s + 2 * u + 1] += d2.y;
@@ -1396,10 +1415,12 @@ __global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
         }
     } else {
         double2 G[4];
-        for (int s = 0; s < ns; ++s)
-            for (int u = 0; u < ns; ++u) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
                 G[s * 2 + u] = make_double2(red[4 * s + 2 * u], red[4 * s + 2 * u + 1]);
-                if (s == u) G[s * 2 + u].x += 1.0 / (0.5 * P->half_dt * P->g[s]);
+                if (s == u && s < ns) G[s * 2 + u].x += 1.0 / (0.5 * P->half_dt * P->g[s]);
             }
         for (int q = 0; q < 4; ++q) Kr->G[q] = G[q];
         if (ns == 1) {
