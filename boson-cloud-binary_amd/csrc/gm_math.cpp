@@ -3,6 +3,8 @@
 
 #include <cmath>
 #include <istream>
+#include <map>
+#include <mutex>
 #include <ostream>
 #include <stdexcept>
 
@@ -84,6 +86,20 @@ double sph_plm0(int l, int m)
 
 void gauss_legendre01(int n, std::vector<double> &x, std::vector<double> &w)
 {
+    // the rule is computed once per order and process (its binary128 Newton
+    // iterations cost milliseconds, and an evolving orbit asks for it at every
+    // step when the absorber profiles move)
+    static std::mutex mu;
+    static std::map<int, std::pair<std::vector<double>, std::vector<double>>> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        const auto it = cache.find(n);
+        if (it != cache.end()) {
+            x = it->second.first;
+            w = it->second.second;
+            return;
+        }
+    }
     x.assign(n, 0.0);
     w.assign(n, 0.0);
     auto legendre = [n](f128 z, f128 &pn, f128 &dpn) {
@@ -110,6 +126,8 @@ void gauss_legendre01(int n, std::vector<double> &x, std::vector<double> &w)
         x[n - 1 - k] = static_cast<double>((1 + z) / 2);
         w[n - 1 - k] = static_cast<double>(1 / ((1 - z * z) * dpn * dpn));
     }
+    std::lock_guard<std::mutex> lk(mu);
+    cache.emplace(n, std::make_pair(x, w));
 }
 
 // ------------------------------------------------------------------ rolling average
