@@ -39,7 +39,8 @@ def announce(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("l_max,absorb,cap", [(2, False, False), (4, True, False), (3, True, True), (6, False, False)])
+@pytest.mark.parametrize("l_max,absorb,cap", [(0, True, False), (1, False, True), (2, False, False), (4, True, False),
+                                               (3, True, True), (6, False, False)])
 def test_apply_H_matches_oracle(l_max, absorb, cap):
     announce(f"apply_H l_max={l_max} absorb={absorb} cap={cap}")
     objs = pair(l_max=l_max, absorption=absorb, Nr=97, L=96.0)
@@ -205,6 +206,23 @@ def test_isolated_black_hole():
     both(objs, "set_initial_state_full", psi)
     both(objs, "initialize")
     hist = run_both(objs, 8, 30.0)
+    check_history(objs, hist)
+
+
+@pytest.mark.parametrize("l_max", [0, 1])
+def test_low_lmax_steps_match_oracle(l_max):
+    """One mode (only the l* = 0 self-coupling) and three modes: an evolving
+    orbit with active absorbers, 15 steps against the oracle."""
+    announce(f"low l_max {l_max}")
+    objs = pair(Nr=129, L=128.0, dt=2.0, l_max=l_max, mu=0.2, M_c=0.05, q=0.5, R=20.0, absorption=True,
+                ramp_periods=0.05, epsilon_smear=4.0)
+    g, o = objs
+    both(objs, "set_flags", evolve_orbit=1, steps_per_period=32.0)
+    both(objs, "set_initial_state_full", generic_state(o))
+    both(objs, "initialize")
+    both(objs, "build_damping_mask", 0.85, 1.0, 6)
+    hist = run_both(objs, 15, 100.0)
+    assert o.scalars()["sink_active"] == 1
     check_history(objs, hist)
 
 
