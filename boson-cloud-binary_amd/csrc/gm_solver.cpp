@@ -2048,19 +2048,30 @@ int gm_ensemble_step(gm_solver *const *members, int32_t n, int32_t step_id, gm_d
         st[i] = guarded(members[i], [&] { step_begin(*members[i]); });
         phase[i] = st[i] == GM_OK ? 1 : -1;
     }
+    // members are advanced as their streams go idle (non-blocking queries), so
+    // a member whose batch finished early is refilled at once instead of
+    // waiting behind the members before it
     for (bool busy = true; busy;) {
         busy = false;
+        bool progressed = false;
         for (int i = 0; i < n; ++i) {
             if (phase[i] != 1) continue;
             gm_solver *S = members[i];
+            const hipError_t q = hipStreamQuery(S->st);
+            if (q == hipErrorNotReady) {
+                busy = true;
+                continue;
+            }
             st[i] = guarded(S, [&] {
-                HIPCK(hipStreamSynchronize(S->st));
+                HIPCK(q);
                 prof_collect(*S);
                 if (solve_advance(*S)) phase[i] = 2;
             });
+            progressed = true;
             if (st[i] != GM_OK) phase[i] = -1;
             busy = busy || phase[i] == 1;
         }
+        if (busy && !progressed) std::this_thread::yield();
     }
     int first = GM_OK;
     for (int i = 0; i < n; ++i) {
