@@ -46,3 +46,6 @@ def test_invalid_arguments_are_rejected_before_the_device():
         gm.CrankNicolsonModes(2, 10.0, 0.1, 0.0, 2, 0.2, 0.01, 0.1, 0.0, 30.0)
     with pytest.raises(gm.InvalidArgument):
         gm.CrankNicolsonModes(64, 10.0, -0.1, 0.0, 2, 0.2, 0.01, 0.1, 0.0, 30.0)
+    # the kernels' fixed tables hold l_max <= 39
+    with pytest.raises(gm.InvalidArgument, match="largest supported"):
+        gm.CrankNicolsonModes(64, 10.0, 0.1, 0.0, 40, 0.2, 0.01, 0.1, 0.0, 30.0)
