@@ -226,6 +226,22 @@ def test_low_lmax_steps_match_oracle(l_max):
     check_history(objs, hist)
 
 
+def test_exact_preconditioner_one_iteration_on_the_gpu():
+    """One mode and no absorbers: the radial preconditioner is A's exact
+    inverse, so both paths stop after exactly one BiCGSTAB iteration."""
+    announce("exact preconditioner")
+    objs = pair(l_max=0, ramp_periods=0.2)
+    g, o = objs
+    both(objs, "set_flags", evolve_orbit=0)
+    both(objs, "set_initial_state_full", generic_state(o))
+    both(objs, "initialize")
+    both(objs, "begin_run", 40.0, 11)
+    for n in range(1, 11):
+        g.step(n, diag=False), o.step(n, diag=False)
+        assert g.last_solver_iterations == 1 and o.last_solver_iterations == 1, n
+    assert rel(g.psi, o.psi) < 1e-12
+
+
 def test_gpu_unitarity_without_absorption():
     """tests/physics.cpp:106-124 on the GPU path."""
     announce("unitarity")

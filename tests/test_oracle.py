@@ -209,6 +209,23 @@ def test_unitarity_without_absorption():
     assert cn.last_solver_residual <= 1e-9
 
 
+def test_exact_preconditioner_converges_in_one_iteration():
+    """A pin of the BiCGSTAB loop count. With one mode (l_max = 0) and no
+    absorbers, A has no inter-mode coupling: the mode's self-coupling is on
+    the diagonal the Thomas pivots include, so the radial preconditioner is A's
+    exact inverse. Eigen's loop (matrices.cpp:188-196) then stops after exactly
+    one iteration, with the residual at rounding."""
+    cn = make_physics(l_max=0, ramp_periods=0.2)
+    cn.set_flags(evolve_orbit=0)
+    cn.set_initial_state_full(generic_state(cn))
+    cn.initialize()
+    cn.begin_run(40, 11)
+    for step in range(1, 11):
+        cn.step(step, diag=False)
+        assert cn.last_solver_iterations == 1, step
+        assert cn.last_solver_residual <= 1e-13
+
+
 def test_hermitian_and_sink_psd():
     """tests/physics.cpp:77-104 on operator probes."""
     cn = make_physics(l_max=4, absorption=True)
