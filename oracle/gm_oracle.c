@@ -17,9 +17,13 @@
  * operators against a dense restatement of tests/parallel_correctness.cpp;
  * one CN step against a dense LU solve; Thomas solve against the synthetic
  * tridiagonal known answer; physics invariants of tests/physics.cpp.
- * BiCGSTAB iteration counts: parity unpinned (no reference test checks them;
- * the recurrence is restated from Eigen 3.4's BiCGSTAB.h as the reference
- * uses it, src/CrankNicolsonModes_matrices.cpp:44-91,188-196).
+ * BiCGSTAB iteration counts: parity unpinned except where the mathematics
+ * fixes them (one mode without absorbers, where the radial preconditioner is
+ * the exact inverse and the loop stops after one iteration); no reference test
+ * checks them. The recurrence is restated from Eigen 3.4's BiCGSTAB.h as the
+ * reference uses it (src/CrankNicolsonModes_matrices.cpp:44-91,188-196).
+ * The Gaussian-quadrature MODESBIN generator and the ILUT solve are pinned in
+ * tests/test_initial.py and tests/test_ilut.py.
  */
 #include <complex.h>
 #include <float.h>
