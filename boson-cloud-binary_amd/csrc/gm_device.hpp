@@ -16,7 +16,8 @@
 
 namespace gmd {
 
-constexpr int TILE = 64;   // radial nodes of a multipole-cut / diagnostics tile
+constexpr int TILE = 64;   // radial nodes of a multipole-cut tile
+constexpr int DIAG_NODES = 256;  // radial nodes per diagnostics wavefront (4 per lane)
 constexpr int AW = 4;      // row chunks (wavefronts) per operator-apply workgroup
 constexpr int MAXLM = 40;  // largest supported l_max + 1
 constexpr int PT = 512;    // threads of a preconditioner or factor workgroup

@@ -585,7 +585,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 // Diagnostics apply on the post-mask state (extraction.cpp:8-129). The
 // coupling is node-local and the mask is one radial profile for every mode, so
 // V psi = |rhs| mask (V x) with V x kept by the acceptance apply; only the
-// stencil is evaluated here. One wavefront per (mode, tile of 64 nodes) writes
+// stencil is evaluated here. One wavefront per (mode, tile of DIAG_NODES nodes) writes
 // the tile sums of |U|^2 and Re<U|H U> below idx_ex, Re<U|H U>, Im(m <U|V U>)
 // and Re<dU/dt|V U> up to idx_ex inclusive, and |U|^2 over the grid.
 __global__ __launch_bounds__(256) void k_diag_apply(ApplyArgs a)
@@ -594,17 +594,23 @@ __global__ __launch_bounds__(256) void k_diag_apply(ApplyArgs a)
     const StepParams *P = a.P;
     const Krylov *Kr = a.Kr;
     const int lane = threadIdx.x & 63;
+    const int N = g.N, nt = (N + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES;
     const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (item >= g.K * g.ntiles) return;
-    const int j = item / g.ntiles, tile = item - j * g.ntiles;
-    const int N = g.N, i = tile * gmd::TILE + lane;
+    if (item >= g.K * nt) return;
+    const int j = item / nt, tile = item - j * nt;
+    const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
+    const int l = (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
+    const int m = 2 * (j - l * (l + 1) / 2) - l;
+    const double nrm = Kr->rhs_nrm;
+    const int ix = P->idx_ex;
     double rp[6] = {0, 0, 0, 0, 0, 0};
-    if (i < N) {
+    // four nodes per lane, 64 apart (coalesced per pass), summed in that order
+#pragma unroll
+    for (int u = 0; u < gmd::DIAG_NODES / 64; ++u) {
+        const int i = tile * gmd::DIAG_NODES + 64 * u + lane;
+        if (i >= N) continue;
         const size_t row = (size_t)j * N + i;
-        const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
-        const int l = (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
-        const int m = 2 * (j - l * (l + 1) / 2) - l;
-        const double2 Cc = cscale(Kr->rhs_nrm * (a.mask ? a.mask[i] : 1.0), a.Cin[row]);
+        const double2 Cc = cscale(nrm * (a.mask ? a.mask[i] : 1.0), a.Cin[row]);
         const double2 xi = load_x(a, j, i);
         const double2 capv = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
         double2 Hx;
@@ -621,17 +627,16 @@ __global__ __launch_bounds__(256) void k_diag_apply(ApplyArgs a)
         if (P->nsink) dpsi = csub(dpsi, cscale(0.5, sink_term(a, P->nsink, Kr->gq[0], Kr->gq[1], j, i)));
         const double2 eh = cdotm(xi, Hx);
         const double n2 = cnorm2(xi);
-        const int ix = P->idx_ex;
         if (i < ix) {
-            rp[0] = n2;
-            rp[1] = eh.x;
+            rp[0] += n2;
+            rp[1] += eh.x;
         }
         if (i <= ix) {
-            rp[2] = eh.x;
-            rp[3] = m * cdotm(xi, Cc).y;
-            rp[4] = cdotm(dpsi, Cc).x;
+            rp[2] += eh.x;
+            rp[3] += m * cdotm(xi, Cc).y;
+            rp[4] += cdotm(dpsi, Cc).x;
         }
-        rp[5] = n2;
+        rp[5] += n2;
         if (i == ix) a.dpsi_ix[j] = dpsi;
     }
 #pragma unroll
@@ -1563,7 +1568,7 @@ hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 
 hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st)
 {
-    const int items = a.g.K * a.g.ntiles;
+    const int items = a.g.K * ((a.g.N + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES);
     hipLaunchKernelGGL(k_diag_apply, dim3((items + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }

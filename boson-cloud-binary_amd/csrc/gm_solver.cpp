@@ -1100,7 +1100,7 @@ void solve_tail_work(gm_solver &S)
         da.N = S.Nr;
         da.K = S.K;
         da.ix = S.idx_ex;
-        da.ntiles = S.ntiles;
+        da.ntiles = (S.Nr + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES;  // the diagnostics apply's tiles
         da.dr = S.dr;
         da.eps = S.epsilon;
         da.mu = S.mu;
