@@ -46,13 +46,16 @@ struct MBlock {
 constexpr int RING = 4;
 // LDS row read by padding steps (every row above a tile's cut holds zeros).
 constexpr int ZROW = 2 * MAXLM - 1;
+// LDS rows of the staged profiles: a paired padding step reads ZROW .. ZROW + 14.
+constexpr int EROWS = ZROW + 15;
 
 // One step of a chunk's coupling stream: four consecutive multipole slots of
-// the (chunk, l1) block at one column m1. A chunk's steps are sorted by the l*
-// of their first slot, so the steps a tile with multipole cut lc needs are
-// the first step_count[lc].
+// the (chunk, l1) block at one column m1, or, in the chunk's paired stream,
+// two such groups of the same column (eight slots, l* from e to e + 14). A
+// chunk's steps are sorted by the l* of their first slot, so the steps a tile
+// with multipole cut lc needs are the first step_count[lc] (pair_count[lc]).
 struct MStep {
-    int g;    // G element of (slot group, row 0) at this m1
+    int g;    // fragment of the step's slots x 16 rows in the stream-ordered G table
     int x;    // column mode kb + m1
     int e;    // l* of the group's first slot (its LDS row)
     int pad;
@@ -65,9 +68,11 @@ struct Geometry {
     double dr, eps, mu, off;   // off = -1/(2 mu dr^2), the stencil off-diagonal of H0
     const MChunk *mchunks;
     const MBlock *mblocks;     // nchunks * (l_max + 1)
-    const int *step_count;     // nchunks x (2 l_max + 1): steps of a chunk's stream at cut lc
+    const int *step_count;     // nchunks x (2 l_max + 1): single-group steps of a chunk at cut lc
     const MStep *msteps;       // steps of every chunk
-    const int *step_start;     // nchunks + 1 offsets into msteps
+    const int *step_start;     // nchunks + 1 offsets into msteps of the single-group streams
+    const int *pair_count;     // nchunks x (2 l_max + 1): paired steps of a chunk at cut lc
+    const int *pair_start;     // nchunks offsets into msteps of the paired streams
     unsigned gm_n;             // elements of GM / GM32
     const double *GM;          // Gaunt coefficients in MFMA block layout
     const double *E;           // 2 * Ls * N

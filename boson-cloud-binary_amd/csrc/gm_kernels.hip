@@ -305,7 +305,12 @@ template <> __device__ __forceinline__ double2 buffer_ld<double2>(__amdgpu_buffe
 // block and the LDS rows above lc hold zeros. After the last step the ring
 // repeats the last step's offsets (unused loads), so the loads stay
 // unconditional.
-template <int NG, typename T>
+//
+// GRP = 2 runs the chunk's paired stream: each step holds two consecutive slot
+// groups of one column (fragments at g and g + 64, LDS rows from e and e + 8),
+// chained on the matrix core (the second MFMA accumulates onto the first), so
+// the column is loaded and multiplied in once for eight slots.
+template <int NG, typename T, int GRP>
 __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, const typename Arith<T>::C *Xt, int t0,
                                             int chn, int lc, const T *sE, typename Arith<T>::C (&acc)[NG][4])
 {
@@ -315,9 +320,10 @@ __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, cons
     constexpr int RING = gmd::RING;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int N = g.N;
-    const int total = g.step_count[chn * (2 * g.lmax + 1) + lc];
+    const int cix = chn * (2 * g.lmax + 1) + lc;
+    const int total = GRP == 2 ? g.pair_count[cix] : g.step_count[cix];
     if (total == 0) return;
-    const gmd::MStep *st = g.msteps + g.step_start[chn];
+    const gmd::MStep *st = g.msteps + (GRP == 2 ? g.pair_start[chn] : g.step_start[chn]);
     const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(GM, g.gm_n * (unsigned)sizeof(T));
     const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(Xt, (unsigned)g.K * (unsigned)N * (unsigned)sizeof(C));
     // per-lane byte offsets; nodes past the grid repeat node N-1 (they only
@@ -338,36 +344,45 @@ __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, cons
     gmd::MStep ent[RING];
 #pragma unroll
     for (int k = 0; k < RING; ++k) ent[k] = entry(k);
-    T fr[RING];
+    T fr[RING][GRP];
     C xv[RING][NG];
     uint32_t eoff[RING];
     // issue the loads of `step` into slot k (ent[k] holds its entry) and read
     // the entry of the step RING later
     auto issue = [&](const int k, const int step) {
         const gmd::MStep e = ent[k];
-        fr[k] = buffer_ld<T>(rg, g_lane, (uint32_t)e.g * (uint32_t)sizeof(T));
+#pragma unroll
+        for (int h = 0; h < GRP; ++h)
+            fr[k][h] = buffer_ld<T>(rg, g_lane + (uint32_t)(64 * h * sizeof(T)), (uint32_t)e.g * (uint32_t)sizeof(T));
 #pragma unroll
         for (int n = 0; n < NG; ++n) xv[k][n] = buffer_ld<C>(rx, x_lane[n], (uint32_t)e.x * xcol);
         eoff[k] = (uint32_t)e.e * (uint32_t)(TN * sizeof(T));
         ent[k] = entry(step + RING);
     };
-    auto bf_load = [&](const int k, T (&bf)[NG]) {
+    auto bf_load = [&](const int k, T (&bf)[GRP][NG]) {
         const T *eb = (const T *)((const char *)sE + eoff[k] + e_lane);
 #pragma unroll
-        for (int n = 0; n < NG; ++n) bf[n] = eb[16 * n];
+        for (int h = 0; h < GRP; ++h)
+#pragma unroll
+            for (int n = 0; n < NG; ++n) bf[h][n] = eb[8 * h * TN + 16 * n];
     };
     const typename AR::V4 z = {T(0), T(0), T(0), T(0)};
-    auto consume = [&](const int k, const T (&bf)[NG]) {
+    auto consume = [&](const int k, const T (&bf)[GRP][NG]) {
         if constexpr (NG == 1) {
-            const typename AR::V4 D0 = AR::mfma(fr[k], bf[0], z);
+            typename AR::V4 D0 = AR::mfma(fr[k][0], bf[0][0], z);
+            if constexpr (GRP == 2) D0 = AR::mfma(fr[k][1], bf[1][0], D0);
             acc_update<NG, T>(acc, 0, D0, xv[k][0]);
         } else {
             // node groups in pairs: the accumulation of group n runs while
             // group n + 1 is on the matrix core
 #pragma unroll
             for (int n = 0; n < NG; n += 2) {
-                const typename AR::V4 D0 = AR::mfma(fr[k], bf[n], z);
-                const typename AR::V4 D1 = AR::mfma(fr[k], bf[n + 1], z);
+                typename AR::V4 D0 = AR::mfma(fr[k][0], bf[0][n], z);
+                typename AR::V4 D1 = AR::mfma(fr[k][0], bf[0][n + 1], z);
+                if constexpr (GRP == 2) {
+                    D0 = AR::mfma(fr[k][1], bf[1][n], D0);
+                    D1 = AR::mfma(fr[k][1], bf[1][n + 1], D1);
+                }
                 __builtin_amdgcn_sched_barrier(0);
                 acc_update<NG, T>(acc, n, D0, xv[k][n]);
                 __builtin_amdgcn_sched_barrier(0);
@@ -383,7 +398,7 @@ __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, cons
         issue(k, k);
         __builtin_amdgcn_sched_barrier(0);
     }
-    T bfa[NG], bfb[NG];
+    T bfa[GRP][NG], bfb[GRP][NG];
     bf_load(0, bfa);
 #pragma unroll 1
     for (int s = 0; s < padded; s += RING) {
@@ -418,7 +433,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 {
     typedef Arith<T> AR;
     constexpr int TN = 16 * NG;
-    __shared__ T sE[(gmd::ZROW + 7) * TN];  // rows l* = 0 .. lc, then zeros
+    __shared__ T sE[gmd::EROWS * TN];  // rows l* = 0 .. lc, then zeros
     // the workgroup's chunk rows (mode, l, m) and the phases exp(i m theta),
     // staged with the profiles for the epilogue
     __shared__ int sJ[gmd::AW][gmd::MR], sL[gmd::AW][gmd::MR], sM[gmd::AW][gmd::MR];
@@ -464,7 +479,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
             const int ls = e / TN, node = t0 + (e & (TN - 1));
             sE[e] = node < N ? E[(size_t)((ls & 1) * g.Ls + ls) * N + node] : T(0);
         }
-        for (int e = (lc + 1) * TN + threadIdx.x; e < (gmd::ZROW + 7) * TN; e += 64 * gmd::AW) sE[e] = T(0);
+        for (int e = (lc + 1) * TN + threadIdx.x; e < gmd::EROWS * TN; e += 64 * gmd::AW) sE[e] = T(0);
         for (int e = threadIdx.x; e < gmd::AW * gmd::MR; e += 64 * gmd::AW) {
             const int w = e / gmd::MR, r = e % gmd::MR, c = grp * gmd::AW + w;
             const bool ok = c < g.nchunks;
@@ -474,8 +489,11 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         }
         for (int e = threadIdx.x; e < 2 * lmax + 1; e += 64 * gmd::AW) sPh[e] = P->ph[e];
         __syncthreads();
-        if (wave_ok)
-            mfma_stream<NG, T>(g, GM, static_cast<const typename AR::C *>(a.Xt), t0, chn, lc, sE, acc);
+        if (wave_ok) {
+            const typename AR::C *Xt = static_cast<const typename AR::C *>(a.Xt);
+            mfma_stream<NG, T, 2>(g, GM, Xt, t0, chn, lc, sE, acc);
+            mfma_stream<NG, T, 1>(g, GM, Xt, t0, chn, lc, sE, acc);
+        }
     }
     if (wave_ok) {
         // epilogue: lane owns rows q + 4r and nodes 16 n + col

@@ -139,7 +139,7 @@ struct gm_solver {
     std::vector<double> GM_h, Gdiag_h;
     DevBuf<gmd::MChunk> mchunks;
     DevBuf<gmd::MBlock> mblocks;
-    DevBuf<int> step_count;
+    DevBuf<int> step_count, pair_count, pair_start;
     unsigned gm_n = 0;  // elements of GM / GM32 (Gaunt fragments in stream order)
     DevBuf<gmd::MStep> msteps;
     DevBuf<int> mstep_start;
@@ -474,6 +474,8 @@ gmd::Geometry geometry(const gm_solver &S)
     g.step_count = S.step_count.p;
     g.msteps = S.msteps.p;
     g.step_start = S.mstep_start.p;
+    g.pair_count = S.pair_count.p;
+    g.pair_start = S.pair_start.p;
     g.gm_n = S.gm_n;
     g.GM = S.GM.p;
     g.GM32 = S.GM32.p;
@@ -1141,19 +1143,34 @@ void alloc_device(gm_solver &S)
     HIPCK(hipMemcpy(S.mchunks.p, S.mchunks_h.data(), S.mchunks_h.size() * sizeof(gmd::MChunk), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(S.mblocks.p, S.mblocks_h.data(), S.mblocks_h.size() * sizeof(gmd::MBlock), hipMemcpyHostToDevice));
     {
-        // The coupling stream of each chunk: for every column (l1, m1), the
+        // The coupling streams of each chunk: for every column (l1, m1), the
         // slot groups of four covering the slots where some row of the chunk
         // has a non-zero Gaunt coefficient (the groups stay inside the
         // block's nTp slots; slots outside that range hold zeros for this
-        // column, so a group may extend over them). Steps are sorted by the
-        // l* of their first slot, so a tile with cut lc runs a prefix.
+        // column, so a group may extend over them). Consecutive groups of a
+        // column go in pairs to the paired stream, an odd last group to the
+        // single-group stream. Each stream is sorted by the l* of its steps'
+        // first slot, so a tile with cut lc runs a prefix of both.
         const int Lm = S.l_max + 1, Lc = 2 * S.l_max + 1;
-        std::vector<gmd::MStep> msteps;
-        std::vector<int> sstart(static_cast<size_t>(S.nchunks) + 1, 0);
-        std::vector<int> count(static_cast<size_t>(S.nchunks) * Lc, 0);
+        std::vector<gmd::MStep> msteps;  // g: source offset into GM_h until staged
+        std::vector<int> nvals;          // fragment values of each step (64, or 128 paired)
+        std::vector<char> padding;
+        std::vector<int> sstart(static_cast<size_t>(S.nchunks) + 1, 0), pstart(static_cast<size_t>(S.nchunks), 0);
+        std::vector<int> count(static_cast<size_t>(S.nchunks) * Lc, 0), pcount(count.size(), 0);
+        auto append = [&](const std::vector<gmd::MStep> &v, int vals) {
+            msteps.insert(msteps.end(), v.begin(), v.end());
+            nvals.insert(nvals.end(), v.size(), vals);
+            padding.insert(padding.end(), v.size(), 0);
+            // padding steps: the last step's fragment and column, the zero LDS rows
+            const gmd::MStep last = v.empty() ? gmd::MStep{} : v.back();
+            for (int k = 0; k + 1 < gmd::RING; ++k) {
+                msteps.push_back(gmd::MStep{last.g, last.x, gmd::ZROW, 0});
+                nvals.push_back(vals);
+                padding.push_back(1);
+            }
+        };
         for (int c = 0; c < S.nchunks; ++c) {
-            sstart[c] = static_cast<int>(msteps.size());
-            std::vector<gmd::MStep> v;
+            std::vector<gmd::MStep> v1, v2;
             for (int l1 = S.l_max; l1 >= 0; --l1) {
                 const gmd::MBlock &B = S.mblocks_h[static_cast<size_t>(c) * Lm + l1];
                 const int p = (S.mchunks_h[c].par + l1) & 1;
@@ -1169,36 +1186,43 @@ void alloc_device(gm_solver &S)
                     if (th < 0) continue;
                     const int ng = (th - tl + 4) / 4;
                     const int t0 = std::min(tl, nTp - 4 * ng);
-                    for (int k = 0; k < ng; ++k) {
+                    for (int k = 0; k < ng; k += 2) {
                         const int t = t0 + 4 * k;
-                        v.push_back(gmd::MStep{B.goff + (m1i * nTp + t) * gmd::MR, l1 * (l1 + 1) / 2 + m1i,
-                                               2 * (B.tlo + t) + p, 0});
+                        const gmd::MStep st{B.goff + (m1i * nTp + t) * gmd::MR, l1 * (l1 + 1) / 2 + m1i,
+                                            2 * (B.tlo + t) + p, 0};
+                        (k + 1 < ng ? v2 : v1).push_back(st);
                     }
                 }
             }
-            std::stable_sort(v.begin(), v.end(), [](const gmd::MStep &x, const gmd::MStep &y) { return x.e < y.e; });
-            for (int lc = 0; lc < Lc; ++lc)
-                for (const gmd::MStep &e : v) count[static_cast<size_t>(c) * Lc + lc] += e.e <= lc;
-            msteps.insert(msteps.end(), v.begin(), v.end());
-            // padding steps: valid G and column offsets, the zero LDS row
-            const gmd::MStep last = v.empty() ? gmd::MStep{} : v.back();
-            for (int k = 0; k + 1 < gmd::RING; ++k) msteps.push_back(gmd::MStep{last.g, last.x, gmd::ZROW, 0});
+            const auto by_e = [](const gmd::MStep &x, const gmd::MStep &y) { return x.e < y.e; };
+            std::stable_sort(v1.begin(), v1.end(), by_e);
+            std::stable_sort(v2.begin(), v2.end(), by_e);
+            for (int lc = 0; lc < Lc; ++lc) {
+                for (const gmd::MStep &e : v1) count[static_cast<size_t>(c) * Lc + lc] += e.e <= lc;
+                for (const gmd::MStep &e : v2) pcount[static_cast<size_t>(c) * Lc + lc] += e.e <= lc;
+            }
+            pstart[c] = static_cast<int>(msteps.size());
+            append(v2, 8 * gmd::MR);
+            sstart[c] = static_cast<int>(msteps.size());
+            append(v1, 4 * gmd::MR);
         }
         sstart[S.nchunks] = static_cast<int>(msteps.size());
         msteps.push_back(gmd::MStep{});
-        // Gaunt fragments in stream order: the 4 slots x 16 rows of every step
-        // stored contiguously (a tile streams a prefix of its chunk's table)
+        nvals.push_back(4 * gmd::MR);
+        padding.push_back(1);
+        // Gaunt fragments in stream order: the slots x 16 rows of every step
+        // stored contiguously (a tile streams a prefix of its chunk's tables)
         std::vector<double> gmc;
         gmc.reserve(msteps.size() * 4 * gmd::MR);
         for (size_t k = 0; k < msteps.size(); ++k) {
             gmd::MStep &e = msteps[k];
-            if (e.e == gmd::ZROW && k > 0) {  // padding step: the previous step's fragment
-                e.g = msteps[k - 1].g;
+            if (padding[k]) {  // the previous step's fragment (the first entry is never padding)
+                e.g = k > 0 ? msteps[k - 1].g : 0;
                 continue;
             }
             const size_t src = static_cast<size_t>(e.g);
             e.g = static_cast<int>(gmc.size());
-            for (int v = 0; v < 4 * gmd::MR; ++v) gmc.push_back(src + v < S.GM_h.size() ? S.GM_h[src + v] : 0.0);
+            for (int v = 0; v < nvals[k]; ++v) gmc.push_back(src + v < S.GM_h.size() ? S.GM_h[src + v] : 0.0);
         }
         S.gm_n = static_cast<unsigned>(gmc.size());
         S.GM.alloc(gmc.size());
@@ -1209,9 +1233,13 @@ void alloc_device(gm_solver &S)
         S.msteps.alloc(msteps.size());
         S.mstep_start.alloc(sstart.size());
         S.step_count.alloc(count.size());
+        S.pair_start.alloc(pstart.size());
+        S.pair_count.alloc(pcount.size());
         HIPCK(hipMemcpy(S.msteps.p, msteps.data(), msteps.size() * sizeof(gmd::MStep), hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(S.mstep_start.p, sstart.data(), sstart.size() * sizeof(int), hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(S.step_count.p, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.pair_start.p, pstart.data(), pstart.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.pair_count.p, pcount.data(), pcount.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     HIPCK(hipMemcpy(S.Gdiag.p, S.Gdiag_h.data(), S.Gdiag_h.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCK(hipMemsetAsync(S.dK.p, 0, sizeof(gmd::Krylov), S.st));
