@@ -53,7 +53,7 @@ constexpr int EROWS = ZROW + 15;
 // the (chunk, l1) block at one column m1, or, in the chunk's paired stream,
 // two such groups of the same column (eight slots, l* from e to e + 14). A
 // chunk's steps are sorted by the l* of their first slot, so the steps a tile
-// with multipole cut lc needs are the first step_count[lc] (pair_count[lc]).
+// with multipole cut lc needs are the first step_count[lc] of its stream.
 struct MStep {
     int g;    // fragment of the step's slots x 16 rows in the stream-ordered G table
     int x;    // column mode kb + m1
@@ -68,12 +68,12 @@ struct Geometry {
     double dr, eps, mu, off;   // off = -1/(2 mu dr^2), the stencil off-diagonal of H0
     const MChunk *mchunks;
     const MBlock *mblocks;     // nchunks * (l_max + 1)
-    const int *step_count;     // nchunks x (2 l_max + 1): single-group steps of a chunk at cut lc
-    const MStep *msteps;       // steps of every chunk
-    const int *step_start;     // nchunks + 1 offsets into msteps of the single-group streams
-    const int *pair_count;     // nchunks x (2 l_max + 1): paired steps of a chunk at cut lc
-    const int *pair_start;     // nchunks offsets into msteps of the paired streams
-    unsigned gm_n;             // elements of GM / GM32
+    // stream tables: 0 = FP64 single-group steps, 1 = FP32 single-group
+    // steps, 2 = FP32 paired steps
+    const int *step_count;     // 3 x nchunks x (2 l_max + 1): steps of a chunk's stream at cut lc
+    const MStep *msteps;       // steps of every stream
+    const int *step_start;     // 3 x nchunks offsets into msteps
+    unsigned gm_n, gm32_n;     // elements of GM, GM32
     const double *GM;          // Gaunt coefficients in MFMA block layout
     const double *E;           // 2 * Ls * N
     const float *GM32;         // FP32 copies for the mixed-precision Krylov applies

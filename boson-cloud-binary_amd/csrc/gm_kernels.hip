@@ -306,13 +306,14 @@ template <> __device__ __forceinline__ double2 buffer_ld<double2>(__amdgpu_buffe
 // repeats the last step's offsets (unused loads), so the loads stay
 // unconditional.
 //
-// GRP = 2 runs the chunk's paired stream: each step holds two consecutive slot
-// groups of one column (fragments at g and g + 64, LDS rows from e and e + 8),
-// chained on the matrix core (the second MFMA accumulates onto the first), so
-// the column is loaded and multiplied in once for eight slots.
+// GRP = 2 runs a chunk's paired stream (FP32): each step holds two consecutive
+// slot groups of one column (fragments at g and g + 64, LDS rows from e and
+// e + 8), chained on the matrix core (the second MFMA accumulates onto the
+// first), so the column is loaded and multiplied in once for eight slots.
+// `tab` selects the stream table.
 template <int NG, typename T, int GRP>
 __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, const typename Arith<T>::C *Xt, int t0,
-                                            int chn, int lc, const T *sE, typename Arith<T>::C (&acc)[NG][4])
+                                            int chn, int lc, int tab, const T *sE, typename Arith<T>::C (&acc)[NG][4])
 {
     typedef Arith<T> AR;
     typedef typename AR::C C;
@@ -320,11 +321,12 @@ __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, cons
     constexpr int RING = gmd::RING;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int N = g.N;
-    const int cix = chn * (2 * g.lmax + 1) + lc;
-    const int total = GRP == 2 ? g.pair_count[cix] : g.step_count[cix];
+    const int ts = tab * g.nchunks + chn;
+    const int total = g.step_count[ts * (2 * g.lmax + 1) + lc];
     if (total == 0) return;
-    const gmd::MStep *st = g.msteps + (GRP == 2 ? g.pair_start[chn] : g.step_start[chn]);
-    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(GM, g.gm_n * (unsigned)sizeof(T));
+    const gmd::MStep *st = g.msteps + g.step_start[ts];
+    const unsigned gmn = std::is_same<T, double>::value ? g.gm_n : g.gm32_n;
+    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(GM, gmn * (unsigned)sizeof(T));
     const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(Xt, (unsigned)g.K * (unsigned)N * (unsigned)sizeof(C));
     // per-lane byte offsets; nodes past the grid repeat node N-1 (they only
     // feed accumulators never stored)
@@ -433,7 +435,10 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 {
     typedef Arith<T> AR;
     constexpr int TN = 16 * NG;
-    __shared__ T sE[gmd::EROWS * TN];  // rows l* = 0 .. lc, then zeros
+    // rows l* = 0 .. lc, then zeros (single-group padding steps read up to
+    // ZROW + 6, paired ones up to ZROW + 14)
+    constexpr int ER = std::is_same<T, double>::value ? gmd::ZROW + 7 : gmd::EROWS;
+    __shared__ T sE[ER * TN];
     // the workgroup's chunk rows (mode, l, m) and the phases exp(i m theta),
     // staged with the profiles for the epilogue
     __shared__ int sJ[gmd::AW][gmd::MR], sL[gmd::AW][gmd::MR], sM[gmd::AW][gmd::MR];
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
             const int ls = e / TN, node = t0 + (e & (TN - 1));
             sE[e] = node < N ? E[(size_t)((ls & 1) * g.Ls + ls) * N + node] : T(0);
         }
-        for (int e = (lc + 1) * TN + threadIdx.x; e < gmd::EROWS * TN; e += 64 * gmd::AW) sE[e] = T(0);
+        for (int e = (lc + 1) * TN + threadIdx.x; e < ER * TN; e += 64 * gmd::AW) sE[e] = T(0);
         for (int e = threadIdx.x; e < gmd::AW * gmd::MR; e += 64 * gmd::AW) {
             const int w = e / gmd::MR, r = e % gmd::MR, c = grp * gmd::AW + w;
             const bool ok = c < g.nchunks;
@@ -491,8 +496,12 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         __syncthreads();
         if (wave_ok) {
             const typename AR::C *Xt = static_cast<const typename AR::C *>(a.Xt);
-            mfma_stream<NG, T, 2>(g, GM, Xt, t0, chn, lc, sE, acc);
-            mfma_stream<NG, T, 1>(g, GM, Xt, t0, chn, lc, sE, acc);
+            if constexpr (std::is_same<T, double>::value) {
+                mfma_stream<NG, T, 1>(g, GM, Xt, t0, chn, lc, 0, sE, acc);
+            } else {
+                mfma_stream<NG, T, 2>(g, GM, Xt, t0, chn, lc, 2, sE, acc);
+                mfma_stream<NG, T, 1>(g, GM, Xt, t0, chn, lc, 1, sE, acc);
+            }
         }
     }
     if (wave_ok) {
