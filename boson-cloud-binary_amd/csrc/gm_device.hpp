@@ -50,10 +50,10 @@ constexpr int ZROW = 2 * MAXLM - 1;
 constexpr int EROWS = ZROW + 15;
 
 // One step of a chunk's coupling stream: four consecutive multipole slots of
-// the (chunk, l1) block at one column m1, or, in the chunk's paired stream,
-// two such groups of the same column (eight slots, l* from e to e + 14). A
-// chunk's steps are sorted by the l* of their first slot, so the steps a tile
-// with multipole cut lc needs are the first step_count[lc] of its stream.
+// the (chunk, l1) block at one column m1 (a single), or two such groups of the
+// same column (a pair: eight slots, l* from e to e + 14, 128 fragment values).
+// A chunk's pairs and singles are each sorted by the l* of their first slot,
+// so a tile with multipole cut lc runs prefixes and ranges of the lists.
 struct MStep {
     int g;    // fragment of the step's slots x 16 rows in the stream-ordered G table
     int x;    // column mode kb + m1
@@ -68,12 +68,13 @@ struct Geometry {
     double dr, eps, mu, off;   // off = -1/(2 mu dr^2), the stencil off-diagonal of H0
     const MChunk *mchunks;
     const MBlock *mblocks;     // nchunks * (l_max + 1)
-    // stream tables: 0 = FP64 single-group steps, 1 = FP32 single-group
-    // steps, 2 = FP32 paired steps
-    const int *step_count;     // 3 x nchunks x (2 l_max + 1): steps of a chunk's stream at cut lc
-    const MStep *msteps;       // steps of every stream
-    const int *step_start;     // 3 x nchunks offsets into msteps
-    unsigned gm_n, gm32_n;     // elements of GM, GM32
+    const MStep *msteps;       // steps of every chunk's pair and single lists
+    const int *step_start;     // nchunks offsets into msteps of the single lists
+    const int *step_count;     // nchunks x (2 l_max + 1): singles a tile with cut lc runs
+    const int *pair_start;     // nchunks offsets into msteps of the pair lists
+    const int *pair_full;      // nchunks x (2 l_max + 1): pairs run whole at cut lc (a prefix)
+    const int *pair_end;       // nchunks x (2 l_max + 1): end of the pairs run as their first group
+    unsigned gm_n;             // elements of GM / GM32
     const double *GM;          // Gaunt coefficients in MFMA block layout
     const double *E;           // 2 * Ls * N
     const float *GM32;         // FP32 copies for the mixed-precision Krylov applies

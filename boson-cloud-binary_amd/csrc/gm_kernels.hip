@@ -295,9 +295,9 @@ template <> __device__ __forceinline__ double2 buffer_ld<double2>(__amdgpu_buffe
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
 }
 
-// The coupling of one chunk at one tile as a single stream of steps from the
-// chunk's step table: a tile with multipole cut lc runs the first
-// step_count[lc] steps (the table is sorted by the first slot's l*). The A
+// The coupling of one chunk at one tile, one stream of steps from one of the
+// chunk's step lists (each sorted by the first slot's l*, so that a tile with
+// multipole cut lc runs a prefix or a range of it). The A
 // fragment G[m1][slot][row] and the phased column vectors of the next RING
 // steps are in flight, as buffer loads with fixed per-lane offsets and the
 // step's scalar offsets; the B fragment E_{l*}(node) is read from LDS one step
@@ -306,14 +306,17 @@ template <> __device__ __forceinline__ double2 buffer_ld<double2>(__amdgpu_buffe
 // repeats the last step's offsets (unused loads), so the loads stay
 // unconditional.
 //
-// GRP = 2 runs a chunk's paired stream (FP32): each step holds two consecutive
-// slot groups of one column (fragments at g and g + 64, LDS rows from e and
-// e + 8), chained on the matrix core (the second MFMA accumulates onto the
-// first), so the column is loaded and multiplied in once for eight slots.
-// `tab` selects the stream table.
+// GRP = 2 runs pairs: each step holds two consecutive slot groups of one
+// column (fragments at g and g + 64, LDS rows from e and e + 8), chained on
+// the matrix core (the second MFMA accumulates onto the first), so the column
+// is loaded and multiplied in once for eight slots. GRP = 1 runs singles, or
+// pairs as their first group alone. The stream is the `total` steps from
+// `st`, rounded up to whole ring rounds; the caller makes sure that the
+// entries past `total` add nothing or belong to no other stream.
 template <int NG, typename T, int GRP>
 __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, const typename Arith<T>::C *Xt, int t0,
-                                            int chn, int lc, int tab, const T *sE, typename Arith<T>::C (&acc)[NG][4])
+                                            const gmd::MStep *st, int total, const T *sE,
+                                            typename Arith<T>::C (&acc)[NG][4])
 {
     typedef Arith<T> AR;
     typedef typename AR::C C;
@@ -321,12 +324,8 @@ __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, cons
     constexpr int RING = gmd::RING;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int N = g.N;
-    const int ts = tab * g.nchunks + chn;
-    const int total = g.step_count[ts * (2 * g.lmax + 1) + lc];
     if (total == 0) return;
-    const gmd::MStep *st = g.msteps + g.step_start[ts];
-    const unsigned gmn = std::is_same<T, double>::value ? g.gm_n : g.gm32_n;
-    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(GM, gmn * (unsigned)sizeof(T));
+    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(GM, g.gm_n * (unsigned)sizeof(T));
     const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(Xt, (unsigned)g.K * (unsigned)N * (unsigned)sizeof(C));
     // per-lane byte offsets; nodes past the grid repeat node N-1 (they only
     // feed accumulators never stored)
@@ -437,7 +436,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     constexpr int TN = 16 * NG;
     // rows l* = 0 .. lc, then zeros (single-group padding steps read up to
     // ZROW + 6, paired ones up to ZROW + 14)
-    constexpr int ER = std::is_same<T, double>::value ? gmd::ZROW + 7 : gmd::EROWS;
+    constexpr int ER = gmd::EROWS;
     __shared__ T sE[ER * TN];
     // the workgroup's chunk rows (mode, l, m) and the phases exp(i m theta),
     // staged with the profiles for the epilogue
@@ -496,12 +495,18 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         __syncthreads();
         if (wave_ok) {
             const typename AR::C *Xt = static_cast<const typename AR::C *>(a.Xt);
-            if constexpr (std::is_same<T, double>::value) {
-                mfma_stream<NG, T, 1>(g, GM, Xt, t0, chn, lc, 0, sE, acc);
-            } else {
-                mfma_stream<NG, T, 2>(g, GM, Xt, t0, chn, lc, 2, sE, acc);
-                mfma_stream<NG, T, 1>(g, GM, Xt, t0, chn, lc, 1, sE, acc);
-            }
+            // Whole pairs, then pairs cut after their first group, then
+            // singles. The pair stream runs its count rounded up to whole
+            // ring rounds; the pairs it runs past the whole ones are exact as
+            // pairs too (a second group above the cut reads zero rows), so
+            // the cut pairs start after them.
+            const int cix = chn * (2 * lmax + 1) + lc;
+            const gmd::MStep *pl = g.msteps + g.pair_start[chn];
+            const int pf = g.pair_full[cix], pe = g.pair_end[cix];
+            const int hb = min((pf + gmd::RING - 1) / gmd::RING * gmd::RING, pe);
+            mfma_stream<NG, T, 2>(g, GM, Xt, t0, pl, pf, sE, acc);
+            mfma_stream<NG, T, 1>(g, GM, Xt, t0, pl + hb, pe - hb, sE, acc);
+            mfma_stream<NG, T, 1>(g, GM, Xt, t0, g.msteps + g.step_start[chn], g.step_count[cix], sE, acc);
         }
     }
     if (wave_ok) {

@@ -139,8 +139,8 @@ struct gm_solver {
     std::vector<double> GM_h, Gdiag_h;
     DevBuf<gmd::MChunk> mchunks;
     DevBuf<gmd::MBlock> mblocks;
-    DevBuf<int> step_count;
-    unsigned gm_n = 0, gm32_n = 0;  // elements of GM / GM32 (Gaunt fragments in stream order)
+    DevBuf<int> step_count, pair_full, pair_end, pair_start;
+    unsigned gm_n = 0;  // elements of GM / GM32 (Gaunt fragments in stream order)
     DevBuf<gmd::MStep> msteps;
     DevBuf<int> mstep_start;
     DevBuf<double> GM, Gdiag, E, dmask;
@@ -474,8 +474,10 @@ gmd::Geometry geometry(const gm_solver &S)
     g.step_count = S.step_count.p;
     g.msteps = S.msteps.p;
     g.step_start = S.mstep_start.p;
+    g.pair_start = S.pair_start.p;
+    g.pair_full = S.pair_full.p;
+    g.pair_end = S.pair_end.p;
     g.gm_n = S.gm_n;
-    g.gm32_n = S.gm32_n;
     g.GM = S.GM.p;
     g.GM32 = S.GM32.p;
     g.E32 = S.E32.p;
@@ -1146,37 +1148,37 @@ void alloc_device(gm_solver &S)
         // slot groups of four covering the slots where some row of the chunk
         // has a non-zero Gaunt coefficient (the groups stay inside the
         // block's nTp slots; slots outside that range hold zeros for this
-        // column, so a group may extend over them). Three tables:
-        //   0  FP64: every group a step of its own;
-        //   1  FP32 single-group steps: the odd last group of a column;
-        //   2  FP32 paired steps: consecutive groups of a column, two at a time.
-        // Pairing saves an accumulation and a column load per pair but runs the
-        // second group's MFMA also in tiles whose cut falls inside the pair: a
-        // gain at 32 cycles per FP32 MFMA, not at 64 per FP64 one. Each stream
-        // is sorted by the l* of its steps' first slot, so a tile with cut lc
-        // runs a prefix of it.
+        // column, so a group may extend over them). Consecutive groups of a
+        // column go in pairs (groups 0 and 1, 2 and 3, ...) to the chunk's
+        // pair list, an odd last group to its single list; both are sorted by
+        // the l* of their first slot e. A tile with cut lc runs
+        //   - the pairs with e + 8 <= lc, both groups chained (a prefix),
+        //   - the pairs with e <= lc < e + 8 as their first group alone (the
+        //     next entries of the same list: the second group lies above the
+        //     cut),
+        //   - the singles with e <= lc (a prefix),
+        // so it runs the same MFMAs as with one step per group.
         const int Lm = S.l_max + 1, Lc = 2 * S.l_max + 1;
         const size_t CL = static_cast<size_t>(S.nchunks) * Lc;
         std::vector<gmd::MStep> msteps;  // g: source offset into GM_h until staged
         std::vector<int> nvals;          // fragment values of each step (64, or 128 paired)
-        std::vector<char> padding, f32;
-        std::vector<int> sstart(static_cast<size_t>(3) * S.nchunks, 0), count(3 * CL, 0);
-        auto append = [&](const std::vector<gmd::MStep> &v, int vals, bool in32) {
+        std::vector<char> padding;
+        std::vector<int> sstart(S.nchunks, 0), pstart(S.nchunks, 0);
+        std::vector<int> scount(CL, 0), pfull(CL, 0), pend(CL, 0);
+        auto append = [&](const std::vector<gmd::MStep> &v, int vals) {
             msteps.insert(msteps.end(), v.begin(), v.end());
             nvals.insert(nvals.end(), v.size(), vals);
             padding.insert(padding.end(), v.size(), 0);
-            f32.insert(f32.end(), v.size(), in32);
             // padding steps: the last step's fragment and column, the zero LDS rows
             const gmd::MStep last = v.empty() ? gmd::MStep{} : v.back();
             for (int k = 0; k + 1 < gmd::RING; ++k) {
                 msteps.push_back(gmd::MStep{last.g, last.x, gmd::ZROW, 0});
                 nvals.push_back(vals);
                 padding.push_back(1);
-                f32.push_back(in32);
             }
         };
         for (int c = 0; c < S.nchunks; ++c) {
-            std::vector<gmd::MStep> v[3];
+            std::vector<gmd::MStep> v1, v2;
             for (int l1 = S.l_max; l1 >= 0; --l1) {
                 const gmd::MBlock &B = S.mblocks_h[static_cast<size_t>(c) * Lm + l1];
                 const int p = (S.mchunks_h[c].par + l1) & 1;
@@ -1192,56 +1194,66 @@ void alloc_device(gm_solver &S)
                     if (th < 0) continue;
                     const int ng = (th - tl + 4) / 4;
                     const int t0 = std::min(tl, nTp - 4 * ng);
-                    for (int k = 0; k < ng; ++k) {
+                    for (int k = 0; k < ng; k += 2) {
                         const int t = t0 + 4 * k;
                         const gmd::MStep st{B.goff + (m1i * nTp + t) * gmd::MR, l1 * (l1 + 1) / 2 + m1i,
                                             2 * (B.tlo + t) + p, 0};
-                        v[0].push_back(st);
-                        if (k % 2 == 0) v[k + 1 < ng ? 2 : 1].push_back(st);
+                        (k + 1 < ng ? v2 : v1).push_back(st);
                     }
                 }
             }
-            for (int tab = 0; tab < 3; ++tab) {
-                std::stable_sort(v[tab].begin(), v[tab].end(),
-                                 [](const gmd::MStep &x, const gmd::MStep &y) { return x.e < y.e; });
-                for (int lc = 0; lc < Lc; ++lc)
-                    for (const gmd::MStep &e : v[tab]) count[tab * CL + static_cast<size_t>(c) * Lc + lc] += e.e <= lc;
-                sstart[static_cast<size_t>(tab) * S.nchunks + c] = static_cast<int>(msteps.size());
-                append(v[tab], (tab == 2 ? 8 : 4) * gmd::MR, tab > 0);
+            const auto by_e = [](const gmd::MStep &x, const gmd::MStep &y) { return x.e < y.e; };
+            std::stable_sort(v1.begin(), v1.end(), by_e);
+            std::stable_sort(v2.begin(), v2.end(), by_e);
+            for (int lc = 0; lc < Lc; ++lc) {
+                const size_t ix = static_cast<size_t>(c) * Lc + lc;
+                for (const gmd::MStep &e : v1) scount[ix] += e.e <= lc;
+                for (const gmd::MStep &e : v2) {
+                    pfull[ix] += e.e + 8 <= lc;
+                    pend[ix] += e.e <= lc;
+                }
             }
+            pstart[c] = static_cast<int>(msteps.size());
+            append(v2, 8 * gmd::MR);
+            sstart[c] = static_cast<int>(msteps.size());
+            append(v1, 4 * gmd::MR);
         }
         msteps.push_back(gmd::MStep{});
         nvals.push_back(4 * gmd::MR);
         padding.push_back(1);
-        f32.push_back(0);
-        // Gaunt fragments in stream order, one table per precision: the slots x
-        // 16 rows of every step stored contiguously (a tile streams a prefix of
-        // its chunk's streams)
-        std::vector<double> gmc, gmc32;
+        // Gaunt fragments in stream order: the slots x 16 rows of every step
+        // stored contiguously (a tile streams prefixes and ranges of its
+        // chunk's lists front to back)
+        std::vector<double> gmc;
+        gmc.reserve(msteps.size() * 4 * gmd::MR);
         for (size_t k = 0; k < msteps.size(); ++k) {
             gmd::MStep &e = msteps[k];
             if (padding[k]) {  // the previous step's fragment (the first entry is never padding)
                 e.g = k > 0 ? msteps[k - 1].g : 0;
                 continue;
             }
-            std::vector<double> &dst = f32[k] ? gmc32 : gmc;
             const size_t src = static_cast<size_t>(e.g);
-            e.g = static_cast<int>(dst.size());
-            for (int v = 0; v < nvals[k]; ++v) dst.push_back(src + v < S.GM_h.size() ? S.GM_h[src + v] : 0.0);
+            e.g = static_cast<int>(gmc.size());
+            for (int v = 0; v < nvals[k]; ++v) gmc.push_back(src + v < S.GM_h.size() ? S.GM_h[src + v] : 0.0);
         }
         S.gm_n = static_cast<unsigned>(gmc.size());
-        S.gm32_n = static_cast<unsigned>(gmc32.size());
         S.GM.alloc(gmc.size());
-        S.GM32.alloc(gmc32.size());
+        S.GM32.alloc(gmc.size());
         HIPCK(hipMemcpy(S.GM.p, gmc.data(), gmc.size() * sizeof(double), hipMemcpyHostToDevice));
-        const std::vector<float> gm32(gmc32.begin(), gmc32.end());
+        const std::vector<float> gm32(gmc.begin(), gmc.end());
         HIPCK(hipMemcpy(S.GM32.p, gm32.data(), gm32.size() * sizeof(float), hipMemcpyHostToDevice));
         S.msteps.alloc(msteps.size());
         S.mstep_start.alloc(sstart.size());
-        S.step_count.alloc(count.size());
+        S.pair_start.alloc(pstart.size());
+        S.step_count.alloc(scount.size());
+        S.pair_full.alloc(pfull.size());
+        S.pair_end.alloc(pend.size());
         HIPCK(hipMemcpy(S.msteps.p, msteps.data(), msteps.size() * sizeof(gmd::MStep), hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(S.mstep_start.p, sstart.data(), sstart.size() * sizeof(int), hipMemcpyHostToDevice));
-        HIPCK(hipMemcpy(S.step_count.p, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.pair_start.p, pstart.data(), pstart.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.step_count.p, scount.data(), scount.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.pair_full.p, pfull.data(), pfull.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(S.pair_end.p, pend.data(), pend.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     HIPCK(hipMemcpy(S.Gdiag.p, S.Gdiag_h.data(), S.Gdiag_h.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCK(hipMemsetAsync(S.dK.p, 0, sizeof(gmd::Krylov), S.st));
