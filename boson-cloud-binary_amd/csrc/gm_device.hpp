@@ -6,9 +6,11 @@
 //   E table           2 * (2 l_max + 1) * Nr doubles: the radial multipole
 //                     profiles c_l* (f1 + sign prf f2) for even (index 0) and
 //                     odd (index 1) m*-parity pairs;
-//   GM table          per (16-row chunk, l1) block, [m1][t][row] doubles: the
-//                     Gaunt coefficient of the row's mode with (l1, m1) at
-//                     l* = 2 (tlo + t) + (l + l1 mod 2), the MFMA A operand;
+//   GM table          the MFMA A operands in coupling-stream order: per step
+//                     of a chunk's pair or single list, [slot][row] doubles
+//                     (8 or 4 slots x 16 rows), the Gaunt coefficient of the
+//                     row's mode with the step's column at each slot's l*
+//                     (host-side they come from per-(chunk, l1) blocks, MBlock);
 //   preconditioner    K*Nr double2 inverse pivots of each mode's radial tridiagonal.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -41,8 +43,8 @@ struct MBlock {
     int goff, tlo, nT, pad;
 };
 
-// Depth of the operand ring of the coupling stream; each chunk's step list
-// ends with RING - 1 padding steps that add nothing.
+// Depth of the operand ring of the coupling stream; each of a chunk's step
+// lists ends with RING - 1 padding steps that add nothing.
 constexpr int RING = 4;
 // LDS row read by padding steps (every row above a tile's cut holds zeros).
 constexpr int ZROW = 2 * MAXLM - 1;
