@@ -1685,6 +1685,18 @@ int gm_coupling_work(gm_solver *S, double *terms, double *pairs)
     });
 }
 
+int gm_multipole_cut(gm_solver *S, int32_t *lcut, int32_t n)
+{
+    return guarded(S, [&] {
+        require_initialized(*S);
+        if (!lcut || n < S->ntiles) throw std::invalid_argument("lcut needs one entry per 64-node tile");
+        std::vector<int> lc(S->ntiles);
+        HIPCK(hipMemcpyAsync(lc.data(), S->lcut.p, sizeof(int) * S->ntiles, hipMemcpyDeviceToHost, S->st));
+        HIPCK(hipStreamSynchronize(S->st));
+        for (int t = 0; t < S->ntiles; ++t) lcut[t] = lc[t];
+    });
+}
+
 int gm_profile_enable(gm_solver *S, int32_t on)
 {
     return guarded(S, [&] {

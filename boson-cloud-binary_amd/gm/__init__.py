@@ -7,6 +7,8 @@ is visible, constructing a solver raises.
 import ctypes as C
 import os
 
+import numpy as np
+
 from . import capi  # noqa: F401
 from .capi import GM_PRECISION_FP64, GM_PRECISION_MIXED, GMError, InvalidArgument, LogicError  # noqa: F401
 
@@ -27,6 +29,7 @@ def lib():
         vp = C.c_void_p
         _lib.gm_set_coupling_tolerance.argtypes = [vp, C.c_double]
         _lib.gm_coupling_work.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib.gm_multipole_cut.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         _lib.gm_profile_enable.argtypes = [vp, C.c_int32]
         _lib.gm_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
         _lib.gm_ensemble_step.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(capi.Diag)]
@@ -83,6 +86,13 @@ class CrankNicolsonModes(capi.Solver):
         t, p = C.c_double(), C.c_double()
         self._check(self._lib.gm_coupling_work(self._h, C.byref(t), C.byref(p)))
         return t.value, p.value
+
+    def multipole_cut(self):
+        """Largest multipole l* evaluated at each 64-node tile (int32 array)."""
+        n = (self.Nr + 63) // 64
+        out = np.zeros(n, dtype=np.int32)
+        self._check(self._lib.gm_multipole_cut(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), n))
+        return out
 
     def apply_config(self, precision=GM_PRECISION_FP64):
         """(tile nodes chosen by the autotuner, {16: ms, 32: ms, 64: ms})."""

@@ -209,6 +209,10 @@ int gm_set_coupling_tolerance(gm_solver *solver, double tol);
 /* Work of one coupling apply under the current truncation, summed over radial
  * nodes: Gaunt terms evaluated (terms) and mode pairs coupled (pairs). */
 int gm_coupling_work(gm_solver *solver, double *terms, double *pairs);
+/* The truncation itself (no counterpart in the reference): the largest
+ * multipole l* evaluated at each tile of 64 radial nodes, tile t covering
+ * nodes [64 t, 64 t + 64); n must be at least ceil(Nr / 64). */
+int gm_multipole_cut(gm_solver *solver, int32_t *lcut, int32_t n);
 
 /* Operator-apply tile chosen by the autotuner (no counterpart in the
  * reference): on a solver's first step each tile of 16, 32 or 64 radial nodes

@@ -293,6 +293,7 @@ struct gm_solver {
     /* per-step potential */
     double *f1, *f2; /* (2 l_max + 1) * Nr */
     double *W;       /* Nr * K * K, node-major */
+    int32_t *lcut;   /* Nr: largest l* kept at each node, or NULL for every term (a test probe) */
     zc *phases;      /* 4 l_max + 1 */
     double ramp_factor, prf, Omega_V;
     /* absorption */
@@ -398,7 +399,7 @@ void om_destroy(gm_solver *s)
     if (!s) return;
     free(s->ml); free(s->mm); free(s->r); free(s->psi); free(s->psi_last);
     free(s->damping); free(s->capv); free(s->pairs); free(s->term_l); free(s->term_ang);
-    free(s->f1); free(s->f2); free(s->W); free(s->phases);
+    free(s->f1); free(s->f2); free(s->W); free(s->lcut); free(s->phases);
     free_profile(&s->prof1); free_profile(&s->prof2);
     free(s->sink1.v); free(s->sink2.v);
     free(s->avg.seg);
@@ -558,6 +559,7 @@ static void build_static_perturbation(gm_solver *s)
             double v1 = 0.0, v2 = 0.0;
             for (int t = pr->t0; t < pr->t0 + pr->nt; ++t) {
                 const int ls = s->term_l[t];
+                if (s->lcut && ls > s->lcut[i]) continue;
                 const double common = -s->mu * Mt * 4.0 * M_PI / (2.0 * ls + 1.0) * s->term_ang[t];
                 v1 += common * s->f1[(size_t)ls * N + i];
                 v2 += common * (parity * s->f2[(size_t)ls * N + i]);
@@ -1280,6 +1282,23 @@ int om_apply_H(gm_solver *s, const double *in, double *out)
     apply_op(s, OP_H, x, y);
     pack(y, out, n);
     free(x); free(y);
+    return GM_OK;
+}
+
+/* Test probe, no counterpart in the reference: evaluate only the multipoles
+ * l* <= lcut[i / tile] at node i (a truncated coupling given from outside, to
+ * check a truncating implementation term for term); lcut = NULL restores every
+ * term. The coupling is rebuilt at once and at every later rebuild. */
+int om_set_multipole_cut(gm_solver *s, const int32_t *lcut, int32_t tile)
+{
+    free(s->lcut);
+    s->lcut = NULL;
+    if (lcut) {
+        if (tile <= 0) return GM_E_INVALID;
+        s->lcut = (int32_t *)malloc(sizeof(int32_t) * (size_t)s->Nr);
+        for (int i = 0; i < s->Nr; ++i) s->lcut[i] = lcut[i / tile];
+    }
+    if (s->f1) build_static_perturbation(s);
     return GM_OK;
 }
 

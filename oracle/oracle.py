@@ -43,6 +43,8 @@ def lib():
         _lib.om_apply_op.restype = C.c_int
         _lib.om_apply_op.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib.om_sink_vector.restype = C.c_int
+        _lib.om_set_multipole_cut.restype = C.c_int
+        _lib.om_set_multipole_cut.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
         _lib.om_sink_vector.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         ip = C.POINTER(C.c_int32)
         _lib.om_coupling_terms.restype = C.c_int
@@ -90,6 +92,17 @@ class Oracle(capi.Solver):
         out = np.zeros(self.size, dtype=np.complex128)
         self._lib.om_sink_vector(self._h, body, out.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)))
         return out
+
+    def set_multipole_cut(self, lcut, tile=64):
+        """Evaluate only the multipoles l* <= lcut[i // tile] at node i (None:
+        every term), to check a truncated coupling term for term."""
+        if lcut is None:
+            self._check(self._lib.om_set_multipole_cut(self._h, None, 0))
+            return
+        self._lcut = np.ascontiguousarray(np.asarray(lcut, dtype=np.int32))
+        assert self._lcut.size * tile >= self.Nr
+        self._check(self._lib.om_set_multipole_cut(self._h, self._lcut.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                   int(tile)))
 
     def coupling_terms(self):
         n = self._lib.om_coupling_terms(self._h, None, None, None, None, 0)

@@ -275,6 +275,85 @@ def test_apply_H_high_lmax_matches_oracle(l_max, Nr):
     assert rel(g.apply_H(x), o.apply_H(x)) < 1e-13
 
 
+CUT_ARGS = dict(Nr=1025, L=2048.0, dt=0.5, epsilon=0.0, mu=0.2, M_c=0.01, q=0.5, a=0.0, R=40.0,
+                counter=False, absorption=True, save_every=1000, ramp_periods=0.0, epsilon_smear=2.0)
+
+
+def cut_pair(l_max, Nr=1025, tol=1e-12):
+    """GPU and oracle under a coarse multipole truncation: at tol = 1e-12 of a
+    tile's leading profile the 64-node tiles keep from 2 l_max multipoles near
+    the bodies down to a handful at the far end, so every chunk's pair list
+    runs as whole pairs, as pairs cut after their first group, or both,
+    depending on the tile. Returns the GPU's per-tile cut; the GPU's
+    untruncated apply of x is taken first when x is given."""
+    args = dict(CUT_ARGS, l_max=l_max, Nr=Nr, L=2.0 * (Nr - 1))
+    g, o = gm.CrankNicolsonModes(**args), oracle.Oracle(**args)
+    psi = generic_state(o)
+    for s in (g, o):
+        s.set_flags(evolve_orbit=0)
+        s.set_initial_state_full(psi)
+        s.initialize()
+        s.build_damping_mask(0.85, 1.0, 6)
+        s.begin_run(0.6 * args["L"], 6)
+    g.set_coupling_tolerance(tol)
+    lc = g.multipole_cut()
+    assert lc.max() == 2 * l_max and lc.min() <= 2 * l_max - 16 and len(set(lc.tolist())) >= 4, lc
+    return g, o, lc
+
+
+@pytest.mark.parametrize("l_max,Nr,tol", [(12, 1025, 1e-12), (20, 513, 1e-12), (12, 1025, 9e-7)])
+def test_coarse_multipole_cut_matches_oracle_term_for_term(l_max, Nr, tol):
+    """The coupling streams under a cut that falls inside many slot-group
+    pairs. The apply equals the oracle's sum over the multipoles the GPU's
+    per-tile cut keeps, to rounding (a kept term run twice or not at all would
+    move the weak coupling part far above that). And the change the cut makes
+    (cut minus untruncated apply, in which the stencil cancels) is the same on
+    both sides: the GPU drops exactly the oracle's terms."""
+    announce(f"coarse cut apply l_max={l_max} Nr={Nr} tol={tol}")
+    g, o, lc = cut_pair(l_max, Nr, tol)
+    rng = np.random.default_rng(7 + l_max)
+    x = rng.standard_normal(o.size) + 1j * rng.standard_normal(o.size)
+    g_cut = g.apply_H(x)
+    g.set_coupling_tolerance(0.0)
+    g_full = g.apply_H(x)
+    o_full = o.apply_H(x)
+    o.set_multipole_cut(lc, 64)
+    o_cut = o.apply_H(x)
+    assert rel(g_cut, o_cut) < 1e-13
+    assert rel(g_full, o_full) < 1e-13
+    d_o = o_cut - o_full
+    if np.linalg.norm(d_o) > 1e-11 * np.linalg.norm(o_full):
+        # resolvable above the rounding of the two applies
+        assert rel(g_cut - g_full, d_o) < 1e-3
+    else:
+        assert tol < 1e-9, "the coarsest cut must change the apply measurably"
+
+
+def test_coarse_multipole_cut_steps_match_oracle():
+    """Three steps under the coarse cut, FP64 within 1e-10 of the oracle with
+    the same cut and mixed precision within 1e-8 (the FP32 Krylov applies run
+    the same stream tables), iterations +-1."""
+    announce("coarse cut steps")
+    g, o, lc = cut_pair(12)
+    g32 = gm.CrankNicolsonModes(**dict(CUT_ARGS, l_max=12))
+    g32.set_flags(evolve_orbit=0, precision=gm.GM_PRECISION_MIXED)
+    g32.set_initial_state_full(generic_state(o))
+    g32.initialize()
+    g32.build_damping_mask(0.85, 1.0, 6)
+    g32.begin_run(0.6 * CUT_ARGS["L"], 6)
+    g32.set_coupling_tolerance(1e-12)
+    assert np.array_equal(g32.multipole_cut(), lc)
+    o.set_multipole_cut(lc, 64)
+    for n in range(1, 4):
+        for s in (g, g32, o):
+            s.step(n, diag=False)
+        it = o.scalars()["last_solver_iterations"]
+        assert abs(g.scalars()["last_solver_iterations"] - it) <= 1
+        assert abs(g32.scalars()["last_solver_iterations"] - it) <= 1
+    assert rel(g.psi, o.psi) < 1e-10
+    assert rel(g32.psi, o.psi) < 1e-8
+
+
 def test_high_lmax_steps_match_oracle():
     """Five steps at l_max = 16 with both absorbers and the mask: FP64 within
     1e-10 of the oracle, mixed precision within 1e-8, iterations +-1."""

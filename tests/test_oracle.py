@@ -226,6 +226,37 @@ def test_exact_preconditioner_converges_in_one_iteration():
         assert cn.last_solver_residual <= 1e-13
 
 
+def test_multipole_cut_probe():
+    """The oracle's truncation probe (a checker for truncating applies), pinned
+    by the Gaunt selection rules: with only l* = 0 kept, the coupling is the
+    monopole, which joins each mode to itself alone, so a single-mode input
+    stays in its mode; a cut at 2 l_max keeps every term; a cut in between
+    changes the apply; no cut restores it."""
+    cn = make_physics(l_max=3, Nr=129, L=128.0)
+    cn.set_flags(evolve_orbit=0)
+    cn.set_initial_state_full(generic_state(cn))
+    cn.initialize()
+    cn.begin_run(40, 2)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(cn.size) + 1j * rng.standard_normal(cn.size)
+    full = cn.apply_H(x)
+    tiles = (cn.Nr + 63) // 64
+    cn.set_multipole_cut(np.full(tiles, 6))
+    assert np.array_equal(cn.apply_H(x), full)
+    cn.set_multipole_cut(np.full(tiles, 0))
+    K, N = cn.n_modes, cn.Nr
+    for j in range(K):
+        e = np.zeros(cn.size, dtype=complex)
+        e[j * N:(j + 1) * N] = x[j * N:(j + 1) * N]
+        y = cn.apply_H(e).reshape(K, N)
+        assert np.abs(np.delete(y, j, axis=0)).max() == 0.0, j
+        assert np.abs(y[j]).max() > 0.0
+    cn.set_multipole_cut([2, 6, 6])
+    assert np.linalg.norm(cn.apply_H(x) - full) > 1e-6 * np.linalg.norm(full)
+    cn.set_multipole_cut(None)
+    assert np.array_equal(cn.apply_H(x), full)
+
+
 def test_hermitian_and_sink_psd():
     """tests/physics.cpp:77-104 on operator probes."""
     cn = make_physics(l_max=4, absorption=True)
