@@ -298,7 +298,7 @@ template <> __device__ __forceinline__ double2 buffer_ld<double2>(__amdgpu_buffe
 // The coupling of one chunk at one tile, one stream of steps from one of the
 // chunk's step lists (each sorted by the first slot's l*, so that a tile with
 // multipole cut lc runs a prefix or a range of it). The A
-// fragment G[m1][slot][row] and the phased column vectors of the next RING
+// fragment (the step's [slot][row] Gaunt values) and the phased column vectors of the next RING
 // steps are in flight, as buffer loads with fixed per-lane offsets and the
 // step's scalar offsets; the B fragment E_{l*}(node) is read from LDS one step
 // ahead. No lane masking is needed: G holds zeros for the padding slots of a
