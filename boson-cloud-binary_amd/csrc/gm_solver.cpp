@@ -179,6 +179,7 @@ struct gm_solver {
     double step_start = 0.0;
     bool step_wood = false;
     bool step_ilut = false;  // this step's solve is the host ILUT one
+    bool step_fp64 = false;  // a mixed-precision step solved again with the FP64 operator
     int step_enqueued = 0;
     // captured parts of the step, keyed by every kernel argument that can change
     struct GraphEntry {
@@ -1963,6 +1964,7 @@ void step_begin(gm_solver &S)
     S.phase += S.Omega * S.dt;
     build_perturbation(S, S.t_i);
     S.step_ilut = false;
+    S.step_fp64 = false;
     if (!S.flags.use_block_preconditioner) {
         // the "ilut" preconditioner key: right-hand side on the device, solve
         // on the host, then the tail as after a converged BiCGSTAB
@@ -1999,6 +2001,30 @@ void step_finish(gm_solver &S, gm_diag *d)
             }
             if (S.hflags.p[1] && !bad_factor) break;
             if (S.step_ilut) throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
+            if (S.flags.precision == GM_PRECISION_MIXED && !S.step_fp64 && !S.step_ilut) {
+                // The FP32 coupling kept the solve above the FP64 residual
+                // threshold (its error grows with dt |V|): the same radial
+                // solve again with the FP64 operator, from the step's start
+                // state, before the host ILUT.
+                S.step_fp64 = true;
+                S.flags.precision = GM_PRECISION_FP64;
+                try {
+                    tune_apply(S, 0);
+                    solve_begin(S);
+                    for (;;) {
+                        HIPCK(hipStreamSynchronize(S.st));
+                        prof_collect(S);
+                        if (solve_advance(S)) break;
+                    }
+                    HIPCK(hipStreamSynchronize(S.st));
+                    prof_collect(S);
+                } catch (...) {
+                    S.flags.precision = GM_PRECISION_MIXED;
+                    throw;
+                }
+                S.flags.precision = GM_PRECISION_MIXED;
+                continue;
+            }
             // "Radial preconditioner did not converge; retrying with full ILUT."
             // (matrices.cpp:173,203): psi is still the step's start state
             S.step_ilut = true;
