@@ -48,6 +48,25 @@ def test_ensemble_equals_members_alone():
         assert t.last_solver_iterations == a.last_solver_iterations
 
 
+def test_ensemble_retries_a_failed_mixed_member_in_fp64():
+    """A mixed-precision member whose FP32 coupling error lies above the FP64
+    residual check (dt = 23 at R = 20): inside the ensemble each of its steps
+    is solved again with the FP64 operator, so it steps exactly as the same
+    member in FP64 alone, while its neighbour is unaffected."""
+    print("ensemble with a re-solved mixed member", flush=True)
+    big = dict(Nr=257, L=128.0, dt=23.0, l_max=4, mu=0.2, M_c=0.05, q=0.5, R=20.0)
+    together = [make(dict(big, precision=gm.GM_PRECISION_MIXED)), make(MEMBERS[2])]
+    alone = [make(dict(big, precision=gm.GM_PRECISION_FP64)), make(MEMBERS[2])]
+    for n in range(1, 6):
+        gm.ensemble_step(together, n)
+        for s in alone:
+            s.step(n)
+        assert together[0].last_solver_residual <= 1e-9
+        assert together[0].last_solver_iterations == alone[0].last_solver_iterations, n
+    assert np.linalg.norm(together[0].psi - alone[0].psi) <= 1e-13 * np.linalg.norm(alone[0].psi)
+    assert np.array_equal(together[1].psi, alone[1].psi)
+
+
 def test_ensemble_rejects_duplicates():
     s = make(MEMBERS[2])
     with pytest.raises(gm.InvalidArgument):
