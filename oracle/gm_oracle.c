@@ -1541,12 +1541,58 @@ int om_restore_from_diagnostics(gm_solver *s, int32_t n_rows, const double *rows
     return GM_OK;
 }
 
-/* Rolling average probe (tests/rolling_average.cpp). */
+/* Rolling average probe (tests/rolling_average.cpp). A window of n_orbits <= 0
+ * is refused (NULL), as the constructor's std::invalid_argument
+ * (src/RollingPhase.cpp:14-22). */
 void *om_rolling_new(double n_orbits)
 {
+    if (!(n_orbits > 0.0)) return NULL;
     om_rolling *a = (om_rolling *)calloc(1, sizeof(om_rolling));
     rolling_init(a, n_orbits);
     return a;
+}
+
+/* ready() and value() (src/RollingPhase.cpp:129-137). */
+int om_rolling_ready(void *h) { return ((om_rolling *)h)->ready; }
+
+double om_rolling_value(void *h) { return ((om_rolling *)h)->average; }
+
+/* reset() (src/RollingPhase.cpp:139-150): samples dropped, window kept. */
+void om_rolling_reset(void *h)
+{
+    om_rolling *a = (om_rolling *)h;
+    const double window = a->window_phase;
+    rolling_init(a, 1.0);
+    a->window_phase = window;
+}
+
+/* save() / load() (src/RollingPhase.cpp:167-209) through a file: 0 on
+ * success, -1 when the file cannot be written or read whole (load's
+ * "truncated checkpoint"). */
+int om_rolling_save(void *h, const char *path)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    const int ok = rolling_save((om_rolling *)h, f);
+    return (fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+int om_rolling_load(void *h, const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    om_rolling tmp;
+    memset(&tmp, 0, sizeof(tmp));
+    const int ok = rolling_load(&tmp, f);
+    fclose(f);
+    if (!ok) {
+        free(tmp.seg);
+        return -1;
+    }
+    om_rolling *a = (om_rolling *)h;
+    free(a->seg);
+    *a = tmp;
+    return 0;
 }
 
 int om_rolling_update(void *h, double time, double phase, double value, double *out)

@@ -58,6 +58,16 @@ def lib():
         _lib.om_rolling_update.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, dp]
         _lib.om_rolling_free.restype = None
         _lib.om_rolling_free.argtypes = [C.c_void_p]
+        _lib.om_rolling_ready.restype = C.c_int
+        _lib.om_rolling_ready.argtypes = [C.c_void_p]
+        _lib.om_rolling_value.restype = C.c_double
+        _lib.om_rolling_value.argtypes = [C.c_void_p]
+        _lib.om_rolling_reset.restype = None
+        _lib.om_rolling_reset.argtypes = [C.c_void_p]
+        _lib.om_rolling_save.restype = C.c_int
+        _lib.om_rolling_save.argtypes = [C.c_void_p, C.c_char_p]
+        _lib.om_rolling_load.restype = C.c_int
+        _lib.om_rolling_load.argtypes = [C.c_void_p, C.c_char_p]
     return _lib
 
 
@@ -126,18 +136,42 @@ def thomas(diag, lower, upper, rhs):
 
 
 class Rolling:
+    """The oracle's RollingPhaseAverage (src/RollingPhase.cpp)."""
+
     def __init__(self, n_orbits):
-        self._h = lib().om_rolling_new(n_orbits)
+        self._h = lib().om_rolling_new(float(n_orbits))
+        if not self._h:
+            raise ValueError("RollingPhaseAverage: n_orbits must be positive")
 
     def update(self, t, phase, value):
         out = C.c_double()
         rc = lib().om_rolling_update(self._h, t, phase, value, C.byref(out))
+        if rc == -1:
+            raise RuntimeError("RollingPhaseAverage: time must increase")
         if rc != 0:
-            raise RuntimeError("RollingPhaseAverage: time and phase must increase")
+            raise RuntimeError("RollingPhaseAverage: phase must increase")
         return out.value
+
+    def ready(self):
+        return bool(lib().om_rolling_ready(self._h))
+
+    def value(self):
+        return lib().om_rolling_value(self._h)
+
+    def reset(self):
+        lib().om_rolling_reset(self._h)
+
+    def save(self, path):
+        if lib().om_rolling_save(self._h, str(path).encode()) != 0:
+            raise RuntimeError("RollingPhaseAverage: could not write")
+
+    def load(self, path):
+        if lib().om_rolling_load(self._h, str(path).encode()) != 0:
+            raise RuntimeError("RollingPhaseAverage: truncated checkpoint")
 
     def __del__(self):
         try:
-            lib().om_rolling_free(self._h)
+            if self._h:
+                lib().om_rolling_free(self._h)
         except Exception:
             pass
