@@ -197,11 +197,35 @@ def cpu_baseline(w, runs=3, steps_per_run=8):
 HBM_PASSES = {"precond": 8.5, "precond_setup": 3.0, "update": 8.0, "diag": 2.0}
 
 
-def coupling_flops(terms, pairs, K, N):
-    """FP64 flops of one full operator apply: one multiply-add per Gaunt term
-    (real W), two per coupled mode pair (real W times complex x), and about
-    20 per mode-node for the stencil, phases and the A = 1 + i dt/2 H form."""
+def algorithmic_flops(K, N):
+    """Algorithmic flops of one operator apply: a real symmetric coupling
+    block V0 (K x K per node) times a complex vector is 4 K flops per
+    mode-node, plus 12 per mode-node for the three-point stencil, the
+    centrifugal term and the phases."""
+    return 4.0 * K * K * N + 12.0 * K * N
+
+
+def executed_flops(terms, pairs, K, N):
+    """Flops this build's factored apply executes: one multiply-add per Gaunt
+    term kept by the multipole cut (forming W = sum G E), two per coupled mode
+    pair (real W times complex x), about 20 per mode-node for the stencil,
+    phases and the A = 1 + i dt/2 H form."""
     return 2.0 * terms + 4.0 * pairs + 20.0 * K * N
+
+
+def mfma_macs(kernel):
+    """Matrix-core multiply-adds per launch from the committed SQ counter
+    summary (SQ_INSTS_MFMA x 1024 for 16x16x4 MFMAs; the first table of the
+    file), or None."""
+    path = os.path.join(REPO, "profiles", "k_apply_sq_counters_l20_n16384.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(kernel + " "):
+                    return float(line[len(kernel):].split()[2]) * 1024.0
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
 
 
 def rooflines(s, prof, K, N, iters, mixed):
@@ -224,11 +248,21 @@ def rooflines(s, prof, K, N, iters, mixed):
         cnt = krylov + prof["apply_other"][1]
         peak, name, dt = FP64_PEAK_TFLOPS, "k_apply", "f64"
     avg_s = ms / max(cnt, 1) * 1e-3
-    flops = coupling_flops(terms, pairs, K, N)
+    flops = algorithmic_flops(K, N)
+    executed = executed_flops(terms, pairs, K, N)
     roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": peak, "unit": "TFLOP/s",
             "kernel": name, "dtype": dt, "per_launch": flops, "per_launch_unit": "flop",
+            "per_launch_formula": "4 K^2 N + 12 K N (real symmetric K x K coupling times a complex vector, "
+                                  "plus stencil and phases)",
             "avg_launch_ms": avg_s * 1e3, "launches": cnt}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    # what the factored apply executes instead: Gaunt terms after the multipole
+    # cut, and the matrix-core work they are padded to
+    macs = mfma_macs("k_apply<2, float>" if mixed else "k_apply<2, double>")
+    roof["executed"] = {"flops_per_launch": executed, "gaunt_terms_per_launch": terms,
+                        "mode_pairs_per_launch": pairs, "achieved_TFLOPs": executed / avg_s / 1e12,
+                        "mfma_macs_per_launch": macs,
+                        "mfma_padding": (macs / terms) if macs else None}
     roof["traffic"] = measured_traffic("k_apply_fp32" if mixed else "k_apply")
     # whole step: the sum over kernels of max(bytes / HBM peak, flops / MFMA
     # peak) against the measured time of the profiled steps

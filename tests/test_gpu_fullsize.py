@@ -1,6 +1,7 @@
 """Size-independent properties at the full l_max 20, N 16384 shape (needs an
-MI355X). The oracle would take minutes per step here, so these checks rest
-on what the mathematics fixes instead:
+MI355X). The oracle takes 5-20 s per step here; it is compared with the GPU
+over the first 10 steps in test_gpu_config2.py. These checks run more steps
+and other states, and rest on what the mathematics fixes:
 
 - H is Hermitian on vectors that vanish at the two Dirichlet nodes (the
   three-point stencil, the centrifugal term and the Gaunt coupling are all

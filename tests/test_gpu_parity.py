@@ -536,6 +536,57 @@ def test_config1_thousand_steps_against_the_oracle():
         assert e_f < 1e-9 and e_m < 1e-7, k
 
 
+def test_config1_sinks_switch_on_against_the_oracle():
+    """The minimal example's setup across the end of its two-period ramp
+    (t = 1968.8), where the s-wave sinks switch on: the three solvers start
+    from the translated 2p atom at t = 1700 and step 800 times to t = 2100 in
+    lockstep. Iterations within one on every step; after the last step psi
+    within 1e-10 (FP64) and 1e-8 (mixed) of the oracle, and the absorbed mass,
+    now non-zero, within 1e-9 and 1e-7."""
+    from gm import initial
+    args = dict(Nr=2000, L=1000.0, dt=0.5, epsilon=0.0, l_max=6, mu=0.2, M_c=0.01, q=0.1, a=0.0, R=30.0,
+                counter=False, absorption=True, save_every=1000, ramp_periods=2.0, epsilon_smear=2.0)
+    dr = args["L"] / (args["Nr"] - 1)
+    r = dr * np.arange(args["Nr"])
+    psi = initial.translated_atom(r, args["l_max"], args["mu"], args["q"] / (1 + args["q"]) * args["R"])
+    psi /= math.sqrt(np.vdot(psi, psi).real * dr)
+    f64, mix = gm.CrankNicolsonModes(**args), gm.CrankNicolsonModes(**args)
+    o = oracle.Oracle(**args)
+    t0 = 1700.0
+    for s, prec in ((f64, gm.GM_PRECISION_FP64), (mix, gm.GM_PRECISION_MIXED), (o, None)):
+        if prec is None:
+            s.set_flags(evolve_orbit=0)
+        else:
+            s.set_flags(evolve_orbit=0, precision=prec)
+        s.set_initial_state_full(psi)
+        s.initialize()
+        s.build_damping_mask(0.85, 1.0, 6)
+        s.begin_run(800.0, 801)
+        s.set_scalars(t=t0, theta=0.4, phase=0.4)
+    worst, active_from = 0, None
+    for n in range(1, 801):
+        last = n == 800
+        df, dm, do = f64.step(n, diag=last), mix.step(n, diag=last), o.step(n, diag=last)
+        it = o.last_solver_iterations
+        worst = max(worst, abs(f64.last_solver_iterations - it), abs(mix.last_solver_iterations - it))
+        assert worst <= 1, n
+        if active_from is None and o.scalars()["sink_active"]:
+            active_from = n
+            assert f64.scalars()["sink_active"] and mix.scalars()["sink_active"]
+        if n % 200 == 0:
+            announce(f"config 1 ramp end: step {n}, fp64 {rel(f64.psi, o.psi):.2e}, mixed {rel(mix.psi, o.psi):.2e}")
+    assert active_from is not None and 500 < active_from < 600
+    e64, emx = rel(f64.psi, o.psi), rel(mix.psi, o.psi)
+    announce(f"config 1 ramp end: sinks on from step {active_from}; psi fp64 {e64:.3e}, mixed {emx:.3e}, "
+             f"absorbed {do['absorbed']:.3e}")
+    assert e64 < 1e-10
+    assert emx < 1e-8
+    assert do["absorbed"] > 0
+    for k in ("cloud_mass", "energy", "p_bin_inst", "absorbed"):
+        assert df[k] == pytest.approx(do[k], rel=1e-9, abs=1e-15), k
+        assert dm[k] == pytest.approx(do[k], rel=1e-7, abs=1e-14), k
+
+
 def test_checkpoint_resume_is_bitwise(tmp_path):
     """save_checkpoint at step 10, then a fresh solver resumed from it runs
     steps 11-20 exactly as the uninterrupted run (restart.cpp:139-233); the
