@@ -123,9 +123,10 @@ struct Krylov {
     double2 Ginv[4];            // inverse of the k x k Woodbury capacitance
     double2 G[4];
     int wrange[4];              // [lo, hi) of each Woodbury column W_a
+    double rst_eps2, rst_ref;   // restart thresholds as fractions of |r0|^2 (Eigen's eps^2; 1e-32 with sinks)
     int iter, maxit, done, restart, eigen_mode, restarts, zero_rhs, accepted;
     unsigned counter;
-    int pad;
+    int fresh;                  // no iteration yet since k_init: r is read from r0' / |rhs| and p = r
 };
 
 // Output of the diagnostics reduction.

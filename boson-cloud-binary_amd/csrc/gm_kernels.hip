@@ -115,8 +115,18 @@ __device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned 
             __hip_atomic_store(partial + (size_t)q * gridDim.x + blockIdx.x, s, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
-        // the partial stores have completed before this block is counted
+        // The partial stores have completed before this block is counted.
+        // The ordering rests on the AMDGPU memory model for gfx950: an
+        // agent-scope store is performed at the device coherence point once
+        // vmcnt reaches zero, so after s_waitcnt vmcnt(0) it is visible to
+        // any agent-scope load issued later, and the relaxed counter RMW
+        // cannot be issued before the wait. The signal fences are compiler
+        // barriers: they keep the compiler from moving the stores below the
+        // wait or the RMW above it (a release fence would instead write back
+        // this XCD's whole L2).
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = (t == gridDim.x - 1);
     }
@@ -467,7 +477,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     const bool gq_mode = MODE == AM_RHS;
     const double2 coef0 = gq_mode ? Kr->gq[0] : Kr->coefA[0];
     const double2 coef1 = gq_mode ? Kr->gq[1] : Kr->coefA[1];
-    const double rhs_nrm = MODE == AM_RES ? Kr->rhs_nrm : 0.0;
+    const double rhs_nrm = (MODE == AM_RES || MODE == AM_RST) ? Kr->rhs_nrm : 0.0;
     double red[3] = {0.0, 0.0, 0.0};
     typename AR::C acc[NG][4];
 #pragma unroll
@@ -541,7 +551,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                 }
                 if (MODE == AM_H) {
                     a.Y[row] = Hx;
-                } else if (MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_A) {
+                } else if (MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_A || MODE == AM_RST) {
                     double2 out = cadd(xi, imul(hdt, Hx));
                     if (nsink) out = cadd(out, sink_term(a, nsink, coef0, coef1, j, i));
                     if (MODE == AM_V) {
@@ -557,6 +567,13 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
                         red[2] += cnorm2(out);
                     } else if (MODE == AM_A) {
                         a.Y[row] = out;
+                    } else if (MODE == AM_RST) {
+                        // r = b - A x of the normalised system, r0' = |rhs| r
+                        const double2 bq = make_double2(a.R0[row].x / rhs_nrm, a.R0[row].y / rhs_nrm);
+                        const double2 rq = csub(bq, out);
+                        a.Y[row] = rq;
+                        a.Y2[row] = cscale(rhs_nrm, rq);
+                        red[0] += cnorm2(rq);
                     } else {
                         double2 bq = a.R0[row];
                         if (a.r0_raw && rhs_nrm != 0.0) bq = make_double2(bq.x / rhs_nrm, bq.y / rhs_nrm);
@@ -608,6 +625,30 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
             const double s = sum_partials(a.partial, 0);
             if (threadIdx.x == 0) {
                 Kr->res2 = s;
+                Kr->counter = 0;
+            }
+        }
+    } else if (MODE == AM_RST) {
+        // Lost bi-orthogonality (matrices.cpp:67-74, and Eigen's BiCGSTAB):
+        // restart from the true residual, r0 = r, rho = |r|^2 (scaled by
+        // |rhs| with the unnormalised shadow residual); Eigen also takes the
+        // new |r0|^2 as its threshold base and resets the iteration counter
+        // on the first restart. The iteration then goes on from beta with
+        // the old p, v, alpha and w.
+        if (block_partials<1>({red[0]}, a.partial, &Kr->counter)) {
+            const double s = sum_partials(a.partial, 0);
+            if (threadIdx.x == 0) {
+                const double rho = Kr->rhs_nrm * s;
+                Kr->rho = make_double2(rho, 0.0);
+                if (Kr->eigen_mode) {
+                    Kr->r0_sq = rho;
+                    if (Kr->restarts == 0) Kr->iter = 0;
+                }
+                Kr->restarts += 1;
+                Kr->r2 = s;
+                Kr->beta = cmul(cdiv(Kr->rho, Kr->rho_old), cdiv(Kr->alpha, Kr->w));
+                Kr->restart = 0;
+                Kr->done = 0;
                 Kr->counter = 0;
             }
         }
@@ -958,10 +999,11 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     };
     const int ntile = (N + TE - 1) / TE;
     double2 beta = make_double2(0, 0), wv = make_double2(0, 0), alpha = make_double2(0, 0);
-    // first iteration: p = r + beta (p - w v) with p = v = 0 is r
-    const bool first = VAR == PV_P && Kr->iter == 0;
+    // first iteration: p = r + beta (p - w v) with p = v = 0 is r (a restart
+    // that resets the counter keeps p and v)
+    const bool first = VAR == PV_P && Kr->fresh;
     // the first iteration reads r = r0' / |rhs| from the unnormalised shadow residual
-    const bool it0 = VAR != PV_PLAIN && Kr->iter == 0;
+    const bool it0 = VAR != PV_PLAIN && Kr->fresh;
     const double nrm0 = it0 ? Kr->rhs_nrm : 1.0;
     const double2 *Rin = it0 ? a.R0 : a.R;
     if (VAR == PV_P) {
@@ -1177,8 +1219,13 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     // c = Ginv (V^H y), then the rank-one coefficients of A applied to
     // y - W c: coefA_a = (dt/4) g_a v_a^H (y - W c).
     __shared__ bool last;
-    // every thread's support stores have completed before the block is counted
+    // every thread's support stores have completed before the block is
+    // counted (the same ordering argument as block_partials: agent-scope
+    // stores are at the coherence point once vmcnt is zero; the signal fences
+    // stop the compiler from moving them across the wait)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
     if (tid == 0) last = __hip_atomic_fetch_add(&Kr->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
     __syncthreads();
@@ -1275,7 +1322,7 @@ __device__ void loop_top(Krylov *Kr, double r2, double2 rho_new)
     }
     Kr->rho_old = Kr->rho;
     Kr->rho = rho_new;
-    const double thr = Kr->eigen_mode ? 4.930380657631324e-32 * Kr->r0_sq : Kr->restart_thr;
+    const double thr = Kr->eigen_mode ? Kr->rst_eps2 * Kr->r0_sq : Kr->restart_thr;
     if (sqrt(cnorm2(rho_new)) < thr) {
         Kr->restart = 1;
         Kr->done = 1;
@@ -1305,6 +1352,7 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a)
         const double r2 = sum_partials(a.partial, 2);
         if (threadIdx.x == 0) {
             Kr->iter += 1;
+            Kr->fresh = 0;
             loop_top(Kr, r2, make_double2(rr, ri));
             Kr->counter = 0;
         }
@@ -1339,6 +1387,9 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a)
         Kr->eigen_mode = a.eigen_mode;
         Kr->restarts = 0;
         Kr->restart = 0;
+        Kr->fresh = 1;
+        Kr->rst_eps2 = a.rst_eps2;
+        Kr->rst_ref = a.rst_ref;
         Kr->accepted = 0;
         Kr->done = 0;
         Kr->zero_rhs = nrm == 0.0;
@@ -1346,7 +1397,7 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a)
         Kr->alpha = make_double2(1.0, 0.0);
         Kr->w = make_double2(1.0, 0.0);
         Kr->r0_sq = rho0;               // |rhs| |r0|^2: Eigen's eps^2 |r0|^2 test on the scaled rho
-        Kr->restart_thr = rho0 * 1e-32;
+        Kr->restart_thr = rho0 * a.rst_ref;  // fixed at the start (matrices.cpp:56)
         Kr->tol2 = a.tol * a.tol * b2;
         if (nrm == 0.0) {
             Kr->done = 1;

@@ -4,7 +4,8 @@
 
 namespace gmk {
 
-enum ApplyMode { AM_H = 0, AM_A = 1, AM_V = 2, AM_T = 3, AM_RHS = 4, AM_RES = 5 };
+// AM_RST: the BiCGSTAB restart (r = b - A x into Y, r0' = |rhs| r into Y2, |r|^2)
+enum ApplyMode { AM_H = 0, AM_A = 1, AM_V = 2, AM_T = 3, AM_RHS = 4, AM_RES = 5, AM_RST = 6 };
 enum PrecVar { PV_PLAIN = 0, PV_P = 1, PV_S = 2 };
 enum SinkMode { SK_PROJ = 0, SK_SETUP = 1 };
 enum SinkDest { SD_START = 0, SD_QNEW = 1, SD_POST = 2, SD_COEFA = 3, SD_COEFA_QNEW = 4 };
@@ -15,9 +16,9 @@ struct ApplyArgs {
     gmd::Krylov *Kr;
     const double2 *X;    // input vector
     double2 *Y;          // output
-    double2 *Y2;         // second output (AM_RHS: rhs - A psi, the unnormalised shadow residual)
-    const double2 *R0;   // unnormalised shadow residual (AM_V), s (AM_T), b or the raw rhs (AM_RES)
-    int r0_raw;          // AM_RES: R0 holds the raw rhs; b = rhs / |rhs| is formed here
+    double2 *Y2;         // second output (AM_RHS: rhs - A psi, the unnormalised shadow residual; AM_RST: r0')
+    const double2 *R0;   // unnormalised shadow residual (AM_V), s (AM_T), b or the raw rhs (AM_RES, AM_RST)
+    int r0_raw;          // AM_RES, AM_RST: R0 holds the raw rhs; b = rhs / |rhs| is formed here
     gmd::SinkDev sk[2];
     double *partial;     // reduction partials (3 * grid doubles)
     double *rowpart;     // diagnostics apply: 6 doubles per (tile, mode)
@@ -92,6 +93,7 @@ struct InitArgs {
     double2 *X;
     double tol;
     int maxit, eigen_mode;
+    double rst_eps2, rst_ref;  // restart thresholds relative to |r0|^2
 };
 
 struct AcceptArgs {

@@ -175,6 +175,10 @@ struct gm_solver {
     double w_key[14] = {};
     double w_theta = 0.0;
     int prev_iters = 4;
+    // BiCGSTAB restart thresholds relative to |r0|^2: Eigen's eps^2 without
+    // sinks, the reference's 1e-32 with them (gm_set_restart_thresholds)
+    double rst_eps2 = 4.930380657631324e-32, rst_ref = 1e-32;
+    int last_restarts = 0;
     // progress of the step in flight (step_begin / solve_advance / step_finish)
     double step_start = 0.0;
     bool step_wood = false;
@@ -1034,6 +1038,8 @@ void rhs_and_init(gm_solver &S)
         ia.tol = 1e-10;
         ia.maxit = kMaxIt;
         ia.eigen_mode = wood ? 0 : 1;
+        ia.rst_eps2 = S.rst_eps2;
+        ia.rst_ref = S.rst_ref;
         launch(S, GM_PROF_SMALL, [&] { return gmk::launch_init(ia, S.st); });
     }
 }
@@ -1041,26 +1047,37 @@ void rhs_and_init(gm_solver &S)
 void solve_tail_work(gm_solver &S);
 void ilut_host_solve(gm_solver &S);
 
+// The restart branch of BiCGSTAB (matrices.cpp:67-74 and Eigen's): the
+// iteration that found |rho| below the threshold stopped at its top; r = b -
+// A x is formed with the FP64 operator and the rank-one sinks of x, then the
+// restart apply's last block resets r0, rho, the counter (Eigen, first
+// restart) and beta, and clears the stop so the next batch goes on.
+void enqueue_restart(gm_solver &S)
+{
+    if (S.step_wood) sink_proj(S, S.x.p, gmk::SD_COEFA, 1.0, 0);
+    gmk::ApplyArgs a = apply_args(S, S.x.p, S.rr.p);
+    a.Y2 = S.r0.p;
+    a.R0 = S.b.p;
+    a.r0_raw = 1;
+    launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RST, S.st); });
+}
+
 // After a convergence check (host copy of the Krylov state is current): more
 // iterations (false) or the acceptance check, mask and diagnostics with their
 // readback (true).
 bool solve_advance(gm_solver &S)
 {
     gmd::Krylov &K = *S.hK.p;
-    if (K.restart && !K.done) {
-        // Bi-orthogonality lost on the radial path: BiCGSTAB is not restarted
-        // on the device; the step is solved by the full ILUT instead.
-        S.step_ilut = true;
-        ilut_host_solve(S);
-        K.done = 1;
-        K.iter = S.last_iterations;
-    }
-    if (!K.done) {
-        if (S.step_enqueued > 2 * kMaxIt) throw std::runtime_error("BiCGSTAB did not report convergence");
+    if (!K.done || K.restart) {
+        // Eigen resets its counter on the first restart, so a solve that
+        // restarts may run up to twice the iteration limit
+        if (S.step_enqueued > 3 * kMaxIt) throw std::runtime_error("BiCGSTAB did not report convergence");
+        if (K.restart) enqueue_restart(S);
         enqueue_batch(S, 2);
         return false;
     }
     S.last_iterations = K.iter;
+    S.last_restarts = K.restarts;
     S.prev_iters = std::max(1, S.last_iterations);
     run_captured(S, GK_TAIL, graph_key(S, 0), [&] { solve_tail_work(S); });
     return true;
@@ -1650,6 +1667,25 @@ int gm_set_coupling_tolerance(gm_solver *S, double tol)
             build_perturbation(*S, S->t_i);
             HIPCK(hipStreamSynchronize(S->st));
         }
+    });
+}
+
+int gm_set_restart_thresholds(gm_solver *S, double eigen_eps2, double sink_factor)
+{
+    return guarded(S, [&] {
+        if (!(eigen_eps2 > 0.0) || !(sink_factor > 0.0))
+            throw std::invalid_argument("restart thresholds must be positive");
+        S->rst_eps2 = eigen_eps2;
+        S->rst_ref = sink_factor;
+        drop_graphs(*S);
+    });
+}
+
+int gm_restart_count(const gm_solver *S, int32_t *n)
+{
+    return guarded(const_cast<gm_solver *>(S), [&] {
+        if (!n) throw std::invalid_argument("restart count pointer is NULL");
+        *n = S->last_restarts;
     });
 }
 

@@ -104,6 +104,8 @@ def bind(lib, prefix):
     f("save_checkpoint", C.c_int, vp, C.c_char_p, C.c_int32)
     f("load_checkpoint", C.c_int, vp, C.c_char_p, C.c_int32, C.POINTER(C.c_int32))
     f("restore_from_diagnostics", C.c_int, vp, C.c_int32, _DP)
+    f("set_restart_thresholds", C.c_int, vp, C.c_double, C.c_double)
+    f("restart_count", C.c_int, vp, C.POINTER(C.c_int32))
     return lib
 
 
@@ -272,6 +274,16 @@ class Solver:
 
     def schrodinger_rhs(self, x):
         return self._op("schrodinger_rhs", x)
+
+    def set_restart_thresholds(self, eigen_eps2, sink_factor):
+        """BiCGSTAB restart thresholds as fractions of |r0|^2 (a test hook)."""
+        self._check(self._fn("set_restart_thresholds")(self._h, float(eigen_eps2), float(sink_factor)))
+
+    def restart_count(self):
+        """BiCGSTAB restarts of the last solve."""
+        n = C.c_int32()
+        self._check(self._fn("restart_count")(self._h, C.byref(n)))
+        return n.value
 
     def sink_strength(self, body):
         return self._fn("sink_strength")(self._h, int(body))

@@ -201,6 +201,14 @@ int gm_ilut_bicgstab(int32_t n, const int32_t *ptr, const int32_t *col, const do
                      double *x, double droptol, int32_t fill_factor, double tol, int32_t max_iterations,
                      int32_t eigen_mode, int32_t *iterations);
 
+/* Restart thresholds of BiCGSTAB, as fractions of |r0|^2 (a test hook with no
+ * counterpart in the reference, whose thresholds are fixed): without sinks
+ * Eigen's eps^2 = 4.93e-32, with sinks the reference's 1e-32
+ * (src/CrankNicolsonModes_matrices.cpp:56,67-74). Raising them forces the
+ * restart branch. gm_restart_count gives the restarts of the last solve. */
+int gm_set_restart_thresholds(gm_solver *solver, double eigen_eps2, double sink_factor);
+int gm_restart_count(const gm_solver *solver, int32_t *n);
+
 /* Multipole truncation of the coupling (no counterpart in the reference, which
  * evaluates every l*): at each tile of 64 radial nodes, multipoles whose radial
  * profile is below tol times the tile's leading profile are not evaluated.

@@ -308,6 +308,8 @@ struct gm_solver {
     om_rolling avg;
     double last_residual;
     int last_iterations;
+    double rst_eps2, rst_ref; /* BiCGSTAB restart thresholds relative to |r0|^2 */
+    int last_restarts;
     char err[512];
 };
 
@@ -371,6 +373,8 @@ int om_create(const gm_params *p, gm_solver **out)
     s->flags.precision = GM_PRECISION_FP64;
     s->flags.use_graphs = 1;
     s->flags.apply_tile = 0;
+    s->rst_eps2 = DBL_EPSILON * DBL_EPSILON;
+    s->rst_ref = 1e-32;
     rolling_init(&s->avg, 2.0);
     s->r = (double *)malloc(sizeof(double) * s->Nr);
     for (int i = 0; i < s->Nr; i++) s->r[i] = s->epsilon + i * s->dr;
@@ -957,7 +961,7 @@ static void apply_M_full(const om_mctx *m, size_t n, const zc *x, zc *y)
  * first restart); eigen_mode = 0 restates the reference's own recurrence for
  * the sink path (src/CrankNicolsonModes_matrices.cpp:47-91). */
 static void bicgstab(const om_opctx *A, const om_mctx *M, const zc *b, zc *x, double tol,
-                     int max_iterations, int *iterations, int eigen_mode)
+                     int max_iterations, int *iterations, int eigen_mode, int *restarts_out)
 {
     const size_t n = nvec(A->s);
     const double b_norm2 = vnorm2(b, n);
@@ -966,6 +970,7 @@ static void bicgstab(const om_opctx *A, const om_mctx *M, const zc *b, zc *x, do
     zc *y = (zc *)malloc(sizeof(zc) * n), *z = (zc *)malloc(sizeof(zc) * n);
     zc *sv = (zc *)malloc(sizeof(zc) * n), *t = (zc *)malloc(sizeof(zc) * n);
     *iterations = 0;
+    *restarts_out = 0;
     if (b_norm2 == 0.0) {
         memset(x, 0, sizeof(zc) * n);
         goto done;
@@ -976,8 +981,10 @@ static void bicgstab(const om_opctx *A, const om_mctx *M, const zc *b, zc *x, do
         for (size_t q = 0; q < n; ++q) r[q] = b[q] - r[q];
         memcpy(r0, r, sizeof(zc) * n);
         double r0_norm2 = vnorm2(r0, n);
-        const double fixed_restart = r0_norm2 * 1e-32;
-        const double eps2 = DBL_EPSILON * DBL_EPSILON;
+        /* restart thresholds: 1e-32 |r0|^2 (matrices.cpp:56) and Eigen's
+         * eps^2 |r0|^2, both settable for tests (om_set_restart_thresholds) */
+        const double fixed_restart = r0_norm2 * A->s->rst_ref;
+        const double eps2 = A->s->rst_eps2;
         zc rho = 1.0, alpha = 1.0, w = 1.0;
         int i = 0, restarts = 0;
         while (vnorm2(r, n) > threshold && i < max_iterations) {
@@ -989,7 +996,7 @@ static void bicgstab(const om_opctx *A, const om_mctx *M, const zc *b, zc *x, do
                 for (size_t q = 0; q < n; ++q) r[q] = b[q] - r[q];
                 memcpy(r0, r, sizeof(zc) * n);
                 rho = r0_norm2 = vnorm2(r, n);
-                if (eigen_mode && restarts++ == 0) i = 0;
+                if (restarts++ == 0 && eigen_mode) i = 0;
             }
             const zc beta = (rho / rho_old) * (alpha / w);
 #pragma omp parallel for schedule(static)
@@ -1011,6 +1018,7 @@ static void bicgstab(const om_opctx *A, const om_mctx *M, const zc *b, zc *x, do
             ++i;
         }
         *iterations = i;
+        *restarts_out = restarts;
     }
 done:
     free(r); free(r0); free(v); free(p); free(y); free(z); free(sv); free(t);
@@ -1072,9 +1080,9 @@ static int solve_CN(gm_solver *s, const zc *rhs)
         }
         if (A.k == 1) M.Ginv[0][0] = 1.0 / G[0][0];
         else matrix_inverse_2x2(G, M.Ginv);
-        bicgstab(&A, &M, b, x, 1e-10, 500, &s->last_iterations, 0);
+        bicgstab(&A, &M, b, x, 1e-10, 500, &s->last_iterations, 0, &s->last_restarts);
     } else {
-        bicgstab(&A, &M, b, x, 1e-10, 500, &s->last_iterations, 1);
+        bicgstab(&A, &M, b, x, 1e-10, 500, &s->last_iterations, 1, &s->last_restarts);
     }
     for (size_t q = 0; q < n; ++q) x[q] *= rhs_norm;
     {
@@ -1538,6 +1546,24 @@ int om_restore_from_diagnostics(gm_solver *s, int32_t n_rows, const double *rows
         s->P = 2.0 * M_PI / s->Omega;
         s->dt = s->P / s->flags.steps_per_period;
     }
+    return GM_OK;
+}
+
+/* BiCGSTAB restart thresholds, as fractions of |r0|^2 (Eigen's eps^2 when
+ * there are no sinks, the reference's 1e-32 with them), so that tests can
+ * force the restart branch (src/CrankNicolsonModes_matrices.cpp:67-74); and
+ * the restarts of the last solve. */
+int om_set_restart_thresholds(gm_solver *s, double eigen_eps2, double sink_factor)
+{
+    if (!(eigen_eps2 > 0.0) || !(sink_factor > 0.0)) return fail(s, GM_E_INVALID, "restart thresholds must be positive");
+    s->rst_eps2 = eigen_eps2;
+    s->rst_ref = sink_factor;
+    return GM_OK;
+}
+
+int om_restart_count(const gm_solver *s, int32_t *n)
+{
+    *n = s->last_restarts;
     return GM_OK;
 }
 

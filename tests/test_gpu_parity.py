@@ -587,6 +587,44 @@ def test_config1_sinks_switch_on_against_the_oracle():
         assert dm[k] == pytest.approx(do[k], rel=1e-7, abs=1e-14), k
 
 
+@pytest.mark.parametrize("absorb", [False, True])
+@pytest.mark.parametrize("precision", [gm.GM_PRECISION_FP64, gm.GM_PRECISION_MIXED])
+def test_forced_bicgstab_restart_matches_oracle(absorb, precision):
+    """The restart branch of BiCGSTAB on the device (matrices.cpp:67-74 with
+    the sinks; Eigen's, with its counter reset on the first restart, without
+    them), forced on both paths by raising the restart thresholds to 0.1 of
+    |r0|^2: five steps, psi within 1e-10 (FP64) or 1e-8 (mixed) of the
+    oracle, iterations within one and the same restart count."""
+    announce(f"forced restart: absorb {absorb}, precision {precision}")
+    args = dict(Nr=129, L=128.0, dt=4.0, epsilon=0.0, l_max=4, mu=0.2, M_c=0.1, q=0.5, a=0.0, R=20.0,
+                counter=False, absorption=absorb, save_every=1000, ramp_periods=0.0, epsilon_smear=4.0)
+    g, o = gm.CrankNicolsonModes(**args), oracle.Oracle(**args)
+    psi = None
+    for s in (g, o):
+        s.set_flags(evolve_orbit=0)
+        if s is g:
+            s.set_flags(precision=precision)
+        if psi is None:
+            from test_oracle import generic_state
+            psi = generic_state(s)
+        s.set_initial_state_full(psi)
+        s.initialize()
+        s.set_restart_thresholds(0.1, 0.1)
+        s.begin_run(40.0, 6)
+    tol = 1e-10 if precision == gm.GM_PRECISION_FP64 else 1e-8
+    total = 0
+    for n in range(1, 6):
+        dg, do = g.step(n), o.step(n)
+        assert g.scalars()["sink_active"] == int(absorb)
+        assert abs(g.last_solver_iterations - o.last_solver_iterations) <= 1, n
+        assert g.restart_count() == o.restart_count(), n
+        total += o.restart_count()
+        assert rel(g.psi, o.psi) < tol, n
+        assert dg["energy"] == pytest.approx(do["energy"], rel=1e-9 if tol < 1e-9 else 1e-7)
+    assert total > 0
+    announce(f"  restarts {total}, psi {rel(g.psi, o.psi):.2e}")
+
+
 def test_checkpoint_resume_is_bitwise(tmp_path):
     """save_checkpoint at step 10, then a fresh solver resumed from it runs
     steps 11-20 exactly as the uninterrupted run (restart.cpp:139-233); the

@@ -151,6 +151,23 @@ def test_one_step_matches_dense_solve(scenario):
     assert cn.last_solver_residual <= 1e-9
 
 
+@pytest.mark.parametrize("scenario", [0, 1])
+def test_restarted_solve_matches_dense_solve(scenario):
+    """The restart branch of BiCGSTAB (matrices.cpp:67-74 with the sinks,
+    Eigen's without), forced by raising the thresholds to 0.1 |r0|^2: the
+    restarted solve still meets the dense LU solution of the step."""
+    cn, ref = correctness_case(scenario)
+    cn.set_restart_thresholds(0.1, 0.1)
+    cn.begin_run(60.0, 2)
+    cn.step(1)
+    assert cn.restart_count() > 0
+    exp = ref["expected"]
+    assert np.linalg.norm(cn.psi - exp) / np.linalg.norm(exp) < 1e-9
+    assert cn.last_solver_residual <= 1e-9
+    with pytest.raises(ValueError):
+        cn.set_restart_thresholds(0.0, 1e-32)
+
+
 @pytest.mark.parametrize("scenario", [0, 1, 2, 3])
 def test_operators_match_dense(scenario):
     cn, ref = correctness_case(scenario)
