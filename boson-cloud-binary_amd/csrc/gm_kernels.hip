@@ -1551,6 +1551,91 @@ __global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a)
     }
 }
 
+// ---------------------------------------------------------------- apply_sink on device vectors
+
+// One wavefront per mode k: the projections v_a^H X over the mode's part of
+// each support, stored per mode; the last wavefront to finish sums the modes
+// in order (fixed order, so the result does not depend on timing) and forms
+// coef_a = scale_a (v_a^H X).
+__global__ __launch_bounds__(64) void k_sink_project(SinkApplyArgs a)
+{
+    const int k = blockIdx.x, lane = threadIdx.x;
+    SinkScratch *scr = a.scr;
+    double v[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (s >= a.nsink) break;
+        const SinkDev &sk = a.sk[s];
+        for (int d = lane; d < sk.n; d += 64) {
+            const double2 t = cdotm(sk.v[(size_t)k * sk.n + d], a.X[(size_t)k * a.N + sk.lo + d]);
+            v[2 * s] += t.x;
+            v[2 * s + 1] += t.y;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = wave_sum(v[q]);
+    __shared__ bool last;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            __hip_atomic_store(&scr->part[k][q], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the same ordering argument as block_partials
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        last = __hip_atomic_fetch_add(&scr->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+               gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t[4] = {0, 0, 0, 0};
+    // contiguous slice of modes per lane, then a fixed-order wave sum
+    const int per = (a.K + 63) / 64, k0 = lane * per;
+    for (int kk = k0; kk < k0 + per && kk < a.K; ++kk)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] += vload(&scr->part[kk][q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[q] = wave_sum(t[q]);
+    if (lane == 0) {
+        for (int s = 0; s < 2; ++s)
+            scr->coef[s] = s < a.nsink ? cscale(a.scale[s], make_double2(t[2 * s], t[2 * s + 1]))
+                                       : make_double2(0.0, 0.0);
+        scr->counter = 0;
+    }
+}
+
+// One workgroup per mode: Y (or zero) + coef_0 v_0 + coef_1 v_1, the second
+// support after the first (they may share nodes).
+__global__ __launch_bounds__(256) void k_sink_expand(SinkApplyArgs a)
+{
+    const int k = blockIdx.x;
+    double2 *Yk = a.Y + (size_t)k * a.N;
+    if (a.zero_base)
+        for (int i = threadIdx.x; i < a.N; i += blockDim.x) Yk[i] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int s = 0; s < a.nsink; ++s) {
+        const SinkDev &sk = a.sk[s];
+        const double2 c = vload2(&a.scr->coef[s]);
+        for (int d = threadIdx.x; d < sk.n; d += blockDim.x) {
+            const int i = sk.lo + d;
+            Yk[i] = cadd(Yk[i], cmul(c, sk.v[(size_t)k * sk.n + d]));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_cscale(double2 *y, size_t n, double2 c)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        y[q] = cmul(c, y[q]);
+}
+
+__global__ void k_sink_scratch_clear(SinkScratch *scr)
+{
+    if (threadIdx.x == 0) scr->counter = 0;
+}
+
 // Expand a compact sink projector into a full-length vector.
 __global__ void k_expand(int N, int K, SinkDev sk, double2 *out)
 {
@@ -1729,6 +1814,23 @@ hipError_t launch_expand(int N, int K, const SinkDev &sk, double2 *out, hipStrea
 hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_diag_final, dim3(a.K), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sink_apply(const SinkApplyArgs &a, hipStream_t st)
+{
+    if (a.K < 1 || a.K > SINK_MAX_MODES || !a.scr || !a.X || !a.Y || a.nsink < 0 || a.nsink > 2)
+        return hipErrorInvalidValue;
+    // the counter starts at zero (a fill kernel, so the sequence may be captured)
+    hipLaunchKernelGGL(k_sink_scratch_clear, dim3(1), dim3(64), 0, st, a.scr);
+    if (a.nsink > 0) hipLaunchKernelGGL(k_sink_project, dim3(a.K), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_sink_expand, dim3(a.K), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cscale(double2 *y, size_t n, double2 c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cscale, dim3(stream_grid(n)), dim3(256), 0, st, y, n, c);
     return hipGetLastError();
 }
 

@@ -127,6 +127,29 @@ struct DiagArgs {
     double *out;         // flux, energy_flux, norms, energies, then E, p_bin, dV, mass shares (K each)
 };
 
+// apply_sink (absorption.cpp:100-109) on device vectors: Y = base + sum_a
+// scale_a (v_a^H X) v_a over the supports, base being Y itself (accumulate) or
+// zero. The projections reduce through a scratch block (SinkScratch).
+constexpr int SINK_MAX_MODES = gmd::MAXLM * (gmd::MAXLM + 1) / 2;  // l_max <= MAXLM - 1
+struct SinkScratch {
+    unsigned counter;
+    unsigned pad;
+    double2 coef[2];
+    double part[SINK_MAX_MODES][4];  // per-mode projections of both sinks (re, im)
+};
+struct SinkApplyArgs {
+    int N, K, nsink;
+    gmd::SinkDev sk[2];
+    double scale[2];     // factor g_a of each sink
+    const double2 *X;
+    double2 *Y;
+    int zero_base;       // Y = sum ... (1) or Y += sum ... (0)
+    SinkScratch *scr;    // device scratch of at least sizeof(SinkScratch)
+};
+hipError_t launch_sink_apply(const SinkApplyArgs &a, hipStream_t st);
+// y = c y over n complex values
+hipError_t launch_cscale(double2 *y, size_t n, double2 c, hipStream_t st);
+
 // grid of the operator apply for tiles of 16 ng nodes (ng = 1, 2 or 4)
 int apply_grid(int N, int nchunks, int ng, int *ngroups);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);

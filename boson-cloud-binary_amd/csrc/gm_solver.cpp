@@ -152,6 +152,7 @@ struct gm_solver {
     DevBuf<float2> inv32, y32;  // FP32 pivots and sweep scratch (mixed-precision preconditioner)
     DevBuf<double2> xt;         // phased column vectors of the coupling (FP64; FP32 uses the first half)
     DevBuf<double> partial, rowpart, diagout;
+    DevBuf<gmk::SinkScratch> sinkscr;  // reduction scratch of the device apply_sink
     DevBuf<double2> dpsi_ix;
     DevBuf<gmd::StepParams> dP;
     DevBuf<gmd::Krylov> dK;
@@ -1151,6 +1152,7 @@ void alloc_device(gm_solver &S)
     int per = 0;
     const int grid = gmk::apply_grid(S.Nr, S.nchunks, 1, &per);
     S.partial.alloc(static_cast<size_t>(3) * std::max(grid, gmd::RED_BLOCKS));
+    S.sinkscr.alloc(1);
     S.rowpart.alloc(static_cast<size_t>(S.ntiles) * S.K * 6);
     S.dpsi_ix.alloc(S.K);
     S.diagout.alloc(static_cast<size_t>(8) * S.K);
@@ -1317,36 +1319,41 @@ void to_dev(gm_solver &S, const double *h, double2 *d)
     HIPCK(hipStreamSynchronize(S.st));
 }
 
-void host_sink(const gm_solver &S, const double *in, double *out, double factor)
+// apply_sink (absorption.cpp:100-109) on device vectors: Y = factor sum_a g_a
+// (v_a^H X) v_a over the active sinks' supports, added to Y when accumulate
+// is set; the projections reduce through scr.
+void sink_apply_dev(const gm_solver &S, const double2 *X, double2 *Y, double factor, bool accumulate,
+                    gmk::SinkScratch *scr, hipStream_t st)
 {
-    // out += factor * sum_a g_a (v_a^H x) v_a on the supports (apply_sink, absorption.cpp:100-109)
-    if (!S.sink_active) return;
-    for (const Sink &sk : S.sink) {
-        if (sk.strength == 0.0 || sk.n == 0) continue;
-        double pr = 0.0, pi = 0.0;
-        for (int j = 0; j < S.K; ++j)
-            for (int d = 0; d < sk.n; ++d) {
-                const double2 v = sk.v[static_cast<size_t>(j) * sk.n + d];
-                const size_t q = static_cast<size_t>(j) * S.Nr + sk.lo + d;
-                pr += v.x * in[2 * q] + v.y * in[2 * q + 1];
-                pi += v.x * in[2 * q + 1] - v.y * in[2 * q];
-            }
-        const double cr = factor * sk.strength * pr, ci = factor * sk.strength * pi;
-        for (int j = 0; j < S.K; ++j)
-            for (int d = 0; d < sk.n; ++d) {
-                const double2 v = sk.v[static_cast<size_t>(j) * sk.n + d];
-                const size_t q = static_cast<size_t>(j) * S.Nr + sk.lo + d;
-                out[2 * q] += cr * v.x - ci * v.y;
-                out[2 * q + 1] += cr * v.y + ci * v.x;
-            }
-    }
+    gmk::SinkApplyArgs a{};
+    a.N = S.Nr;
+    a.K = S.K;
+    if (S.sink_active)
+        for (const Sink &sk : S.sink) {
+            if (sk.strength == 0.0 || sk.n == 0) continue;
+            a.sk[a.nsink].lo = sk.lo;
+            a.sk[a.nsink].n = sk.n;
+            a.sk[a.nsink].v = sk.dv.p;
+            a.scale[a.nsink] = factor * sk.strength;
+            a.nsink++;
+        }
+    a.X = X;
+    a.Y = Y;
+    a.zero_base = accumulate ? 0 : 1;
+    a.scr = scr;
+    HIPCK(gmk::launch_sink_apply(a, st));
+}
+
+void apply_H_dev(gm_solver &S, const double2 *X, double2 *Y)
+{
+    gmk::ApplyArgs a = apply_args(S, X, Y);
+    HIPCK(gmk::launch_apply(a, gmk::AM_H, S.st));
 }
 
 void apply_H_host(gm_solver &S, const double *in, double *out)
 {
     to_dev(S, in, S.tmp.p);
-    gmk::ApplyArgs a = apply_args(S, S.tmp.p, S.t.p);
-    HIPCK(gmk::launch_apply(a, gmk::AM_H, S.st));
+    apply_H_dev(S, S.tmp.p, S.t.p);
     to_host(S, S.t.p, out);
 }
 
@@ -2435,8 +2442,9 @@ int gm_apply_H(gm_solver *S, const double *in, double *out)
 int gm_apply_sink(gm_solver *S, const double *in, double *out)
 {
     return guarded(S, [&] {
-        std::fill(out, out + 2 * S->nvec(), 0.0);
-        host_sink(*S, in, out, 1.0);
+        to_dev(*S, in, S->tmp.p);
+        sink_apply_dev(*S, S->tmp.p, S->t.p, 1.0, false, S->sinkscr.p, S->st);
+        to_host(*S, S->t.p, out);
     });
 }
 
@@ -2444,13 +2452,12 @@ int gm_schrodinger_rhs(gm_solver *S, const double *in, double *out)
 {
     return guarded(S, [&] {
         require_initialized(*S);
-        apply_H_host(*S, in, out);
-        for (size_t q = 0; q < S->nvec(); ++q) {
-            const double re = out[2 * q], im = out[2 * q + 1];
-            out[2 * q] = im;
-            out[2 * q + 1] = -re;
-        }
-        host_sink(*S, in, out, -0.5);
+        // -i H psi - Gamma psi / 2 (absorption.cpp:111-117)
+        to_dev(*S, in, S->tmp.p);
+        apply_H_dev(*S, S->tmp.p, S->t.p);
+        HIPCK(gmk::launch_cscale(S->t.p, S->nvec(), make_double2(0.0, -1.0), S->st));
+        sink_apply_dev(*S, S->tmp.p, S->t.p, -0.5, true, S->sinkscr.p, S->st);
+        to_host(*S, S->t.p, out);
     });
 }
 
