@@ -21,6 +21,7 @@
 #include "gm_kernels.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 namespace gmk {
@@ -1815,6 +1816,36 @@ hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_diag_final, dim3(a.K), dim3(64), 0, st, a);
     return hipGetLastError();
+}
+
+// Static reduction scratch of the allocation-free apply_sink
+// (gm_apply_sink_device): one SinkScratch per GPU, held for the life of the
+// process. Host code never passes `scratch` itself to a kernel: its device
+// address on each GPU comes from hipGetSymbolAddress, looked up once per
+// device with that device current.
+static __device__ __align__(256) unsigned char scratch[sizeof(SinkScratch)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+SinkScratch *scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t sink_scratch_address(SinkScratch **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[dev] = static_cast<SinkScratch *>(p);
+    }
+    *out = scratch_ptr[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_sink_apply(const SinkApplyArgs &a, hipStream_t st)

@@ -147,6 +147,9 @@ struct SinkApplyArgs {
     SinkScratch *scr;    // device scratch of at least sizeof(SinkScratch)
 };
 hipError_t launch_sink_apply(const SinkApplyArgs &a, hipStream_t st);
+// Device address of the library's static SinkScratch on the current device,
+// looked up with hipGetSymbolAddress on the first call per device.
+hipError_t sink_scratch_address(SinkScratch **out);
 // y = c y over n complex values
 hipError_t launch_cscale(double2 *y, size_t n, double2 c, hipStream_t st);
 

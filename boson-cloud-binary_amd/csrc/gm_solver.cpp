@@ -153,6 +153,7 @@ struct gm_solver {
     DevBuf<double2> xt;         // phased column vectors of the coupling (FP64; FP32 uses the first half)
     DevBuf<double> partial, rowpart, diagout;
     DevBuf<gmk::SinkScratch> sinkscr;  // reduction scratch of the device apply_sink
+    gmk::SinkScratch *static_scr = nullptr;  // this GPU's static scratch (gm_apply_sink_device)
     DevBuf<double2> dpsi_ix;
     DevBuf<gmd::StepParams> dP;
     DevBuf<gmd::Krylov> dK;
@@ -1634,6 +1635,8 @@ int gm_create(const gm_params *p, gm_solver **out)
         std::memset(S->hP.p, 0, sizeof(gmd::StepParams));
         host_tables(*S);
         alloc_device(*S);
+        // set-up of the allocation-free entry point: this GPU's static scratch
+        HIPCK(gmk::sink_scratch_address(&S->static_scr));
         HIPCK(hipMemsetAsync(S->psi.p, 0, S->nvec() * sizeof(double2), S->st));
         HIPCK(hipStreamSynchronize(S->st));
         *out = S.release();
@@ -2445,6 +2448,22 @@ int gm_apply_sink(gm_solver *S, const double *in, double *out)
         to_dev(*S, in, S->tmp.p);
         sink_apply_dev(*S, S->tmp.p, S->t.p, 1.0, false, S->sinkscr.p, S->st);
         to_host(*S, S->t.p, out);
+    });
+}
+
+int gm_apply_sink_device(gm_solver *S, const double *d_in, double *d_out, void *stream)
+{
+    return guarded(S, [&] {
+        if (!d_in || !d_out) throw std::invalid_argument("apply_sink_device: NULL device buffer");
+        if (S->K > gmk::SINK_MAX_MODES)
+            throw std::invalid_argument("apply_sink_device: " + std::to_string(S->K) +
+                                        " modes exceed the static scratch's " +
+                                        std::to_string(gmk::SINK_MAX_MODES));
+        if (!S->static_scr) throw DeviceError("apply_sink_device: static scratch not set up");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : S->st;
+        sink_apply_dev(*S, reinterpret_cast<const double2 *>(d_in), reinterpret_cast<double2 *>(d_out), 1.0, false,
+                       S->static_scr, st);
+        if (!stream) HIPCK(hipStreamSynchronize(st));
     });
 }
 

@@ -77,6 +77,15 @@ class CrankNicolsonModes(capi.Solver):
     def __init__(self, *args, **kw):
         super().__init__(lib(), "gm_", *args, **kw)
 
+    def apply_sink_device(self, d_in, d_out, stream=None):
+        """gm_apply_sink_device on device pointers (ints): allocates nothing.
+        With stream None the call waits for the result."""
+        fn = self._lib.gm_apply_sink_device
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        self._check(fn(self._h, C.c_void_p(int(d_in)), C.c_void_p(int(d_out)),
+                       None if stream is None else C.c_void_p(int(stream))))
+
     def set_coupling_tolerance(self, tol):
         """Multipole truncation of the coupling (0 keeps every term)."""
         self._check(self._lib.gm_set_coupling_tolerance(self._h, float(tol)))

@@ -186,6 +186,18 @@ int gm_apply_sink(gm_solver *solver, const double *in, double *out);
 int gm_schrodinger_rhs(gm_solver *solver, const double *in, double *out);
 double gm_sink_strength(const gm_solver *solver, int32_t body);
 
+/* apply_sink without any allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops that run with allocation forbidden). d_in and d_out are
+ * device buffers of Nr*n_modes complex values on the solver's GPU; the result
+ * equals gm_apply_sink's. The projections reduce through a static scratch
+ * array of about 26 KB per GPU, set up by gm_create and held until the
+ * process ends; solvers with more than 820 modes (l_max above 39) are refused.
+ * stream NULL runs on the solver's stream and waits for the result; otherwise
+ * the work is only enqueued on that hipStream_t. Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU race. */
+int gm_apply_sink_device(gm_solver *solver, const double *d_in, double *d_out, void *stream);
+
 /* The reference's ilut_solver (include/CrankNicolsonModes.hpp:264, configured at
  * src/CrankNicolsonModes_matrices.cpp:21-24) on a caller's system, host only:
  * ILUT(fill_factor, droptol) of the n x n complex CSR matrix (ptr[n+1],
