@@ -64,20 +64,27 @@ def build(clean=False, verbose=True):
     return LIB
 
 
-DRIVER_SRC = os.path.join(REPO, "tests", "cpp", "cnm_driver.cpp")
-DRIVER = os.path.join(REPO, "tests", "cpp", "_bin", "cnm_driver")
+CPP_TESTS = os.path.join(REPO, "tests", "cpp")
+# C++ programs over the headers in include/: the gm.hpp class driver, the
+# reference's main driver over the drop-in class, and the config reader probe
+DRIVERS = (("cnm_driver.cpp", True), ("gm_main.cpp", True), ("config_probe.cpp", False))
 
 
 def build_driver(verbose=True):
-    """The C++ test driver of include/gm.hpp, linked against libgm.so."""
-    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"), DRIVER_SRC, "-o", DRIVER,
-           "-L", os.path.dirname(LIB), "-lgm", "-Wl,-rpath,$ORIGIN/../../../boson-cloud-binary_amd/gm"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), res.stdout + res.stderr))
-    if verbose:
-        print("built", DRIVER)
+    """The C++ test programs of tests/cpp, linked against libgm.so."""
+    out_dir = os.path.join(CPP_TESTS, "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    for src, link in DRIVERS:
+        out = os.path.join(out_dir, os.path.splitext(src)[0])
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"),
+               os.path.join(CPP_TESTS, src), "-o", out]
+        if link:
+            cmd += ["-L", os.path.dirname(LIB), "-lgm", "-Wl,-rpath,$ORIGIN/../../../boson-cloud-binary_amd/gm"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), res.stdout + res.stderr))
+        if verbose:
+            print("built", out)
 
 
 if __name__ == "__main__":
