@@ -28,6 +28,9 @@ struct ServeRequest {
     double r_ex = 0;
     int n_steps = 0;
     double footprint = 0;
+    int probe_step = 0;
+    double *probe_psi = nullptr;
+    Clock::time_point submitted;
 };
 
 struct ServeActive {
@@ -70,6 +73,7 @@ struct gm_server {
     std::vector<ServeActive> active;
     std::map<int64_t, ServeResult> results;
     std::vector<double> lat_ms;
+    std::vector<double> req_ms;  // submission to completion of finished requests
     double work = 0;  // mode-node-steps completed
     double busy_s = 0;  // wall time inside gm_server_run
     std::string err;
@@ -125,6 +129,8 @@ void serve_retire(gm_server &V, size_t i, bool ok)
         if (gm_get_state(a.S, r.psi.data()) != GM_OK || gm_get_scalars(a.S, &r.sc) != GM_OK) {
             r.state = GM_SERVE_FAILED;
             r.error = a.S->err;
+        } else {
+            V.req_ms.push_back(std::chrono::duration<double, std::milli>(Clock::now() - a.req.submitted).count());
         }
     }
     gm_destroy(a.S);
@@ -175,6 +181,9 @@ int gm_server_submit(gm_server *V, const gm_request *req, int64_t *ticket)
     q.mask_n = req->mask_n;
     q.r_ex = req->r_ex;
     q.n_steps = req->n_steps;
+    q.probe_step = req->probe_step;
+    q.probe_psi = req->probe_psi;
+    q.submitted = Clock::now();
     q.ticket = V->next_ticket++;
     *ticket = q.ticket;
     V->results[q.ticket].state = GM_SERVE_QUEUED;
@@ -231,6 +240,9 @@ int gm_server_run(gm_server *V, double seconds, int32_t *completed)
                             V->work += static_cast<double>(S->K) * S->Nr;
                             ++a.steps_done;
                             V->results[a.req.ticket].steps_done = a.steps_done;
+                            // spot-check probe: the state after the requested step
+                            if (a.req.probe_psi && a.steps_done == a.req.probe_step)
+                                st = gm_get_state(S, a.req.probe_psi);
                         }
                         a.phase = 0;
                     }
@@ -302,17 +314,21 @@ int gm_server_stats(const gm_server *V, gm_serve_stats *out)
     out->queued = static_cast<int32_t>(V->queue.size());
     out->active = static_cast<int32_t>(V->active.size());
     out->hbm_in_use = V->in_use;
-    if (!V->lat_ms.empty()) {
-        std::vector<double> v(V->lat_ms);
+    out->completed = static_cast<int64_t>(V->req_ms.size());
+    // nearest-rank percentiles
+    auto pct = [](std::vector<double> v, double &p50, double &p99, double &mx) {
+        if (v.empty()) return;
         std::sort(v.begin(), v.end());
-        auto pct = [&](double p) {
-            const size_t k = static_cast<size_t>(std::ceil(p * v.size())) - 1;
+        auto at = [&](double f) {
+            const size_t k = static_cast<size_t>(std::ceil(f * v.size())) - 1;
             return v[std::min(k, v.size() - 1)];
         };
-        out->p50_ms = pct(0.50);
-        out->p99_ms = pct(0.99);
-        out->max_ms = v.back();
-    }
+        p50 = at(0.50);
+        p99 = at(0.99);
+        mx = v.back();
+    };
+    pct(V->lat_ms, out->p50_ms, out->p99_ms, out->max_ms);
+    pct(V->req_ms, out->req_p50_ms, out->req_p99_ms, out->req_max_ms);
     return GM_OK;
 }
 

@@ -270,7 +270,8 @@ int gm_profile_read(const gm_solver *solver, double *ms, int64_t *launches, int3
  * idle: steps of different sizes overlap on the device and requests that
  * arrive while others run join at once. Results are bitwise those of the same
  * request run alone. Per-step latency (start of a step to the end of its
- * host-side finish) is recorded for every step.
+ * host-side finish) is recorded for every step, and every finished request's
+ * latency from its submission (queueing included) to its completion.
  * ------------------------------------------------------------------------- */
 typedef struct gm_server gm_server;
 
@@ -285,6 +286,8 @@ typedef struct gm_request {
     int32_t mask_n;
     double r_ex;           /* extraction radius */
     int32_t n_steps;       /* steps to run */
+    int32_t probe_step;    /* > 0: after this step, psi is copied into probe_psi (for spot checks) */
+    double *probe_psi;     /* caller buffer of Nr*n_modes complex values, kept valid until the request ends */
 } gm_request;
 
 typedef struct gm_serve_stats {
@@ -294,6 +297,8 @@ typedef struct gm_serve_stats {
     double p50_ms, p99_ms, max_ms;  /* per-step latency */
     int32_t queued, active;
     double hbm_in_use;       /* footprint of the active solvers, bytes */
+    int64_t completed;       /* requests finished */
+    double req_p50_ms, req_p99_ms, req_max_ms;  /* request latency: submission to completion */
 } gm_serve_stats;
 
 /* Estimated device bytes of one solver with these constructor arguments. */

@@ -1,9 +1,15 @@
 """Serving loop (gm_server_*): requests of different sizes that arrive while
-others run give, bit for bit, the result of the same request run alone."""
+others run give, bit for bit, the result of the same request run alone; a
+Poisson stream with the served workload's physics (back-reaction on, cloud
+masses drawn, a two-period startup ramp, mixed precision) at small shapes,
+with 5 % of its requests spot-checked against the FP64 oracle."""
+import math
+
 import numpy as np
 import pytest
 
 import gm
+import oracle
 from gm import serve
 
 pytestmark = pytest.mark.gpu
@@ -69,3 +75,46 @@ def test_admission_respects_the_budget():
     with pytest.raises(gm.InvalidArgument):
         srv.submit(dict(p, Nr=4096, L=1024.0), np.zeros(15 * 4096, dtype=complex), 1, 800.0, flags=f)
     srv.close()
+
+
+def oracle_steps(params, psi0, n_steps, steps):
+    """The FP64 oracle on a served request's inputs, `steps` steps."""
+    o = oracle.Oracle(**params)
+    f = serve.default_flags()
+    o.set_flags(**{k: getattr(f, k) for k, _ in gm.capi.Flags._fields_})
+    o.set_initial_state_full(psi0)
+    o.initialize()
+    o.build_damping_mask(0.85, 1.0, 6)
+    o.begin_run(0.8 * params["L"], n_steps + 1)
+    for n in range(1, steps + 1):
+        o.step(n, diag=False)
+    return o.psi, o.scalars()
+
+
+def test_served_stream_with_oracle_spot_checks():
+    """Poisson arrivals at 10/s for 2 s of 20-40-step requests (N 1024-2048,
+    l_max 2-8), served in mixed precision while they arrive; every request
+    finishes, per-step and request latencies are reported, and every 20th
+    request (5 %) is checked against the FP64 oracle: its state after two
+    steps, taken inside the mix, and its final state and orbit, within the
+    mixed-precision tolerance 1e-8."""
+    print("serving: Poisson stream with oracle spot checks", flush=True)
+    st = serve.served_stream(duration=2.0, rate=10.0, seed=3, hbm_budget=8e9, max_active=6, drain=True,
+                             drain_limit=240, sizes=[(1024, 4), (2048, 6), (1536, 8), (1024, 2)], steps=(20, 40),
+                             probe_every=20, probe_step=2)
+    print({k: v for k, v in st.items() if k != "probes"}, flush=True)
+    assert st["submitted"] >= 10
+    assert st["completed"] == st["submitted"] and st["queued"] == 0 and st["active"] == 0
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    assert st["req_p99_ms"] >= st["req_p50_ms"] > 0
+    assert len(st["probes"]) == math.ceil(st["submitted"] / 20)
+    for rec in st["probes"]:
+        p, n = rec["params"], rec["n_steps"]
+        ref2, _ = oracle_steps(p, rec["psi0"], n, 2)
+        e2 = np.linalg.norm(rec["psi"] - ref2) / np.linalg.norm(ref2)
+        ref, sref = oracle_steps(p, rec["psi0"], n, n)
+        e = np.linalg.norm(rec["final"] - ref) / np.linalg.norm(ref)
+        print(f"  spot check ticket {rec['ticket']}: after 2 steps {e2:.2e}, after {n} steps {e:.2e}", flush=True)
+        assert e2 < 1e-8 and e < 1e-8
+        for k in ("R", "M1", "M2", "Omega", "theta", "t"):
+            assert rec["scalars"][k] == pytest.approx(sref[k], rel=1e-9, abs=1e-12), k
