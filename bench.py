@@ -459,7 +459,9 @@ def main():
                     help="untimed steps of the mixed-vs-FP64 comparison (0 skips it)")
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the batched ragged ensemble measurement (0 skips it)")
-    ap.add_argument("--serve-seconds", type=float, default=15.0,
+    ap.add_argument("--ensemble-steps", type=int, default=20,
+                    help="timed steps of the ensemble (after 3 warmup steps)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the served Poisson stream for p50/p99 step latency (0 skips it)")
     ap.add_argument("--large-steps", type=int, default=10,
                     help="timed steps per precision at l_max 30, N 32768 (0 skips the second size)")
@@ -516,10 +518,16 @@ def main():
         out["second_size"] = large_case(args.large_steps, 2, args.large_accuracy_steps)
     if args.ensemble_members > 0:
         from gm import ensemble
-        ens = ensemble.run(n=args.ensemble_members, steps=10, warmup=3, seed=0)
-        ens["config"] = ("ragged fixed-orbit binaries from seed 0: q log-uniform [0.05, 1], R [20, 60], "
-                         "mu [0.1, 0.3], N in {4096..32768} (L = N/4), l_max in {8..24}; mixed precision")
+        cfg = ("ragged fixed-orbit binaries from seed 0: q log-uniform [0.05, 1], R [20, 60], mu [0.1, 0.3], "
+               "N in {4096..32768} (L = N/4), l_max in {8..24}, 2p atom on the primary + 1e-3 perturbation; "
+               "mixed precision")
+        ens = ensemble.run(n=args.ensemble_members, steps=args.ensemble_steps, warmup=3, seed=0, mode="batch")
+        ens["config"] = cfg + "; one ragged grid per launch over all members, graph-captured step"
         ens["over_fp64_single_step"] = ens["mode_node_steps_per_s"] / f64["value"]
+        streams = ensemble.run(n=args.ensemble_members, steps=args.ensemble_steps, warmup=3, seed=0,
+                               mode="streams")
+        streams["over_fp64_single_step"] = streams["mode_node_steps_per_s"] / f64["value"]
+        ens["per_member_streams"] = streams
         out["ensemble"] = ens
     if args.serve_seconds > 0:
         out["serving"] = serving(args.serve_seconds)

@@ -90,11 +90,35 @@ __device__ __forceinline__ int mode_m(int j)
     return 2 * (j - l * (l + 1) / 2) - l;
 }
 
+// The blocks of one launch that belong to one solver: this block's index
+// among them and their count. A plain launch gives each solver the whole
+// grid; a batched ragged launch (one grid over many solvers) gives each its
+// own contiguous range of blocks.
+struct Blk {
+    int id, n;
+};
+__device__ __forceinline__ Blk grid_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
+
+// The member of a batched launch that owns this block: the largest m with
+// off[m] <= blockIdx.x (off holds M + 1 non-decreasing block offsets).
+__device__ __forceinline__ int batch_member(const int *off, int M)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = M;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (__builtin_amdgcn_readfirstlane(off[mid]) <= b) lo = mid;
+        else hi = mid;
+    }
+    // uniform over the block: the member's arguments load into scalar registers
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
 // Sum NQ per-thread values over the block; every block stores its totals in
 // partial[q * nblocks + block]. Returns true in the last block to finish, whose
 // threads then see every partial. blockDim.x must be a multiple of 64, <= 1024.
 template <int NQ>
-__device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned *counter)
+__device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned *counter, Blk B)
 {
     __shared__ double ws[16][NQ];
     __shared__ bool last;
@@ -113,7 +137,7 @@ __device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned 
             // point, so no release fence (a write-back of this XCD's whole
             // L2, which every block of a streaming kernel would wait on) is
             // needed for the partials
-            __hip_atomic_store(partial + (size_t)q * gridDim.x + blockIdx.x, s, __ATOMIC_RELAXED,
+            __hip_atomic_store(partial + (size_t)q * B.n + B.id, s, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
         // The partial stores have completed before this block is counted.
@@ -129,7 +153,7 @@ __device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned 
         __builtin_amdgcn_s_waitcnt(0);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == gridDim.x - 1);
+        last = (t == (unsigned)B.n - 1);
     }
     __syncthreads();
     // acquire in the one block that reads every partial (coherent loads)
@@ -138,10 +162,10 @@ __device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned 
 }
 
 // In the last block: sum the partials of quantity q in block order.
-__device__ double sum_partials(const double *partial, int q)
+__device__ double sum_partials(const double *partial, int q, Blk B)
 {
     __shared__ double ws[16];
-    const int nb = gridDim.x;
+    const int nb = B.n;
     double s = 0.0;
     // contiguous slice per thread keeps the order fixed for a fixed grid
     const int per = (nb + blockDim.x - 1) / blockDim.x;
@@ -441,7 +465,7 @@ __device__ __forceinline__ void mfma_stream(const Geometry &g, const T *GM, cons
 // (matrices.cpp:27-40) with the rank-one sinks (matrices.cpp:111-116), and
 // fuses the reductions.
 template <int NG, typename T>
-__global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
+__device__ __forceinline__ void k_apply_body(const ApplyArgs &a, Blk B)
 {
     typedef Arith<T> AR;
     constexpr int TN = 16 * NG;
@@ -464,7 +488,7 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
     // block b runs on XCD b % 8 (round-robin dispatch): the chunk groups of a
     // tile share one XCD's L2, tiles are spread over the XCDs and, within an
     // XCD, dispatched in order of increasing radius (decreasing multipole work)
-    const int b = blockIdx.x;
+    const int b = B.id;
     const int ng = a.ngroups;
     const int local = b >> 3;
     const int grp = local % ng;
@@ -596,25 +620,25 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         }
     }
     if (MODE == AM_V) {
-        if (block_partials<2>({red[0], red[1]}, a.partial, &Kr->counter)) {
-            const double dr_ = sum_partials(a.partial, 0), di = sum_partials(a.partial, 1);
+        if (block_partials<2>({red[0], red[1]}, a.partial, &Kr->counter, B)) {
+            const double dr_ = sum_partials(a.partial, 0, B), di = sum_partials(a.partial, 1, B);
             if (threadIdx.x == 0) {
                 Kr->alpha = cdiv(Kr->rho, make_double2(dr_, di));
                 Kr->counter = 0;
             }
         }
     } else if (MODE == AM_T) {
-        if (block_partials<3>({red[0], red[1], red[2]}, a.partial, &Kr->counter)) {
-            const double tr = sum_partials(a.partial, 0), ti = sum_partials(a.partial, 1);
-            const double tt = sum_partials(a.partial, 2);
+        if (block_partials<3>({red[0], red[1], red[2]}, a.partial, &Kr->counter, B)) {
+            const double tr = sum_partials(a.partial, 0, B), ti = sum_partials(a.partial, 1, B);
+            const double tt = sum_partials(a.partial, 2, B);
             if (threadIdx.x == 0) {
                 Kr->w = tt > 0.0 ? make_double2(tr / tt, ti / tt) : make_double2(0.0, 0.0);
                 Kr->counter = 0;
             }
         }
     } else if (MODE == AM_RHS) {
-        if (block_partials<2>({red[0], red[1]}, a.partial, &Kr->counter)) {
-            const double s = sum_partials(a.partial, 0), s0 = sum_partials(a.partial, 1);
+        if (block_partials<2>({red[0], red[1]}, a.partial, &Kr->counter, B)) {
+            const double s = sum_partials(a.partial, 0, B), s0 = sum_partials(a.partial, 1, B);
             if (threadIdx.x == 0) {
                 Kr->rhs_nrm = sqrt(s);
                 Kr->r0_raw2 = s0;
@@ -622,8 +646,8 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
             }
         }
     } else if (MODE == AM_RES) {
-        if (block_partials<1>({red[0]}, a.partial, &Kr->counter)) {
-            const double s = sum_partials(a.partial, 0);
+        if (block_partials<1>({red[0]}, a.partial, &Kr->counter, B)) {
+            const double s = sum_partials(a.partial, 0, B);
             if (threadIdx.x == 0) {
                 Kr->res2 = s;
                 Kr->counter = 0;
@@ -636,8 +660,8 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         // new |r0|^2 as its threshold base and resets the iteration counter
         // on the first restart. The iteration then goes on from beta with
         // the old p, v, alpha and w.
-        if (block_partials<1>({red[0]}, a.partial, &Kr->counter)) {
-            const double s = sum_partials(a.partial, 0);
+        if (block_partials<1>({red[0]}, a.partial, &Kr->counter, B)) {
+            const double s = sum_partials(a.partial, 0, B);
             if (threadIdx.x == 0) {
                 const double rho = Kr->rhs_nrm * s;
                 Kr->rho = make_double2(rho, 0.0);
@@ -655,6 +679,17 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
         }
     }
 }
+template <int NG, typename T>
+__global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a) { k_apply_body<NG, T>(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+template <int NG, typename T>
+__global__ __launch_bounds__(64 * gmd::AW) void k_apply_batch(const ApplyArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const ApplyArgs a = args[m];
+    k_apply_body<NG, T>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+}
 
 // Diagnostics apply on the post-mask state (extraction.cpp:8-129). The
 // coupling is node-local and the mask is one radial profile for every mode, so
@@ -662,14 +697,14 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply(ApplyArgs a)
 // stencil is evaluated here. One wavefront per (mode, tile of DIAG_NODES nodes) writes
 // the tile sums of |U|^2 and Re<U|H U> below idx_ex, Re<U|H U>, Im(m <U|V U>)
 // and Re<dU/dt|V U> up to idx_ex inclusive, and |U|^2 over the grid.
-__global__ __launch_bounds__(256) void k_diag_apply(ApplyArgs a)
+__device__ __forceinline__ void k_diag_apply_body(const ApplyArgs &a, Blk B)
 {
     const Geometry &g = a.g;
     const StepParams *P = a.P;
     const Krylov *Kr = a.Kr;
     const int lane = threadIdx.x & 63;
     const int N = g.N, nt = (N + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES;
-    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int item = B.id * 4 + (threadIdx.x >> 6);
     if (item >= g.K * nt) return;
     const int j = item / nt, tile = item - j * nt;
     const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
@@ -718,6 +753,15 @@ __global__ __launch_bounds__(256) void k_diag_apply(ApplyArgs a)
         const double s = wave_sum(rp[qq]);
         if (lane == 0) a.rowpart[((size_t)tile * g.K + j) * 6 + qq] = s;
     }
+}
+__global__ __launch_bounds__(256) void k_diag_apply(ApplyArgs a) { k_diag_apply_body(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+__global__ __launch_bounds__(256) void k_diag_apply_batch(const ApplyArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const ApplyArgs a = args[m];
+    k_diag_apply_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
 // ---------------------------------------------------------------- Thomas factor
@@ -978,7 +1022,7 @@ __device__ __forceinline__ void store_phased(void *Xt, int f32, size_t q, double
 }
 
 template <int VAR, typename RT>
-__global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
+__device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
 {
     // RT = float: the two sweeps in FP32 on FP32 pivots (the forward result in
     // an FP32 scratch vector); the p / s updates and the output stay FP64
@@ -989,7 +1033,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
     __shared__ C sa[TE + TE / E];
     __shared__ C sf[TE + TE / E];
-    const int j = blockIdx.x;
+    const int j = B.id;
     const int N = a.N;
     const int tid = threadIdx.x;
     const size_t base_j = (size_t)j * N;
@@ -1228,7 +1272,7 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
-    if (tid == 0) last = __hip_atomic_fetch_add(&Kr->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    if (tid == 0) last = __hip_atomic_fetch_add(&Kr->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == B.n - 1;
     __syncthreads();
     if (!last) return;
     __threadfence();
@@ -1310,6 +1354,17 @@ __global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a)
         Kr->counter = 0;
     }
 }
+template <int VAR, typename RT>
+__global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a) { k_precond_body<VAR, RT>(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+template <int VAR, typename RT>
+__global__ __launch_bounds__(gmd::PT) void k_precond_batch(const PrecArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const PrecArgs a = args[m];
+    k_precond_body<VAR, RT>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+}
 
 // ---------------------------------------------------------------- BiCGSTAB vector update
 
@@ -1332,13 +1387,13 @@ __device__ void loop_top(Krylov *Kr, double r2, double2 rho_new)
     Kr->beta = cmul(cdiv(Kr->rho, Kr->rho_old), cdiv(Kr->alpha, Kr->w));
 }
 
-__global__ __launch_bounds__(256) void k_update(UpdateArgs a)
+__device__ __forceinline__ void k_update_body(const UpdateArgs &a, Blk B)
 {
     Krylov *Kr = a.Kr;
     if (Kr->done) return;
     const double2 al = Kr->alpha, w = Kr->w;
     double v[3] = {0, 0, 0};
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (size_t)gridDim.x * blockDim.x) {
+    for (size_t q = (size_t)B.id * blockDim.x + threadIdx.x; q < a.n; q += (size_t)B.n * blockDim.x) {
         const double2 x = cadd(a.X[q], cadd(cmul(al, a.Yc[q]), cmul(w, a.Zc[q])));
         a.X[q] = x;
         const double2 r = csub(a.S[q], cmul(w, a.T[q]));
@@ -1348,9 +1403,9 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a)
         v[1] += d.y;
         v[2] += cnorm2(r);
     }
-    if (block_partials<3>(v, a.partial, &Kr->counter)) {
-        const double rr = sum_partials(a.partial, 0), ri = sum_partials(a.partial, 1);
-        const double r2 = sum_partials(a.partial, 2);
+    if (block_partials<3>(v, a.partial, &Kr->counter, B)) {
+        const double rr = sum_partials(a.partial, 0, B), ri = sum_partials(a.partial, 1, B);
+        const double r2 = sum_partials(a.partial, 2, B);
         if (threadIdx.x == 0) {
             Kr->iter += 1;
             Kr->fresh = 0;
@@ -1358,6 +1413,15 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a)
             Kr->counter = 0;
         }
     }
+}
+__global__ __launch_bounds__(256) void k_update(UpdateArgs a) { k_update_body(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+__global__ __launch_bounds__(256) void k_update_batch(const UpdateArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const UpdateArgs a = args[m];
+    k_update_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
 // Start of the normalised solve (Eigen BiCGSTAB on A x = b, b = rhs/|rhs|,
@@ -1367,17 +1431,17 @@ __global__ __launch_bounds__(256) void k_update(UpdateArgs a)
 // |rhs| r0): rho and r0.v carry the factor |rhs|, which cancels in alpha and
 // beta, and the restart thresholds are scaled by it. r = r0'/|rhs| is formed
 // where the first iteration reads it; p = v = 0 are never read.
-__global__ __launch_bounds__(256) void k_init(InitArgs a)
+__device__ __forceinline__ void k_init_body(const InitArgs &a, Blk B)
 {
     Krylov *Kr = a.Kr;
     const double nrm = Kr->rhs_nrm;
     if (nrm != 0.0) {
-        for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (size_t)gridDim.x * blockDim.x) {
+        for (size_t q = (size_t)B.id * blockDim.x + threadIdx.x; q < a.n; q += (size_t)B.n * blockDim.x) {
             const double2 x = a.Psi[q];
             a.X[q] = make_double2(x.x / nrm, x.y / nrm);
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (B.id == 0 && threadIdx.x == 0) {
         // |r|^2 and |b|^2 of the normalised system
         const double r2 = nrm != 0.0 ? Kr->r0_raw2 / (nrm * nrm) : 0.0;
         const double b2 = nrm != 0.0 ? 1.0 : 0.0;
@@ -1407,33 +1471,51 @@ __global__ __launch_bounds__(256) void k_init(InitArgs a)
         }
     }
 }
+__global__ __launch_bounds__(256) void k_init(InitArgs a) { k_init_body(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+__global__ __launch_bounds__(256) void k_init_batch(const InitArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const InitArgs a = args[m];
+    k_init_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+}
 
 // Acceptance (matrices.cpp:118-121,166-170): psi = mask * |rhs| x when
 // |A x - b| <= 1e-9, or 0 for a zero right-hand side.
-__global__ __launch_bounds__(256) void k_accept(AcceptArgs a)
+__device__ __forceinline__ void k_accept_body(const AcceptArgs &a, Blk B)
 {
     const Krylov *Kr = a.Kr;
     const double nrm = Kr->rhs_nrm;
     const double res = sqrt(Kr->res2);
     const bool zero = Kr->zero_rhs;
     const bool ok = zero || res <= 1e-9;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (B.id == 0 && threadIdx.x == 0) {
         a.status[0] = ok ? 1 : 0;
         a.resid[0] = zero ? 0.0 : res;
     }
     if (!ok) return;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += (size_t)gridDim.x * blockDim.x) {
+    for (size_t q = (size_t)B.id * blockDim.x + threadIdx.x; q < a.n; q += (size_t)B.n * blockDim.x) {
         double2 x = zero ? make_double2(0.0, 0.0) : cscale(nrm, a.X[q]);
         if (a.mask) x = cscale(a.mask[q % a.N], x);
         a.Psi[q] = x;
     }
+}
+__global__ __launch_bounds__(256) void k_accept(AcceptArgs a) { k_accept_body(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+__global__ __launch_bounds__(256) void k_accept_batch(const AcceptArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const AcceptArgs a = args[m];
+    k_accept_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
 // ---------------------------------------------------------------- sinks
 
 // One workgroup. mode SK_PROJ: out[a] = scale * v_a^H X (stored per the dest
 // selector); SK_SETUP: Woodbury capacitance G = diag(1/((dt/4) g)) + V^H W and its inverse.
-__global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
+__device__ __forceinline__ void k_sink_body(const SinkArgs &a, Blk B)
 {
     Krylov *Kr = a.Kr;
     const StepParams *P = a.P;
@@ -1524,11 +1606,20 @@ __global__ __launch_bounds__(1024) void k_sink(SinkArgs a)
         }
     }
 }
+__global__ __launch_bounds__(1024) void k_sink(SinkArgs a) { k_sink_body(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+__global__ __launch_bounds__(1024) void k_sink_batch(const SinkArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const SinkArgs a = args[m];
+    k_sink_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+}
 
 // Woodbury correction of a preconditioned vector, in place over the node
 // range where the columns W_a can be non-zero (matrices.cpp:151):
 // y -= sum_a c_a W_a.
-__global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a)
+__device__ __forceinline__ void k_wcorr_body(const WcorrArgs &a, Blk B)
 {
     const Krylov *Kr = a.Kr;
     if (Kr->done) return;
@@ -1539,7 +1630,7 @@ __global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a)
     const int width = hi - lo;
     const double2 c0 = Kr->c[0], c1 = Kr->c[1];
     const size_t n = (size_t)width * a.K;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    for (size_t q = (size_t)B.id * blockDim.x + threadIdx.x; q < n; q += (size_t)B.n * blockDim.x) {
         const int k = (int)(q / width), i = lo + (int)(q - (size_t)k * width);
         const size_t gi = (size_t)k * a.N + i;
         // stored columns W(theta_W); W(theta) = exp(-i m (theta - theta_W)) W(theta_W)
@@ -1550,6 +1641,15 @@ __global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a)
         a.Y[gi] = y;
         if (a.Xt) store_phased(a.Xt, a.xt_f32, gi, a.P->ph[mk + a.lmax], y);
     }
+}
+__global__ __launch_bounds__(256) void k_wcorr(WcorrArgs a) { k_wcorr_body(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+__global__ __launch_bounds__(256) void k_wcorr_batch(const WcorrArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const WcorrArgs a = args[m];
+    k_wcorr_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
 // ---------------------------------------------------------------- apply_sink on device vectors
@@ -1652,9 +1752,9 @@ __global__ void k_expand(int N, int K, SinkDev sk, double2 *out)
 
 // One wavefront per mode: sums of the per-tile partials of the diagnostics
 // apply, the flux at r_ex, and the mode's share of the global sums.
-__global__ __launch_bounds__(64) void k_diag_final(DiagArgs a)
+__device__ __forceinline__ void k_diag_final_body(const DiagArgs &a, Blk B)
 {
-    const int j = blockIdx.x, lane = threadIdx.x;
+    const int j = B.id, lane = threadIdx.x;
     const int K = a.K, N = a.N, ix = a.ix, nt = a.ntiles;
     const double dr = a.dr, rix = a.eps + ix * dr;
     double v[6] = {0, 0, 0, 0, 0, 0};
@@ -1676,6 +1776,15 @@ __global__ __launch_bounds__(64) void k_diag_final(DiagArgs a)
         for (int q = 0; q < 4; ++q) a.out[(4 + q) * K + j] = v[2 + q];
     }
 }
+__global__ __launch_bounds__(64) void k_diag_final(DiagArgs a) { k_diag_final_body(a, grid_blk()); }
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+__global__ __launch_bounds__(64) void k_diag_final_batch(const DiagArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const DiagArgs a = args[m];
+    k_diag_final_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+}
 
 // ---------------------------------------------------------------- launchers
 
@@ -1688,21 +1797,28 @@ int apply_grid(int N, int nchunks, int ng, int *ngroups)
 }
 
 // Column vectors of the coupling, exp(i m theta) x in the arithmetic of the
-// apply (FP64, or FP32 for the mixed Krylov applies): one row of blocks per mode.
-template <typename C>
-__global__ __launch_bounds__(256) void k_phase(const double2 *X, C *Xt, int N, int lmax, const StepParams *P)
+// apply (FP64, or FP32 for the mixed Krylov applies): nbx blocks per mode.
+__device__ __forceinline__ void k_phase_body(const PhaseArgs &a, Blk B)
 {
-    const int j = blockIdx.y;
+    const int j = B.id / a.nbx, bx = B.id - j * a.nbx;
     const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
     const int l = (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
     const int m = 2 * (j - l * (l + 1) / 2) - l;
-    const double2 ph = P->ph[m + lmax];
-    const size_t base = (size_t)j * N;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const double2 v = cmul(ph, X[base + i]);
-        if constexpr (std::is_same<C, float2>::value) Xt[base + i] = make_float2((float)v.x, (float)v.y);
-        else Xt[base + i] = v;
+    const double2 ph = a.P->ph[m + a.lmax];
+    const size_t base = (size_t)j * a.N;
+    for (int i = bx * blockDim.x + threadIdx.x; i < a.N; i += a.nbx * blockDim.x) {
+        const double2 v = cmul(ph, a.X[base + i]);
+        if (a.f32) static_cast<float2 *>(a.Xt)[base + i] = make_float2((float)v.x, (float)v.y);
+        else static_cast<double2 *>(a.Xt)[base + i] = v;
     }
+}
+__global__ __launch_bounds__(256) void k_phase(PhaseArgs a) { k_phase_body(a, grid_blk()); }
+__global__ __launch_bounds__(256) void k_phase_batch(const PhaseArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    const PhaseArgs a = args[m];
+    k_phase_body(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
 template <int NG>
@@ -1715,6 +1831,19 @@ static hipError_t launch_apply_ng(const ApplyArgs &a, int grid, hipStream_t st)
     return hipGetLastError();
 }
 
+PhaseArgs phase_args(const ApplyArgs &a)
+{
+    PhaseArgs p{};
+    p.X = a.X;
+    p.Xt = a.Xt;
+    p.N = a.g.N;
+    p.lmax = a.g.lmax;
+    p.nbx = (a.g.N + 255) / 256 < 16 ? (a.g.N + 255) / 256 : 16;
+    p.f32 = a.mixed;
+    p.P = a.P;
+    return p;
+}
+
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
     if (a.ng != 1 && a.ng != 2 && a.ng != 4) return hipErrorInvalidValue;
@@ -1722,11 +1851,8 @@ hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
     const int grid = apply_grid(a.g.N, a.g.nchunks, a.ng, &a.ngroups);
     a.mode = mode;
     if (!a.xt_ready) {
-        const dim3 pg((a.g.N + 255) / 256 < 16 ? (a.g.N + 255) / 256 : 16, a.g.K);
-        if (a.mixed)
-            hipLaunchKernelGGL(k_phase<float2>, pg, dim3(256), 0, st, a.X, (float2 *)a.Xt, a.g.N, a.g.lmax, a.P);
-        else
-            hipLaunchKernelGGL(k_phase<double2>, pg, dim3(256), 0, st, a.X, (double2 *)a.Xt, a.g.N, a.g.lmax, a.P);
+        const PhaseArgs pa = phase_args(a);
+        hipLaunchKernelGGL(k_phase, dim3(pa.nbx * a.g.K), dim3(256), 0, st, pa);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1780,6 +1906,87 @@ static int stream_grid(size_t n)
     size_t g = (n + 255) / 256;
     if (g > (size_t)gmd::RED_BLOCKS) g = gmd::RED_BLOCKS;
     return (int)g;
+}
+
+int precond_blocks(int K) { return K; }
+int stream_blocks(size_t n) { return stream_grid(n); }
+int phase_blocks(int N, int K) { return ((N + 255) / 256 < 16 ? (N + 255) / 256 : 16) * K; }
+int diag_apply_blocks(int N, int K) { return (K * ((N + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES) + 3) / 4; }
+
+__global__ void k_krylov_gather(gmd::Krylov *const *kr, int *out, int M)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const gmd::Krylov *k = kr[m];
+    out[3 * m] = k->done;
+    out[3 * m + 1] = k->restart;
+    out[3 * m + 2] = k->iter;
+}
+
+hipError_t launch_krylov_gather(gmd::Krylov *const *kr, int *out, int M, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_krylov_gather, dim3((M + 63) / 64), dim3(64), 0, st, kr, out, M);
+    return hipGetLastError();
+}
+
+template <int NG>
+static void launch_apply_batch_ng(const BatchLaunch &b, hipStream_t st)
+{
+    const ApplyArgs *a = static_cast<const ApplyArgs *>(b.args);
+    if (b.mixed)
+        hipLaunchKernelGGL((k_apply_batch<NG, float>), dim3(b.total), dim3(64 * gmd::AW), 0, st, a, b.off, b.M);
+    else
+        hipLaunchKernelGGL((k_apply_batch<NG, double>), dim3(b.total), dim3(64 * gmd::AW), 0, st, a, b.off, b.M);
+}
+
+hipError_t launch_batch(const BatchLaunch &b, hipStream_t st)
+{
+    if (b.total <= 0 || b.M < 1) return hipSuccess;
+    const dim3 g(b.total);
+    switch (b.kernel) {
+    case BK_APPLY:
+        if (b.ng == 1) launch_apply_batch_ng<1>(b, st);
+        else if (b.ng == 2) launch_apply_batch_ng<2>(b, st);
+        else if (b.ng == 4) launch_apply_batch_ng<4>(b, st);
+        else return hipErrorInvalidValue;
+        break;
+    case BK_PRECOND: {
+        const PrecArgs *a = static_cast<const PrecArgs *>(b.args);
+        if (b.var == PV_P && b.f32) hipLaunchKernelGGL((k_precond_batch<PV_P, float>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        else if (b.var == PV_S && b.f32) hipLaunchKernelGGL((k_precond_batch<PV_S, float>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        else if (b.var == PV_P) hipLaunchKernelGGL((k_precond_batch<PV_P, double>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        else if (b.var == PV_S) hipLaunchKernelGGL((k_precond_batch<PV_S, double>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        else return hipErrorInvalidValue;
+        break;
+    }
+    case BK_UPDATE:
+        hipLaunchKernelGGL(k_update_batch, g, dim3(256), 0, st, static_cast<const UpdateArgs *>(b.args), b.off, b.M);
+        break;
+    case BK_INIT:
+        hipLaunchKernelGGL(k_init_batch, g, dim3(256), 0, st, static_cast<const InitArgs *>(b.args), b.off, b.M);
+        break;
+    case BK_ACCEPT:
+        hipLaunchKernelGGL(k_accept_batch, g, dim3(256), 0, st, static_cast<const AcceptArgs *>(b.args), b.off, b.M);
+        break;
+    case BK_SINK:
+        hipLaunchKernelGGL(k_sink_batch, g, dim3(1024), 0, st, static_cast<const SinkArgs *>(b.args), b.off, b.M);
+        break;
+    case BK_WCORR:
+        hipLaunchKernelGGL(k_wcorr_batch, g, dim3(256), 0, st, static_cast<const WcorrArgs *>(b.args), b.off, b.M);
+        break;
+    case BK_DIAG_APPLY:
+        hipLaunchKernelGGL(k_diag_apply_batch, g, dim3(256), 0, st, static_cast<const ApplyArgs *>(b.args), b.off, b.M);
+        break;
+    case BK_DIAG_FINAL:
+        hipLaunchKernelGGL(k_diag_final_batch, g, dim3(64), 0, st, static_cast<const DiagArgs *>(b.args), b.off, b.M);
+        break;
+    case BK_PHASE:
+        hipLaunchKernelGGL(k_phase_batch, g, dim3(256), 0, st, static_cast<const PhaseArgs *>(b.args), b.off, b.M);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_update(const UpdateArgs &a, hipStream_t st)

@@ -34,6 +34,14 @@ struct ApplyArgs {
     int mode;            // ApplyMode, set by the launcher
 };
 
+// exp(i m theta) X into Xt (double2, or float2 when f32), nbx blocks per mode
+struct PhaseArgs {
+    const double2 *X;
+    void *Xt;
+    int N, lmax, nbx, f32;
+    const gmd::StepParams *P;
+};
+
 struct FactorArgs {
     int N, K, lmax;
     double dr, eps, mu, off, half_dt;
@@ -153,9 +161,34 @@ hipError_t sink_scratch_address(SinkScratch **out);
 // y = c y over n complex values
 hipError_t launch_cscale(double2 *y, size_t n, double2 c, hipStream_t st);
 
+// One launch over many solvers (a batched ragged launch): member m's
+// arguments are args[m] (device memory) and it owns the blocks
+// [off[m], off[m + 1]) of a grid of `total` blocks. Kernels and their block
+// counts are those of the plain launches; a member with no blocks sits out.
+enum BatchKernel {
+    BK_APPLY = 0, BK_PRECOND, BK_UPDATE, BK_INIT, BK_ACCEPT, BK_SINK, BK_WCORR, BK_DIAG_APPLY, BK_DIAG_FINAL, BK_PHASE
+};
+struct BatchLaunch {
+    int kernel;        // BatchKernel
+    const void *args;  // device array of the kernel's argument struct, one per member
+    const int *off;    // device array of M + 1 block offsets
+    int M, total;
+    int ng, mixed;     // BK_APPLY: tile and precision of every member of the launch
+    int var, f32;      // BK_PRECOND: variant and sweep precision
+};
+hipError_t launch_batch(const BatchLaunch &b, hipStream_t st);
+// Blocks of the plain launches, for laying out batched ones.
+int precond_blocks(int K);
+int stream_blocks(size_t n);
+int phase_blocks(int N, int K);
+int diag_apply_blocks(int N, int K);
+// kr[m]->done, ->restart and ->iter into out[3 m .. 3 m + 2]
+hipError_t launch_krylov_gather(gmd::Krylov *const *kr, int *out, int M, hipStream_t st);
+
 // grid of the operator apply for tiles of 16 ng nodes (ng = 1, 2 or 4)
 int apply_grid(int N, int nchunks, int ng, int *ngroups);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
+PhaseArgs phase_args(const ApplyArgs &a);
 hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
                           double mu, double prf, double *E, float *E32, hipStream_t st);
 hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hipStream_t st);

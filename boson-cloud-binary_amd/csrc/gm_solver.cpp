@@ -757,7 +757,7 @@ void factorize(gm_solver &S)
     S.fac_valid = true;
 }
 
-void sink_proj(gm_solver &S, const double2 *X, int dest, double scale, int by_nrm)
+gmk::SinkArgs sink_proj_args(gm_solver &S, const double2 *X, int dest, double scale, int by_nrm)
 {
     gmk::SinkArgs a{};
     a.N = S.Nr;
@@ -770,11 +770,33 @@ void sink_proj(gm_solver &S, const double2 *X, int dest, double scale, int by_nr
     fill_sinks(S, a.sk);
     a.Kr = S.dK.p;
     a.P = S.dP.p;
+    return a;
+}
+
+void sink_proj(gm_solver &S, const double2 *X, int dest, double scale, int by_nrm)
+{
+    const gmk::SinkArgs a = sink_proj_args(S, X, dest, scale, by_nrm);
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_sink(a, S.st); });
 }
 
-// Woodbury columns W_a = P^-1 v_a and the capacitance (matrices.cpp:133-145).
-void woodbury_setup(gm_solver &S)
+// The capacitance G = diag(1/((dt/4) g)) + V^H W and its inverse, every step
+// (the stored columns are rotated to the step's angle).
+gmk::SinkArgs woodbury_setup_args(gm_solver &S)
+{
+    gmk::SinkArgs a{};
+    a.N = S.Nr;
+    a.K = S.K;
+    a.mode = gmk::SK_SETUP;
+    a.lmax = S.l_max;
+    fill_sinks(S, a.sk);
+    a.Kr = S.dK.p;
+    a.P = S.dP.p;
+    return a;
+}
+
+// Woodbury columns W_a = P^-1 v_a (matrices.cpp:133-145), solved again only
+// when P or an absorber support changed.
+void woodbury_columns(gm_solver &S)
 {
     const gmd::StepParams &P = *S.hP.p;
     gmk::PrecArgs pa = prec_args(S);
@@ -800,19 +822,18 @@ void woodbury_setup(gm_solver &S)
         }
         S.w_valid = true;
     }
-    gmk::SinkArgs a{};
-    a.N = S.Nr;
-    a.K = S.K;
-    a.mode = gmk::SK_SETUP;
-    a.lmax = S.l_max;
-    fill_sinks(S, a.sk);
-    a.Kr = S.dK.p;
-    a.P = S.dP.p;
+}
+
+// Woodbury columns and capacitance (matrices.cpp:133-145).
+void woodbury_setup(gm_solver &S)
+{
+    woodbury_columns(S);
+    const gmk::SinkArgs a = woodbury_setup_args(S);
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_sink(a, S.st); });
 }
 
 // y -= sum_a c_a W_a over the columns' support (matrices.cpp:151)
-void woodbury_correct(gm_solver &S, double2 *Y, void *Xt = nullptr, int xt_f32 = 0)
+gmk::WcorrArgs wcorr_args(gm_solver &S, double2 *Y, void *Xt, int xt_f32)
 {
     const gmd::StepParams &P = *S.hP.p;
     gmk::WcorrArgs w{};
@@ -827,50 +848,54 @@ void woodbury_correct(gm_solver &S, double2 *Y, void *Xt = nullptr, int xt_f32 =
     w.xt_f32 = xt_f32;
     w.P = S.dP.p;
     w.lmax = S.l_max;
+    return w;
+}
+
+void woodbury_correct(gm_solver &S, double2 *Y, void *Xt = nullptr, int xt_f32 = 0)
+{
+    const gmk::WcorrArgs w = wcorr_args(S, Y, Xt, xt_f32);
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_wcorr(w, S.st); });
 }
 
-void enqueue_iteration(gm_solver &S, bool wood)
+// Arguments of the launches of one BiCGSTAB iteration (matrices.cpp:62-89):
+// y = M^-1 p with p = r + beta (p - w v), v = A y, z = M^-1 s with s = r -
+// alpha v, t = A z, then the x and r updates. Mixed precision: the Thomas
+// sweeps in FP32 on FP32 pivots and the coupling of the two applies in FP32;
+// rhs, residual check and diagnostics keep the FP64 operator. The
+// preconditioner (and the Woodbury correction) also write the phased copy
+// exp(i m theta) y that the next apply's coupling reads.
+gmk::PrecArgs iter_prec_args(gm_solver &S, bool wood, int var)
 {
     gmk::PrecArgs pa = prec_args(S);
     pa.wood = wood;
-    if (S.flags.precision == GM_PRECISION_MIXED) {
-        // mixed precision: the Thomas sweeps in FP32 on FP32 pivots
+    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
+    if (mixed) {
         pa.inv32 = S.inv32.p;
         pa.Y32 = S.y32.p;
     }
-    // mixed precision: the coupling of the two Krylov applies runs in FP32;
-    // rhs, residual check and diagnostics keep the FP64 operator
-    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
-    // the preconditioner (and the Woodbury correction) also write the phased
-    // copy exp(i m theta) y that the next apply's coupling reads
     pa.Xt = S.xt.p;
     pa.xt_f32 = mixed;
-    // y = M^-1 p with p = r + beta (p - w v)
     pa.R = S.rr.p;
     pa.R0 = S.r0.p;
-    pa.Pv = S.p.p;
     pa.V = S.v.p;
-    pa.Y = S.y.p;
-    launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_P, S.st); });
-    if (wood) woodbury_correct(S, S.y.p, S.xt.p, mixed);
-    gmk::ApplyArgs a = apply_args(S, S.y.p, S.v.p);
-    a.R0 = S.r0.p;
+    pa.Pv = var == gmk::PV_P ? S.p.p : S.s.p;
+    pa.Y = var == gmk::PV_P ? S.y.p : S.z.p;
+    return pa;
+}
+
+gmk::ApplyArgs iter_apply_args(gm_solver &S, int mode)
+{
+    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
+    gmk::ApplyArgs a = mode == gmk::AM_V ? apply_args(S, S.y.p, S.v.p) : apply_args(S, S.z.p, S.t.p);
+    a.R0 = mode == gmk::AM_V ? S.r0.p : S.s.p;
     a.mixed = mixed;
     a.xt_ready = 1;
     if (mixed && S.apply_ng[1]) a.ng = S.apply_ng[1];
-    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_V, S.st); });
-    // z = M^-1 s with s = r - alpha v
-    pa.Pv = S.s.p;
-    pa.Y = S.z.p;
-    launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pa, gmk::PV_S, S.st); });
-    if (wood) woodbury_correct(S, S.z.p, S.xt.p, mixed);
-    a = apply_args(S, S.z.p, S.t.p);
-    a.R0 = S.s.p;
-    a.mixed = mixed;
-    a.xt_ready = 1;
-    if (mixed && S.apply_ng[1]) a.ng = S.apply_ng[1];
-    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(a, gmk::AM_T, S.st); });
+    return a;
+}
+
+gmk::UpdateArgs update_args(gm_solver &S)
+{
     gmk::UpdateArgs u{};
     u.n = S.nvec();
     u.Kr = S.dK.p;
@@ -882,6 +907,23 @@ void enqueue_iteration(gm_solver &S, bool wood)
     u.R0 = S.r0.p;
     u.R = S.rr.p;
     u.partial = S.partial.p;
+    return u;
+}
+
+void enqueue_iteration(gm_solver &S, bool wood)
+{
+    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
+    const gmk::PrecArgs pp = iter_prec_args(S, wood, gmk::PV_P);
+    launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pp, gmk::PV_P, S.st); });
+    if (wood) woodbury_correct(S, S.y.p, S.xt.p, mixed);
+    const gmk::ApplyArgs av = iter_apply_args(S, gmk::AM_V);
+    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(av, gmk::AM_V, S.st); });
+    const gmk::PrecArgs ps = iter_prec_args(S, wood, gmk::PV_S);
+    launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(ps, gmk::PV_S, S.st); });
+    if (wood) woodbury_correct(S, S.z.p, S.xt.p, mixed);
+    const gmk::ApplyArgs at = iter_apply_args(S, gmk::AM_T);
+    launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(at, gmk::AM_T, S.st); });
+    const gmk::UpdateArgs u = update_args(S);
     launch(S, GM_PROF_UPDATE, [&] { return gmk::launch_update(u, S.st); });
 }
 
@@ -1017,33 +1059,40 @@ void solve_begin_work(gm_solver &S, int batch)
     HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
 }
 
+// rhs = B psi - (dt/4) Gamma psi and the raw shadow residual rhs - A psi in
+// one apply (b stays unnormalised in S.b; the acceptance divides).
+gmk::ApplyArgs rhs_apply_args(gm_solver &S)
+{
+    gmk::ApplyArgs a = apply_args(S, S.psi.p, S.b.p);
+    a.Y2 = S.r0.p;
+    return a;
+}
+
+// Eigen's BiCGSTAB without sinks, the reference's own recurrence with them.
+gmk::InitArgs init_args(gm_solver &S, bool wood)
+{
+    gmk::InitArgs ia{};
+    ia.n = S.nvec();
+    ia.Kr = S.dK.p;
+    ia.Psi = S.psi.p;
+    ia.X = S.x.p;
+    ia.tol = 1e-10;
+    ia.maxit = kMaxIt;
+    ia.eigen_mode = wood ? 0 : 1;
+    ia.rst_eps2 = S.rst_eps2;
+    ia.rst_ref = S.rst_ref;
+    return ia;
+}
+
 // Right-hand side, its norm and the BiCGSTAB start (b, x, r, r0 normalised).
 void rhs_and_init(gm_solver &S)
 {
     const bool wood = S.step_wood;
-    gmd::Krylov *dk = S.dK.p;
-    // rhs = B psi - (dt/4) Gamma psi and the raw shadow residual rhs - A psi
-    // in one apply (b stays unnormalised in S.b; the acceptance divides)
     if (wood) sink_proj(S, S.psi.p, gmk::SD_START, 1.0, 0);
-    {
-        gmk::ApplyArgs a = apply_args(S, S.psi.p, S.b.p);
-        a.Y2 = S.r0.p;
-        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RHS, S.st); });
-    }
-    // Eigen's BiCGSTAB without sinks, the reference's own recurrence with them
-    {
-        gmk::InitArgs ia{};
-        ia.n = S.nvec();
-        ia.Kr = dk;
-        ia.Psi = S.psi.p;
-        ia.X = S.x.p;
-        ia.tol = 1e-10;
-        ia.maxit = kMaxIt;
-        ia.eigen_mode = wood ? 0 : 1;
-        ia.rst_eps2 = S.rst_eps2;
-        ia.rst_ref = S.rst_ref;
-        launch(S, GM_PROF_SMALL, [&] { return gmk::launch_init(ia, S.st); });
-    }
+    const gmk::ApplyArgs a = rhs_apply_args(S);
+    launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RHS, S.st); });
+    const gmk::InitArgs ia = init_args(S, wood);
+    launch(S, GM_PROF_SMALL, [&] { return gmk::launch_init(ia, S.st); });
 }
 
 void solve_tail_work(gm_solver &S);
@@ -1085,58 +1134,82 @@ bool solve_advance(gm_solver &S)
     return true;
 }
 
-void solve_tail_work(gm_solver &S)
+// The acceptance apply: |A_full x - b| with the rank-one sinks of x (their
+// coefficients from one projection of x), keeping V x for the diagnostics.
+gmk::ApplyArgs res_apply_args(gm_solver &S)
 {
-    const gmd::StepParams &P = *S.hP.p;
-    const bool wood = S.step_wood;
-    gmd::Krylov *dk = S.dK.p;
-    // acceptance: |A_full x - b| with the rank-one sinks of x; one projection
-    // of x gives their coefficients and |rhs| v^H x for the absorbed mass
-    if (wood) sink_proj(S, S.x.p, gmk::SD_COEFA_QNEW, 1.0, 0);
-    {
-        gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
-        a.R0 = S.b.p;
-        a.r0_raw = 1;  // b = rhs / |rhs|
-        a.Cout = S.tmp.p;  // V x, reused by the diagnostics of the accepted state
-        launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_RES, S.st); });
-    }
+    gmk::ApplyArgs a = apply_args(S, S.x.p, nullptr);
+    a.R0 = S.b.p;
+    a.r0_raw = 1;  // b = rhs / |rhs|
+    a.Cout = S.tmp.p;  // V x, reused by the diagnostics of the accepted state
+    return a;
+}
+
+gmk::AcceptArgs accept_args(gm_solver &S)
+{
     gmk::AcceptArgs ac{};
     ac.n = S.nvec();
     ac.N = S.Nr;
-    ac.Kr = dk;
+    ac.Kr = S.dK.p;
     ac.X = S.x.p;
     ac.mask = S.mask ? S.dmask.p : nullptr;
     ac.Psi = S.psi.p;
     ac.status = S.dflags.p + 1;
     ac.resid = S.dresid.p;
+    return ac;
+}
+
+// Diagnostics on the post-mask state (extraction.cpp), with Gamma psi. V psi
+// = |rhs| mask V x: the coupling is node-local and the mask is the same for
+// every mode, so only the stencil is recomputed.
+gmk::ApplyArgs diag_apply_args(gm_solver &S)
+{
+    gmk::ApplyArgs a = apply_args(S, S.psi.p, nullptr);
+    a.Cin = S.tmp.p;
+    a.mask = S.mask ? S.dmask.p : nullptr;
+    return a;
+}
+
+gmk::DiagArgs diag_final_args(gm_solver &S)
+{
+    gmk::DiagArgs da{};
+    da.N = S.Nr;
+    da.K = S.K;
+    da.ix = S.idx_ex;
+    da.ntiles = (S.Nr + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES;  // the diagnostics apply's tiles
+    da.dr = S.dr;
+    da.eps = S.epsilon;
+    da.mu = S.mu;
+    da.Psi = S.psi.p;
+    da.dpsi_ix = S.dpsi_ix.p;
+    da.rowpart = S.rowpart.p;
+    da.out = S.diagout.p;
+    return da;
+}
+
+// The step's results back to the host: diagnostics, flags, residual, Krylov state.
+void tail_readback(gm_solver &S, hipStream_t st)
+{
+    HIPCK(hipMemcpyAsync(S.hdiag.p, S.diagout.p, sizeof(double) * (8 * S.K), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(S.hflags.p, S.dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(S.hresid.p, S.dresid.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, st));
+}
+
+void solve_tail_work(gm_solver &S)
+{
+    const gmd::StepParams &P = *S.hP.p;
+    if (S.step_wood) sink_proj(S, S.x.p, gmk::SD_COEFA_QNEW, 1.0, 0);
+    const gmk::ApplyArgs ar = res_apply_args(S);
+    launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(ar, gmk::AM_RES, S.st); });
+    const gmk::AcceptArgs ac = accept_args(S);
     launch(S, GM_PROF_SMALL, [&] { return gmk::launch_accept(ac, S.st); });
-    // diagnostics on the post-mask state (extraction.cpp), with Gamma psi
     if (P.nsink) sink_proj(S, S.psi.p, gmk::SD_POST, 1.0, 0);
-    {
-        gmk::ApplyArgs a = apply_args(S, S.psi.p, nullptr);
-        // V psi = |rhs| mask V x: the coupling is node-local and the mask is
-        // the same for every mode, so only the stencil is recomputed
-        a.Cin = S.tmp.p;
-        a.mask = S.mask ? S.dmask.p : nullptr;
-        launch(S, GM_PROF_DIAG, [&] { return gmk::launch_diag_apply(a, S.st); });
-        gmk::DiagArgs da{};
-        da.N = S.Nr;
-        da.K = S.K;
-        da.ix = S.idx_ex;
-        da.ntiles = (S.Nr + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES;  // the diagnostics apply's tiles
-        da.dr = S.dr;
-        da.eps = S.epsilon;
-        da.mu = S.mu;
-        da.Psi = S.psi.p;
-        da.dpsi_ix = S.dpsi_ix.p;
-        da.rowpart = S.rowpart.p;
-        da.out = S.diagout.p;
-        launch(S, GM_PROF_DIAG, [&] { return gmk::launch_diag_final(da, S.st); });
-    }
-    HIPCK(hipMemcpyAsync(S.hdiag.p, S.diagout.p, sizeof(double) * (8 * S.K), hipMemcpyDeviceToHost, S.st));
-    HIPCK(hipMemcpyAsync(S.hflags.p, S.dflags.p, 2 * sizeof(int), hipMemcpyDeviceToHost, S.st));
-    HIPCK(hipMemcpyAsync(S.hresid.p, S.dresid.p, sizeof(double), hipMemcpyDeviceToHost, S.st));
-    HIPCK(hipMemcpyAsync(S.hK.p, S.dK.p, sizeof(gmd::Krylov), hipMemcpyDeviceToHost, S.st));
+    const gmk::ApplyArgs ad = diag_apply_args(S);
+    launch(S, GM_PROF_DIAG, [&] { return gmk::launch_diag_apply(ad, S.st); });
+    const gmk::DiagArgs da = diag_final_args(S);
+    launch(S, GM_PROF_DIAG, [&] { return gmk::launch_diag_final(da, S.st); });
+    tail_readback(S, S.st);
 }
 
 void alloc_device(gm_solver &S)
@@ -2491,3 +2564,6 @@ double gm_sink_strength(const gm_solver *S, int32_t body)
 
 // serving loop (gm_server_*): same translation unit, it drives the step phases
 #include "gm_server.inl"
+// batched ragged ensemble (gm_batch_*): same translation unit, it reuses the
+// step's argument builders
+#include "gm_batch.inl"

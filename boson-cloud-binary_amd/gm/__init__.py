@@ -153,3 +153,63 @@ def ensemble_step(members, step_id=1, diag=True):
         d.update(arrs[i])
         out.append(d)
     return out
+
+
+class Batch:
+    """gm_batch: the members stepped together, every launch of the step one
+    ragged grid over all of them (each member's result bitwise its own
+    step's). Members stay usable alone between batch steps."""
+
+    def __init__(self, members):
+        L = lib()
+        L.gm_batch_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        L.gm_batch_step.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.gm_batch_destroy.argtypes = [C.c_void_p]
+        L.gm_batch_destroy.restype = None
+        L.gm_batch_last_error.argtypes = [C.c_void_p]
+        L.gm_batch_last_error.restype = C.c_char_p
+        self._lib = L
+        self.members = list(members)
+        n = len(self.members)
+        self._handles = (C.c_void_p * n)(*[m._h for m in self.members])
+        h = C.c_void_p()
+        rc = L.gm_batch_create(self._handles, n, C.byref(h))
+        if rc != 0:
+            raise capi.GMError(rc, L.gm_last_error(None).decode())
+        self._h = h
+
+    def step(self, step_id=1, diag=True):
+        """One step of every member; diagnostics dicts as CrankNicolsonModes.step."""
+        import numpy as np
+        n = len(self.members)
+        arrs, diags = [], None
+        if diag:
+            diags = (capi.Diag * n)()
+            for i, m in enumerate(self.members):
+                a = {k: np.zeros(m.n_modes) for k in ("flux", "energy_flux", "norms", "energies")}
+                for k, v in a.items():
+                    setattr(diags[i], k, capi._ptr(v))
+                arrs.append(a)
+        st = self._lib.gm_batch_step(self._h, int(step_id), C.cast(diags, C.c_void_p) if diags else None)
+        if st != 0:
+            raise capi.GMError(st, self._lib.gm_batch_last_error(self._h).decode())
+        if not diag:
+            return [None] * n
+        out = []
+        for i in range(n):
+            d = {k: getattr(diags[i], k) for k in ("energy", "p_bin", "p_bin_inst", "dV_bin_dt", "cloud_mass",
+                                                    "absorbed")}
+            d.update(arrs[i])
+            out.append(d)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.gm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,21 +1,23 @@
 """Batched ragged ensemble: many independent fixed-orbit binaries stepped
-together by gm_ensemble_step (each member on its own HIP stream, captured
-graphs replayed per member, apply tile autotuned per shape).
+together, either as one ragged grid per launch of the step over all members
+(gm_batch, graph-captured) or with each member on its own HIP stream
+(gm_ensemble_step); the apply tile is autotuned per shape.
 
 Members are drawn from one seed: mass ratio log-uniform in [0.05, 1],
 separation uniform in [20, 60], boson mass uniform in [0.1, 0.3], N from
 {4096, 8192, 16384, 32768} with L = N / 4, l_max from {8, 12, 16, 20, 24},
 dt = 0.25, absorbers on, damping mask (0.85, 1, 6), r_ex = 0.8 L. Initial
-state: the 2p atom at the origin plus a seeded 1e-3 perturbation, one per
-(N, l_max) shape.
+state of each member: the hydrogenic 2p atom (n=2, l=1, m=1) of its own boson
+mass on the primary, translated to the centre of mass, plus a 1e-3 complex
+Gaussian perturbation seeded by the member's index.
 """
 import math
 import time
 
 import numpy as np
 
-from . import GM_PRECISION_MIXED, CrankNicolsonModes, ensemble_step
-from .serve import atom_state
+from . import GM_PRECISION_MIXED, Batch, CrankNicolsonModes, ensemble_step
+from . import initial
 
 
 def draw_members(n, seed=0):
@@ -32,15 +34,34 @@ def draw_members(n, seed=0):
     return out
 
 
-def build(members, precision=GM_PRECISION_MIXED):
-    solvers, states = [], {}
-    for p in members:
-        shape = (p["Nr"], p["l_max"])
-        if shape not in states:
-            states[shape] = atom_state(p["Nr"], p["L"], p["l_max"], p["mu"], len(states))
+_STATES = {}
+
+
+def member_state(p, index):
+    """The member's 2p atom on the primary (offset q/(1+q) R from the centre
+    of mass) plus the perturbation seeded by its index (kept for the process:
+    the quadrature of the translated atom is the slow part of a build)."""
+    key = (index,) + tuple(sorted(p.items()))
+    if key not in _STATES:
+        _STATES[key] = _member_state(p, index)
+    return _STATES[key]
+
+
+def _member_state(p, index):
+    dr = p["L"] / (p["Nr"] - 1)
+    r = p["epsilon"] + dr * np.arange(p["Nr"])
+    base = initial.translated_atom(r, p["l_max"], p["mu"], p["q"] / (1 + p["q"]) * p["R"])
+    base /= math.sqrt(np.vdot(base, base).real * dr)
+    K = (p["l_max"] + 1) * (p["l_max"] + 2) // 2
+    return initial.perturbed_state(base, p["Nr"], K, rel=1e-3, seed=index, dr=dr)
+
+
+def build(members, precision=GM_PRECISION_MIXED, states=None):
+    solvers = []
+    for i, p in enumerate(members):
         s = CrankNicolsonModes(**p)
         s.set_flags(evolve_orbit=0, precision=precision)
-        s.set_initial_state_full(states[shape])
+        s.set_initial_state_full(states[i] if states is not None else member_state(p, i))
         s.initialize()
         s.build_damping_mask(0.85, 1.0, 6)
         s.begin_run(0.8 * p["L"], 100000)
@@ -48,26 +69,38 @@ def build(members, precision=GM_PRECISION_MIXED):
     return solvers
 
 
-def run(n=64, steps=10, warmup=3, seed=0, precision=GM_PRECISION_MIXED):
-    """Aggregate mode.node-steps/s of `steps` ensemble steps after `warmup`."""
+def run(n=64, steps=10, warmup=3, seed=0, precision=GM_PRECISION_MIXED, mode="batch"):
+    """Aggregate mode.node-steps/s of `steps` ensemble steps after `warmup`;
+    mode "batch" (one ragged grid per launch, gm_batch) or "streams" (a
+    stream per member, gm_ensemble_step)."""
     members = draw_members(n, seed)
     t0 = time.perf_counter()
     solvers = build(members, precision)
     t_build = time.perf_counter() - t0
+    batch = Batch(solvers) if mode == "batch" else None
+
+    def one(k):
+        if batch is not None:
+            batch.step(k, diag=False)
+        else:
+            ensemble_step(solvers, k, diag=False)
+
     k = 0
     for _ in range(warmup):
         k += 1
-        ensemble_step(solvers, k, diag=False)
+        one(k)
     t0 = time.perf_counter()
     for _ in range(steps):
         k += 1
-        ensemble_step(solvers, k, diag=False)
+        one(k)
     el = time.perf_counter() - t0
     work = sum(s.n_modes * s.Nr for s in solvers) * steps
-    res = {"members": n, "steps": steps, "warmup": warmup, "seconds": el, "build_seconds": t_build,
+    res = {"mode": mode, "members": n, "steps": steps, "warmup": warmup, "seconds": el, "build_seconds": t_build,
            "mode_node_steps_per_s": work / el, "ms_per_ensemble_step": el / steps * 1e3,
            "mode_nodes_per_step": work / steps, "iterations_per_step": float(np.mean(
                [s.last_solver_iterations for s in solvers]))}
+    if batch is not None:
+        batch.close()
     for s in solvers:
         s.close()
     return res

@@ -154,6 +154,21 @@ int gm_step(gm_solver *solver, int32_t step_id, gm_diag *diag);
  * gm_last_error and the others complete their step. A solver may appear once. */
 int gm_ensemble_step(gm_solver *const *members, int32_t n, int32_t step_id, gm_diag *diags);
 
+/* Batched ragged ensemble (no counterpart in the reference): n independent
+ * solvers (any q, R, mu, N, l_max, precision) stepped together, every launch
+ * of the CN step being one grid over all members, each member owning the
+ * blocks its own launch would use, so each member's result is bitwise that of
+ * its own gm_step (at a fixed apply tile). The step is captured in HIP graphs
+ * and replayed while the members' launch arguments repeat. The members stay
+ * usable on their own between batch steps; a solver may be in one batch. A
+ * member that fails keeps its message in gm_last_error; gm_batch_step
+ * returns the first failing status. */
+typedef struct gm_batch gm_batch;
+int gm_batch_create(gm_solver *const *members, int32_t n, gm_batch **out);
+void gm_batch_destroy(gm_batch *batch);
+int gm_batch_step(gm_batch *batch, int32_t step_id, gm_diag *diags);
+const char *gm_batch_last_error(const gm_batch *batch);
+
 /* psi (include/CrankNicolsonModes.hpp:162): read, or overwrite without scaling. */
 int gm_get_state(const gm_solver *solver, double *psi);
 int gm_set_state(gm_solver *solver, const double *psi);

@@ -1,6 +1,10 @@
-"""Ensemble step (gm_ensemble_step): independent solvers of different shapes
-stepped together on their own streams give exactly the states they give when
-stepped alone (the kernels are deterministic and members share nothing)."""
+"""Ensembles: gm_ensemble_step (each member on its own stream) and gm_batch
+(every launch of the step one ragged grid over all members, graph-captured):
+independent solvers of different shapes stepped together give exactly the
+states they give when stepped alone (the kernels are deterministic, members
+share nothing, and a member's blocks in a batched grid are those of its own
+launch). The batched ensemble of 64 binaries drawn from seed 0 is checked
+member by member, and its smallest member against the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -100,3 +104,72 @@ def test_apply_tile_flag_is_validated():
     s = make(MEMBERS[2])
     with pytest.raises(gm.InvalidArgument):
         s.set_flags(apply_tile=48)
+
+
+def test_batch_equals_members_alone():
+    print("batched ensemble of 3 ragged members, 12 steps", flush=True)
+    together = [make(m) for m in MEMBERS]
+    alone = [make(m) for m in MEMBERS]
+    batch = gm.Batch(together)
+    for n in range(1, 13):
+        dt = batch.step(n)
+        da = [s.step(n) for s in alone]
+        for t, a in zip(dt, da):
+            for k in ("energy", "p_bin_inst", "cloud_mass", "absorbed", "dV_bin_dt"):
+                assert t[k] == a[k], (n, k)
+            for k in ("flux", "norms", "energies", "energy_flux"):
+                assert np.array_equal(t[k], a[k]), (n, k)
+    for t, a in zip(together, alone):
+        assert np.array_equal(t.psi, a.psi)
+        assert t.last_solver_iterations == a.last_solver_iterations
+    # members step alone between batch steps, and the batch goes on
+    together[0].step(13)
+    alone[0].step(13)
+    batch.step(14)
+    for a in alone:
+        a.step(14)
+    for t, a in zip(together, alone):
+        assert np.array_equal(t.psi, a.psi)
+    batch.close()
+
+
+@pytest.mark.timeout(900)
+def test_config4_batched_ensemble_64_members():
+    """The 64 binaries of the batched ensemble from seed 0 (q log-uniform in
+    [0.05, 1], R in [20, 60], mu in [0.1, 0.3], N from 4096 to 32768 with
+    L = N/4, l_max from 8 to 24, the 2p atom on the primary plus a 1e-3
+    perturbation, mixed precision): three batched steps equal each member's
+    own three steps bitwise, and the smallest member's FP64 twin, batched the
+    same way, matches the oracle to 1e-10."""
+    from gm import ensemble
+    print("config 4: 64 batched members", flush=True)
+    members = ensemble.draw_members(64, seed=0)
+    states = [ensemble.member_state(p, i) for i, p in enumerate(members)]
+    batched = ensemble.build(members, states=states)
+    print("  built", flush=True)
+    small = min(range(64), key=lambda i: members[i]["Nr"] * (members[i]["l_max"] + 1) ** 2)
+    p = members[small]
+    psi0 = states[small]
+    twin = gm.CrankNicolsonModes(**p)
+    o = oracle.Oracle(**p)
+    for s in (twin, o):
+        s.set_flags(evolve_orbit=0)
+        s.set_initial_state_full(psi0)
+        s.initialize()
+        s.build_damping_mask(0.85, 1.0, 6)
+        s.begin_run(0.8 * p["L"], 100000)
+    batch = gm.Batch(batched + [twin])
+    for n in range(1, 4):
+        batch.step(n, diag=False)
+        o.step(n, diag=False)
+    e = np.linalg.norm(twin.psi - o.psi) / np.linalg.norm(o.psi)
+    print(f"  FP64 twin of member {small} (N {p['Nr']}, l_max {p['l_max']}) vs oracle {e:.2e}", flush=True)
+    assert e < 1e-10
+    for i, s in enumerate(batched):
+        solo = ensemble.build([members[i]], states=[states[i]])[0]
+        for n in range(1, 4):
+            solo.step(n, diag=False)
+        assert np.array_equal(s.psi, solo.psi), i
+        assert s.last_solver_iterations == solo.last_solver_iterations, i
+        solo.close()
+    batch.close()
