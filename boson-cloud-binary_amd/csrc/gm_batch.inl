@@ -54,7 +54,9 @@ void add_seg(std::vector<std::unique_ptr<BatchSeg>> &out, int kernel, const std:
 
 }  // namespace
 
-struct gm_batch {
+// One lane of a batch: a subset of the members stepped as ragged grids on
+// the lane's own stream.
+struct BatchLane {
     std::vector<gm_solver *> mem;
     hipStream_t st = nullptr;
     // the batched step: begin (rhs and BiCGSTAB start), one iteration, tail
@@ -70,18 +72,28 @@ struct gm_batch {
     };
     std::vector<Graph> graphs;
     std::vector<uint64_t> last_key[3];
-    std::string err;
-    ~gm_batch()
+    // progress of the step in flight: 1 iterating, 2 tail enqueued, 3 done
+    int phase = 0, enqueued = 0;
+    bool capture_ok = false;
+    ~BatchLane()
     {
         for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
         if (st) (void)hipStreamDestroy(st);
     }
 };
 
+// Lanes overlap on the device: while one lane's grid of a memory-bound launch
+// runs, another lane's matrix-core apply can fill the chip.
+struct gm_batch {
+    std::vector<gm_solver *> mem;
+    std::vector<std::unique_ptr<BatchLane>> lanes;
+    std::string err;
+};
+
 namespace {
 
 // Everything the plan's launch arguments depend on, per member.
-std::vector<uint64_t> batch_key(gm_batch &B)
+std::vector<uint64_t> batch_key(BatchLane &B)
 {
     std::vector<uint64_t> k;
     for (gm_solver *S : B.mem) {
@@ -134,7 +146,7 @@ void add_phase_seg(std::vector<std::unique_ptr<BatchSeg>> &out, const std::vecto
     add_seg(out, gmk::BK_PHASE, pa, b);
 }
 
-void build_plan(gm_batch &B)
+void build_plan(BatchLane &B)
 {
     const int M = static_cast<int>(B.mem.size());
     B.begin.clear();
@@ -268,12 +280,12 @@ void build_plan(gm_batch &B)
     }
 }
 
-void run_segs(gm_batch &B, const std::vector<std::unique_ptr<BatchSeg>> &segs)
+void run_segs(BatchLane &B, const std::vector<std::unique_ptr<BatchSeg>> &segs)
 {
     for (const auto &s : segs) HIPCK(gmk::launch_batch(s->L, B.st));
 }
 
-void batch_gather(gm_batch &B)
+void batch_gather(BatchLane &B)
 {
     const int M = static_cast<int>(B.mem.size());
     HIPCK(gmk::launch_krylov_gather(B.kr.p, B.kstate.p, M, B.st));
@@ -285,7 +297,7 @@ enum { BG_BEGIN = 0, BG_BATCH = 1, BG_TAIL = 2 };
 // Enqueue `work` on the batch stream, through a captured graph once the same
 // plan and batch size were requested twice in a row.
 template <class F>
-void batch_captured(gm_batch &B, int kind, int batch, bool capture_ok, F &&work)
+void batch_captured(BatchLane &B, int kind, int batch, bool capture_ok, F &&work)
 {
     std::vector<uint64_t> key = B.plan_key;
     key.push_back(static_cast<uint64_t>(batch));
@@ -322,7 +334,7 @@ void batch_captured(gm_batch &B, int kind, int batch, bool capture_ok, F &&work)
     HIPCK(hipGraphLaunch(ex, B.st));
 }
 
-void batch_drop_graphs(gm_batch &B)
+void batch_drop_graphs(BatchLane &B)
 {
     for (auto &g : B.graphs) (void)hipGraphExecDestroy(g.exec);
     B.graphs.clear();
@@ -350,28 +362,109 @@ void batch_member_begin(gm_solver &S)
     if (S.step_wood) woodbury_columns(S);
 }
 
+
+// Enqueue the begin of a lane's step (after its members' host parts).
+void lane_begin(BatchLane &L)
+{
+    bool dirty = false, graphs_ok = true;
+    for (gm_solver *S : L.mem) {
+        dirty = dirty || S->w_dirty;
+        graphs_ok = graphs_ok && graphs_enabled(*S);
+    }
+    const std::vector<uint64_t> key = batch_key(L);
+    if (key != L.plan_key) {
+        HIPCK(hipStreamSynchronize(L.st));
+        batch_drop_graphs(L);
+        build_plan(L);
+        L.plan_key = key;
+    }
+    L.capture_ok = graphs_ok && !dirty;
+    int batch = 1;
+    for (gm_solver *S : L.mem) batch = std::max(batch, S->prev_iters);
+    batch_captured(L, BG_BEGIN, batch, L.capture_ok, [&] {
+        run_segs(L, L.begin);
+        for (int k = 0; k < batch; ++k) run_segs(L, L.iter);
+        batch_gather(L);
+    });
+    L.enqueued = batch;
+    L.phase = 1;
+}
+
+// After the lane's stream went idle: restarts, more iterations, or the tail.
+void lane_advance(BatchLane &L)
+{
+    if (L.phase == 2) {
+        L.phase = 3;
+        return;
+    }
+    bool all_done = true;
+    for (size_t m = 0; m < L.mem.size(); ++m) {
+        const int *k = L.hkstate.p + 3 * m;
+        if (k[1]) {
+            enqueue_restart(*L.mem[m]);
+            all_done = false;
+        } else if (!k[0]) {
+            all_done = false;
+        }
+    }
+    if (all_done) {
+        batch_captured(L, BG_TAIL, 0, L.capture_ok, [&] {
+            run_segs(L, L.tail);
+            for (gm_solver *S : L.mem) tail_readback(*S, L.st);
+        });
+        L.phase = 2;
+        return;
+    }
+    if (L.enqueued > 3 * kMaxIt) throw std::runtime_error("BiCGSTAB did not report convergence");
+    batch_captured(L, BG_BATCH, 2, L.capture_ok, [&] {
+        for (int k = 0; k < 2; ++k) run_segs(L, L.iter);
+        batch_gather(L);
+    });
+    L.enqueued += 2;
+}
+
 }  // namespace
 
 extern "C" {
 
-int gm_batch_create(gm_solver *const *members, int32_t n, gm_batch **out)
+int gm_batch_create_lanes(gm_solver *const *members, int32_t n, int32_t lanes, gm_batch **out)
 {
     if (!out) return GM_E_INVALID;
     *out = nullptr;
-    if (n < 1 || !members) return GM_E_INVALID;
+    if (n < 1 || !members || lanes < 1) return GM_E_INVALID;
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < i; ++k)
             if (!members[i] || members[i] == members[k]) return GM_E_INVALID;
     try {
         auto B = std::make_unique<gm_batch>();
         B->mem.assign(members, members + n);
-        HIPCK(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking));
-        B->kr.alloc(n);
-        std::vector<gmd::Krylov *> kp(n);
-        for (int i = 0; i < n; ++i) kp[i] = members[i]->dK.p;
-        HIPCK(hipMemcpy(B->kr.p, kp.data(), sizeof(gmd::Krylov *) * n, hipMemcpyHostToDevice));
-        B->kstate.alloc(3 * static_cast<size_t>(n));
-        B->hkstate.alloc(3 * static_cast<size_t>(n));
+        lanes = std::min(lanes, n);
+        // members by decreasing work, each to the lane with the least work so far
+        std::vector<int> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        auto work = [&](int i) { return static_cast<double>(members[i]->K) * members[i]->Nr; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+        std::vector<std::vector<int>> part(lanes);
+        std::vector<double> load(lanes, 0.0);
+        for (int i : order) {
+            const int l = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            part[l].push_back(i);
+            load[l] += work(i);
+        }
+        for (auto &idx : part) {
+            std::sort(idx.begin(), idx.end());
+            auto L = std::make_unique<BatchLane>();
+            for (int i : idx) L->mem.push_back(members[i]);
+            HIPCK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+            const int M = static_cast<int>(idx.size());
+            L->kr.alloc(M);
+            std::vector<gmd::Krylov *> kp(M);
+            for (int m = 0; m < M; ++m) kp[m] = L->mem[m]->dK.p;
+            HIPCK(hipMemcpy(L->kr.p, kp.data(), sizeof(gmd::Krylov *) * M, hipMemcpyHostToDevice));
+            L->kstate.alloc(3 * static_cast<size_t>(M));
+            L->hkstate.alloc(3 * static_cast<size_t>(M));
+            B->lanes.push_back(std::move(L));
+        }
         *out = B.release();
         return GM_OK;
     } catch (const std::exception &e) {
@@ -380,101 +473,79 @@ int gm_batch_create(gm_solver *const *members, int32_t n, gm_batch **out)
     }
 }
 
+int gm_batch_create(gm_solver *const *members, int32_t n, gm_batch **out)
+{
+    return gm_batch_create_lanes(members, n, 1, out);
+}
+
 void gm_batch_destroy(gm_batch *B) { delete B; }
 
 const char *gm_batch_last_error(const gm_batch *B) { return B ? B->err.c_str() : ""; }
 
 int gm_batch_step(gm_batch *B, int32_t step_id, gm_diag *diags)
 {
-    (void)step_id;
     if (!B) return GM_E_INVALID;
     const int M = static_cast<int>(B->mem.size());
-    // members on the "ilut" key take the per-member path after the batch
-    std::vector<gm_solver *> batched, single;
-    std::vector<int> idx;
-    for (int m = 0; m < M; ++m) {
-        if (B->mem[m]->flags.use_block_preconditioner) {
-            batched.push_back(B->mem[m]);
-            idx.push_back(m);
-        } else {
-            single.push_back(B->mem[m]);
+    bool ilut = false;
+    for (gm_solver *S : B->mem) ilut = ilut || !S->flags.use_block_preconditioner;
+    int first = GM_OK;
+    if (ilut) {
+        // a member on the host ILUT path: the whole step per member
+        for (int m = 0; m < M; ++m) {
+            gm_solver *S = B->mem[m];
+            const int st = gm_ensemble_step(&S, 1, step_id, diags ? diags + m : nullptr);
+            if (first == GM_OK && st != GM_OK) first = st;
         }
+        return first;
     }
     std::vector<hipStream_t> own(M);
     for (int m = 0; m < M; ++m) own[m] = B->mem[m]->st;
-    int first = GM_OK;
+    auto restore = [&] {
+        for (int m = 0; m < M; ++m) B->mem[m]->st = own[m];
+    };
     try {
-        if (single.empty()) {
-            // every member's stream is the batch stream for the step
-            for (gm_solver *S : B->mem) S->st = B->st;
-            for (gm_solver *S : B->mem) batch_member_begin(*S);
-            bool dirty = false, graphs_ok = true;
-            for (gm_solver *S : B->mem) {
-                dirty = dirty || S->w_dirty;
-                graphs_ok = graphs_ok && graphs_enabled(*S);
+        // every member's work of the step goes to its lane's stream
+        for (auto &L : B->lanes)
+            for (gm_solver *S : L->mem) {
+                S->st = L->st;
+                batch_member_begin(*S);
             }
-            const std::vector<uint64_t> key = batch_key(*B);
-            if (key != B->plan_key) {
-                HIPCK(hipStreamSynchronize(B->st));
-                batch_drop_graphs(*B);
-                build_plan(*B);
-                B->plan_key = key;
-            }
-            const bool capture_ok = graphs_ok && !dirty;
-            int batch = 1;
-            for (gm_solver *S : B->mem) batch = std::max(batch, S->prev_iters);
-            batch_captured(*B, BG_BEGIN, batch, capture_ok, [&] {
-                run_segs(*B, B->begin);
-                for (int k = 0; k < batch; ++k) run_segs(*B, B->iter);
-                batch_gather(*B);
-            });
-            int enqueued = batch;
-            for (;;) {
-                HIPCK(hipStreamSynchronize(B->st));
-                bool all_done = true;
-                for (int m = 0; m < M; ++m) {
-                    const int *k = B->hkstate.p + 3 * m;
-                    if (k[1]) {
-                        enqueue_restart(*B->mem[m]);
-                        all_done = false;
-                    } else if (!k[0]) {
-                        all_done = false;
-                    }
+        for (auto &L : B->lanes) lane_begin(*L);
+        // lanes advance as their streams go idle
+        for (bool busy = true; busy;) {
+            busy = false;
+            bool progressed = false;
+            for (auto &L : B->lanes) {
+                if (L->phase == 3) continue;
+                const hipError_t q = hipStreamQuery(L->st);
+                if (q == hipErrorNotReady) {
+                    busy = true;
+                    continue;
                 }
-                if (all_done) break;
-                if (enqueued > 3 * kMaxIt) throw std::runtime_error("BiCGSTAB did not report convergence");
-                batch_captured(*B, BG_BATCH, 2, capture_ok, [&] {
-                    for (int k = 0; k < 2; ++k) run_segs(*B, B->iter);
-                    batch_gather(*B);
-                });
-                enqueued += 2;
+                HIPCK(q);
+                lane_advance(*L);
+                progressed = true;
+                busy = busy || L->phase != 3;
             }
-            batch_captured(*B, BG_TAIL, 0, capture_ok, [&] {
-                run_segs(*B, B->tail);
-                for (gm_solver *S : B->mem) tail_readback(*S, B->st);
-            });
-            HIPCK(hipStreamSynchronize(B->st));
-            for (int m = 0; m < M; ++m) {
-                gm_solver &S = *B->mem[m];
+            if (busy && !progressed) std::this_thread::yield();
+        }
+        for (auto &L : B->lanes)
+            for (size_t i = 0; i < L->mem.size(); ++i) {
+                gm_solver &S = *L->mem[i];
                 const gmd::Krylov &K = *S.hK.p;
                 S.last_iterations = K.iter;
                 S.last_restarts = K.restarts;
                 S.prev_iters = std::max(1, S.last_iterations);
-                const int st = guarded(&S, [&] { step_finish(S, diags ? diags + m : nullptr); });
-                if (first == GM_OK && st != GM_OK) first = st;
             }
-            for (int m = 0; m < M; ++m) B->mem[m]->st = own[m];
-        } else {
-            // a member on the host ILUT path: the whole step per member
-            for (int m = 0; m < M; ++m) {
-                gm_solver *S = B->mem[m];
-                const int st = gm_ensemble_step(&S, 1, step_id, diags ? diags + m : nullptr);
-                if (first == GM_OK && st != GM_OK) first = st;
-            }
+        for (int m = 0; m < M; ++m) {
+            gm_solver &S = *B->mem[m];
+            const int st = guarded(&S, [&] { step_finish(S, diags ? diags + m : nullptr); });
+            if (first == GM_OK && st != GM_OK) first = st;
         }
+        restore();
     } catch (const std::exception &e) {
-        for (int m = 0; m < M; ++m) B->mem[m]->st = own[m];
-        (void)hipStreamSynchronize(B->st);
+        restore();
+        for (auto &L : B->lanes) (void)hipStreamSynchronize(L->st);
         B->err = e.what();
         return GM_E_DEVICE;
     }

@@ -105,6 +105,9 @@ struct StepParams {
     int sink_body[2];           // body (0 or 1) of each active sink
     int idx_ex;                 // extraction index
     int pad;
+    // orbit of the radial profiles: positions of the bodies, mass ratio,
+    // total mass and the companion's ramp weight
+    double R1, R2, q, Mt, prf;
 };
 
 // BiCGSTAB state, device resident (src/CrankNicolsonModes_matrices.cpp:47-91).

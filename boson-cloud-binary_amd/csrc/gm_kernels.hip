@@ -83,10 +83,15 @@ __device__ __forceinline__ double2 vload2(const double2 *p)
 }
 
 // Azimuthal number m of mode j (modes ordered by l, then m = -l, -l+2, ..., l).
-__device__ __forceinline__ int mode_m(int j)
+__device__ __forceinline__ int mode_l(int j)
 {
     const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
-    const int l = (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
+    return (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
+}
+
+__device__ __forceinline__ int mode_m(int j)
+{
+    const int l = mode_l(j);
     return 2 * (j - l * (l + 1) / 2) - l;
 }
 
@@ -185,11 +190,11 @@ __device__ double sum_partials(const double *partial, int q, Blk B)
 
 // ---------------------------------------------------------------- E table
 
-__global__ void k_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
-                          double mu, double prf, double *E, float *E32)
+__global__ void k_build_E(int N, int Ls, double dr, double eps, double mu, const StepParams *P, double *E, float *E32)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= Ls * N) return;
+    const double R1 = P->R1, R2 = P->R2, q = P->q, Mt = P->Mt, prf = P->prf;
     const int ls = idx / N, i = idx - ls * N;
     double e0 = 0.0, e1 = 0.0;
     if (i > 0 && i < N - 1) {
@@ -780,7 +785,8 @@ __device__ __forceinline__ double2 cn_diag(const FactorArgs &f, const double *Gj
         h.x = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * f.mu) + v;
     }
     if (f.cap) h = cadd(h, f.cap[i]);
-    return make_double2(1.0 - f.half_dt * h.y, f.half_dt * h.x);
+    const double hdt = f.P->half_dt;
+    return make_double2(1.0 - hdt * h.y, hdt * h.x);
 }
 
 struct Mob {
@@ -834,7 +840,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
     const int mi = j - l * (l + 1) / 2;
     (void)mi;
     const double *Gjj = f.Gdiag + (size_t)j * (f.lmax + 1);
-    const double2 ell = make_double2(0.0, f.half_dt * f.off);
+    const double2 ell = make_double2(0.0, f.P->half_dt * f.off);
     const double2 ell2 = cmul(ell, ell);
     const int N = f.N;
     const int ntile = (N + TE - 1) / TE;
@@ -1029,15 +1035,20 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
     typedef typename Arith<RT>::C C;
     constexpr bool F32 = std::is_same<RT, float>::value;
     Krylov *Kr = a.Kr;
-    if (VAR != PV_PLAIN && Kr->done) return;
+    if (VAR != PV_WCOL && Kr->done) return;
     constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
     __shared__ C sa[TE + TE / E];
     __shared__ C sf[TE + TE / E];
-    const int j = B.id;
+    // PV_WCOL: block col * K + j solves mode j of the column of sink col
+    const int col = VAR == PV_WCOL ? B.id / a.K : 0;
+    const int j = B.id - col * a.K;
+    // (the sink columns, read-only to every other kernel, are this variant's output)
+    double2 *const Y = VAR == PV_WCOL ? const_cast<double2 *>(a.sk[col].W) : a.Y;
+    int *const range = VAR == PV_WCOL ? a.range + 2 * col : nullptr;
     const int N = a.N;
     const int tid = threadIdx.x;
     const size_t base_j = (size_t)j * N;
-    const C ell = Arith<RT>::from(make_double2(0.0, a.half_dt * a.off));
+    const C ell = Arith<RT>::from(make_double2(0.0, a.Ps->half_dt * a.off));
     auto pivot = [&](size_t q) -> C {
         if constexpr (F32) return a.inv32[q];
         else return a.inv[q];
@@ -1048,7 +1059,7 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
     // that resets the counter keeps p and v)
     const bool first = VAR == PV_P && Kr->fresh;
     // the first iteration reads r = r0' / |rhs| from the unnormalised shadow residual
-    const bool it0 = VAR != PV_PLAIN && Kr->fresh;
+    const bool it0 = VAR != PV_WCOL && Kr->fresh;
     const double nrm0 = it0 ? Kr->rhs_nrm : 1.0;
     const double2 *Rin = it0 ? a.R0 : a.R;
     if (VAR == PV_P) {
@@ -1068,9 +1079,12 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
         const int tc = min(t, ntile - 1);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const size_t q = base_j + min(tc * TE + tid + T * e, N - 1);
-            if (VAR == PV_PLAIN) {
-                b[e].r = a.X[q];
+            const int idx = min(tc * TE + tid + T * e, N - 1);
+            const size_t q = base_j + idx;
+            if (VAR == PV_WCOL) {
+                // the sink vector v_col, zero outside its support
+                const int d = idx - a.sk[col].lo;
+                b[e].r = d >= 0 && d < a.sk[col].n ? a.sk[col].v[(size_t)j * a.sk[col].n + d] : make_double2(0.0, 0.0);
             } else {
                 b[e].r = Rin[q];
                 b[e].v = a.V[q];
@@ -1137,7 +1151,7 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
             const int idx = t0 + pos;
             if (idx < N) {
                 if constexpr (F32) a.Y32[base_j + idx] = sf[lds_slot(pos)];
-                else a.Y[base_j + idx] = sf[lds_slot(pos)];
+                else Y[base_j + idx] = sf[lds_slot(pos)];
             }
         }
         lds_barrier();
@@ -1164,7 +1178,7 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
         for (int e = 0; e < E; ++e) {
             const size_t q = base_j + min(tc * TE + tid + T * e, N - 1);
             if constexpr (F32) b[e].y = a.Y32[q];
-            else b[e].y = a.Y[q];
+            else b[e].y = Y[q];
             b[e].iv = pivot(q);
         }
     };
@@ -1174,7 +1188,7 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
     // Woodbury epilogue: the last block reads y on the sink supports, so those
     // values are stored at agent scope (written through to the device
     // coherence point) instead of fencing the whole of each XCD's L2
-    const bool wood_out = VAR != PV_PLAIN && a.wood;
+    const bool wood_out = VAR != PV_WCOL && a.wood;
     const int ns_w = wood_out ? a.Ps->nsink : 0;
     const int s0lo = ns_w > 0 ? a.sk[0].lo : 0, s0hi = ns_w > 0 ? a.sk[0].lo + a.sk[0].n : 0;
     const int s1lo = ns_w > 1 ? a.sk[1].lo : 0, s1hi = ns_w > 1 ? a.sk[1].lo + a.sk[1].n : 0;
@@ -1223,14 +1237,14 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
             if (idx < N) {
                 const double2 yv = Arith<RT>::to_d(sf[lds_slot(pos)]);
                 if ((idx >= s0lo && idx < s0hi) || (idx >= s1lo && idx < s1hi)) {
-                    double *yp = reinterpret_cast<double *>(a.Y + base_j + idx);
+                    double *yp = reinterpret_cast<double *>(Y + base_j + idx);
                     __hip_atomic_store(yp, yv.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_store(yp + 1, yv.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 } else {
-                    a.Y[base_j + idx] = yv;
+                    Y[base_j + idx] = yv;
                 }
                 if (a.Xt) store_phased(a.Xt, a.xt_f32, base_j + idx, phj, yv);
-                if (VAR == PV_PLAIN && a.range && (yv.x != 0.0 || yv.y != 0.0)) {
+                if (VAR == PV_WCOL && (yv.x != 0.0 || yv.y != 0.0)) {
                     nz_lo = min(nz_lo, idx);
                     nz_hi = max(nz_hi, idx + 1);
                 }
@@ -1248,18 +1262,18 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
             if (t - 1 >= 0) bproc(t - 1, bb);
         }
     }
-    if (VAR == PV_PLAIN && a.range) {
+    if (VAR == PV_WCOL) {
         // exact non-zero node range of the solution over all modes
         for (int o = 32; o > 0; o >>= 1) {
             nz_lo = min(nz_lo, __shfl_xor(nz_lo, o));
             nz_hi = max(nz_hi, __shfl_xor(nz_hi, o));
         }
         if ((tid & 63) == 0) {
-            atomicMin(a.range, nz_lo);
-            atomicMax(a.range + 1, nz_hi);
+            atomicMin(range, nz_lo);
+            atomicMax(range + 1, nz_hi);
         }
     }
-    if (!a.wood) return;
+    if (VAR == PV_WCOL || !a.wood) return;
     // Woodbury epilogue in the last block (matrices.cpp:146-153):
     // c = Ginv (V^H y), then the rank-one coefficients of A applied to
     // y - W c: coefA_a = (dt/4) g_a v_a^H (y - W c).
@@ -1288,7 +1302,7 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
         const int tot = a.K * sk.n;
         for (int q = tid; q < tot; q += T) {
             const int k = q / sk.n, d = q - k * sk.n;
-            const double2 d2 = cdotm(sk.v[q], vload2(a.Y + (size_t)k * N + sk.lo + d));
+            const double2 d2 = cdotm(sk.v[q], vload2(Y + (size_t)k * N + sk.lo + d));
             v[2 * s] += d2.x;
             v[2 * s + 1] += d2.y;
         }
@@ -1324,7 +1338,7 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
         for (int q = tid; q < tot; q += T) {
             const int k = q / sk.n, d = q - k * sk.n;
             const size_t gi = (size_t)k * N + sk.lo + d;
-            double2 y = vload2(a.Y + gi);
+            double2 y = vload2(Y + gi);
             const double2 rot = P->wrot[mode_m(k) + a.lmax];
 #pragma unroll
             for (int u = 0; u < 2; ++u)
@@ -1732,20 +1746,110 @@ __global__ void k_cscale(double2 *y, size_t n, double2 c)
         y[q] = cmul(c, y[q]);
 }
 
+// ---------------------------------------------------------------- absorber profiles and projectors
+
+// update_absorption_profile (absorption.cpp:12-81) for one support node per
+// block (the first nb0 blocks: body 0): the cap of half-width `width` around
+// the body seen from radius r, integrated over z = 1 - cos(gamma) with the
+// Gauss-Legendre rule, multipole by multipole,
+//   mult_l(r) = sum_k 2 pi capw w_k exp(-1 / (1 - dist^2 / width^2)) P_l(1 - z_k).
+// Thread k evaluates quadrature point k and its Legendre recurrence; the
+// terms of 32 multipoles at a time meet in LDS and thread l adds its row in
+// quadrature order, so each sum has the order of the host restatement.
+__global__ __launch_bounds__(ABS_MAX_NQ) void k_abs_profile(AbsorberArgs a, int nb0)
+{
+#pragma clang fp contract(off)
+    const int b = (int)blockIdx.x < nb0 ? 0 : 1;
+    const int d = (int)blockIdx.x - (b ? nb0 : 0);
+    const AbsorberBody &B = a.b[b];
+    const int n = B.n, degree = 2 * a.lmax, k = threadIdx.x;
+    const double rv = a.r[B.lo + d], radius = B.radius, width = B.width;
+    const double delta = rv - radius;
+    double *mult = B.mult;
+    if (fabs(delta) >= width || radius == 0.0) {
+        // outside the cap: no weight; the body at the origin: the s-wave only
+        for (int l = k; l <= degree; l += blockDim.x) {
+            double v = 0.0;
+            if (l == 0 && fabs(delta) < width) {
+                const double xx = rv / width;
+                v = 4.0 * M_PI * exp(-1.0 / (1.0 - xx * xx));
+            }
+            mult[(size_t)l * n + d] = v;
+        }
+        return;
+    }
+    const double product = 2.0 * rv * radius;
+    const double capw = fmin(2.0, (width * width - delta * delta) / product);
+    double weight = 0.0, u = 0.0;
+    if (k < a.nq) {
+        const double z = capw * a.glx[k];
+        u = 1.0 - z;
+        const double dist2 = delta * delta + product * z;
+        weight = 2.0 * M_PI * capw * a.glw[k] * exp(-1.0 / (1.0 - dist2 / (width * width)));
+    }
+    constexpr int LG = 32;
+    __shared__ double term[LG][ABS_MAX_NQ];
+    double previous = 1.0, current = u;
+    for (int l0 = 0; l0 <= degree; l0 += LG) {
+        for (int q = 0; q < LG && l0 + q <= degree; ++q) {
+            const int l = l0 + q;
+            double pl;
+            if (l == 0) {
+                pl = 1.0;
+            } else if (l == 1) {
+                pl = current;
+            } else {
+                const double next = ((2 * l - 1) * u * current - (l - 1) * previous) / l;
+                previous = current;
+                current = next;
+                pl = next;
+            }
+            term[q][k] = l == 0 ? weight : weight * pl;
+        }
+        __syncthreads();
+        if (k < LG && l0 + k <= degree) {
+            double acc = 0.0;
+            for (int q = 0; q < a.nq; ++q) acc += term[k][q];
+            mult[(size_t)(l0 + k) * n + d] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// build_swave_projector (absorption.cpp:127-151): block (j, body) writes row
+// j of the body's projector,
+//   v_j(r) = sqrt(4 pi) r (mult_{l_j}(r) / norm) c_j exp(-i m_j theta) (x (-1)^m_j for the secondary),
+// with norm = sum_d dr r^2 mult_0(r) (each block adds it in node order) and
+// c_j the angular coefficient of the (j, 00) pair.
+__global__ __launch_bounds__(64) void k_abs_project(AbsorberArgs a)
+{
+#pragma clang fp contract(off)
+    const AbsorberBody &B = a.b[blockIdx.y];
+    const int j = blockIdx.x, n = B.n;
+    if (n == 0) return;
+    double norm = 0.0;
+    for (int d = 0; d < n; ++d) norm += a.dr * a.r[B.lo + d] * a.r[B.lo + d] * B.mult[d];
+    const int l = mode_l(j), ms = -mode_m(j);
+    const double ang = a.ang[j];
+    const double2 ph = a.P->ph[ms + a.lmax];
+    const double parity = B.secondary && ms % 2 != 0 ? -1.0 : 1.0;
+    const double sqrt4pi = sqrt(4.0 * M_PI);
+    for (int d = threadIdx.x; d < n; d += blockDim.x) {
+        double2 val = make_double2(0.0, 0.0);
+        if (ang != 0.0) {
+            double mv = B.mult[(size_t)l * n + d];
+            if (norm > 0.0) mv /= norm;
+            const double m = mv * ang;
+            const double sc = a.r[B.lo + d] * sqrt4pi * parity;
+            val = make_double2(sc * (m * ph.x), sc * (m * ph.y));
+        }
+        B.v[(size_t)j * n + d] = val;
+    }
+}
+
 __global__ void k_sink_scratch_clear(SinkScratch *scr)
 {
     if (threadIdx.x == 0) scr->counter = 0;
-}
-
-// Expand a compact sink projector into a full-length vector.
-__global__ void k_expand(int N, int K, SinkDev sk, double2 *out)
-{
-    const size_t n = (size_t)N * K;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(q / N), i = (int)(q - (size_t)k * N);
-        const int d = i - sk.lo;
-        out[q] = (d >= 0 && d < sk.n) ? sk.v[(size_t)k * sk.n + d] : make_double2(0.0, 0.0);
-    }
 }
 
 // ---------------------------------------------------------------- diagnostics
@@ -1868,12 +1972,11 @@ hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt, double mu,
-                          double prf, double *E, float *E32, hipStream_t st)
+hipError_t launch_build_E(int N, int Ls, double dr, double eps, double mu, const StepParams *P, double *E, float *E32,
+                          hipStream_t st)
 {
     const int n = Ls * N;
-    hipLaunchKernelGGL(k_build_E, dim3((n + 255) / 256), dim3(256), 0, st, N, Ls, dr, eps, R1, R2, q, Mt, mu, prf, E,
-                       E32);
+    hipLaunchKernelGGL(k_build_E, dim3((n + 255) / 256), dim3(256), 0, st, N, Ls, dr, eps, mu, P, E, E32);
     return hipGetLastError();
 }
 
@@ -1892,12 +1995,12 @@ hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st)
 {
-    const bool f32 = a.inv32 != nullptr && a.Y32 != nullptr && var != PV_PLAIN;
+    const bool f32 = a.inv32 != nullptr && a.Y32 != nullptr && var != PV_WCOL;
     if (var == PV_P && f32) hipLaunchKernelGGL((k_precond<PV_P, float>), dim3(a.K), dim3(gmd::PT), 0, st, a);
     else if (var == PV_S && f32) hipLaunchKernelGGL((k_precond<PV_S, float>), dim3(a.K), dim3(gmd::PT), 0, st, a);
     else if (var == PV_P) hipLaunchKernelGGL((k_precond<PV_P, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
     else if (var == PV_S) hipLaunchKernelGGL((k_precond<PV_S, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
-    else hipLaunchKernelGGL((k_precond<PV_PLAIN, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
+    else hipLaunchKernelGGL((k_precond<PV_WCOL, double>), dim3(a.K * a.ncol), dim3(gmd::PT), 0, st, a);
     return hipGetLastError();
 }
 
@@ -2013,12 +2116,6 @@ hipError_t launch_sink(const SinkArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_expand(int N, int K, const SinkDev &sk, double2 *out, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_expand, dim3(stream_grid((size_t)N * K)), dim3(256), 0, st, N, K, sk, out);
-    return hipGetLastError();
-}
-
 hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_diag_final, dim3(a.K), dim3(64), 0, st, a);
@@ -2063,6 +2160,15 @@ hipError_t launch_sink_apply(const SinkApplyArgs &a, hipStream_t st)
     hipLaunchKernelGGL(k_sink_scratch_clear, dim3(1), dim3(64), 0, st, a.scr);
     if (a.nsink > 0) hipLaunchKernelGGL(k_sink_project, dim3(a.K), dim3(64), 0, st, a);
     hipLaunchKernelGGL(k_sink_expand, dim3(a.K), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_absorbers(const AbsorberArgs &a, hipStream_t st)
+{
+    if (a.nq < 1 || a.nq > ABS_MAX_NQ || 2 * a.lmax + 1 > ABS_MAX_NQ) return hipErrorInvalidValue;
+    const int nb0 = a.b[0].build ? a.b[0].n : 0, nb1 = a.b[1].build ? a.b[1].n : 0;
+    if (nb0 + nb1 > 0) hipLaunchKernelGGL(k_abs_profile, dim3(nb0 + nb1), dim3(ABS_MAX_NQ), 0, st, a, nb0);
+    if (a.b[0].n + a.b[1].n > 0) hipLaunchKernelGGL(k_abs_project, dim3(a.K, 2), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

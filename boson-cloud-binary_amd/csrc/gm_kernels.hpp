@@ -6,7 +6,10 @@ namespace gmk {
 
 // AM_RST: the BiCGSTAB restart (r = b - A x into Y, r0' = |rhs| r into Y2, |r|^2)
 enum ApplyMode { AM_H = 0, AM_A = 1, AM_V = 2, AM_T = 3, AM_RHS = 4, AM_RES = 5, AM_RST = 6 };
-enum PrecVar { PV_PLAIN = 0, PV_P = 1, PV_S = 2 };
+// PV_WCOL: the Woodbury columns W_s = P^-1 v_s of the first `ncol` sinks in
+// one launch (sk[s].W written, range[2 s, 2 s + 1] widened); PV_P / PV_S: the
+// two preconditioner applies of a BiCGSTAB iteration.
+enum PrecVar { PV_WCOL = 0, PV_P = 1, PV_S = 2 };
 enum SinkMode { SK_PROJ = 0, SK_SETUP = 1 };
 enum SinkDest { SD_START = 0, SD_QNEW = 1, SD_POST = 2, SD_COEFA = 3, SD_COEFA_QNEW = 4 };
 
@@ -44,7 +47,8 @@ struct PhaseArgs {
 
 struct FactorArgs {
     int N, K, lmax;
-    double dr, eps, mu, off, half_dt;
+    double dr, eps, mu, off;
+    const gmd::StepParams *P;  // dt / 2
     const double *Gdiag;  // K * (l_max + 1): coefficient of (j, j) at l* = 2 s
     const double *E;
     const double2 *cap;
@@ -55,19 +59,19 @@ struct FactorArgs {
 
 struct PrecArgs {
     int N, K;
-    double half_dt, off;
+    double off;          // the stencil off-diagonal; dt / 2 comes from Ps
     const double2 *inv;
     gmd::Krylov *Kr;
     const double2 *R;
     const double2 *R0;   // unnormalised shadow residual: r = R0 / |rhs| at the first iteration
     double2 *Pv;         // p (PV_P, updated in place) or s (PV_S, written)
     const double2 *V;
-    const double2 *X;    // PV_PLAIN input
-    double2 *Y;          // output
+    int ncol;            // PV_WCOL: number of sink columns
+    double2 *Y;          // output (PV_P / PV_S)
     int wood;            // run the Woodbury epilogue
     gmd::SinkDev sk[2];
     const gmd::StepParams *Ps;
-    int *range;          // PV_PLAIN: atomically widened [lo, hi) of non-zero output nodes, or nullptr
+    int *range;          // PV_WCOL: per column, atomically widened [lo, hi) of non-zero output nodes
     int lmax;            // Woodbury epilogue: mode -> m for the column rotation
     const float2 *inv32; // FP32 pivots: with Y32, the p / s variants sweep in FP32 (mixed precision)
     float2 *Y32;         // FP32 forward-sweep scratch
@@ -161,6 +165,29 @@ hipError_t sink_scratch_address(SinkScratch **out);
 // y = c y over n complex values
 hipError_t launch_cscale(double2 *y, size_t n, double2 c, hipStream_t st);
 
+// Absorber of one body (absorption.cpp:12-81,127-151): the smeared cap's
+// multipole profile over the interior support [lo, lo + n) and its s-wave
+// projector v (K x n, at the step's angle).
+struct AbsorberBody {
+    int lo, n;            // support nodes (n = 0: no absorber)
+    int build;            // rebuild the profile: the body moved or its width changed
+    int secondary;        // the secondary's sign on odd m
+    double radius, width;
+    double *mult;         // (2 l_max + 1) x n multipoles (unnormalised)
+    double2 *v;           // K x n projector
+};
+constexpr int ABS_MAX_NQ = 128;  // quadrature points of one profile node (one per thread)
+struct AbsorberArgs {
+    int K, lmax, nq;
+    double dr;
+    const double *r;      // radial nodes
+    const double *glx, *glw;  // Gauss-Legendre rule on [0, 1], nq points
+    const double *ang;    // per mode j: angular coefficient of the (j, 00) pair with l* = l_j
+    const gmd::StepParams *P;  // exp(i m theta)
+    AbsorberBody b[2];
+};
+hipError_t launch_absorbers(const AbsorberArgs &a, hipStream_t st);
+
 // One launch over many solvers (a batched ragged launch): member m's
 // arguments are args[m] (device memory) and it owns the blocks
 // [off[m], off[m + 1]) of a grid of `total` blocks. Kernels and their block
@@ -189,8 +216,9 @@ hipError_t launch_krylov_gather(gmd::Krylov *const *kr, int *out, int M, hipStre
 int apply_grid(int N, int nchunks, int ng, int *ngroups);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
 PhaseArgs phase_args(const ApplyArgs &a);
-hipError_t launch_build_E(int N, int Ls, double dr, double eps, double R1, double R2, double q, double Mt,
-                          double mu, double prf, double *E, float *E32, hipStream_t st);
+// radial profiles from the orbit in P (R1, R2, q, Mt, prf)
+hipError_t launch_build_E(int N, int Ls, double dr, double eps, double mu, const gmd::StepParams *P, double *E,
+                          float *E32, hipStream_t st);
 hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hipStream_t st);
 hipError_t launch_factor(const FactorArgs &f, hipStream_t st);
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st);
@@ -198,7 +226,6 @@ hipError_t launch_update(const UpdateArgs &a, hipStream_t st);
 hipError_t launch_init(const InitArgs &a, hipStream_t st);
 hipError_t launch_accept(const AcceptArgs &a, hipStream_t st);
 hipError_t launch_sink(const SinkArgs &a, hipStream_t st);
-hipError_t launch_expand(int N, int K, const gmd::SinkDev &sk, double2 *out, hipStream_t st);
 hipError_t launch_wcorr(const WcorrArgs &a, hipStream_t st);
 hipError_t launch_diag_apply(const ApplyArgs &a, hipStream_t st);
 hipError_t launch_diag_final(const DiagArgs &a, hipStream_t st);

@@ -81,19 +81,21 @@ struct PinBuf {
 struct Profile {
     double radius = -1.0, width = -1.0;
     bool valid = false;
+    bool build = false;          // the device multipoles are to be rebuilt
     int lo = 0, n = 0;           // interior support nodes [lo, lo+n)
-    std::vector<double> mult;    // (2 l_max + 1) * n, over the support
 };
 
 struct Sink {
-    std::vector<double2> v;      // K * n over the profile support
     int lo = 0, n = 0;
     double strength = 0.0;
-    double norm2 = 0.0;
-    DevBuf<double2> dv;          // device copy of v
+    DevBuf<double> dmult;        // (2 l_max + 1) * n profile multipoles of the support
+    DevBuf<double2> dv;          // K * n projector over the support (device only)
     DevBuf<double2> W;           // P^-1 v, full length
-    PinBuf<double2> stage;       // pinned staging of v
-    size_t stage_n = 0;
+    // host copy of v and its squared norm, read back on demand (host ILUT,
+    // sink strength queries)
+    std::vector<double2> v;
+    double norm2 = 0.0;
+    bool v_host = false;
 };
 
 }  // namespace
@@ -146,6 +148,9 @@ struct gm_solver {
     DevBuf<double> GM, Gdiag, E, dmask;
     DevBuf<float> GM32, E32;  // FP32 copies for the mixed-precision Krylov applies
     DevBuf<int> lcut;
+    // absorbers: radial nodes, the cap quadrature and the (j, 00) angular coefficients
+    DevBuf<double> dr_nodes, dglx, dglw, dang;
+    int abs_nq = 0;
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
     DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, inv, tmp;
@@ -168,6 +173,7 @@ struct gm_solver {
     int ntiles = 0, nchunks = 0;
     // caches
     bool E_valid = false;
+    bool E_pending = false;  // profiles of a new orbit not yet enqueued
     double E_key[6] = {0, 0, 0, 0, 0, 0};
     bool fac_valid = false;
     double fac_key[3] = {0, 0, 0};
@@ -184,6 +190,7 @@ struct gm_solver {
     // progress of the step in flight (step_begin / solve_advance / step_finish)
     double step_start = 0.0;
     bool step_wood = false;
+    bool step_E = false, step_fac = false;  // profiles / pivots rebuilt at the solve start
     bool step_ilut = false;  // this step's solve is the host ILUT one
     bool step_fp64 = false;  // a mixed-precision step solved again with the FP64 operator
     int step_enqueued = 0;
@@ -496,100 +503,132 @@ gmd::Geometry geometry(const gm_solver &S)
 
 // ------------------------------------------------------------------ absorbers
 
-// update_absorption_profile (src/CrankNicolsonModes_absorption.cpp:12-81),
-// kept on the body's support only.
+// update_absorption_profile (src/CrankNicolsonModes_absorption.cpp:12-81):
+// the interior nodes within `width` of the body; the multipoles over them are
+// rebuilt on the device (k_abs_profile) when the body or its width changed.
 void update_profile(gm_solver &S, Profile &pr, double radius, double width)
 {
     if (pr.valid && pr.radius == radius && pr.width == width) return;
     pr.valid = true;
     pr.radius = radius;
     pr.width = width;
-    pr.mult.clear();
     pr.lo = 0;
     pr.n = 0;
+    pr.build = false;
     if (width < S.dr || width <= 0.0 || radius < 0.0) return;
-    const int degree = 2 * S.l_max, N = S.Nr;
+    // r_i = epsilon + i dr is monotone: scan a window around the body
+    const int N = S.Nr;
+    const int c = static_cast<int>(std::floor((radius - S.epsilon) / S.dr));
+    const int span = static_cast<int>(std::ceil(width / S.dr)) + 2;
+    const int a = std::max(1, c - span), b = std::min(N - 1, c + span + 1);
     int lo = -1, hi = -1;
-    for (int i = 1; i < N - 1; ++i)
+    for (int i = a; i < b; ++i)
         if (std::abs(S.r[i] - radius) < width) {
             if (lo < 0) lo = i;
             hi = i + 1;
         }
     if (lo < 0) return;
-    const int n = hi - lo;
     pr.lo = lo;
-    pr.n = n;
-    pr.mult.assign(static_cast<size_t>(degree + 1) * n, 0.0);
-    const int nq = std::max(128, degree + 16);
-    std::vector<double> nodes, weights;
-    gmh::gauss_legendre01(nq, nodes, weights);
-    for (int d = 0; d < n; ++d) {
-        const int i = lo + d;
-        const double delta = S.r[i] - radius;
-        if (std::abs(delta) >= width) continue;
-        if (radius == 0.0) {
-            const double xx = S.r[i] / width;
-            pr.mult[d] = 4.0 * M_PI * std::exp(-1.0 / (1.0 - xx * xx));
-            continue;
-        }
-        const double product = 2.0 * S.r[i] * radius;
-        const double capw = std::min(2.0, (width * width - delta * delta) / product);
-        for (int k = 0; k < nq; ++k) {
-            const double z = capw * nodes[k];
-            const double u = 1.0 - z;
-            const double dist2 = delta * delta + product * z;
-            const double weight = 2.0 * M_PI * capw * weights[k] * std::exp(-1.0 / (1.0 - dist2 / (width * width)));
-            double previous = 1.0, current = u;
-            pr.mult[d] += weight;
-            if (degree > 0) pr.mult[n + d] += weight * current;
-            for (int l = 2; l <= degree; ++l) {
-                const double next = ((2 * l - 1) * u * current - (l - 1) * previous) / l;
-                pr.mult[static_cast<size_t>(l) * n + d] += weight * next;
-                previous = current;
-                current = next;
-            }
-        }
-    }
-    double norm = 0.0;
-    for (int d = 0; d < n; ++d) norm += S.dr * S.r[lo + d] * S.r[lo + d] * pr.mult[d];
-    if (norm > 0.0)
-        for (double &value : pr.mult) value /= norm;
+    pr.n = hi - lo;
+    pr.build = true;
 }
 
-// build_swave_projector (absorption.cpp:127-151): the (j, 00) pair has the
-// single multipole l* = l_j.
-void build_projector(gm_solver &S, const Profile &pr, bool secondary, Sink &sk)
+// The cap quadrature (nq = max(128, 2 l_max + 16) Gauss-Legendre points on
+// [0, 1], absorption.cpp:27-34), the radial nodes and the angular
+// coefficients of the (j, 00) pairs on the device, once per solver.
+std::mutex gl_mu;
+std::map<int, std::pair<std::vector<double>, std::vector<double>>> gl_cache;
+
+void absorber_tables(gm_solver &S)
 {
-    sk.lo = pr.lo;
-    sk.n = pr.n;
-    sk.v.assign(static_cast<size_t>(S.K) * pr.n, make_double2(0.0, 0.0));
-    sk.norm2 = 0.0;
-    if (S.mode00 < 0 || pr.n == 0) return;
-    const double sqrt4pi = std::sqrt(4.0 * M_PI);
-    for (int j = 0; j < S.K; ++j) {
-        const int ms = 0 - S.mm[j];
-        const double ang = angular(S, j, S.mode00, S.ml[j]);
-        if (ang == 0.0) continue;
-        const double ph_re = std::cos(ms * S.theta), ph_im = std::sin(ms * S.theta);
-        const double parity = secondary && ms % 2 != 0 ? -1.0 : 1.0;
-        for (int d = 0; d < pr.n; ++d) {
-            const double m = pr.mult[static_cast<size_t>(S.ml[j]) * pr.n + d] * ang;
-            const double sc = S.r[pr.lo + d] * sqrt4pi * parity;
-            double2 val = make_double2(sc * (m * ph_re), sc * (m * ph_im));
-            sk.v[static_cast<size_t>(j) * pr.n + d] = val;
-            sk.norm2 += val.x * val.x + val.y * val.y;
+    const int nq = std::max(128, 2 * S.l_max + 16);
+    if (nq > gmk::ABS_MAX_NQ) throw std::invalid_argument("l_max too large for the absorber quadrature");
+    std::pair<std::vector<double>, std::vector<double>> gl;
+    {
+        std::lock_guard<std::mutex> lk(gl_mu);
+        auto it = gl_cache.find(nq);
+        if (it == gl_cache.end()) {
+            gmh::gauss_legendre01(nq, gl.first, gl.second);
+            gl_cache.emplace(nq, gl);
+        } else {
+            gl = it->second;
         }
     }
-    const size_t cnt = std::max<size_t>(sk.v.size(), 1);
-    if (sk.dv.n < cnt) {
-        sk.dv.alloc(cnt);
-        sk.stage.alloc(cnt);
-        sk.stage_n = cnt;
+    std::vector<double> ang(S.K, 0.0);
+    if (S.mode00 >= 0)
+        for (int j = 0; j < S.K; ++j) ang[j] = angular(S, j, S.mode00, S.ml[j]);
+    if (S.dr_nodes.n != static_cast<size_t>(S.Nr)) S.dr_nodes.alloc(S.Nr);
+    if (S.dglx.n != static_cast<size_t>(nq)) {
+        S.dglx.alloc(nq);
+        S.dglw.alloc(nq);
     }
-    if (!sk.v.empty()) {
-        std::memcpy(sk.stage.p, sk.v.data(), sk.v.size() * sizeof(double2));
-        HIPCK(hipMemcpyAsync(sk.dv.p, sk.stage.p, sk.v.size() * sizeof(double2), hipMemcpyHostToDevice, S.st));
+    if (S.dang.n != static_cast<size_t>(S.K)) S.dang.alloc(S.K);
+    // pageable sources: synchronous copies
+    HIPCK(hipMemcpy(S.dr_nodes.p, S.r.data(), S.Nr * sizeof(double), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(S.dglx.p, gl.first.data(), nq * sizeof(double), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(S.dglw.p, gl.second.data(), nq * sizeof(double), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(S.dang.p, ang.data(), S.K * sizeof(double), hipMemcpyHostToDevice));
+    S.abs_nq = nq;
+}
+
+// Device storage of a body's absorber over its support.
+void size_sink(gm_solver &S, Sink &sk, int n)
+{
+    const size_t nm = static_cast<size_t>(2 * S.l_max + 1) * std::max(n, 1);
+    const size_t nv = static_cast<size_t>(S.K) * std::max(n, 1);
+    if (sk.dmult.n < nm) sk.dmult.alloc(nm);
+    if (sk.dv.n < nv) sk.dv.alloc(nv);
+}
+
+// The two bodies' profiles (when rebuilt) and projectors at the step's angle
+// (build_swave_projector, absorption.cpp:127-151), after the step parameters
+// (exp(i m theta)) are on the device.
+void launch_absorbers(gm_solver &S)
+{
+    gmk::AbsorberArgs a{};
+    a.K = S.K;
+    a.lmax = S.l_max;
+    a.nq = S.abs_nq;
+    a.dr = S.dr;
+    a.r = S.dr_nodes.p;
+    a.glx = S.dglx.p;
+    a.glw = S.dglw.p;
+    a.ang = S.dang.p;
+    a.P = S.dP.p;
+    for (int b = 0; b < 2; ++b) {
+        Profile &pr = S.prof[b];
+        Sink &sk = S.sink[b];
+        gmk::AbsorberBody &B = a.b[b];
+        B.lo = pr.lo;
+        B.n = pr.n;
+        B.build = pr.build;
+        B.secondary = b == 1;
+        B.radius = pr.radius;
+        B.width = pr.width;
+        B.mult = sk.dmult.p;
+        B.v = sk.dv.p;
+        pr.build = false;
+        sk.v_host = false;
     }
+    launch(S, GM_PROF_SMALL, [&] { return gmk::launch_absorbers(a, S.st); });
+}
+
+// The host copy of a sink's projector (and its squared norm), read back from
+// the device after the stream has drained.
+const Sink &sink_host(const gm_solver &S, int body)
+{
+    Sink &sk = const_cast<Sink &>(S.sink[body]);
+    if (!sk.v_host) {
+        sk.v.assign(static_cast<size_t>(S.K) * sk.n, make_double2(0.0, 0.0));
+        sk.norm2 = 0.0;
+        if (!sk.v.empty()) {
+            HIPCK(hipStreamSynchronize(S.st));
+            HIPCK(hipMemcpy(sk.v.data(), sk.dv.p, sk.v.size() * sizeof(double2), hipMemcpyDeviceToHost));
+            for (const double2 &val : sk.v) sk.norm2 += val.x * val.x + val.y * val.y;
+        }
+        sk.v_host = true;
+    }
+    return sk;
 }
 
 // build_spherical_absorption (absorption.cpp:153-182).
@@ -605,16 +644,48 @@ void build_absorption(gm_solver &S)
     }
     S.Ah1 = 16.0 * M_PI * S.M1 * S.M1 * ramp;
     S.Ah2 = 16.0 * M_PI * S.M2 * S.M2 * ramp;
-    build_projector(S, S.prof[0], false, S.sink[0]);
+    for (int b = 0; b < 2; ++b) {
+        Sink &sk = S.sink[b];
+        sk.lo = S.prof[b].lo;
+        sk.n = S.prof[b].n;
+        size_sink(S, sk, sk.n);
+    }
     S.sink[0].strength = S.Ah1 * S.dr;
-    build_projector(S, S.prof[1], true, S.sink[1]);
     S.sink[1].strength = S.Ah2 * S.dr;
 }
 
 // ------------------------------------------------------------------ per-step operator
 
-// build_time_dependent_perturbation (hamiltonian.cpp:245-268) on the device tables.
-void build_perturbation(gm_solver &S, double time)
+// The reference drops a sink whose projector has |v|^2 = 0 (absorption.cpp:
+// 153-182). A body off the origin with a non-empty support has |v|^2 > 0: the
+// (00, 00) row holds sqrt(4 pi) r mult_0(r) c_00 with c_00 != 0, and at the
+// support node nearest the body (within dr / 2 <= width / 2) the cap weight
+// exp(-1 / (1 - dist^2 / width^2)) is at least exp(-4/3) near the cap's
+// centre. At the origin the weights of a support just inside `width` can
+// underflow to zero; such a sink is kept with v = 0, where its Woodbury terms
+// and absorbed mass vanish, the same step as without it.
+bool sink_nonzero(const gm_solver &S, int body)
+{
+    return S.sink[body].n > 0 && S.mode00 >= 0;
+}
+
+// Radial profiles E of the orbit in the device step parameters, and the
+// multipole cut of each tile.
+void launch_profiles(gm_solver &S)
+{
+    launch(S, GM_PROF_FACTOR, [&] {
+        return gmk::launch_build_E(S.Nr, 2 * S.l_max + 1, S.dr, S.epsilon, S.mu, S.dP.p, S.E.p, S.E32.p, S.st);
+    });
+    launch(S, GM_PROF_FACTOR, [&] {
+        return gmk::launch_lcut(S.Nr, 2 * S.l_max + 1, S.coupling_tol, S.E.p, S.lcut.p, S.st);
+    });
+}
+
+// build_time_dependent_perturbation (hamiltonian.cpp:245-268) on the device
+// tables. The step scalars go to the device first; the radial profiles, when
+// the orbit moved, are rebuilt at once, or with `defer` inside the step's
+// captured start (E_pending).
+void build_perturbation(gm_solver &S, double time, bool defer = false)
 {
     S.ramp_factor = 1.0;
     if (S.isolated) {
@@ -630,18 +701,17 @@ void build_perturbation(gm_solver &S, double time)
     // radial profiles: rebuilt only when the orbit or the companion weight changed
     const double key[6] = {S.R1, S.R2, S.q, S.M1 + S.M2, S.prf, S.mu};
     if (!S.E_valid || std::memcmp(key, S.E_key, sizeof(key)) != 0) {
-        launch(S, GM_PROF_FACTOR, [&] {
-            return gmk::launch_build_E(S.Nr, 2 * S.l_max + 1, S.dr, S.epsilon, S.R1, S.R2, S.q, S.M1 + S.M2, S.mu,
-                                       S.prf, S.E.p, S.E32.p, S.st);
-        });
-        launch(S, GM_PROF_FACTOR, [&] {
-            return gmk::launch_lcut(S.Nr, 2 * S.l_max + 1, S.coupling_tol, S.E.p, S.lcut.p, S.st);
-        });
+        S.E_pending = true;
         std::memcpy(S.E_key, key, sizeof(key));
         S.E_valid = true;
         S.fac_valid = false;
     }
     gmd::StepParams &P = *S.hP.p;
+    P.R1 = S.R1;
+    P.R2 = S.R2;
+    P.q = S.q;
+    P.Mt = S.M1 + S.M2;
+    P.prf = S.prf;
     for (int m = -S.l_max; m <= S.l_max; ++m)
         P.ph[m + S.l_max] = make_double2(std::cos(m * S.theta), std::sin(m * S.theta));
     S.Omega_V = S.Omega;
@@ -654,7 +724,7 @@ void build_perturbation(gm_solver &S, double time)
     P.g[0] = P.g[1] = 0.0;
     if (S.sink_active)
         for (int bdy = 0; bdy < 2; ++bdy)
-            if (S.sink[bdy].strength != 0.0 && S.sink[bdy].norm2 > 0.0) {
+            if (S.sink[bdy].strength != 0.0 && sink_nonzero(S, bdy)) {
                 P.sink_body[P.nsink] = bdy;
                 P.g[P.nsink] = S.sink[bdy].strength;
                 P.nsink++;
@@ -682,6 +752,11 @@ void build_perturbation(gm_solver &S, double time)
             P.wrot[m + S.l_max] = make_double2(std::cos(m * (S.theta - S.w_theta)), -std::sin(m * (S.theta - S.w_theta)));
     }
     HIPCK(hipMemcpyAsync(S.dP.p, S.hP.p, sizeof(gmd::StepParams), hipMemcpyHostToDevice, S.st));
+    if (S.absorption) launch_absorbers(S);
+    if (!defer && S.E_pending) {
+        launch_profiles(S);
+        S.E_pending = false;
+    }
 }
 
 void fill_sinks(const gm_solver &S, gmd::SinkDev (&sk)[2])
@@ -721,7 +796,6 @@ gmk::PrecArgs prec_args(gm_solver &S)
     gmk::PrecArgs a{};
     a.N = S.Nr;
     a.K = S.K;
-    a.half_dt = S.dt * 0.5;
     a.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
     a.inv = S.inv.p;
     a.Kr = S.dK.p;
@@ -731,11 +805,20 @@ gmk::PrecArgs prec_args(gm_solver &S)
     return a;
 }
 
-// Thomas pivots, recomputed when dt, the E table or the CAP changed.
-void factorize(gm_solver &S)
+// Thomas pivots are recomputed when dt, the E table or the CAP changed;
+// factor_needed records the new key.
+bool factor_needed(gm_solver &S)
 {
     const double key[3] = {S.dt, S.E_key[4], S.E_key[3]};
-    if (S.fac_valid && std::memcmp(key, S.fac_key, sizeof(key)) == 0 && S.fac_cap_version == S.cap_version) return;
+    if (S.fac_valid && std::memcmp(key, S.fac_key, sizeof(key)) == 0 && S.fac_cap_version == S.cap_version) return false;
+    std::memcpy(S.fac_key, key, sizeof(key));
+    S.fac_cap_version = S.cap_version;
+    S.fac_valid = true;
+    return true;
+}
+
+void launch_factor(gm_solver &S)
+{
     gmk::FactorArgs f{};
     f.N = S.Nr;
     f.K = S.K;
@@ -744,7 +827,7 @@ void factorize(gm_solver &S)
     f.eps = S.epsilon;
     f.mu = S.mu;
     f.off = -(1.0 / (S.dr * S.dr)) / (2 * S.mu);
-    f.half_dt = S.dt * 0.5;
+    f.P = S.dP.p;
     f.Gdiag = S.Gdiag.p;
     f.E = S.E.p;
     f.cap = S.cap ? S.dcap.p : nullptr;
@@ -752,9 +835,11 @@ void factorize(gm_solver &S)
     f.inv32 = S.inv32.p;
     f.failed = S.dflags.p;
     launch(S, GM_PROF_FACTOR, [&] { return gmk::launch_factor(f, S.st); });
-    std::memcpy(S.fac_key, key, sizeof(key));
-    S.fac_cap_version = S.cap_version;
-    S.fac_valid = true;
+}
+
+void factorize(gm_solver &S)
+{
+    if (factor_needed(S)) launch_factor(S);
 }
 
 gmk::SinkArgs sink_proj_args(gm_solver &S, const double2 *X, int dest, double scale, int by_nrm)
@@ -809,17 +894,13 @@ void woodbury_columns(gm_solver &S)
         for (int s = 0; s < P.nsink; ++s) {
             Sink &sk = S.sink[P.sink_body[s]];
             if (sk.W.n != S.nvec()) sk.W.alloc(S.nvec());
-            gmd::SinkDev sd{};
-            sd.lo = sk.lo;
-            sd.n = sk.n;
-            sd.v = sk.dv.p;
-            launch(S, GM_PROF_SMALL, [&] { return gmk::launch_expand(S.Nr, S.K, sd, S.tmp.p, S.st); });
-            pa.X = S.tmp.p;
-            pa.Y = sk.W.p;
-            pa.wood = 0;
-            pa.range = &dk->wrange[2 * s];
-            launch(S, GM_PROF_PRECOND_SETUP, [&] { return gmk::launch_precond(pa, gmk::PV_PLAIN, S.st); });
         }
+        // (fill_sinks again: the columns may just have been allocated)
+        fill_sinks(S, pa.sk);
+        pa.wood = 0;
+        pa.ncol = P.nsink;
+        pa.range = &dk->wrange[0];
+        launch(S, GM_PROF_PRECOND_SETUP, [&] { return gmk::launch_precond(pa, gmk::PV_WCOL, S.st); });
         S.w_valid = true;
     }
 }
@@ -930,30 +1011,27 @@ void enqueue_iteration(gm_solver &S, bool wood)
 
 constexpr int kMaxIt = 500;  // matrices.cpp:104
 
-// Graph-captured parts of a step: the start of the solve (Woodbury columns,
-// rhs, init, first iterations), a batch of iterations, and the tail (residual
-// check, mask, diagnostics, readbacks). Per-step values (theta, phases, sink
-// strengths, Krylov scalars) live in device memory, so a graph stays valid
-// while the kernel arguments below do not change; it is captured once the same
-// arguments were requested on two consecutive steps (an evolving orbit moves
-// the absorber supports every step and runs without graphs).
+// Graph-captured parts of a step: the start of the solve (orbit profiles and
+// pivots when the orbit moved, Woodbury columns, rhs, init, first
+// iterations), a batch of iterations, and the tail (residual check, mask,
+// diagnostics, readbacks). Per-step values (dt, orbit radii and weights,
+// theta, phases, sink strengths, Krylov scalars) live in device memory, so a
+// graph stays valid while the kernel arguments below do not change; it is
+// captured once the same arguments were requested on two consecutive steps.
+// An evolving orbit replays the same graphs until an absorber support crosses
+// a radial node.
 enum GraphKind { GK_BEGIN = 0, GK_BATCH = 1, GK_TAIL = 2 };
 
 std::vector<uint64_t> graph_key(const gm_solver &S, int batch)
 {
     std::vector<uint64_t> k;
-    auto put_d = [&](double v) {
-        uint64_t u;
-        std::memcpy(&u, &v, 8);
-        k.push_back(u);
-    };
     k.push_back(static_cast<uint64_t>(batch));
     k.push_back(S.step_wood);
     k.push_back(static_cast<uint64_t>(S.flags.precision));
     k.push_back(S.mask);
     k.push_back(static_cast<uint64_t>(S.apply_ng[0] | (S.apply_ng[1] << 8)));
     k.push_back(S.w_dirty);
-    put_d(S.dt);
+    k.push_back(static_cast<uint64_t>(S.step_E) | (static_cast<uint64_t>(S.step_fac) << 1));
     gmd::SinkDev sk[2];
     fill_sinks(S, sk);
     for (const auto &d : sk) {
@@ -1042,7 +1120,9 @@ void solve_begin(gm_solver &S)
     const bool wood = P.nsink > 0;
     S.step_wood = wood;
     S.step_enqueued = 0;
-    factorize(S);
+    S.step_E = S.E_pending;
+    S.E_pending = false;
+    S.step_fac = factor_needed(S);
     const int batch = std::max(1, S.prev_iters);
     run_captured(S, GK_BEGIN, graph_key(S, batch), [&] { solve_begin_work(S, batch); });
     S.step_enqueued = batch;
@@ -1052,6 +1132,8 @@ void rhs_and_init(gm_solver &S);
 
 void solve_begin_work(gm_solver &S, int batch)
 {
+    if (S.step_E) launch_profiles(S);
+    if (S.step_fac) launch_factor(S);
     const bool wood = S.step_wood;
     if (wood) woodbury_setup(S);
     rhs_and_init(S);
@@ -1566,7 +1648,7 @@ void ilut_host_solve(gm_solver &S)
     std::vector<std::vector<std::pair<size_t, gmh::cplx>>> sv(ns);
     std::vector<double> beta(ns);
     for (int a = 0; a < ns; ++a) {
-        const Sink &sk = S.sink[P.sink_body[a]];
+        const Sink &sk = sink_host(S, P.sink_body[a]);
         beta[a] = 0.5 * P.half_dt * P.g[a];
         for (int k = 0; k < K; ++k)
             for (int d = 0; d < sk.n; ++d) {
@@ -1929,11 +2011,13 @@ int gm_initialize(gm_solver *S)
             S->mode00 = -1;
             for (int j = 0; j < S->K; ++j)
                 if (S->ml[j] == 0 && S->mm[j] == 0) S->mode00 = j;
+            absorber_tables(*S);
             for (Sink &sk : S->sink) {
                 sk.v.clear();
                 sk.n = 0;
                 sk.strength = 0.0;
                 sk.norm2 = 0.0;
+                sk.v_host = false;
             }
         }
         S->t_i = 0.0;
@@ -2081,7 +2165,9 @@ void step_begin(gm_solver &S)
     S.t_i += S.dt;
     S.theta += (S.counter ? -1.0 : 1.0) * S.Omega * S.dt;
     S.phase += S.Omega * S.dt;
-    build_perturbation(S, S.t_i);
+    // with the block preconditioner the profiles of a moved orbit are
+    // rebuilt inside the captured start of the solve
+    build_perturbation(S, S.t_i, S.flags.use_block_preconditioner != 0);
     S.step_ilut = false;
     S.step_fp64 = false;
     if (!S.flags.use_block_preconditioner) {
@@ -2557,7 +2643,7 @@ double gm_sink_strength(const gm_solver *S, int32_t body)
 {
     const Sink &sk = S->sink[body == 1 ? 0 : 1];
     if (!S->sink_active || sk.strength == 0.0) return 0.0;
-    return sk.strength * sk.norm2;
+    return sk.strength * sink_host(*S, body == 1 ? 0 : 1).norm2;
 }
 
 }  // extern "C"

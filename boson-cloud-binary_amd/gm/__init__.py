@@ -157,12 +157,13 @@ def ensemble_step(members, step_id=1, diag=True):
 
 class Batch:
     """gm_batch: the members stepped together, every launch of the step one
-    ragged grid over all of them (each member's result bitwise its own
-    step's). Members stay usable alone between batch steps."""
+    ragged grid over all of them, or over each of `lanes` subsets on their
+    own streams (each member's result bitwise its own step's). Members stay
+    usable alone between batch steps."""
 
-    def __init__(self, members):
+    def __init__(self, members, lanes=1):
         L = lib()
-        L.gm_batch_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        L.gm_batch_create_lanes.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
         L.gm_batch_step.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.gm_batch_destroy.argtypes = [C.c_void_p]
         L.gm_batch_destroy.restype = None
@@ -173,7 +174,7 @@ class Batch:
         n = len(self.members)
         self._handles = (C.c_void_p * n)(*[m._h for m in self.members])
         h = C.c_void_p()
-        rc = L.gm_batch_create(self._handles, n, C.byref(h))
+        rc = L.gm_batch_create_lanes(self._handles, n, int(lanes), C.byref(h))
         if rc != 0:
             raise capi.GMError(rc, L.gm_last_error(None).decode())
         self._h = h

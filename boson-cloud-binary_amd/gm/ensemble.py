@@ -69,15 +69,15 @@ def build(members, precision=GM_PRECISION_MIXED, states=None):
     return solvers
 
 
-def run(n=64, steps=10, warmup=3, seed=0, precision=GM_PRECISION_MIXED, mode="batch"):
+def run(n=64, steps=10, warmup=3, seed=0, precision=GM_PRECISION_MIXED, mode="batch", lanes=1):
     """Aggregate mode.node-steps/s of `steps` ensemble steps after `warmup`;
-    mode "batch" (one ragged grid per launch, gm_batch) or "streams" (a
-    stream per member, gm_ensemble_step)."""
+    mode "batch" (ragged grids per launch over `lanes` member subsets,
+    gm_batch) or "streams" (a stream per member, gm_ensemble_step)."""
     members = draw_members(n, seed)
     t0 = time.perf_counter()
     solvers = build(members, precision)
     t_build = time.perf_counter() - t0
-    batch = Batch(solvers) if mode == "batch" else None
+    batch = Batch(solvers, lanes) if mode == "batch" else None
 
     def one(k):
         if batch is not None:
@@ -95,7 +95,7 @@ def run(n=64, steps=10, warmup=3, seed=0, precision=GM_PRECISION_MIXED, mode="ba
         one(k)
     el = time.perf_counter() - t0
     work = sum(s.n_modes * s.Nr for s in solvers) * steps
-    res = {"mode": mode, "members": n, "steps": steps, "warmup": warmup, "seconds": el, "build_seconds": t_build,
+    res = {"mode": mode, "lanes": lanes if mode == "batch" else n, "members": n, "steps": steps, "warmup": warmup, "seconds": el, "build_seconds": t_build,
            "mode_node_steps_per_s": work / el, "ms_per_ensemble_step": el / steps * 1e3,
            "mode_nodes_per_step": work / steps, "iterations_per_step": float(np.mean(
                [s.last_solver_iterations for s in solvers]))}

@@ -165,6 +165,10 @@ int gm_ensemble_step(gm_solver *const *members, int32_t n, int32_t step_id, gm_d
  * returns the first failing status. */
 typedef struct gm_batch gm_batch;
 int gm_batch_create(gm_solver *const *members, int32_t n, gm_batch **out);
+/* The same with the members split into `lanes` subsets of about equal work,
+ * each with its own stream, ragged grids and graphs: the lanes' launches
+ * overlap on the device (gm_batch_create is one lane). */
+int gm_batch_create_lanes(gm_solver *const *members, int32_t n, int32_t lanes, gm_batch **out);
 void gm_batch_destroy(gm_batch *batch);
 int gm_batch_step(gm_batch *batch, int32_t step_id, gm_diag *diags);
 const char *gm_batch_last_error(const gm_batch *batch);

@@ -106,11 +106,12 @@ def test_apply_tile_flag_is_validated():
         s.set_flags(apply_tile=48)
 
 
-def test_batch_equals_members_alone():
-    print("batched ensemble of 3 ragged members, 12 steps", flush=True)
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_batch_equals_members_alone(lanes):
+    print(f"batched ensemble of 3 ragged members in {lanes} lanes, 12 steps", flush=True)
     together = [make(m) for m in MEMBERS]
     alone = [make(m) for m in MEMBERS]
-    batch = gm.Batch(together)
+    batch = gm.Batch(together, lanes)
     for n in range(1, 13):
         dt = batch.step(n)
         da = [s.step(n) for s in alone]
