@@ -84,9 +84,13 @@ def coupling_terms(l_max):
     return n
 
 
-def make_solver(w, precision):
+def make_solver(w, precision, evolve=False):
     s = gm.CrankNicolsonModes(**w)
-    s.set_flags(evolve_orbit=0, precision=precision)
+    if evolve:
+        # back-reaction on; dt = P / steps_per_period stays at the workload's dt
+        s.set_flags(evolve_orbit=1, precision=precision, steps_per_period=s.P / w["dt"])
+    else:
+        s.set_flags(evolve_orbit=0, precision=precision)
     s.set_initial_state_full(initial_state(w))
     s.initialize()
     s.build_damping_mask(*MASK)
@@ -140,6 +144,35 @@ def run_gpu(w, precision, steps, warmup, prof_steps=5, graphs=True):
     prof = profile_read(s)
     profile(s, False)
     return s, t1 - t0, prof, iters, piters
+
+
+def evolving(w, steps, warmup, fixed):
+    """The workload with the orbit evolving (GW and cloud back-reaction, the
+    bodies accreting through the absorbers): ms per step per precision and the
+    ratio to the fixed-orbit step measured above."""
+    out = {}
+    for name, precision in (("fp64", gm.GM_PRECISION_FP64), ("mixed", gm.GM_PRECISION_MIXED)):
+        s = make_solver(w, precision, evolve=True)
+        k = 0
+        for _ in range(warmup):
+            k += 1
+            s.step(k, diag=True)
+        iters = []
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            k += 1
+            s.step(k, diag=True)
+            iters.append(s.last_solver_iterations)
+        device_sync()
+        el = time.perf_counter() - t0
+        sc = s.scalars()
+        out[name] = {"ms_per_step": el / steps * 1e3, "iterations_per_step": float(np.mean(iters)),
+                     "over_fixed_orbit_step": el / steps * 1e3 / fixed[name]["ms_per_step"],
+                     "R_after": sc["R"], "dt_after": sc["dt"]}
+        del s
+    out["steps"] = steps
+    return out
 
 
 def cpu_baseline(w, runs=3, steps_per_run=8):
@@ -459,14 +492,19 @@ def main():
                     help="untimed steps of the mixed-vs-FP64 comparison (0 skips it)")
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the batched ragged ensemble measurement (0 skips it)")
-    ap.add_argument("--ensemble-steps", type=int, default=20,
-                    help="timed steps of the ensemble (after 3 warmup steps)")
+    ap.add_argument("--ensemble-steps", type=int, default=500,
+                    help="timed steps of the batched ensemble (after 3 warmup steps); the per-member-stream "
+                         "comparison runs a tenth of them")
+    ap.add_argument("--ensemble-lanes", type=int, default=4,
+                    help="member subsets of the batched ensemble, each on its own stream")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the served Poisson stream for p50/p99 step latency (0 skips it)")
     ap.add_argument("--large-steps", type=int, default=10,
                     help="timed steps per precision at l_max 30, N 32768 (0 skips the second size)")
     ap.add_argument("--large-accuracy-steps", type=int, default=100,
                     help="lockstep mixed-vs-FP64 steps at l_max 30, N 32768 (0 skips them)")
+    ap.add_argument("--evolving-steps", type=int, default=50,
+                    help="timed steps per precision with the orbit evolving (0 skips them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="timed steps without graph replay (for kernel-trace profiling runs)")
@@ -512,6 +550,8 @@ def main():
         "coupling_work": {"terms_per_apply": terms, "pairs_per_apply": pairs,
                           "untruncated_terms_per_apply": float(coupling_terms(w["l_max"])) * (N - 2)},
     }
+    if args.evolving_steps > 0:
+        out["evolving_orbit"] = evolving(w, args.evolving_steps, 5, runs)
     if args.accuracy_steps > 0:
         out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
     if args.large_steps > 0:
@@ -521,10 +561,12 @@ def main():
         cfg = ("ragged fixed-orbit binaries from seed 0: q log-uniform [0.05, 1], R [20, 60], mu [0.1, 0.3], "
                "N in {4096..32768} (L = N/4), l_max in {8..24}, 2p atom on the primary + 1e-3 perturbation; "
                "mixed precision")
-        ens = ensemble.run(n=args.ensemble_members, steps=args.ensemble_steps, warmup=3, seed=0, mode="batch")
-        ens["config"] = cfg + "; one ragged grid per launch over all members, graph-captured step"
+        ens = ensemble.run(n=args.ensemble_members, steps=args.ensemble_steps, warmup=3, seed=0, mode="batch",
+                           lanes=args.ensemble_lanes)
+        ens["config"] = cfg + ("; every launch of the step one ragged grid over the members of a lane (%d lanes "
+                               "of balanced work, each on its own stream), graph-captured" % args.ensemble_lanes)
         ens["over_fp64_single_step"] = ens["mode_node_steps_per_s"] / f64["value"]
-        streams = ensemble.run(n=args.ensemble_members, steps=args.ensemble_steps, warmup=3, seed=0,
+        streams = ensemble.run(n=args.ensemble_members, steps=max(1, args.ensemble_steps // 10), warmup=3, seed=0,
                                mode="streams")
         streams["over_fp64_single_step"] = streams["mode_node_steps_per_s"] / f64["value"]
         ens["per_member_streams"] = streams

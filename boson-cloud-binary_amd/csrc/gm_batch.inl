@@ -23,6 +23,11 @@ struct BatchSeg {
     gmk::BatchLaunch L{};
     DevBuf<unsigned char> args;
     DevBuf<int> off;
+    int pool_device = -1;  // BK_APPLY: the argument entries held in the constant pool
+    ~BatchSeg()
+    {
+        if (L.pool >= 0) gmk::apply_pool_release(pool_device, L.pool, L.M);
+    }
 };
 
 // One ragged launch of `kernel` over the members in `ms`, with per-member
@@ -49,6 +54,17 @@ void add_seg(std::vector<std::unique_ptr<BatchSeg>> &out, int kernel, const std:
     seg->L.mixed = mixed;
     seg->L.var = var;
     seg->L.f32 = f32;
+    seg->L.pool = -1;
+    if constexpr (std::is_same<A, gmk::ApplyArgs>::value) {
+        // the apply reads its arguments from the constant pool while it has room
+        int first = -1, dev = -1;
+        HIPCK(hipGetDevice(&dev));
+        if (kernel == gmk::BK_APPLY && gmk::apply_pool_acquire(M, &first) == hipSuccess) {
+            seg->L.pool = first;
+            seg->pool_device = dev;
+            HIPCK(gmk::apply_pool_write(first, args.data(), M));
+        }
+    }
     out.push_back(std::move(seg));
 }
 

@@ -216,25 +216,26 @@ __global__ void k_build_E(int N, int Ls, double dr, double eps, double mu, const
 
 // Largest multipole l* of each node tile whose profile exceeds tau times the
 // tile's leading profile (one wavefront per tile). tau = 0 keeps every l*.
+// Each lane first takes its own node's maximum over l* and its own highest
+// l* above the threshold; a wave maximum of each gives the tile's values
+// (max over nodes of the per-node highest l* = highest l* whose maximum over
+// nodes passes).
 __global__ __launch_bounds__(64) void k_lcut(int N, int Ls, double tau, const double *E, int *lcut)
 {
     const int tile = blockIdx.x;
     const int i = tile * gmd::TILE + threadIdx.x;
+    auto prof = [&](int ls) {
+        return i < N ? fmax(fabs(E[(size_t)ls * N + i]), fabs(E[(size_t)(Ls + ls) * N + i])) : 0.0;
+    };
     double gmax = 0.0;
-    for (int ls = 0; ls < Ls; ++ls) {
-        double v = 0.0;
-        if (i < N) v = fmax(fabs(E[(size_t)ls * N + i]), fabs(E[(size_t)(Ls + ls) * N + i]));
-        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-        gmax = fmax(gmax, v);
-    }
+    for (int ls = 0; ls < Ls; ++ls) gmax = fmax(gmax, prof(ls));
+    for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o));
     int keep = tau > 0.0 ? 0 : Ls - 1;
-    if (tau > 0.0)
-        for (int ls = 0; ls < Ls; ++ls) {
-            double v = 0.0;
-            if (i < N) v = fmax(fabs(E[(size_t)ls * N + i]), fabs(E[(size_t)(Ls + ls) * N + i]));
-            for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-            if (v > tau * gmax) keep = ls;
-        }
+    if (tau > 0.0) {
+        for (int ls = 0; ls < Ls; ++ls)
+            if (prof(ls) > tau * gmax) keep = ls;
+        for (int o = 32; o > 0; o >>= 1) keep = max(keep, __shfl_xor(keep, o));
+    }
     if (threadIdx.x == 0) lcut[tile] = keep;
 }
 
@@ -650,35 +651,33 @@ __device__ __forceinline__ void k_apply_body(const ApplyArgs &a, Blk B)
                 Kr->counter = 0;
             }
         }
-    } else if (MODE == AM_RES) {
+    } else if (MODE == AM_RES || MODE == AM_RST) {
+        // (one reduction site for both: a second copy of the block reduction
+        // raised the register count of every apply instance by a fifth)
         if (block_partials<1>({red[0]}, a.partial, &Kr->counter, B)) {
             const double s = sum_partials(a.partial, 0, B);
             if (threadIdx.x == 0) {
-                Kr->res2 = s;
-                Kr->counter = 0;
-            }
-        }
-    } else if (MODE == AM_RST) {
-        // Lost bi-orthogonality (matrices.cpp:67-74, and Eigen's BiCGSTAB):
-        // restart from the true residual, r0 = r, rho = |r|^2 (scaled by
-        // |rhs| with the unnormalised shadow residual); Eigen also takes the
-        // new |r0|^2 as its threshold base and resets the iteration counter
-        // on the first restart. The iteration then goes on from beta with
-        // the old p, v, alpha and w.
-        if (block_partials<1>({red[0]}, a.partial, &Kr->counter, B)) {
-            const double s = sum_partials(a.partial, 0, B);
-            if (threadIdx.x == 0) {
-                const double rho = Kr->rhs_nrm * s;
-                Kr->rho = make_double2(rho, 0.0);
-                if (Kr->eigen_mode) {
-                    Kr->r0_sq = rho;
-                    if (Kr->restarts == 0) Kr->iter = 0;
+                if (MODE == AM_RES) {
+                    Kr->res2 = s;
+                } else {
+                    // Lost bi-orthogonality (matrices.cpp:67-74, and Eigen's BiCGSTAB):
+                    // restart from the true residual, r0 = r, rho = |r|^2 (scaled by
+                    // |rhs| with the unnormalised shadow residual); Eigen also takes the
+                    // new |r0|^2 as its threshold base and resets the iteration counter
+                    // on the first restart. The iteration then goes on from beta with
+                    // the old p, v, alpha and w.
+                    const double rho = Kr->rhs_nrm * s;
+                    Kr->rho = make_double2(rho, 0.0);
+                    if (Kr->eigen_mode) {
+                        Kr->r0_sq = rho;
+                        if (Kr->restarts == 0) Kr->iter = 0;
+                    }
+                    Kr->restarts += 1;
+                    Kr->r2 = s;
+                    Kr->beta = cmul(cdiv(Kr->rho, Kr->rho_old), cdiv(Kr->alpha, Kr->w));
+                    Kr->restart = 0;
+                    Kr->done = 0;
                 }
-                Kr->restarts += 1;
-                Kr->r2 = s;
-                Kr->beta = cmul(cdiv(Kr->rho, Kr->rho_old), cdiv(Kr->alpha, Kr->w));
-                Kr->restart = 0;
-                Kr->done = 0;
                 Kr->counter = 0;
             }
         }
@@ -694,6 +693,18 @@ __global__ __launch_bounds__(64 * gmd::AW) void k_apply_batch(const ApplyArgs *_
     const ApplyArgs a = args[m];
     k_apply_body<NG, T>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
                              __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+}
+// The same launch with the members' arguments at apply_pool[first + m]: from
+// constant memory the compiler reloads fields where they are used, and the
+// kernel keeps the register count of a plain launch (a global argument table
+// cost about 30 more VGPRs, and with them waves per SIMD).
+__constant__ ApplyArgs apply_pool[APPLY_POOL];
+template <int NG, typename T>
+__global__ __launch_bounds__(64 * gmd::AW) void k_apply_pooled(int first, const int *__restrict__ off, int M)
+{
+    const int m = batch_member(off, M);
+    k_apply_body<NG, T>(apply_pool[first + m], Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                                                   __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
 // Diagnostics apply on the post-mask state (extraction.cpp:8-129). The
@@ -793,14 +804,9 @@ struct Mob {
     double2 a, b, c, d;  // [[a, b], [c, d]] acting on (u, v), pivot = u / v
 };
 
-// first, then second: second * first, rescaled (the pivot is a ratio)
-__device__ __forceinline__ Mob mob_compose(const Mob &x, const Mob &y)
+// Divide the map by its largest entry (the pivot is a ratio).
+__device__ __forceinline__ void mob_rescale(Mob &r)
 {
-    Mob r;
-    r.a = cadd(cmul(y.a, x.a), cmul(y.b, x.c));
-    r.b = cadd(cmul(y.a, x.b), cmul(y.b, x.d));
-    r.c = cadd(cmul(y.c, x.a), cmul(y.d, x.c));
-    r.d = cadd(cmul(y.c, x.b), cmul(y.d, x.d));
     double m = fmax(fmax(fabs(r.a.x), fabs(r.a.y)), fmax(fabs(r.b.x), fabs(r.b.y)));
     m = fmax(m, fmax(fmax(fabs(r.c.x), fabs(r.c.y)), fmax(fabs(r.d.x), fabs(r.d.y))));
     if (m > 0.0 && isfinite(m)) {
@@ -810,6 +816,39 @@ __device__ __forceinline__ Mob mob_compose(const Mob &x, const Mob &y)
         r.c = cscale(sc, r.c);
         r.d = cscale(sc, r.d);
     }
+}
+
+// first, then second: second * first, rescaled
+__device__ __forceinline__ Mob mob_compose(const Mob &x, const Mob &y)
+{
+    Mob r;
+    r.a = cadd(cmul(y.a, x.a), cmul(y.b, x.c));
+    r.b = cadd(cmul(y.a, x.b), cmul(y.b, x.d));
+    r.c = cadd(cmul(y.c, x.a), cmul(y.d, x.c));
+    r.d = cadd(cmul(y.c, x.b), cmul(y.d, x.d));
+    mob_rescale(r);
+    return r;
+}
+
+// x, then an interior node's map (u, v) -> (d u + nl2 v, u), unscaled
+__device__ __forceinline__ Mob mob_interior(const Mob &x, double2 d, double2 nl2)
+{
+    Mob r;
+    r.a = cadd(cmul(d, x.a), cmul(nl2, x.c));
+    r.b = cadd(cmul(d, x.b), cmul(nl2, x.d));
+    r.c = x.a;
+    r.d = x.b;
+    return r;
+}
+
+// x, then a boundary node's reset (u, v) -> (d v, v), unscaled
+__device__ __forceinline__ Mob mob_reset(const Mob &x, double2 d)
+{
+    Mob r;
+    r.a = cmul(d, x.c);
+    r.b = cmul(d, x.d);
+    r.c = x.c;
+    r.d = x.d;
     return r;
 }
 
@@ -842,6 +881,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
     const double *Gjj = f.Gdiag + (size_t)j * (f.lmax + 1);
     const double2 ell = make_double2(0.0, f.P->half_dt * f.off);
     const double2 ell2 = cmul(ell, ell);
+    const double2 nl2 = make_double2(-ell2.x, -ell2.y);
     const int N = f.N;
     const int ntile = (N + TE - 1) / TE;
     double2 carry = make_double2(1.0, 0.0);  // pivot before node 0 (unused: node 0 resets)
@@ -854,35 +894,19 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
         loc.b = make_double2(0.0, 0.0);
         loc.c = make_double2(0.0, 0.0);
         loc.d = make_double2(1.0, 0.0);
+        // the thread's E maps (identity beyond the grid), rescaled once: the
+        // entries of a product of E maps stay far inside the double range
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int i = i0 + e;
-            Mob m;
             if (i >= N) {
-                m = loc;  // identity beyond the grid
-                m.a = make_double2(1.0, 0.0);
-                m.b = make_double2(0.0, 0.0);
-                m.c = make_double2(0.0, 0.0);
-                m.d = make_double2(1.0, 0.0);
                 dv[e] = make_double2(1.0, 0.0);
             } else {
                 dv[e] = cn_diag(f, Gjj, l, i);
-                if (i >= 2 && i <= N - 2) {
-                    // (u, v) -> (d u - l^2 v, u)
-                    m.a = dv[e];
-                    m.b = make_double2(-ell2.x, -ell2.y);
-                    m.c = make_double2(1.0, 0.0);
-                    m.d = make_double2(0.0, 0.0);
-                } else {
-                    // reset: (u, v) -> (d v, v)
-                    m.a = make_double2(0.0, 0.0);
-                    m.b = dv[e];
-                    m.c = make_double2(0.0, 0.0);
-                    m.d = make_double2(1.0, 0.0);
-                }
+                loc = i >= 2 && i <= N - 2 ? mob_interior(loc, dv[e], nl2) : mob_reset(loc, dv[e]);
             }
-            loc = mob_compose(loc, m);
         }
+        mob_rescale(loc);
         // block exclusive scan of the thread maps (increasing node order)
         Mob inc = loc;
 #pragma unroll
@@ -2036,7 +2060,11 @@ template <int NG>
 static void launch_apply_batch_ng(const BatchLaunch &b, hipStream_t st)
 {
     const ApplyArgs *a = static_cast<const ApplyArgs *>(b.args);
-    if (b.mixed)
+    if (b.pool >= 0 && b.mixed)
+        hipLaunchKernelGGL((k_apply_pooled<NG, float>), dim3(b.total), dim3(64 * gmd::AW), 0, st, b.pool, b.off, b.M);
+    else if (b.pool >= 0)
+        hipLaunchKernelGGL((k_apply_pooled<NG, double>), dim3(b.total), dim3(64 * gmd::AW), 0, st, b.pool, b.off, b.M);
+    else if (b.mixed)
         hipLaunchKernelGGL((k_apply_batch<NG, float>), dim3(b.total), dim3(64 * gmd::AW), 0, st, a, b.off, b.M);
     else
         hipLaunchKernelGGL((k_apply_batch<NG, double>), dim3(b.total), dim3(64 * gmd::AW), 0, st, a, b.off, b.M);
@@ -2150,6 +2178,65 @@ hipError_t sink_scratch_address(SinkScratch **out)
     }
     *out = scratch_ptr[dev];
     return hipSuccess;
+}
+
+namespace {
+std::mutex pool_mu;
+std::vector<char> pool_used[kMaxDevices];
+ApplyArgs *pool_addr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t apply_pool_acquire(int count, int *first)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (count < 1 || count > APPLY_POOL) return hipErrorOutOfMemory;
+    std::lock_guard<std::mutex> lk(pool_mu);
+    std::vector<char> &used = pool_used[dev];
+    if (used.empty()) used.assign(APPLY_POOL, 0);
+    // first fit
+    int run = 0;
+    for (int i = 0; i < APPLY_POOL; ++i) {
+        run = used[i] ? 0 : run + 1;
+        if (run == count) {
+            const int f = i + 1 - count;
+            for (int k = f; k <= i; ++k) used[k] = 1;
+            *first = f;
+            return hipSuccess;
+        }
+    }
+    return hipErrorOutOfMemory;
+}
+
+void apply_pool_release(int device, int first, int count)
+{
+    if (device < 0 || device >= kMaxDevices || first < 0) return;
+    std::lock_guard<std::mutex> lk(pool_mu);
+    std::vector<char> &used = pool_used[device];
+    for (int k = first; k < first + count && k < (int)used.size(); ++k) used[k] = 0;
+}
+
+hipError_t apply_pool_write(int first, const ApplyArgs *src, int count)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (first < 0 || count < 0 || first + count > APPLY_POOL) return hipErrorInvalidValue;
+    ApplyArgs *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(pool_mu);
+        if (!pool_addr[dev]) {
+            void *p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(apply_pool));
+            if (e != hipSuccess) return e;
+            pool_addr[dev] = static_cast<ApplyArgs *>(p);
+        }
+        base = pool_addr[dev];
+    }
+    return hipMemcpy(base + first, src, sizeof(ApplyArgs) * count, hipMemcpyHostToDevice);
 }
 
 hipError_t launch_sink_apply(const SinkApplyArgs &a, hipStream_t st)

@@ -202,7 +202,15 @@ struct BatchLaunch {
     int M, total;
     int ng, mixed;     // BK_APPLY: tile and precision of every member of the launch
     int var, f32;      // BK_PRECOND: variant and sweep precision
+    int pool;          // BK_APPLY: first of the M argument entries in the constant pool, or -1 (args)
 };
+// Argument entries of batched applies in constant memory, per device: the
+// kernel reads its member's fields with scalar, invariant loads where it
+// needs them instead of holding them in registers.
+constexpr int APPLY_POOL = 2048;
+hipError_t apply_pool_acquire(int count, int *first);  // current device; hipErrorOutOfMemory when full
+void apply_pool_release(int device, int first, int count);
+hipError_t apply_pool_write(int first, const ApplyArgs *src, int count);  // synchronous, current device
 hipError_t launch_batch(const BatchLaunch &b, hipStream_t st);
 // Blocks of the plain launches, for laying out batched ones.
 int precond_blocks(int K);
