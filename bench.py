@@ -146,13 +146,15 @@ def run_gpu(w, precision, steps, warmup, prof_steps=5, graphs=True):
     return s, t1 - t0, prof, iters, piters
 
 
-def evolving(w, steps, warmup, fixed):
+def evolving(w, steps, warmup, fixed, graphs=True):
     """The workload with the orbit evolving (GW and cloud back-reaction, the
     bodies accreting through the absorbers): ms per step per precision and the
     ratio to the fixed-orbit step measured above."""
     out = {}
     for name, precision in (("fp64", gm.GM_PRECISION_FP64), ("mixed", gm.GM_PRECISION_MIXED)):
         s = make_solver(w, precision, evolve=True)
+        if not graphs:
+            s.set_flags(use_graphs=0)
         k = 0
         for _ in range(warmup):
             k += 1
@@ -483,6 +485,183 @@ def copy_calibration(nbytes=1 << 29, reps=5):
                       "write_kib_expected": nbytes / 1024}))
 
 
+# ---------------------------------------------------------------- rocprofv3 summaries
+
+def _short_kernel(name):
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    name = name.replace("gmk::", "")
+    i = name.find("(")
+    return name[:i] if i > 0 else name
+
+
+def _read_csv(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def trace_launches(path):
+    """kernel -> list of launch durations (us) from a --kernel-trace csv."""
+    out = {}
+    for r in _read_csv(path):
+        k = _short_kernel(r["Kernel_Name"])
+        out.setdefault(k, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    return out
+
+
+def working_us(durs, floor_us=50.0):
+    w = [d for d in durs if d >= floor_us]
+    return w
+
+
+def counters(path):
+    """kernel -> list of {counter: value} per dispatch from a --pmc csv."""
+    per = {}
+    for r in _read_csv(path):
+        key = (r.get("Dispatch_Id") or r.get("Correlation_Id"), _short_kernel(r["Kernel_Name"]))
+        per.setdefault(key, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+    out = {}
+    for (_, k), d in per.items():
+        out.setdefault(k, []).append(d)
+    return out
+
+
+def _mean(v):
+    return sum(v) / len(v) if v else float("nan")
+
+
+def summary_trace(trace_csv):
+    t = trace_launches(trace_csv)
+    total = sum(sum(v) for v in t.values())
+    lines = ["kernel                                 calls     total_ms       avg_us     pct |  working  avg_work_us"]
+    for k, v in sorted(t.items(), key=lambda kv: -sum(kv[1])):
+        w = working_us(v)
+        lines.append(f"{k:38s} {len(v):6d} {sum(v) / 1e3:12.3f} {_mean(v):12.1f} {100 * sum(v) / total:7.2f} | "
+                     f"{len(w):8d} {_mean(w):12.1f}")
+    lines.append("")
+    lines.append("working = launches of at least 50 us (launches enqueued after BiCGSTAB convergence return at once)")
+    return "\n".join(lines) + "\n"
+
+
+def _traffic(fetch_csv, write_csv, cfetch_csv, cwrite_csv):
+    f, w = counters(fetch_csv), counters(write_csv)
+    cf, cw = counters(cfetch_csv), counters(cwrite_csv)
+    copy = [k for k in cf if "copyBuffer" in k]
+    moved_kib = (1 << 29) / 1024.0
+    fs = moved_kib / _mean([d["FETCH_SIZE"] for d in cf[copy[0]]]) if copy else 1.0
+    ws = moved_kib / _mean([d["WRITE_SIZE"] for d in cw[copy[0]]]) if copy else 1.0
+    rows = {}
+    for k in f:
+        fv = [d["FETCH_SIZE"] for d in f[k] if d["FETCH_SIZE"] > 1024.0]
+        wv = [d["WRITE_SIZE"] for d in w.get(k, [])]
+        wv = sorted(wv)[-len(fv):] if fv and wv else wv
+        if not fv:
+            continue
+        fk, wk = _mean(fv), _mean(wv) if wv else 0.0
+        rows[k] = (len(fv), fk, wk, (fk * fs + wk * ws) * 1024.0)
+    return rows, fs, ws, (_mean([d["FETCH_SIZE"] for d in cf[copy[0]]]) if copy else None), \
+        (_mean([d["WRITE_SIZE"] for d in cw[copy[0]]]) if copy else None)
+
+
+def _main_variant(rows, prefix, dtype):
+    cands = [(v[0], k) for k, v in rows.items() if k.startswith(prefix + "<") and k.endswith(dtype + ">")]
+    return max(cands)[1] if cands else None
+
+
+def summary_traffic(fetch_csv, write_csv, cfetch_csv, cwrite_csv):
+    rows, fs, ws, cfk, cwk = _traffic(fetch_csv, write_csv, cfetch_csv, cwrite_csv)
+    lines = []
+    if cfk is not None:
+        lines.append(f"calibration: a 512 MiB device copy (__amd_rocclr_copyBuffer) reads FETCH_SIZE {cfk:.1f} KiB and "
+                     f"WRITE_SIZE {cwk:.1f} KiB per launch against 524288 KiB moved each way")
+    lines.append(f"scale factors: fetch x{fs:.3f}, write x{ws:.3f}")
+    lines.append("")
+    lines.append("kernel                                 working      fetch_KiB      write_KiB  bytes(scaled)")
+    for k, (n, fk, wk, b) in sorted(rows.items(), key=lambda kv: -kv[1][3] * kv[1][0]):
+        lines.append(f"{k:38s} {n:8d} {fk:14.1f} {wk:14.1f} {b:14.4g}")
+    lines.append("")
+    lines.append("working = launches that fetch more than 1 MiB (launches after convergence return at once)")
+    for key, dt in (("k_apply", "double"), ("k_apply_fp32", "float")):
+        k = _main_variant(rows, "k_apply", dt)
+        if k:
+            lines.append(f"traffic_bytes_per_launch {key} {rows[k][3]:.6g}")
+    return "\n".join(lines) + "\n"
+
+
+def summary_rates(trace_csv, fetch_csv, write_csv, cfetch_csv, cwrite_csv):
+    t = trace_launches(trace_csv)
+    rows, fs, ws, _, _ = _traffic(fetch_csv, write_csv, cfetch_csv, cwrite_csv)
+    lines = ["kernel                                    bytes    time_us       GB/s  of_peak"]
+    out = []
+    for k, (n, fk, wk, b) in rows.items():
+        if k not in t:
+            continue
+        w = working_us(t[k], 10.0) or t[k]
+        us = _mean(w)
+        out.append((b / us / 1e3, k, b, us))
+    for gbs, k, b, us in sorted(out, reverse=True):
+        lines.append(f"{k:38s} {b:10.4g} {us:10.1f} {gbs:10.0f} {100 * gbs / HBM_PEAK_GBS:7.1f}%")
+    return "\n".join(lines) + "\n"
+
+
+SQ1 = ("SQ_WAVES", "SQ_INSTS_MFMA", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_WAVE_CYCLES",
+       "SQ_WAIT_INST_ANY")
+SQ2 = ("GRBM_GUI_ACTIVE", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_BUSY_CU_CYCLES", "SQ_INSTS_LDS", "SQ_INSTS_SMEM")
+
+
+def summary_sq(pass1_csv, pass2_csv, trace_csv):
+    t = trace_launches(trace_csv)
+    tabs = []
+    vals = {}
+    for path, names, weight in ((pass1_csv, SQ1, "SQ_INSTS_VALU"), (pass2_csv, SQ2, "SQ_BUSY_CU_CYCLES")):
+        c = counters(path)
+        head = "kernel                                      ms " + "".join(f"{n[3:] if n.startswith('SQ_') else n:>22s}"
+                                                                             for n in names)
+        lines = [head]
+        for k in sorted(c, key=lambda k: -_mean([d.get(weight, 0.0) for d in c[k]])):
+            ds = [d for d in c[k] if weight in d]
+            if not ds:
+                continue
+            mx = max(d[weight] for d in ds)
+            ds = [d for d in ds if d[weight] >= 0.25 * mx]
+            if mx <= 0 or not (k.startswith("k_apply<") or k.startswith("k_precond<") or k in ("k_update", "k_diag_apply")):
+                continue
+            w = working_us(t.get(k, []), 10.0)
+            ms = _mean(w) / 1e3 if w else float("nan")
+            row = {n: _mean([d.get(n, 0.0) for d in ds]) for n in names}
+            vals.setdefault(k, {}).update(row)
+            lines.append(f"{k:38s} {ms:10.4f}" + "".join(f"{row[n]:22.4g}" for n in names))
+        tabs.append("\n".join(lines))
+    notes = []
+    simds, xcds = 1024, 8
+    for k, v in sorted(vals.items()):
+        if not k.startswith("k_apply<") or "GRBM_GUI_ACTIVE" not in v or "SQ_INSTS_MFMA" not in v:
+            continue
+        cyc = v["GRBM_GUI_ACTIVE"] / xcds
+        per = 64 if k.endswith("double>") else 32
+        notes.append(f"{k}: {cyc:.0f} cycles per XCD; matrix core busy {100 * v['SQ_INSTS_MFMA'] * per / (cyc * simds):.0f}% "
+                     f"of each SIMD's cycles ({per} cycles per MFMA); VALU at 4 cycles per wave64 instruction "
+                     f"{100 * v['SQ_INSTS_VALU'] * 4 / (cyc * simds):.0f}%; mean resident waves per SIMD "
+                     f"{v['SQ_WAVE_CYCLES'] * 4 / (cyc * simds):.1f}; waves waiting on s_waitcnt "
+                     f"{100 * v['SQ_WAIT_INST_ANY'] / v['SQ_WAVE_CYCLES']:.0f}% of their cycles")
+    return "\n\n".join(tabs) + "\n\n" + "\n".join(notes) + "\n"
+
+
+def rocprof_summary(argv):
+    kind, rest = argv[0], argv[1:]
+    if kind == "trace":
+        return summary_trace(*rest)
+    if kind == "traffic":
+        return summary_traffic(*rest)
+    if kind == "rates":
+        return summary_rates(*rest)
+    if kind == "sq":
+        return summary_sq(*rest)
+    raise SystemExit("unknown summary kind " + kind)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -510,7 +689,14 @@ def main():
                     help="timed steps without graph replay (for kernel-trace profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
+    ap.add_argument("--rocprof-summary", nargs="+", metavar="ARG",
+                    help="summarise rocprofv3 csv output instead of measuring: trace TRACE_CSV | "
+                         "traffic FETCH_CSV WRITE_CSV COPY_FETCH_CSV COPY_WRITE_CSV | sq PASS1_CSV PASS2_CSV "
+                         "TRACE_CSV | rates TRACE_CSV FETCH_CSV WRITE_CSV COPY_FETCH_CSV COPY_WRITE_CSV")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        print(rocprof_summary(args.rocprof_summary), end="")
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -551,7 +737,7 @@ def main():
                           "untruncated_terms_per_apply": float(coupling_terms(w["l_max"])) * (N - 2)},
     }
     if args.evolving_steps > 0:
-        out["evolving_orbit"] = evolving(w, args.evolving_steps, 5, runs)
+        out["evolving_orbit"] = evolving(w, args.evolving_steps, 5, runs, graphs=not args.no_graphs)
     if args.accuracy_steps > 0:
         out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
     if args.large_steps > 0:

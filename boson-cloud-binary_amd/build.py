@@ -66,8 +66,9 @@ def build(clean=False, verbose=True):
 
 CPP_TESTS = os.path.join(REPO, "tests", "cpp")
 # C++ programs over the headers in include/: the gm.hpp class driver, the
-# reference's main driver over the drop-in class, and the config reader probe
-DRIVERS = (("cnm_driver.cpp", True), ("gm_main.cpp", True), ("config_probe.cpp", False))
+# reference's main driver over the drop-in class, the config reader probe and
+# the reference's operator checks over the drop-in class
+DRIVERS = (("cnm_driver.cpp", True), ("gm_main.cpp", True), ("config_probe.cpp", False), ("physics_ops.cpp", True))
 
 
 def build_driver(verbose=True):

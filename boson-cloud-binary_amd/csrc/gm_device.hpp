@@ -20,7 +20,10 @@ namespace gmd {
 
 constexpr int TILE = 64;   // radial nodes of a multipole-cut tile
 constexpr int DIAG_NODES = 256;  // radial nodes per diagnostics wavefront (4 per lane)
-constexpr int AW = 4;      // row chunks (wavefronts) per operator-apply workgroup
+#ifndef GM_AW
+#define GM_AW 4
+#endif
+constexpr int AW = GM_AW;  // row chunks (wavefronts) per operator-apply workgroup
 constexpr int MAXLM = 40;  // largest supported l_max + 1
 constexpr int PT = 512;    // threads of a preconditioner or factor workgroup
 constexpr int PE = 4;      // consecutive nodes per preconditioner thread
@@ -45,7 +48,10 @@ struct MBlock {
 
 // Depth of the operand ring of the coupling stream; each of a chunk's step
 // lists ends with RING - 1 padding steps that add nothing.
-constexpr int RING = 4;
+#ifndef GM_RING
+#define GM_RING 4
+#endif
+constexpr int RING = GM_RING;
 // LDS row read by padding steps (every row above a tile's cut holds zeros).
 constexpr int ZROW = 2 * MAXLM - 1;
 // LDS rows of the staged profiles: a paired padding step reads ZROW .. ZROW + 14.
@@ -108,6 +114,7 @@ struct StepParams {
     // orbit of the radial profiles: positions of the bodies, mass ratio,
     // total mass and the companion's ramp weight
     double R1, R2, q, Mt, prf;
+    double w1;                  // weight of the primary's profile (1; 0 only to read off V2 alone)
 };
 
 // BiCGSTAB state, device resident (src/CrankNicolsonModes_matrices.cpp:47-91).

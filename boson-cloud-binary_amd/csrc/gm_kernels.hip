@@ -194,7 +194,7 @@ __global__ void k_build_E(int N, int Ls, double dr, double eps, double mu, const
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= Ls * N) return;
-    const double R1 = P->R1, R2 = P->R2, q = P->q, Mt = P->Mt, prf = P->prf;
+    const double R1 = P->R1, R2 = P->R2, q = P->q, Mt = P->Mt, prf = P->prf, w1 = P->w1;
     const int ls = idx / N, i = idx - ls * N;
     double e0 = 0.0, e1 = 0.0;
     if (i > 0 && i < N - 1) {
@@ -205,8 +205,8 @@ __global__ void k_build_E(int N, int Ls, double dr, double eps, double mu, const
         if (r >= R2) f2 = q / (1 + q) * pow(R2, (double)ls) / pow(r, (double)(ls + 1));
         else f2 = q / (1 + q) * pow(r, (double)ls) / pow(R2, (double)(ls + 1));
         const double c = -mu * Mt * 4.0 * M_PI / (2.0 * ls + 1.0);
-        e0 = c * (f1 + prf * f2);
-        e1 = c * (f1 - prf * f2);
+        e0 = c * (w1 * f1 + prf * f2);
+        e1 = c * (w1 * f1 - prf * f2);
     }
     E[(size_t)ls * N + i] = e0;
     E[(size_t)(Ls + ls) * N + i] = e1;

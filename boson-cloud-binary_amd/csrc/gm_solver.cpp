@@ -78,6 +78,13 @@ struct PinBuf {
     ~PinBuf() { release(); }
 };
 
+// A sparse operator in CSR, rows sorted by column (gm_operator_csr).
+struct OpCsr {
+    std::vector<int64_t> ptr;
+    std::vector<int32_t> col;
+    std::vector<std::complex<double>> val;
+};
+
 struct Profile {
     double radius = -1.0, width = -1.0;
     bool valid = false;
@@ -187,6 +194,9 @@ struct gm_solver {
     // sinks, the reference's 1e-32 with them (gm_set_restart_thresholds)
     double rst_eps2 = 4.930380657631324e-32, rst_ref = 1e-32;
     int last_restarts = 0;
+    // gm_operator_csr: the operator of a size query, kept for the fill call
+    OpCsr op_cache;
+    int op_which = -1;
     // progress of the step in flight (step_begin / solve_advance / step_finish)
     double step_start = 0.0;
     bool step_wood = false;
@@ -712,6 +722,7 @@ void build_perturbation(gm_solver &S, double time, bool defer = false)
     P.q = S.q;
     P.Mt = S.M1 + S.M2;
     P.prf = S.prf;
+    P.w1 = 1.0;
     for (int m = -S.l_max; m <= S.l_max; ++m)
         P.ph[m + S.l_max] = make_double2(std::cos(m * S.theta), std::sin(m * S.theta));
     S.Omega_V = S.Omega;
@@ -1575,6 +1586,173 @@ gmh::CsrZ extract_H(gm_solver &S)
         std::vector<std::pair<int, gmh::cplx>>().swap(row);
     }
     return H;
+}
+
+// ------------------------------------------------------------------ public operators
+
+// The operators the reference keeps as public sparse members (H, H0, A, B, V1,
+// V2, V, dVdt, Id, CAP; include/CrankNicolsonModes.hpp:197-206) at the current
+// step, read off the device tables in the reference's stacking (row j Nr + i).
+// H and the coupling V come from 3K probes of the apply (mode j at nodes c,
+// c + 3, ...; H couples modes only within a node and a node only to its
+// neighbours, V only within a node); V1 and V2 from the same probes with the
+// radial profiles of one body (the primary's profile weight or the
+// companion's set to zero); H0 and CAP from the host formulas
+// (hamiltonian.cpp:68-86); dVdt, A and B from V and H (hamiltonian.cpp:
+// 205-219, matrices.cpp:27-40). Explicit zeros of the reference's patterns
+// are left out.
+std::vector<std::vector<std::pair<int32_t, gmh::cplx>>> probe_rows(gm_solver &S, bool coupling)
+{
+    const int K = S.K, N = S.Nr;
+    const size_t n = S.nvec();
+    std::vector<std::vector<std::pair<int32_t, gmh::cplx>>> rows(n);
+    std::vector<double> probe(2 * n), out(2 * n);
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < K; ++j) {
+            std::fill(probe.begin(), probe.end(), 0.0);
+            for (int i = c; i < N; i += 3) probe[2 * (static_cast<size_t>(j) * N + i)] = 1.0;
+            to_dev(S, probe.data(), S.s.p);
+            gmk::ApplyArgs a = apply_args(S, S.s.p, S.t.p);
+            a.Cout = S.tmp.p;
+            launch(S, GM_PROF_APPLY_OTHER, [&] { return gmk::launch_apply(a, gmk::AM_H, S.st); });
+            to_host(S, coupling ? S.tmp.p : S.t.p, out.data());
+            for (int k = 0; k < K; ++k)
+                for (int i2 = 0; i2 < N; ++i2) {
+                    const size_t q = static_cast<size_t>(k) * N + i2;
+                    const gmh::cplx v(out[2 * q], out[2 * q + 1]);
+                    if (v == gmh::cplx(0.0)) continue;
+                    const int d = ((c - i2) % 3 + 3) % 3;
+                    const int i = i2 + (d == 2 ? -1 : d);
+                    if (coupling && d != 0) throw std::logic_error("operator probe: coupling across nodes");
+                    if (i < 0 || i >= N) throw std::logic_error("operator probe: entry outside the stencil");
+                    rows[q].emplace_back(static_cast<int32_t>(static_cast<size_t>(j) * N + i), v);
+                }
+        }
+    return rows;
+}
+
+OpCsr to_csr(std::vector<std::vector<std::pair<int32_t, gmh::cplx>>> &rows)
+{
+    OpCsr o;
+    o.ptr.assign(rows.size() + 1, 0);
+    for (size_t r = 0; r < rows.size(); ++r) {
+        auto &row = rows[r];
+        std::sort(row.begin(), row.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (const auto &e : row) {
+            o.col.push_back(e.first);
+            o.val.push_back(e.second);
+        }
+        o.ptr[r + 1] = static_cast<int64_t>(o.col.size());
+        std::vector<std::pair<int32_t, gmh::cplx>>().swap(row);
+    }
+    return o;
+}
+
+// The coupling with the radial profiles weighted (w1, w2) for the primary and
+// the companion; the step's profiles are restored afterwards.
+std::vector<std::vector<std::pair<int32_t, gmh::cplx>>> probe_coupling(gm_solver &S, double w1, double w2)
+{
+    gmd::StepParams &P = *S.hP.p;
+    const gmd::StepParams keep = P;
+    P.w1 = w1;
+    P.prf = w2;
+    HIPCK(hipMemcpyAsync(S.dP.p, S.hP.p, sizeof(gmd::StepParams), hipMemcpyHostToDevice, S.st));
+    launch_profiles(S);
+    std::vector<std::vector<std::pair<int32_t, gmh::cplx>>> rows;
+    try {
+        rows = probe_rows(S, true);
+    } catch (...) {
+        P = keep;
+        HIPCK(hipMemcpyAsync(S.dP.p, S.hP.p, sizeof(gmd::StepParams), hipMemcpyHostToDevice, S.st));
+        launch_profiles(S);
+        throw;
+    }
+    P = keep;
+    HIPCK(hipMemcpyAsync(S.dP.p, S.hP.p, sizeof(gmd::StepParams), hipMemcpyHostToDevice, S.st));
+    launch_profiles(S);
+    HIPCK(hipStreamSynchronize(S.st));
+    return rows;
+}
+
+OpCsr materialise(gm_solver &S, int which)
+{
+    require_initialized(S);
+    const int K = S.K, N = S.Nr;
+    const size_t n = S.nvec();
+    if (static_cast<double>(n) * (K + 3) > static_cast<double>(1 << 24))
+        throw std::invalid_argument("operator materialisation is for small systems: rows x (n_modes + 3) <= 2^24");
+    if (which < GM_OP_H || which > GM_OP_CAP) throw std::invalid_argument("unknown operator");
+    if (S.E_pending) {
+        launch_profiles(S);
+        S.E_pending = false;
+    }
+    HIPCK(hipStreamSynchronize(S.st));
+    std::vector<std::vector<std::pair<int32_t, gmh::cplx>>> rows(n);
+    const double hdt = 0.5 * S.dt;
+    switch (which) {
+    case GM_OP_ID:
+        for (size_t r = 0; r < n; ++r) rows[r].emplace_back(static_cast<int32_t>(r), 1.0);
+        break;
+    case GM_OP_CAP:
+        if (S.cap)
+            for (int j = 0; j < K; ++j)
+                for (int i = 0; i < N; ++i) {
+                    const size_t r = static_cast<size_t>(j) * N + i;
+                    const gmh::cplx v(S.capv[i].x, S.capv[i].y);
+                    if (v != gmh::cplx(0.0)) rows[r].emplace_back(static_cast<int32_t>(r), v);
+                }
+        break;
+    case GM_OP_H0: {
+        const double alpha = 1.0 / (S.dr * S.dr), off = -alpha / (2 * S.mu);
+        for (int j = 0; j < K; ++j)
+            for (int i = 1; i < N - 1; ++i) {
+                const size_t r = static_cast<size_t>(j) * N + i;
+                const double l = S.ml[j];
+                rows[r].emplace_back(static_cast<int32_t>(r - 1), off);
+                rows[r].emplace_back(static_cast<int32_t>(r), -(-2.0 * alpha - l * (l + 1) / (S.r[i] * S.r[i])) / (2 * S.mu));
+                rows[r].emplace_back(static_cast<int32_t>(r + 1), off);
+            }
+        break;
+    }
+    case GM_OP_V1:
+        rows = probe_coupling(S, 1.0, 0.0);
+        break;
+    case GM_OP_V2:
+        rows = probe_coupling(S, 0.0, 1.0);
+        break;
+    case GM_OP_V:
+    case GM_OP_DVDT:
+        rows = probe_rows(S, true);
+        if (which == GM_OP_DVDT) {
+            const double sign = S.counter ? -1.0 : 1.0;
+            for (size_t r = 0; r < n; ++r) {
+                const int j = static_cast<int>(r / N);
+                for (auto &e : rows[r]) {
+                    const int k = e.first / N;
+                    e.second *= gmh::cplx(0.0, sign * static_cast<double>(S.mm[k] - S.mm[j]) * S.Omega_V);
+                }
+            }
+        }
+        break;
+    default: {  // H, A, B
+        rows = probe_rows(S, false);
+        if (which != GM_OP_H) {
+            const gmh::cplx sc(0.0, which == GM_OP_A ? hdt : -hdt);
+            for (size_t r = 0; r < n; ++r) {
+                bool diag = false;
+                for (auto &e : rows[r]) {
+                    e.second *= sc;
+                    if (static_cast<size_t>(e.first) == r) {
+                        e.second += 1.0;
+                        diag = true;
+                    }
+                }
+                if (!diag) rows[r].emplace_back(static_cast<int32_t>(r), 1.0);
+            }
+        }
+    }
+    }
+    return to_csr(rows);
 }
 
 // A = I + i (dt/2) H (update_CN_values, matrices.cpp:27-42), with every
@@ -2636,6 +2814,34 @@ int gm_schrodinger_rhs(gm_solver *S, const double *in, double *out)
         HIPCK(gmk::launch_cscale(S->t.p, S->nvec(), make_double2(0.0, -1.0), S->st));
         sink_apply_dev(*S, S->tmp.p, S->t.p, -0.5, true, S->sinkscr.p, S->st);
         to_host(*S, S->t.p, out);
+    });
+}
+
+int gm_operator_csr(gm_solver *S, int32_t which, int64_t *nnz, int64_t *row_ptr, int32_t *col, double *val)
+{
+    return guarded(S, [&] {
+        if (!nnz) throw std::invalid_argument("nnz must not be null");
+        if (!row_ptr) {
+            S->op_cache = materialise(*S, which);
+            S->op_which = which;
+            *nnz = static_cast<int64_t>(S->op_cache.col.size());
+            return;
+        }
+        if (S->op_which != which) {
+            S->op_cache = materialise(*S, which);
+            S->op_which = which;
+        }
+        const OpCsr &o = S->op_cache;
+        const int64_t m = static_cast<int64_t>(o.col.size());
+        if (*nnz < m || (m > 0 && (!col || !val))) throw std::invalid_argument("col and val must hold nnz entries");
+        std::memcpy(row_ptr, o.ptr.data(), o.ptr.size() * sizeof(int64_t));
+        if (m > 0) {
+            std::memcpy(col, o.col.data(), m * sizeof(int32_t));
+            std::memcpy(val, o.val.data(), m * sizeof(gmh::cplx));
+        }
+        *nnz = m;
+        S->op_cache = OpCsr{};
+        S->op_which = -1;
     });
 }
 

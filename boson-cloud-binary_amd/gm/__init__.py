@@ -86,6 +86,30 @@ class CrankNicolsonModes(capi.Solver):
         self._check(fn(self._h, C.c_void_p(int(d_in)), C.c_void_p(int(d_out)),
                        None if stream is None else C.c_void_p(int(stream))))
 
+    OPERATORS = ("H", "H0", "A", "B", "V1", "V2", "V", "dVdt", "Id", "CAP")
+
+    def operator(self, name):
+        """The public sparse member `name` of the reference class (H, H0, A, B,
+        V1, V2, V, dVdt, Id, CAP; include/CrankNicolsonModes.hpp:197-206) at the
+        current step, as a scipy CSR matrix in the reference's stacking (row
+        j*Nr + i). Small systems only (gm_operator_csr)."""
+        import scipy.sparse
+        which = self.OPERATORS.index(name)
+        fn = self._lib.gm_operator_csr
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                       C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        nnz = C.c_int64(0)
+        self._check(fn(self._h, which, C.byref(nnz), None, None, None))
+        n = self.size
+        ptr = np.zeros(n + 1, dtype=np.int64)
+        col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        val = np.zeros(max(nnz.value, 1), dtype=np.complex128)
+        self._check(fn(self._h, which, C.byref(nnz), ptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                       col.ctypes.data_as(C.POINTER(C.c_int32)), val.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))))
+        m = nnz.value
+        return scipy.sparse.csr_matrix((val[:m], col[:m], ptr), shape=(n, n))
+
     def set_coupling_tolerance(self, tol):
         """Multipole truncation of the coupling (0 keeps every term)."""
         self._check(self._lib.gm_set_coupling_tolerance(self._h, float(tol)))

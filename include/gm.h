@@ -205,6 +205,20 @@ int gm_apply_sink(gm_solver *solver, const double *in, double *out);
 int gm_schrodinger_rhs(gm_solver *solver, const double *in, double *out);
 double gm_sink_strength(const gm_solver *solver, int32_t body);
 
+/* The public sparse operators of the reference class (H, H0, A, B, V1, V2, V,
+ * dVdt, Id, CAP; include/CrankNicolsonModes.hpp:197-206) at the current step,
+ * materialised on the host in the reference's stacking (row j*Nr + i) as CSR
+ * with columns sorted and complex values as (re, im) pairs; explicit zeros of
+ * the reference's sparsity patterns are left out. For small systems only
+ * (Nr*n_modes*(n_modes+3) <= 2^24). Call with row_ptr NULL for the entry
+ * count in *nnz, then with row_ptr (Nr*n_modes+1), col and val (2 *nnz
+ * doubles) to fill them; the operator is built once for the pair of calls. */
+enum {
+    GM_OP_H = 0, GM_OP_H0 = 1, GM_OP_A = 2, GM_OP_B = 3, GM_OP_V1 = 4, GM_OP_V2 = 5,
+    GM_OP_V = 6, GM_OP_DVDT = 7, GM_OP_ID = 8, GM_OP_CAP = 9
+};
+int gm_operator_csr(gm_solver *solver, int32_t which, int64_t *nnz, int64_t *row_ptr, int32_t *col, double *val);
+
 /* apply_sink without any allocation (no counterpart in the reference; for
  * callers that cannot allocate device memory: code under HIP graph capture,
  * serving loops that run with allocation forbidden). d_in and d_out are

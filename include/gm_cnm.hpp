@@ -152,6 +152,33 @@ inline bool load_snapshot(const std::string &path, size_t size, std::vector<std:
 
 }  // namespace cnm_detail
 
+// Stand-in for the reference's SparseComplex (Eigen::SparseMatrix<dcomplex,
+// RowMajor>): CSR with sorted columns, explicit zeros of the reference's
+// pattern left out.
+struct SparseComplex {
+    int64_t n = 0;
+    std::vector<int64_t> outer;
+    std::vector<int32_t> inner;
+    std::vector<std::complex<double>> values;
+    int64_t rows() const { return n; }
+    int64_t cols() const { return n; }
+    int64_t nonZeros() const { return static_cast<int64_t>(values.size()); }
+    std::complex<double> coeff(int64_t row, int64_t col) const
+    {
+        if (row < 0 || row >= n) return {0.0, 0.0};
+        const auto b = inner.begin() + outer[row], e = inner.begin() + outer[row + 1];
+        const auto it = std::lower_bound(b, e, static_cast<int32_t>(col));
+        return it != e && *it == col ? values[it - inner.begin()] : std::complex<double>(0.0, 0.0);
+    }
+    std::vector<std::complex<double>> operator*(const std::vector<std::complex<double>> &x) const
+    {
+        std::vector<std::complex<double>> y(n);
+        for (int64_t r = 0; r < n; ++r)
+            for (int64_t q = outer[r]; q < outer[r + 1]; ++q) y[r] += values[q] * x[inner[q]];
+        return y;
+    }
+};
+
 class CrankNicolsonModes {
 public:
     using cvec = std::vector<std::complex<double>>;
@@ -232,6 +259,28 @@ public:
     // this build: arithmetic of the step, and the psi sync after a caller's step()
     int precision = GM_PRECISION_FP64;
     bool compat_sync = true;
+    // operators (include/CrankNicolsonModes.hpp:197-206): the device keeps them
+    // as tables; refresh_operators() materialises them here (small systems)
+    SparseComplex H, H0, A, B, V1, V2, V, dVdt, Id, CAP;
+
+    void refresh_operators()
+    {
+        SparseComplex *dst[] = {&H, &H0, &A, &B, &V1, &V2, &V, &dVdt, &Id, &CAP};
+        const int32_t which[] = {GM_OP_H, GM_OP_H0, GM_OP_A, GM_OP_B, GM_OP_V1,
+                                 GM_OP_V2, GM_OP_V, GM_OP_DVDT, GM_OP_ID, GM_OP_CAP};
+        for (int k = 0; k < 10; ++k) {
+            SparseComplex &m = *dst[k];
+            int64_t nnz = 0;
+            cnm_detail::check(gm_operator_csr(h_, which[k], &nnz, nullptr, nullptr, nullptr), h_);
+            m.n = static_cast<int64_t>(Nr) * n_modes;
+            m.outer.assign(m.n + 1, 0);
+            m.inner.assign(nnz, 0);
+            m.values.assign(nnz, {0.0, 0.0});
+            cnm_detail::check(gm_operator_csr(h_, which[k], &nnz, m.outer.data(), m.inner.data(),
+                                              reinterpret_cast<double *>(m.values.data())),
+                              h_);
+        }
+    }
 
     void set_power_average_orbits(double n_orbits)
     {
