@@ -115,13 +115,16 @@ def profile_read(s):
     return s.profile_read()
 
 
-def run_gpu(w, precision, steps, warmup, prof_steps=5, graphs=True):
+def run_gpu(w, precision, steps, warmup, prof_steps=5, graphs=True, tile=0):
     """Warmup, then `steps` timed steps (graph replay on unless `graphs` is
     false, no per-kernel events), then `prof_steps` steps with HIP-event kernel
-    timing (graphs off) for the kernel table and the roofline."""
+    timing (graphs off) for the kernel table and the roofline. `tile` fixes
+    the apply tile (nodes; 0 autotunes)."""
     s = make_solver(w, precision)
     if not graphs:
         s.set_flags(use_graphs=0)
+    if tile:
+        s.set_flags(apply_tile=tile)
     k = 0
     for _ in range(warmup):
         k += 1
@@ -146,7 +149,7 @@ def run_gpu(w, precision, steps, warmup, prof_steps=5, graphs=True):
     return s, t1 - t0, prof, iters, piters
 
 
-def evolving(w, steps, warmup, fixed, graphs=True):
+def evolving(w, steps, warmup, fixed, graphs=True, tile=0):
     """The workload with the orbit evolving (GW and cloud back-reaction, the
     bodies accreting through the absorbers): ms per step per precision and the
     ratio to the fixed-orbit step measured above."""
@@ -155,6 +158,8 @@ def evolving(w, steps, warmup, fixed, graphs=True):
         s = make_solver(w, precision, evolve=True)
         if not graphs:
             s.set_flags(use_graphs=0)
+        if tile:
+            s.set_flags(apply_tile=tile)
         k = 0
         for _ in range(warmup):
             k += 1
@@ -684,6 +689,9 @@ def main():
                     help="lockstep mixed-vs-FP64 steps at l_max 30, N 32768 (0 skips them)")
     ap.add_argument("--evolving-steps", type=int, default=50,
                     help="timed steps per precision with the orbit evolving (0 skips them)")
+    ap.add_argument("--apply-tile", type=int, default=0, choices=(0, 16, 32, 64),
+                    help="fix the coupling apply's tile in nodes (0: autotune; profiler runs fix the tuned one, "
+                         "since counter collection serialises the tuner's timings)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="timed steps without graph replay (for kernel-trace profiling runs)")
@@ -707,7 +715,8 @@ def main():
     w = dict(WORKLOAD)
     runs = {}
     for name, precision in (("fp64", gm.GM_PRECISION_FP64), ("mixed", gm.GM_PRECISION_MIXED)):
-        s, elapsed, prof, iters, piters = run_gpu(w, precision, args.steps, args.warmup, graphs=not args.no_graphs)
+        s, elapsed, prof, iters, piters = run_gpu(w, precision, args.steps, args.warmup, graphs=not args.no_graphs,
+                                                  tile=args.apply_tile)
         K, N = s.n_modes, s.Nr
         roof, kernels = rooflines(s, prof, K, N, piters, name == "mixed")
         runs[name] = {"value": K * N * args.steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
@@ -737,7 +746,8 @@ def main():
                           "untruncated_terms_per_apply": float(coupling_terms(w["l_max"])) * (N - 2)},
     }
     if args.evolving_steps > 0:
-        out["evolving_orbit"] = evolving(w, args.evolving_steps, 5, runs, graphs=not args.no_graphs)
+        out["evolving_orbit"] = evolving(w, args.evolving_steps, 5, runs, graphs=not args.no_graphs,
+                                         tile=args.apply_tile)
     if args.accuracy_steps > 0:
         out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
     if args.large_steps > 0:
