@@ -84,6 +84,11 @@ def coupling_terms(l_max):
     return n
 
 
+def progress(msg):
+    """One line per finished leg on stderr (the JSON line alone goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def make_solver(w, precision, evolve=False):
     s = gm.CrankNicolsonModes(**w)
     if evolve:
@@ -726,6 +731,7 @@ def main():
                       "kernel_launches": {k: int(v[1]) for k, v in prof.items()}}
         tile, tms = s.apply_config(precision)
         runs[name]["apply_tile"] = {"nodes": tile, "tuning_ms": tms}
+        progress("%s leg: %.4f ms per step" % (name, runs[name]["ms_per_step"]))
         terms, pairs = s.coupling_work()
         del s
     mx, f64 = runs["mixed"], runs["fp64"]
@@ -748,10 +754,13 @@ def main():
     if args.evolving_steps > 0:
         out["evolving_orbit"] = evolving(w, args.evolving_steps, 5, runs, graphs=not args.no_graphs,
                                          tile=args.apply_tile)
+        progress("evolving-orbit leg done")
     if args.accuracy_steps > 0:
         out["mixed_vs_fp64"] = accuracy(w, args.accuracy_steps)
+        progress("mixed-vs-FP64 comparison done")
     if args.large_steps > 0:
         out["second_size"] = large_case(args.large_steps, 2, args.large_accuracy_steps)
+        progress("second size done")
     if args.ensemble_members > 0:
         from gm import ensemble
         cfg = ("ragged fixed-orbit binaries from seed 0: q log-uniform [0.05, 1], R [20, 60], mu [0.1, 0.3], "
@@ -762,13 +771,17 @@ def main():
         ens["config"] = cfg + ("; every launch of the step one ragged grid over the members of a lane (%d lanes "
                                "of balanced work, each on its own stream), graph-captured" % args.ensemble_lanes)
         ens["over_fp64_single_step"] = ens["mode_node_steps_per_s"] / f64["value"]
+        progress("batched ensemble: %d steps, %.1f ms per ensemble step" % (args.ensemble_steps,
+                                                                            ens["ms_per_ensemble_step"]))
         streams = ensemble.run(n=args.ensemble_members, steps=max(1, args.ensemble_steps // 10), warmup=3, seed=0,
                                mode="streams")
         streams["over_fp64_single_step"] = streams["mode_node_steps_per_s"] / f64["value"]
         ens["per_member_streams"] = streams
+        progress("per-member-stream ensemble done")
         out["ensemble"] = ens
     if args.serve_seconds > 0:
         out["serving"] = serving(args.serve_seconds)
+        progress("serving done")
         out["p99_served_step_latency_ms"] = out["serving"]["as_specified"]["p99_ms"]
     if not args.no_cpu_baseline:
         try:
