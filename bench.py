@@ -49,6 +49,61 @@ WORKLOAD = dict(Nr=16384, L=4000.0, dt=0.25, epsilon=0.0, l_max=20, mu=0.15, M_c
 MASK = (0.85, 1.0, 6)
 
 
+def initial_states(w, seeds):
+    """The translated 2p atom with one perturbation per seed (the atom built once)."""
+    dr = w["L"] / (w["Nr"] - 1)
+    r = w["epsilon"] + dr * np.arange(w["Nr"])
+    base = initial.translated_atom(r, w["l_max"], w["mu"], w["q"] / (1 + w["q"]) * w["R"])
+    base /= math.sqrt(np.vdot(base, base).real * dr)
+    K = (w["l_max"] + 1) * (w["l_max"] + 2) // 2
+    return [initial.perturbed_state(base, w["Nr"], K, rel=1e-3, seed=sd, dr=dr) for sd in seeds]
+
+
+def headline_batch(w, members, steps, warmup, fp64_value, graphs=True):
+    """The mixed-precision batched step at the headline shape: `members`
+    binaries of the workload (perturbations seeded 0, 1, ...) stepped as
+    batched ragged launches, one lane per member, against the plain FP64
+    single step."""
+    states = initial_states(w, range(members))
+    sol = []
+    for psi in states:
+        s = gm.CrankNicolsonModes(**w)
+        s.set_flags(evolve_orbit=0, precision=gm.GM_PRECISION_MIXED)
+        if not graphs:
+            s.set_flags(use_graphs=0)
+        s.set_initial_state_full(psi)
+        s.initialize()
+        s.build_damping_mask(*MASK)
+        s.begin_run(0.8 * w["L"], 100000)
+        sol.append(s)
+    batch = gm.Batch(sol, members)
+    k = 0
+    for _ in range(warmup):
+        k += 1
+        batch.step(k, diag=True)
+    device_sync()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(steps):
+        k += 1
+        batch.step(k, diag=True)
+        iters.append(float(np.mean([s.last_solver_iterations for s in sol])))
+    device_sync()
+    el = time.perf_counter() - t0
+    K, N = sol[0].n_modes, sol[0].Nr
+    value = K * N * members * steps / el
+    batch.close()
+    for s in sol:
+        s.close()
+    return {"members": members, "lanes": members, "steps": steps, "warmup": warmup,
+            "ms_per_batch_step": el / steps * 1e3, "ms_per_member_step": el / steps / members * 1e3,
+            "mode_node_steps_per_s": value, "over_fp64_single_step": value / fp64_value,
+            "iterations_per_step": float(np.mean(iters)),
+            "config": "%d binaries of the bench workload (perturbations seeded 0..%d), mixed precision, every "
+                      "launch of the step a ragged grid over one member's lane, graph-captured"
+                      % (members, members - 1)}
+
+
 def initial_state(w, seed=0):
     """Translated 2p atom + perturbation, cached in /tmp by parameters."""
     key = "gm_init_%d_%g_%d_%g_%g_%g_%d.npy" % (w["Nr"], w["L"], w["l_max"], w["mu"], w["q"], w["R"], seed)
@@ -697,6 +752,8 @@ def main():
     ap.add_argument("--apply-tile", type=int, default=0, choices=(0, 16, 32, 64),
                     help="fix the coupling apply's tile in nodes (0: autotune; profiler runs fix the tuned one, "
                          "since counter collection serialises the tuner's timings)")
+    ap.add_argument("--headline-batch", type=int, default=8,
+                    help="binaries of the bench workload in the mixed-precision batched step (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="timed steps without graph replay (for kernel-trace profiling runs)")
@@ -751,6 +808,11 @@ def main():
         "coupling_work": {"terms_per_apply": terms, "pairs_per_apply": pairs,
                           "untruncated_terms_per_apply": float(coupling_terms(w["l_max"])) * (N - 2)},
     }
+    if args.headline_batch > 0:
+        # the single legs' window: the same warmup and timed steps
+        out["batched"] = headline_batch(w, args.headline_batch, args.steps, args.warmup, f64["value"],
+                                        graphs=not args.no_graphs)
+        progress("batched headline step: %.1f ms per batch step" % out["batched"]["ms_per_batch_step"])
     if args.evolving_steps > 0:
         out["evolving_orbit"] = evolving(w, args.evolving_steps, 5, runs, graphs=not args.no_graphs,
                                          tile=args.apply_tile)
