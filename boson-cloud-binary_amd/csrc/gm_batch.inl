@@ -363,6 +363,7 @@ void batch_drop_graphs(BatchLane &B)
 void batch_member_begin(gm_solver &S)
 {
     require_initialized(S);
+    restore_precision(S);
     S.step_start = S.t_i;
     S.t_i += S.dt;
     S.theta += (S.counter ? -1.0 : 1.0) * S.Omega * S.dt;

@@ -229,10 +229,23 @@ int gm_server_run(gm_server *V, double seconds, int32_t *completed)
                             if (solve_advance(*S)) a.phase = 2;
                         });
                     } else {
+                        bool retry = false;
                         st = guarded(S, [&] {
                             prof_collect(*S);
+                            // a mixed step that failed the FP64 check is enqueued
+                            // again in FP64 and polled like any other solve
+                            if (!step_accept(*S)) {
+                                retry = true;
+                                return;
+                            }
                             step_finish(*S, nullptr);
                         });
+                        if (st == GM_OK && retry) {
+                            a.phase = 1;
+                            progressed = true;
+                            ++i;
+                            continue;
+                        }
                         const double ms =
                             std::chrono::duration<double, std::milli>(Clock::now() - a.t0).count();
                         if (st == GM_OK) {

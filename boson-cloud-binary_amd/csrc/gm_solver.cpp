@@ -203,6 +203,7 @@ struct gm_solver {
     bool step_E = false, step_fac = false;  // profiles / pivots rebuilt at the solve start
     bool step_ilut = false;  // this step's solve is the host ILUT one
     bool step_fp64 = false;  // a mixed-precision step solved again with the FP64 operator
+    bool retry_mixed = false;  // flags.precision switched from mixed for that retry (restored on acceptance)
     int step_enqueued = 0;
     // captured parts of the step, keyed by every kernel argument that can change
     struct GraphEntry {
@@ -211,7 +212,9 @@ struct gm_solver {
         hipGraphExec_t exec = nullptr;
     };
     std::vector<GraphEntry> graphs;
-    std::vector<uint64_t> last_key[3];  // last key requested per kind (capture once it repeats)
+    // last key requested per kind and precision (capture once it repeats; a
+    // mixed step retried in FP64 alternates the two)
+    std::vector<uint64_t> last_key[3][2];
     // operator-apply tile (16-node groups) per precision, 0 until tuned, and
     // the measured time per candidate (ms; index ng = 1, 2, 4 -> 0, 1, 2)
     int apply_ng[2] = {0, 0};
@@ -1074,8 +1077,9 @@ void run_captured(gm_solver &S, int kind, const std::vector<uint64_t> &key, F &&
             HIPCK(hipGraphLaunch(e.exec, S.st));
             return;
         }
-    const bool repeat = S.last_key[kind] == key;
-    S.last_key[kind] = key;
+    std::vector<uint64_t> &last = S.last_key[kind][S.flags.precision == GM_PRECISION_MIXED];
+    const bool repeat = last == key;
+    last = key;
     if (!repeat) {
         work();
         return;
@@ -1107,7 +1111,8 @@ void drop_graphs(gm_solver &S)
 {
     for (auto &e : S.graphs) (void)hipGraphExecDestroy(e.exec);
     S.graphs.clear();
-    for (auto &k : S.last_key) k.clear();
+    for (auto &kp : S.last_key)
+        for (auto &k : kp) k.clear();
 }
 
 // A batch of BiCGSTAB iterations followed by the readback of the Krylov state.
@@ -2129,6 +2134,7 @@ int gm_set_flags(gm_solver *S, const gm_flags *f)
         if (f->ramp_secondary_potential != S->flags.ramp_secondary_potential) S->E_valid = false;
         if (f->apply_tile != S->flags.apply_tile) S->apply_ng[0] = S->apply_ng[1] = 0;
         S->flags = *f;
+        S->retry_mixed = false;
     });
 }
 
@@ -2336,9 +2342,19 @@ void tune_apply(gm_solver &S, int mixed)
 // step (src/CrankNicolsonModes.cpp:145-234), phase 1: advance t, theta and the
 // phase, rebuild the perturbation and enqueue the solve up to its first
 // convergence check.
+// A mixed step's FP64 retry that ended in an error leaves the precision to restore.
+void restore_precision(gm_solver &S)
+{
+    if (S.retry_mixed) {
+        S.flags.precision = GM_PRECISION_MIXED;
+        S.retry_mixed = false;
+    }
+}
+
 void step_begin(gm_solver &S)
 {
     require_initialized(S);
+    restore_precision(S);
     S.step_start = S.t_i;
     S.t_i += S.dt;
     S.theta += (S.counter ? -1.0 : 1.0) * S.Omega * S.dt;
@@ -2367,52 +2383,73 @@ void step_begin(gm_solver &S)
     solve_begin(S);
 }
 
+// Phase 3a, once the stream has drained after the tail's readback: the
+// residual check. A mixed-precision step that failed it is enqueued again
+// with the FP64 operator (from the step's start state: psi is untouched until
+// the step is accepted) and false is returned: the caller polls solve_advance
+// again and calls step_accept once more, so ensemble and server loops keep
+// advancing their other solvers meanwhile. A second failure, or a failure in
+// FP64, goes to the host ILUT (matrices.cpp:173,203). True once the step's
+// result is accepted; repeated calls then return true at once.
+bool step_accept(gm_solver &S)
+{
+    for (;;) {
+        const bool bad_factor = S.hflags.p[0] != 0;
+        if (bad_factor) {
+            S.fac_valid = false;
+            S.hflags.p[0] = 0;
+            int zero = 0;
+            HIPCK(hipMemcpy(S.dflags.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (S.hflags.p[1] && !bad_factor) {
+            restore_precision(S);
+            return true;
+        }
+        if (S.step_ilut) {
+            restore_precision(S);
+            throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
+        }
+        if (S.flags.precision == GM_PRECISION_MIXED && !S.step_fp64) {
+            // The FP32 coupling kept the solve above the FP64 residual
+            // threshold (its error grows with dt |V|): the same radial solve
+            // again with the FP64 operator, before the host ILUT.
+            S.step_fp64 = true;
+            S.retry_mixed = true;
+            S.flags.precision = GM_PRECISION_FP64;
+            try {
+                tune_apply(S, 0);
+                solve_begin(S);
+            } catch (...) {
+                S.flags.precision = GM_PRECISION_MIXED;
+                S.retry_mixed = false;
+                throw;
+            }
+            return false;
+        }
+        // "Radial preconditioner did not converge; retrying with full ILUT."
+        // (matrices.cpp:173,203): psi is still the step's start state
+        S.step_ilut = true;
+        ilut_host_solve(S);
+        run_captured(S, GK_TAIL, graph_key(S, 0), [&] { solve_tail_work(S); });
+        HIPCK(hipStreamSynchronize(S.st));
+        prof_collect(S);
+    }
+}
+
 // Phase 3, once the stream has drained after the diagnostics readback: the
-// residual check, absorbed mass, orbit update and diagnostics.
+// residual check (a failed mixed step is solved again here, synchronously),
+// absorbed mass, orbit update and diagnostics.
 void step_finish(gm_solver &S, gm_diag *d)
 {
     const double step_start = S.step_start;
     const gmd::StepParams &P = *S.hP.p;
     {
-        for (;;) {
-            const bool bad_factor = S.hflags.p[0] != 0;
-            if (bad_factor) {
-                S.fac_valid = false;
-                S.hflags.p[0] = 0;
-                int zero = 0;
-                HIPCK(hipMemcpy(S.dflags.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+        while (!step_accept(S)) {
+            for (;;) {
+                HIPCK(hipStreamSynchronize(S.st));
+                prof_collect(S);
+                if (solve_advance(S)) break;
             }
-            if (S.hflags.p[1] && !bad_factor) break;
-            if (S.step_ilut) throw std::runtime_error("Crank-Nicolson solve failed the relative residual check");
-            if (S.flags.precision == GM_PRECISION_MIXED && !S.step_fp64 && !S.step_ilut) {
-                // The FP32 coupling kept the solve above the FP64 residual
-                // threshold (its error grows with dt |V|): the same radial
-                // solve again with the FP64 operator, from the step's start
-                // state, before the host ILUT.
-                S.step_fp64 = true;
-                S.flags.precision = GM_PRECISION_FP64;
-                try {
-                    tune_apply(S, 0);
-                    solve_begin(S);
-                    for (;;) {
-                        HIPCK(hipStreamSynchronize(S.st));
-                        prof_collect(S);
-                        if (solve_advance(S)) break;
-                    }
-                    HIPCK(hipStreamSynchronize(S.st));
-                    prof_collect(S);
-                } catch (...) {
-                    S.flags.precision = GM_PRECISION_MIXED;
-                    throw;
-                }
-                S.flags.precision = GM_PRECISION_MIXED;
-                continue;
-            }
-            // "Radial preconditioner did not converge; retrying with full ILUT."
-            // (matrices.cpp:173,203): psi is still the step's start state
-            S.step_ilut = true;
-            ilut_host_solve(S);
-            run_captured(S, GK_TAIL, graph_key(S, 0), [&] { solve_tail_work(S); });
             HIPCK(hipStreamSynchronize(S.st));
             prof_collect(S);
         }
@@ -2513,12 +2550,13 @@ int gm_ensemble_step(gm_solver *const *members, int32_t n, int32_t step_id, gm_d
     }
     // members are advanced as their streams go idle (non-blocking queries), so
     // a member whose batch finished early is refilled at once instead of
-    // waiting behind the members before it
+    // waiting behind the members before it; a mixed member that fails the
+    // FP64 residual check goes back to phase 1 with its FP64 retry enqueued
     for (bool busy = true; busy;) {
         busy = false;
         bool progressed = false;
         for (int i = 0; i < n; ++i) {
-            if (phase[i] != 1) continue;
+            if (phase[i] != 1 && phase[i] != 2) continue;
             gm_solver *S = members[i];
             const hipError_t q = hipStreamQuery(S->st);
             if (q == hipErrorNotReady) {
@@ -2528,17 +2566,21 @@ int gm_ensemble_step(gm_solver *const *members, int32_t n, int32_t step_id, gm_d
             st[i] = guarded(S, [&] {
                 HIPCK(q);
                 prof_collect(*S);
-                if (solve_advance(*S)) phase[i] = 2;
+                if (phase[i] == 1) {
+                    if (solve_advance(*S)) phase[i] = 2;
+                } else {
+                    phase[i] = step_accept(*S) ? 3 : 1;
+                }
             });
             progressed = true;
             if (st[i] != GM_OK) phase[i] = -1;
-            busy = busy || phase[i] == 1;
+            busy = busy || phase[i] == 1 || phase[i] == 2;
         }
         if (busy && !progressed) std::this_thread::yield();
     }
     int first = GM_OK;
     for (int i = 0; i < n; ++i) {
-        if (phase[i] == 2) {
+        if (phase[i] == 3) {
             gm_solver *S = members[i];
             st[i] = guarded(S, [&] {
                 HIPCK(hipStreamSynchronize(S->st));

@@ -77,6 +77,52 @@ def test_admission_respects_the_budget():
     srv.close()
 
 
+def test_served_mixed_request_retried_in_fp64():
+    """A served mixed-precision request whose FP32 coupling error lies above
+    the FP64 residual check (dt = 23 at R = 20, as in the ensemble's retry
+    test): each of its steps is enqueued again with the FP64 operator while
+    the server keeps advancing its neighbour, so it ends as the same request
+    run alone in FP64, and the neighbour as its own solo run, bitwise."""
+    print("serving: a mixed request re-solved in FP64 beside another", flush=True)
+    big = dict(Nr=257, L=128.0, dt=23.0, epsilon=0.0, l_max=4, mu=0.2, M_c=0.05, q=0.5, a=0.0, R=20.0,
+               counter=False, absorption=True, save_every=1000, ramp_periods=0.0, epsilon_smear=2.0)
+    from test_oracle import generic_state
+    psi = generic_state(oracle.Oracle(**big))
+    n = 5
+    mixed = serve.default_flags(precision=1, evolve_orbit=0)
+    fp64 = serve.default_flags(precision=0, evolve_orbit=0)
+    other = spec(1024, 4, 0.3, 30.0, 0.2, 1, 8)
+    srv = serve.Server(hbm_budget=8e9, max_active=4)
+    t_big = srv.submit(big, psi, n, 100.0, flags=mixed, mask=(0.85, 1.0, 6))
+    t_other = srv.submit(other[0], other[1], other[2], 0.8 * other[0]["L"], flags=other[3], mask=(0.85, 1.0, 6))
+    for _ in range(600):
+        srv.run(1.0)
+        if all(srv.status(t)[0] >= serve.GM_SERVE_DONE for t in (t_big, t_other)):
+            break
+    got, _ = srv.take(t_big)
+    got_other, _ = srv.take(t_other)
+    srv.close()
+    solo = {}
+    for name, f in (("mixed", mixed), ("fp64", fp64)):
+        s = gm.CrankNicolsonModes(**big)
+        s.set_flags(**{k: getattr(f, k) for k, _ in gm.capi.Flags._fields_})
+        s.set_initial_state_full(psi)
+        s.initialize()
+        s.build_damping_mask(0.85, 1.0, 6)
+        s.begin_run(100.0, n + 1)
+        for k in range(1, n + 1):
+            s.step(k, diag=False)
+            assert s.last_solver_residual <= 1e-9
+        assert s.get_flags().precision == f.precision
+        solo[name] = s.psi
+        s.close()
+    # served = alone; and alone every mixed step was re-solved in FP64
+    assert np.array_equal(got, solo["mixed"])
+    assert np.linalg.norm(solo["mixed"] - solo["fp64"]) <= 1e-13 * np.linalg.norm(solo["fp64"])
+    ref, _ = alone(*other)
+    assert np.array_equal(got_other, ref)
+
+
 def oracle_steps(params, psi0, n_steps, steps):
     """The FP64 oracle on a served request's inputs, `steps` steps."""
     o = oracle.Oracle(**params)
