@@ -134,6 +134,50 @@ def test_batch_equals_members_alone(lanes):
     batch.close()
 
 
+def test_batches_beyond_the_constant_argument_pool():
+    """The batched applies read their members' arguments from a constant
+    pool of 2048 entries per GPU; a batch's plan takes 4 ragged applies x 64
+    members. Nine batches of 64 alive at once need more: the last ones read
+    global argument tables instead, and every member still steps as alone.
+    Closing the batches frees their entries for a new batch."""
+    print("nine batches of 64 small members", flush=True)
+    args = dict(Nr=65, L=64.0, dt=1.0, epsilon=0.0, l_max=1, mu=0.2, M_c=0.01, q=0.5, a=0.0, R=20.0,
+                counter=False, absorption=True, save_every=1000, ramp_periods=0.0, epsilon_smear=2.0)
+    psi = generic_state(oracle.Oracle(**args))
+
+    def small():
+        s = gm.CrankNicolsonModes(**args)
+        s.set_flags(evolve_orbit=0, apply_tile=32)
+        s.set_initial_state_full(psi)
+        s.initialize()
+        s.build_damping_mask(0.85, 1.0, 6)
+        s.begin_run(40.0, 100)
+        return s
+
+    ref = small()
+    for n in (1, 2):
+        ref.step(n, diag=False)
+    groups = [[small() for _ in range(64)] for _ in range(9)]
+    batches = [gm.Batch(g) for g in groups]
+    for n in (1, 2):
+        for b in batches:
+            b.step(n, diag=False)
+    for g in groups:
+        for s in g:
+            assert np.array_equal(s.psi, ref.psi)
+    for b in batches:
+        b.close()
+    again = [small() for _ in range(64)]
+    b = gm.Batch(again)
+    for n in (1, 2):
+        b.step(n, diag=False)
+    assert all(np.array_equal(s.psi, ref.psi) for s in again)
+    b.close()
+    for g in groups + [again, [ref]]:
+        for s in g:
+            s.close()
+
+
 @pytest.mark.timeout(900)
 def test_config4_batched_ensemble_64_members():
     """The 64 binaries of the batched ensemble from seed 0 (q log-uniform in
