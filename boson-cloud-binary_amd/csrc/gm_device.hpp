@@ -25,8 +25,14 @@ constexpr int DIAG_NODES = 256;  // radial nodes per diagnostics wavefront (4 pe
 #endif
 constexpr int AW = GM_AW;  // row chunks (wavefronts) per operator-apply workgroup
 constexpr int MAXLM = 40;  // largest supported l_max + 1
-constexpr int PT = 512;    // threads of a preconditioner or factor workgroup
-constexpr int PE = 4;      // consecutive nodes per preconditioner thread
+#ifndef GM_PT
+#define GM_PT 512
+#endif
+#ifndef GM_PE
+#define GM_PE 4
+#endif
+constexpr int PT = GM_PT;  // threads of a preconditioner or factor workgroup
+constexpr int PE = GM_PE;  // consecutive nodes per preconditioner thread
 constexpr int RED_BLOCKS = 1024;  // max partial-sum slots per reduction
 
 constexpr int MR = 16;     // rows of an MFMA chunk
