@@ -21,21 +21,22 @@ LIB = os.path.join(HERE, "gm", "libgm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["gm_kernels.hip", "gm_math.cpp", "gm_ilut.cpp", "gm_solver.cpp"]
+SOURCES = ["gm_kernels.hip", "gm_theta.hip", "gm_math.cpp", "gm_theta_tables.cpp", "gm_ilut.cpp", "gm_solver.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I", CSRC, "-I", os.path.join(REPO, "include")]
 # Packed FP32 VALU ops stay off in every device pass (v_pk_* have returned wrong
 # results on this hardware); the host pass warns that it does not know the feature.
-DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          # matrix-core results in ordinary VGPRs (no accumulator copies in the coupling loop)
-          "-mllvm", "-amdgpu-mfma-vgpr-form"]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# matrix-core results in ordinary VGPRs (no accumulator copies in the Gaunt-form
+# coupling loop); the polar-quadrature apply keeps its accumulators in AGPRs
+DEVICE_EXTRA = {"gm_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def compile_one(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC] + COMMON
     if src.endswith(".hip"):
-        cmd += DEVICE + ["-x", "hip"]
+        cmd += DEVICE + DEVICE_EXTRA.get(src, []) + ["-x", "hip"]
     else:
         cmd += ["--offload-host-only"]  # host code: no device pass
     cmd += ["-c", os.path.join(CSRC, src), "-o", out]

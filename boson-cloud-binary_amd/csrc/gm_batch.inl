@@ -59,7 +59,7 @@ void add_seg(std::vector<std::unique_ptr<BatchSeg>> &out, int kernel, const std:
         // the apply reads its arguments from the constant pool while it has room
         int first = -1, dev = -1;
         HIPCK(hipGetDevice(&dev));
-        if (kernel == gmk::BK_APPLY && gmk::apply_pool_acquire(M, &first) == hipSuccess) {
+        if (kernel == gmk::BK_APPLY && ng != gmk::TAPPLY_NG && gmk::apply_pool_acquire(M, &first) == hipSuccess) {
             seg->L.pool = first;
             seg->pool_device = dev;
             HIPCK(gmk::apply_pool_write(first, args.data(), M));
@@ -123,10 +123,11 @@ std::vector<uint64_t> batch_key(BatchLane &B)
 
 int apply_blocks(gm_solver &S, gmk::ApplyArgs &a, int mode)
 {
+    a.mode = mode;
+    if (a.ng == gmk::TAPPLY_NG) return gmk::tapply_grid(S.Nr);
     int ngroups = 0;
     const int grid = gmk::apply_grid(S.Nr, S.nchunks, a.ng, &ngroups);
     a.ngroups = ngroups;
-    a.mode = mode;
     return grid;
 }
 
@@ -135,7 +136,7 @@ void add_apply_segs(std::vector<std::unique_ptr<BatchSeg>> &out, std::vector<gmk
                     std::vector<int> &blocks, const std::vector<int> &run)
 {
     const int M = static_cast<int>(args.size());
-    for (int mixed = 0; mixed < 2; ++mixed)
+    for (int mixed = 0; mixed < 2; ++mixed) {
         for (int ng : {1, 2, 4}) {
             std::vector<int> b(M, 0);
             bool any = false;
@@ -146,6 +147,19 @@ void add_apply_segs(std::vector<std::unique_ptr<BatchSeg>> &out, std::vector<gmk
                 }
             if (any) add_seg(out, gmk::BK_APPLY, args, b, ng, mixed);
         }
+        // polar-form members, one launch per l_max class
+        for (int cls = 4; cls <= 32; cls += 4) {
+            std::vector<int> b(M, 0);
+            bool any = false;
+            for (int m = 0; m < M; ++m)
+                if (run[m] && args[m].ng == gmk::TAPPLY_NG && args[m].mixed == mixed &&
+                    gmk::tapply_class(args[m].g.lmax, mixed) == cls) {
+                    b[m] = blocks[m];
+                    any = true;
+                }
+            if (any) add_seg(out, gmk::BK_APPLY, args, b, gmk::TAPPLY_NG, mixed, cls);
+        }
+    }
 }
 
 // The phase pass exp(i m theta) X that an apply without a ready copy needs.
@@ -157,7 +171,7 @@ void add_phase_seg(std::vector<std::unique_ptr<BatchSeg>> &out, const std::vecto
     std::vector<int> b(M, 0);
     for (int m = 0; m < M; ++m) {
         pa[m] = gmk::phase_args(args[m]);
-        if (run[m]) b[m] = pa[m].nbx * args[m].g.K;
+        if (run[m] && args[m].ng != gmk::TAPPLY_NG) b[m] = pa[m].nbx * args[m].g.K;
     }
     add_seg(out, gmk::BK_PHASE, pa, b);
 }
@@ -225,7 +239,7 @@ void build_plan(BatchLane &B)
         for (int m = 0; m < M; ++m) {
             gm_solver &S = *B.mem[m];
             if (!wood[m]) continue;
-            a[m] = wcorr_args(S, z ? S.z.p : S.y.p, S.xt.p, S.flags.precision == GM_PRECISION_MIXED);
+            a[m] = wcorr_args(S, z ? S.z.p : S.y.p, krylov_xt(S), S.flags.precision == GM_PRECISION_MIXED);
             b[m] = gmk::stream_blocks(static_cast<size_t>(S.Nr) * S.K);
         }
         add_seg(B.iter, gmk::BK_WCORR, a, b);

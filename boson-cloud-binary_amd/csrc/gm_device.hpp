@@ -95,6 +95,10 @@ struct Geometry {
     const float *E32;
     const int *lcut;           // per node tile: largest multipole l* evaluated
     const double2 *cap;        // Nr diagonal CAP or nullptr
+    // polar-quadrature form of the coupling (gm_theta.hpp): MFMA operand
+    // tables of phases A, C and the U sums, all rules; tks / tkd blocks per point group
+    const double *TA, *TC, *TU;
+    int tks, tkd;
 };
 
 // One s-wave sink restricted to its radial support [lo, lo+n).

@@ -19,6 +19,8 @@
 // Reductions are deterministic: every block writes one partial per quantity
 // and the last block to finish sums the partials in block order.
 #include "gm_kernels.hpp"
+// device helpers (complex arithmetic, block ranges, reductions, apply elements)
+#include "gm_devfn.hpp"
 
 #include <cmath>
 #include <mutex>
@@ -33,160 +35,6 @@ using gmd::Krylov;
 using gmd::SinkDev;
 using gmd::StepParams;
 
-// ---------------------------------------------------------------- helpers
-
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
-{
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-// conj(a) * b
-__device__ __forceinline__ double2 cdotm(double2 a, double2 b)
-{
-    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
-__device__ __forceinline__ double2 cdiv(double2 a, double2 b)
-{
-    const double d = b.x * b.x + b.y * b.y;
-    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
-}
-__device__ __forceinline__ double cnorm2(double2 a) { return a.x * a.x + a.y * a.y; }
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
-{
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-template <typename C> __device__ __forceinline__ C cone();
-template <> __device__ __forceinline__ double2 cone<double2>() { return make_double2(1.0, 0.0); }
-template <> __device__ __forceinline__ float2 cone<float2>() { return make_float2(1.0f, 0.0f); }
-template <typename C> __device__ __forceinline__ C czero();
-template <> __device__ __forceinline__ double2 czero<double2>() { return make_double2(0.0, 0.0); }
-template <> __device__ __forceinline__ float2 czero<float2>() { return make_float2(0.0f, 0.0f); }
-// i * h * a for real h
-__device__ __forceinline__ double2 imul(double h, double2 a) { return make_double2(-h * a.y, h * a.x); }
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-__device__ __forceinline__ double vload(const double *p) { return *(const volatile double *)p; }
-
-__device__ __forceinline__ double2 vload2(const double2 *p)
-{
-    const volatile double *q = (const volatile double *)p;
-    return make_double2(q[0], q[1]);
-}
-
-// Azimuthal number m of mode j (modes ordered by l, then m = -l, -l+2, ..., l).
-__device__ __forceinline__ int mode_l(int j)
-{
-    const int kk = (int)(sqrt(8.0 * j + 1.0) - 1.0) / 2;
-    return (kk + 1) * (kk + 2) / 2 <= j ? kk + 1 : kk;
-}
-
-__device__ __forceinline__ int mode_m(int j)
-{
-    const int l = mode_l(j);
-    return 2 * (j - l * (l + 1) / 2) - l;
-}
-
-// The blocks of one launch that belong to one solver: this block's index
-// among them and their count. A plain launch gives each solver the whole
-// grid; a batched ragged launch (one grid over many solvers) gives each its
-// own contiguous range of blocks.
-struct Blk {
-    int id, n;
-};
-__device__ __forceinline__ Blk grid_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x}; }
-
-// The member of a batched launch that owns this block: the largest m with
-// off[m] <= blockIdx.x (off holds M + 1 non-decreasing block offsets).
-__device__ __forceinline__ int batch_member(const int *off, int M)
-{
-    const int b = blockIdx.x;
-    int lo = 0, hi = M;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (__builtin_amdgcn_readfirstlane(off[mid]) <= b) lo = mid;
-        else hi = mid;
-    }
-    // uniform over the block: the member's arguments load into scalar registers
-    return __builtin_amdgcn_readfirstlane(lo);
-}
-
-// Sum NQ per-thread values over the block; every block stores its totals in
-// partial[q * nblocks + block]. Returns true in the last block to finish, whose
-// threads then see every partial. blockDim.x must be a multiple of 64, <= 1024.
-template <int NQ>
-__device__ bool block_partials(const double (&v)[NQ], double *partial, unsigned *counter, Blk B)
-{
-    __shared__ double ws[16][NQ];
-    __shared__ bool last;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const double s = wave_sum(v[q]);
-        if (lane == 0) ws[wv][q] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < NQ; ++q) {
-            double s = 0.0;
-            for (int w = 0; w < nw; ++w) s += ws[w][q];
-            // agent-scope store: written through to the device coherence
-            // point, so no release fence (a write-back of this XCD's whole
-            // L2, which every block of a streaming kernel would wait on) is
-            // needed for the partials
-            __hip_atomic_store(partial + (size_t)q * B.n + B.id, s, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // The partial stores have completed before this block is counted.
-        // The ordering rests on the AMDGPU memory model for gfx950: an
-        // agent-scope store is performed at the device coherence point once
-        // vmcnt reaches zero, so after s_waitcnt vmcnt(0) it is visible to
-        // any agent-scope load issued later, and the relaxed counter RMW
-        // cannot be issued before the wait. The signal fences are compiler
-        // barriers: they keep the compiler from moving the stores below the
-        // wait or the RMW above it (a release fence would instead write back
-        // this XCD's whole L2).
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == (unsigned)B.n - 1);
-    }
-    __syncthreads();
-    // acquire in the one block that reads every partial (coherent loads)
-    if (last) __threadfence();
-    return last;
-}
-
-// In the last block: sum the partials of quantity q in block order.
-__device__ double sum_partials(const double *partial, int q, Blk B)
-{
-    __shared__ double ws[16];
-    const int nb = B.n;
-    double s = 0.0;
-    // contiguous slice per thread keeps the order fixed for a fixed grid
-    const int per = (nb + blockDim.x - 1) / blockDim.x;
-    const int b0 = threadIdx.x * per;
-    for (int b = b0; b < b0 + per && b < nb; ++b) s += vload(partial + (size_t)q * nb + b);
-    // fixed-order combine across threads
-    s = wave_sum(s);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) ws[wv] = s;
-    __syncthreads();
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += ws[w];
-    __syncthreads();
-    return t;
-}
 
 // ---------------------------------------------------------------- E table
 
@@ -243,25 +91,6 @@ __global__ __launch_bounds__(64) void k_lcut(int N, int Ls, double tau, const do
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ double2 load_x(const ApplyArgs &a, int k, int i)
-{
-    return a.X[(size_t)k * a.g.N + i];
-}
-
-// Rank-one sink term sum_a coef_a v_a at (mode j, node i).
-__device__ __forceinline__ double2 sink_term(const ApplyArgs &a, int nsink, double2 coef0, double2 coef1, int j, int i)
-{
-    double2 t = make_double2(0.0, 0.0);
-    if (nsink > 0) {
-        const int d = i - a.sk[0].lo;
-        if (d >= 0 && d < a.sk[0].n) t = cadd(t, cmul(coef0, a.sk[0].v[(size_t)j * a.sk[0].n + d]));
-    }
-    if (nsink > 1) {
-        const int d = i - a.sk[1].lo;
-        if (d >= 0 && d < a.sk[1].n) t = cadd(t, cmul(coef1, a.sk[1].v[(size_t)j * a.sk[1].n + d]));
-    }
-    return t;
-}
 
 // Coupling of the 16 rows of a chunk at the NG x 16 nodes of a tile, on the
 // matrix cores. The chunk's pass table lists its (l1, slot group) blocks; each
@@ -1974,6 +1803,11 @@ PhaseArgs phase_args(const ApplyArgs &a)
 
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st)
 {
+    if (a.ng == TAPPLY_NG) {
+        // the polar form reads X and applies the phases itself
+        a.mode = mode;
+        return launch_tapply(a, st);
+    }
     if (a.ng != 1 && a.ng != 2 && a.ng != 4) return hipErrorInvalidValue;
     if (!a.Xt) return hipErrorInvalidValue;
     const int grid = apply_grid(a.g.N, a.g.nchunks, a.ng, &a.ngroups);
@@ -2076,6 +1910,8 @@ hipError_t launch_batch(const BatchLaunch &b, hipStream_t st)
     const dim3 g(b.total);
     switch (b.kernel) {
     case BK_APPLY:
+        if (b.ng == TAPPLY_NG)
+            return launch_tapply_batch(static_cast<const ApplyArgs *>(b.args), b.off, b.M, b.total, b.var, b.mixed, st);
         if (b.ng == 1) launch_apply_batch_ng<1>(b, st);
         else if (b.ng == 2) launch_apply_batch_ng<2>(b, st);
         else if (b.ng == 4) launch_apply_batch_ng<4>(b, st);

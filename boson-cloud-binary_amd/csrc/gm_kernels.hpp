@@ -31,7 +31,7 @@ struct ApplyArgs {
     const double *mask;  // diagnostics apply: damping mask, or nullptr
     int ngroups;         // chunk groups per tile, set by the launcher
     int mixed;           // coupling in FP32 (E32, GM32) with FP32 accumulation
-    int ng;              // 16-node groups per tile (1, 2 or 4), chosen by the autotuner
+    int ng;              // 16-node groups per tile (1, 2 or 4), or 8: the polar-quadrature form (8-node tiles)
     void *Xt;            // scratch: exp(i m theta) X, double2 (FP64) or float2 (mixed), K * N
     int xt_ready;        // Xt already holds the phased X (written by its producer): no phase pass
     int mode;            // ApplyMode, set by the launcher
@@ -201,7 +201,7 @@ struct BatchLaunch {
     const int *off;    // device array of M + 1 block offsets
     int M, total;
     int ng, mixed;     // BK_APPLY: tile and precision of every member of the launch
-    int var, f32;      // BK_PRECOND: variant and sweep precision
+    int var, f32;      // BK_PRECOND: variant and sweep precision; BK_APPLY with ng = TAPPLY_NG: var = l_max class
     int pool;          // BK_APPLY: first of the M argument entries in the constant pool, or -1 (args)
 };
 // Argument entries of batched applies in constant memory, per device: the
@@ -222,6 +222,15 @@ hipError_t launch_krylov_gather(gmd::Krylov *const *kr, int *out, int M, hipStre
 
 // grid of the operator apply for tiles of 16 ng nodes (ng = 1, 2 or 4)
 int apply_grid(int N, int nchunks, int ng, int *ngroups);
+// The apply with the coupling in polar-quadrature form (gm_theta.hip), chosen
+// by ng = TAPPLY_NG: one 64-thread block per 8 radial nodes, compiled per
+// l_max class (a multiple of 4, up to 24 in FP64 and 32 in FP32).
+constexpr int TAPPLY_NG = 8;
+int tapply_class(int lmax, int mixed);  // 0: not available
+int tapply_grid(int N);
+hipError_t launch_tapply(const ApplyArgs &a, hipStream_t st);
+hipError_t launch_tapply_batch(const ApplyArgs *args, const int *off, int M, int total, int cls, int mixed,
+                               hipStream_t st);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
 PhaseArgs phase_args(const ApplyArgs &a);
 // radial profiles from the orbit in P (R1, R2, q, Mt, prf)

@@ -28,6 +28,7 @@
 #include "gm_kernels.hpp"
 #include "gm_ilut.hpp"
 #include "gm_math.hpp"
+#include "gm_theta.hpp"
 
 namespace {
 
@@ -155,6 +156,9 @@ struct gm_solver {
     DevBuf<double> GM, Gdiag, E, dmask;
     DevBuf<float> GM32, E32;  // FP32 copies for the mixed-precision Krylov applies
     DevBuf<int> lcut;
+    // polar-quadrature form of the coupling (gm_theta.hpp)
+    std::shared_ptr<const gmt::ThetaTables> theta_h;
+    DevBuf<double> TA, TC, TU;
     // absorbers: radial nodes, the cap quadrature and the (j, 00) angular coefficients
     DevBuf<double> dr_nodes, dglx, dglw, dang;
     int abs_nq = 0;
@@ -215,10 +219,11 @@ struct gm_solver {
     // last key requested per kind and precision (capture once it repeats; a
     // mixed step retried in FP64 alternates the two)
     std::vector<uint64_t> last_key[3][2];
-    // operator-apply tile (16-node groups) per precision, 0 until tuned, and
-    // the measured time per candidate (ms; index ng = 1, 2, 4 -> 0, 1, 2)
+    // operator-apply form per precision, 0 until tuned: Gaunt-form tiles of
+    // 16 ng nodes (ng = 1, 2, 4) or the polar form (gmk::TAPPLY_NG), and the
+    // measured time per candidate (ms; 16, 32, 64 nodes, polar; 0 if not timed)
     int apply_ng[2] = {0, 0};
-    double apply_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double apply_ms[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     std::string err;
     // profiling (gm_profile_enable)
     bool prof_on = false;
@@ -436,6 +441,7 @@ struct HostTables {
     std::vector<gmd::MChunk> mchunks;
     std::vector<gmd::MBlock> mblocks;
     int nchunks = 0;
+    std::shared_ptr<const gmt::ThetaTables> theta;
 };
 std::mutex tables_mu;
 std::map<int, std::shared_ptr<const HostTables>> tables_cache;
@@ -459,6 +465,8 @@ void host_tables(gm_solver &S)
         H->mchunks = S.mchunks_h;
         H->mblocks = S.mblocks_h;
         H->nchunks = S.nchunks;
+        H->theta = std::make_shared<const gmt::ThetaTables>(gmt::build_theta_tables(S.l_max));
+        S.theta_h = H->theta;
         std::lock_guard<std::mutex> lk(tables_mu);
         tables_cache.emplace(S.l_max, H);
         return;
@@ -470,6 +478,7 @@ void host_tables(gm_solver &S)
     S.mchunks_h = T->mchunks;
     S.mblocks_h = T->mblocks;
     S.nchunks = T->nchunks;
+    S.theta_h = T->theta;
     S.ntiles = (S.Nr + gmd::TILE - 1) / gmd::TILE;
 }
 
@@ -511,6 +520,11 @@ gmd::Geometry geometry(const gm_solver &S)
     g.E = S.E.p;
     g.lcut = S.lcut.p;
     g.cap = S.cap ? S.dcap.p : nullptr;
+    g.TA = S.TA.p;
+    g.TC = S.TC.p;
+    g.TU = S.TU.p;
+    g.tks = S.theta_h ? S.theta_h->ks : 0;
+    g.tkd = S.theta_h ? S.theta_h->kd : 0;
     return g;
 }
 
@@ -959,6 +973,15 @@ void woodbury_correct(gm_solver &S, double2 *Y, void *Xt = nullptr, int xt_f32 =
 // rhs, residual check and diagnostics keep the FP64 operator. The
 // preconditioner (and the Woodbury correction) also write the phased copy
 // exp(i m theta) y that the next apply's coupling reads.
+// The phased copy exp(i m theta) y that a Gaunt-form Krylov apply reads, or
+// nullptr when the Krylov applies take the polar form (it phases x itself).
+void *krylov_xt(gm_solver &S)
+{
+    const int mixed = S.flags.precision == GM_PRECISION_MIXED;
+    const int ng = mixed && S.apply_ng[1] ? S.apply_ng[1] : (S.apply_ng[0] ? S.apply_ng[0] : 2);
+    return ng == gmk::TAPPLY_NG ? nullptr : S.xt.p;
+}
+
 gmk::PrecArgs iter_prec_args(gm_solver &S, bool wood, int var)
 {
     gmk::PrecArgs pa = prec_args(S);
@@ -968,7 +991,7 @@ gmk::PrecArgs iter_prec_args(gm_solver &S, bool wood, int var)
         pa.inv32 = S.inv32.p;
         pa.Y32 = S.y32.p;
     }
-    pa.Xt = S.xt.p;
+    pa.Xt = krylov_xt(S);
     pa.xt_f32 = mixed;
     pa.R = S.rr.p;
     pa.R0 = S.r0.p;
@@ -1010,12 +1033,12 @@ void enqueue_iteration(gm_solver &S, bool wood)
     const int mixed = S.flags.precision == GM_PRECISION_MIXED;
     const gmk::PrecArgs pp = iter_prec_args(S, wood, gmk::PV_P);
     launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(pp, gmk::PV_P, S.st); });
-    if (wood) woodbury_correct(S, S.y.p, S.xt.p, mixed);
+    if (wood) woodbury_correct(S, S.y.p, krylov_xt(S), mixed);
     const gmk::ApplyArgs av = iter_apply_args(S, gmk::AM_V);
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(av, gmk::AM_V, S.st); });
     const gmk::PrecArgs ps = iter_prec_args(S, wood, gmk::PV_S);
     launch(S, GM_PROF_PRECOND, [&] { return gmk::launch_precond(ps, gmk::PV_S, S.st); });
-    if (wood) woodbury_correct(S, S.z.p, S.xt.p, mixed);
+    if (wood) woodbury_correct(S, S.z.p, krylov_xt(S), mixed);
     const gmk::ApplyArgs at = iter_apply_args(S, gmk::AM_T);
     launch(S, GM_PROF_APPLY_KRYLOV, [&] { return gmk::launch_apply(at, gmk::AM_T, S.st); });
     const gmk::UpdateArgs u = update_args(S);
@@ -1323,7 +1346,7 @@ void alloc_device(gm_solver &S)
     S.lcut.alloc(S.ntiles);
     int per = 0;
     const int grid = gmk::apply_grid(S.Nr, S.nchunks, 1, &per);
-    S.partial.alloc(static_cast<size_t>(3) * std::max(grid, gmd::RED_BLOCKS));
+    S.partial.alloc(static_cast<size_t>(3) * std::max({grid, gmk::tapply_grid(S.Nr), gmd::RED_BLOCKS}));
     S.sinkscr.alloc(1);
     S.rowpart.alloc(static_cast<size_t>(S.ntiles) * S.K * 6);
     S.dpsi_ix.alloc(S.K);
@@ -1434,6 +1457,15 @@ void alloc_device(gm_solver &S)
         HIPCK(hipMemcpy(S.GM.p, gmc.data(), gmc.size() * sizeof(double), hipMemcpyHostToDevice));
         const std::vector<float> gm32(gmc.begin(), gmc.end());
         HIPCK(hipMemcpy(S.GM32.p, gm32.data(), gm32.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (S.theta_h) {
+            const gmt::ThetaTables &T = *S.theta_h;
+            S.TA.alloc(T.A.size());
+            S.TC.alloc(T.C.size());
+            S.TU.alloc(T.U.size());
+            HIPCK(hipMemcpy(S.TA.p, T.A.data(), T.A.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCK(hipMemcpy(S.TC.p, T.C.data(), T.C.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCK(hipMemcpy(S.TU.p, T.U.data(), T.U.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
         S.msteps.alloc(msteps.size());
         S.mstep_start.alloc(sstart.size());
         S.pair_start.alloc(pstart.size());
@@ -1998,9 +2030,9 @@ int gm_apply_config(const gm_solver *S, int32_t precision, int32_t *tile_nodes, 
 {
     if (!S || (precision != GM_PRECISION_FP64 && precision != GM_PRECISION_MIXED) || !tile_nodes) return GM_E_INVALID;
     const int m = precision == GM_PRECISION_MIXED;
-    *tile_nodes = 16 * S->apply_ng[m];
+    *tile_nodes = S->apply_ng[m] == gmk::TAPPLY_NG ? 8 : 16 * S->apply_ng[m];
     if (ms)
-        for (int c = 0; c < 3; ++c) ms[c] = S->apply_ms[m][c];
+        for (int c = 0; c < 4; ++c) ms[c] = S->apply_ms[m][c];
     return GM_OK;
 }
 
@@ -2127,12 +2159,17 @@ int gm_set_flags(gm_solver *S, const gm_flags *f)
         if (!(f->power_average_orbits > 0.0)) throw std::invalid_argument("RollingPhaseAverage: n_orbits must be positive");
         if (f->precision != GM_PRECISION_FP64 && f->precision != GM_PRECISION_MIXED)
             throw std::invalid_argument("precision must be GM_PRECISION_FP64 or GM_PRECISION_MIXED");
-        if (f->apply_tile != 0 && f->apply_tile != 16 && f->apply_tile != 32 && f->apply_tile != 64)
-            throw std::invalid_argument("apply_tile must be 0 (autotune), 16, 32 or 64");
+        if (f->apply_tile != 0 && f->apply_tile != 8 && f->apply_tile != 16 && f->apply_tile != 32 &&
+            f->apply_tile != 64)
+            throw std::invalid_argument("apply_tile must be 0 (autotune), 8 (polar form), 16, 32 or 64");
+        if (f->apply_tile == 8 && gmk::tapply_class(S->l_max, 0) == 0)
+            throw std::invalid_argument("apply_tile 8 (the polar-quadrature form) needs l_max <= 24");
         if (f->power_average_orbits != S->flags.power_average_orbits)
             S->avg = std::make_unique<gmh::RollingPhaseAverage>(f->power_average_orbits);
         if (f->ramp_secondary_potential != S->flags.ramp_secondary_potential) S->E_valid = false;
         if (f->apply_tile != S->flags.apply_tile) S->apply_ng[0] = S->apply_ng[1] = 0;
+        // a fixed form applies at once (to the solver's own applies as well)
+        if (f->apply_tile) S->apply_ng[0] = S->apply_ng[1] = f->apply_tile == 8 ? gmk::TAPPLY_NG : f->apply_tile / 16;
         S->flags = *f;
         S->retry_mixed = false;
     });
@@ -2275,12 +2312,13 @@ struct TuneKey {
     }
 };
 std::mutex tune_mu;
-std::map<TuneKey, std::pair<int, std::array<double, 3>>> tune_table;
+std::map<TuneKey, std::pair<int, std::array<double, 4>>> tune_table;
 
 void tune_apply(gm_solver &S, int mixed)
 {
+    const bool polar_ok = gmk::tapply_class(S.l_max, mixed) != 0;
     if (S.flags.apply_tile) {
-        S.apply_ng[mixed] = S.flags.apply_tile / 16;
+        S.apply_ng[mixed] = S.flags.apply_tile == 8 ? gmk::TAPPLY_NG : S.flags.apply_tile / 16;
         return;
     }
     if (S.apply_ng[mixed]) return;
@@ -2290,24 +2328,25 @@ void tune_apply(gm_solver &S, int mixed)
         auto it = tune_table.find(key);
         if (it != tune_table.end()) {
             S.apply_ng[mixed] = it->second.first;
-            for (int c = 0; c < 3; ++c) S.apply_ms[mixed][c] = it->second.second[c];
+            for (int c = 0; c < 4; ++c) S.apply_ms[mixed][c] = it->second.second[c];
             return;
         }
     }
     hipEvent_t e0, e1;
     HIPCK(hipEventCreate(&e0));
     HIPCK(hipEventCreate(&e1));
-    const int cand[3] = {1, 2, 4};
-    std::array<double, 3> ms{};
+    const int cand[4] = {1, 2, 4, gmk::TAPPLY_NG};
+    const int ncand = polar_ok ? 4 : 3;
+    std::array<double, 4> ms{};
     int best = 2;
     double best_ms = 1e300;
     try {
         // the residual form: full coupling plus a fused reduction, like the
         // Krylov applies (its result in the Krylov state is overwritten by the
         // step's own residual check); two interleaved rounds, best of each
-        for (int c = 0; c < 3; ++c) ms[c] = 1e300;
+        for (int c = 0; c < ncand; ++c) ms[c] = 1e300;
         for (int round = 0; round < 2; ++round)
-            for (int c = 0; c < 3; ++c) {
+            for (int c = 0; c < ncand; ++c) {
                 gmk::ApplyArgs a = apply_args(S, S.psi.p, nullptr);
                 a.R0 = S.psi.p;
                 a.mixed = mixed;
@@ -2321,11 +2360,12 @@ void tune_apply(gm_solver &S, int mixed)
                 HIPCK(hipEventElapsedTime(&t, e0, e1));
                 ms[c] = std::min(ms[c], t / 5.0);
             }
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < ncand; ++c)
             if (ms[c] < best_ms) {
                 best_ms = ms[c];
                 best = cand[c];
             }
+        if (!polar_ok) ms[3] = 0.0;
     } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -2334,7 +2374,7 @@ void tune_apply(gm_solver &S, int mixed)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     S.apply_ng[mixed] = best;
-    for (int c = 0; c < 3; ++c) S.apply_ms[mixed][c] = ms[c];
+    for (int c = 0; c < 4; ++c) S.apply_ms[mixed][c] = ms[c];
     std::lock_guard<std::mutex> lk(tune_mu);
     tune_table[key] = {best, ms};
 }

@@ -128,11 +128,12 @@ class CrankNicolsonModes(capi.Solver):
         return out
 
     def apply_config(self, precision=GM_PRECISION_FP64):
-        """(tile nodes chosen by the autotuner, {16: ms, 32: ms, 64: ms})."""
+        """(apply form chosen by the autotuner: tile nodes 16, 32, 64, or 8 for the
+        polar-quadrature form; {16: ms, 32: ms, 64: ms, 8: ms}, 0 for a form not timed)."""
         t = C.c_int32()
-        ms = (C.c_double * 3)()
+        ms = (C.c_double * 4)()
         self._check(self._lib.gm_apply_config(self._h, int(precision), C.byref(t), ms))
-        return t.value, {16: ms[0], 32: ms[1], 64: ms[2]}
+        return t.value, {16: ms[0], 32: ms[1], 64: ms[2], 8: ms[3]}
 
     def profile(self, on):
         self._check(self._lib.gm_profile_enable(self._h, int(on)))

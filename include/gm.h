@@ -76,7 +76,9 @@ typedef struct gm_flags {
     int32_t precision;                /* GM_PRECISION_FP64 (default) or _MIXED */
     int32_t use_graphs;               /* replay captured parts of the step (default 1);
                                          profiling (gm_profile_enable) runs without graphs */
-    int32_t apply_tile;               /* operator-apply tile: 0 = autotune (default), or 16, 32, 64 nodes */
+    int32_t apply_tile;               /* operator-apply form: 0 = autotune (default); 16, 32, 64 = Gaunt-form
+                                         tiles of that many nodes; 8 = the polar-quadrature form
+                                         (8-node tiles; l_max <= 24 in FP64, <= 32 mixed) */
 } gm_flags;
 
 /* Orbit state: the 13 doubles of the checkpoint (restart.cpp:153-163) in the
@@ -268,11 +270,14 @@ int gm_coupling_work(gm_solver *solver, double *terms, double *pairs);
  * nodes [64 t, 64 t + 64); n must be at least ceil(Nr / 64). */
 int gm_multipole_cut(gm_solver *solver, int32_t *lcut, int32_t n);
 
-/* Operator-apply tile chosen by the autotuner (no counterpart in the
- * reference): on a solver's first step each tile of 16, 32 or 64 radial nodes
- * is timed on its stream and the fastest kept, per (N, l_max, precision,
- * coupling tolerance) for the whole process. tile_nodes receives the choice (0
- * before the first step); ms, if not NULL, the 3 measured times in ms. */
+/* Operator-apply form chosen by the autotuner (no counterpart in the
+ * reference): on a solver's first step the Gaunt-form tiles of 16, 32 and 64
+ * radial nodes and the polar-quadrature form (8-node tiles, where its l_max
+ * class exists) are timed on the solver's stream and the fastest kept, per
+ * (N, l_max, precision, coupling tolerance) for the whole process. tile_nodes
+ * receives the choice (16, 32, 64, or 8 for the polar form; 0 before the first
+ * step); ms, if not NULL, the 4 measured times in ms (16, 32, 64, polar; 0 for
+ * a form not timed). */
 int gm_apply_config(const gm_solver *solver, int32_t precision, int32_t *tile_nodes, double *ms);
 
 /* Measurement hook (no counterpart in the reference; for benchmarks): device
