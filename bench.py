@@ -749,7 +749,7 @@ def main():
                     help="lockstep mixed-vs-FP64 steps at l_max 30, N 32768 (0 skips them)")
     ap.add_argument("--evolving-steps", type=int, default=50,
                     help="timed steps per precision with the orbit evolving (0 skips them)")
-    ap.add_argument("--apply-tile", type=int, default=0, choices=(0, 16, 32, 64),
+    ap.add_argument("--apply-tile", type=int, default=0, choices=(0, 8, 16, 32, 64),
                     help="fix the coupling apply's tile in nodes (0: autotune; profiler runs fix the tuned one, "
                          "since counter collection serialises the tuner's timings)")
     ap.add_argument("--headline-batch", type=int, default=8,

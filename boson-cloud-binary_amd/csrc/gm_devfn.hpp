@@ -209,27 +209,47 @@ __device__ __forceinline__ ApplyEpi apply_epi(const ApplyArgs &a)
     return e;
 }
 
+// The loads of one element (mode j, node i) of the apply: x at i and its two
+// neighbours (clamped to the grid; used only on interior rows), the CAP and
+// the reduction partner R0 where the mode has one. Split from the arithmetic
+// so that a caller can issue the loads of several elements before using any.
+struct ElemIn {
+    double2 xi, xm, xp, cap, r0;
+};
+
+__device__ __forceinline__ ElemIn apply_element_load(const ApplyArgs &a, int j, int i)
+{
+    const gmd::Geometry &g = a.g;
+    const int N = g.N, MODE = a.mode;
+    const size_t row = (size_t)j * N + i;
+    ElemIn in;
+    in.xi = a.X[row];
+    in.xm = a.X[row - (i > 0 ? 1 : 0)];
+    in.xp = a.X[row + (i < N - 1 ? 1 : 0)];
+    in.cap = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
+    const bool uses_r0 = MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_RST;
+    in.r0 = uses_r0 ? a.R0[row] : make_double2(0.0, 0.0);
+    return in;
+}
+
 // One element (mode j = (l, m), node i) of the apply, given the coupling Cc
-// = (V x)_j at node i: the three-point stencil, the centrifugal term and the
-// CAP (hamiltonian.cpp:44-66,224-243), A = 1 + i dt/2 H (matrices.cpp:27-40)
-// with the rank-one sinks (matrices.cpp:111-116), the output of the mode and
-// its terms of the fused reductions.
+// = (V x)_j at node i and the element's loads: the three-point stencil, the
+// centrifugal term and the CAP (hamiltonian.cpp:44-66,224-243), A = 1 + i dt/2
+// H (matrices.cpp:27-40) with the rank-one sinks (matrices.cpp:111-116), the
+// output of the mode and its terms of the fused reductions.
 __device__ __forceinline__ void apply_element(const ApplyArgs &a, const ApplyEpi &ep, int j, int l, int i, double2 Cc,
-                                              double (&red)[3])
+                                              const ElemIn &in, double (&red)[3])
 {
     const gmd::Geometry &g = a.g;
     const int N = g.N, MODE = a.mode;
     const size_t row = (size_t)j * N + i;
     if (a.Cout) a.Cout[row] = Cc;
-    const double2 xi = load_x(a, j, i);
-    const double2 capv = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
+    const double2 xi = in.xi, capv = in.cap;
     double2 Hx;
     if (i > 0 && i < N - 1) {
         const double ri = g.eps + i * g.dr;
         const double h0d = -(-2.0 * ep.alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * g.mu);
-        const double2 xm = load_x(a, j, i - 1);
-        const double2 xp = load_x(a, j, i + 1);
-        Hx = cadd(cscale(g.off, cadd(xm, xp)), cadd(cmul(make_double2(h0d + capv.x, capv.y), xi), Cc));
+        Hx = cadd(cscale(g.off, cadd(in.xm, in.xp)), cadd(cmul(make_double2(h0d + capv.x, capv.y), xi), Cc));
     } else {
         Hx = cmul(capv, xi);
     }
@@ -251,12 +271,12 @@ __device__ __forceinline__ void apply_element(const ApplyArgs &a, const ApplyEpi
         if (ep.nsink) out = cadd(out, sink_term(a, ep.nsink, ep.coef0, ep.coef1, j, i));
         if (MODE == AM_V) {
             a.Y[row] = out;
-            const double2 d = cdotm(a.R0[row], out);
+            const double2 d = cdotm(in.r0, out);
             red[0] += d.x;
             red[1] += d.y;
         } else if (MODE == AM_T) {
             a.Y[row] = out;
-            const double2 d = cdotm(out, a.R0[row]);
+            const double2 d = cdotm(out, in.r0);
             red[0] += d.x;
             red[1] += d.y;
             red[2] += cnorm2(out);
@@ -264,13 +284,13 @@ __device__ __forceinline__ void apply_element(const ApplyArgs &a, const ApplyEpi
             a.Y[row] = out;
         } else if (MODE == AM_RST) {
             // r = b - A x of the normalised system, r0' = |rhs| r
-            const double2 bq = make_double2(a.R0[row].x / ep.rhs_nrm, a.R0[row].y / ep.rhs_nrm);
+            const double2 bq = make_double2(in.r0.x / ep.rhs_nrm, in.r0.y / ep.rhs_nrm);
             const double2 rq = csub(bq, out);
             a.Y[row] = rq;
             a.Y2[row] = cscale(ep.rhs_nrm, rq);
             red[0] += cnorm2(rq);
         } else {  // AM_RES
-            double2 bq = a.R0[row];
+            double2 bq = in.r0;
             if (a.r0_raw && ep.rhs_nrm != 0.0) bq = make_double2(bq.x / ep.rhs_nrm, bq.y / ep.rhs_nrm);
             red[0] += cnorm2(csub(out, bq));
         }

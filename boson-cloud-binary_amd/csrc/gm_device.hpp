@@ -99,6 +99,7 @@ struct Geometry {
     // tables of phases A, C and the U sums, all rules; tks / tkd blocks per point group
     const double *TA, *TC, *TU;
     int tks, tkd;
+    const double *Ut;          // U sums of every 8-node tile (rebuilt with E and the cut)
 };
 
 // One s-wave sink restricted to its radial support [lo, lo+n).

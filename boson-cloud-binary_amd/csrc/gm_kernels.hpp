@@ -224,11 +224,15 @@ hipError_t launch_krylov_gather(gmd::Krylov *const *kr, int *out, int M, hipStre
 int apply_grid(int N, int nchunks, int ng, int *ngroups);
 // The apply with the coupling in polar-quadrature form (gm_theta.hip), chosen
 // by ng = TAPPLY_NG: one 64-thread block per 8 radial nodes, compiled per
-// l_max class (a multiple of 4, up to 24 in FP64 and 32 in FP32).
+// l_max class (a multiple of 4, up to 24).
 constexpr int TAPPLY_NG = 8;
 int tapply_class(int lmax, int mixed);  // 0: not available
 int tapply_grid(int N);
 hipError_t launch_tapply(const ApplyArgs &a, hipStream_t st);
+// The polar form's U-sum table (doubles per solver; 0 without a class) and
+// its build from the radial profiles and the cut (g.Ut unused here).
+size_t tapply_u_doubles(int N, int lmax);
+hipError_t launch_tapply_u(const gmd::Geometry &g, double *Ut, hipStream_t st);
 hipError_t launch_tapply_batch(const ApplyArgs *args, const int *off, int M, int total, int cls, int mixed,
                                hipStream_t st);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);

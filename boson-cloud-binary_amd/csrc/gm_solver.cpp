@@ -159,6 +159,7 @@ struct gm_solver {
     // polar-quadrature form of the coupling (gm_theta.hpp)
     std::shared_ptr<const gmt::ThetaTables> theta_h;
     DevBuf<double> TA, TC, TU;
+    DevBuf<double> Ut;  // U sums of every tile, rebuilt with the radial profiles and the cut
     // absorbers: radial nodes, the cap quadrature and the (j, 00) angular coefficients
     DevBuf<double> dr_nodes, dglx, dglw, dang;
     int abs_nq = 0;
@@ -465,7 +466,8 @@ void host_tables(gm_solver &S)
         H->mchunks = S.mchunks_h;
         H->mblocks = S.mblocks_h;
         H->nchunks = S.nchunks;
-        H->theta = std::make_shared<const gmt::ThetaTables>(gmt::build_theta_tables(S.l_max));
+        if (const int cls = gmk::tapply_class(S.l_max, 0))
+            H->theta = std::make_shared<const gmt::ThetaTables>(gmt::build_theta_tables(S.l_max, cls));
         S.theta_h = H->theta;
         std::lock_guard<std::mutex> lk(tables_mu);
         tables_cache.emplace(S.l_max, H);
@@ -525,6 +527,7 @@ gmd::Geometry geometry(const gm_solver &S)
     g.TU = S.TU.p;
     g.tks = S.theta_h ? S.theta_h->ks : 0;
     g.tkd = S.theta_h ? S.theta_h->kd : 0;
+    g.Ut = S.Ut.p;
     return g;
 }
 
@@ -706,6 +709,11 @@ void launch_profiles(gm_solver &S)
     launch(S, GM_PROF_FACTOR, [&] {
         return gmk::launch_lcut(S.Nr, 2 * S.l_max + 1, S.coupling_tol, S.E.p, S.lcut.p, S.st);
     });
+    // the polar form's U sums follow the profiles and the cut (skipped once
+    // both precisions settled on the Gaunt form)
+    const auto polar = [&](int p) { return S.apply_ng[p] == 0 || S.apply_ng[p] == gmk::TAPPLY_NG; };
+    if (S.Ut.p && (polar(0) || polar(1)))
+        launch(S, GM_PROF_FACTOR, [&] { return gmk::launch_tapply_u(geometry(S), S.Ut.p, S.st); });
 }
 
 // build_time_dependent_perturbation (hamiltonian.cpp:245-268) on the device
@@ -1465,6 +1473,7 @@ void alloc_device(gm_solver &S)
             HIPCK(hipMemcpy(S.TA.p, T.A.data(), T.A.size() * sizeof(double), hipMemcpyHostToDevice));
             HIPCK(hipMemcpy(S.TC.p, T.C.data(), T.C.size() * sizeof(double), hipMemcpyHostToDevice));
             HIPCK(hipMemcpy(S.TU.p, T.U.data(), T.U.size() * sizeof(double), hipMemcpyHostToDevice));
+            S.Ut.alloc(gmk::tapply_u_doubles(S.Nr, S.l_max));
         }
         S.msteps.alloc(msteps.size());
         S.mstep_start.alloc(sstart.size());
@@ -2042,7 +2051,9 @@ int gm_set_coupling_tolerance(gm_solver *S, double tol)
         if (!(tol >= 0.0) || tol >= 1e-6) throw std::invalid_argument("coupling tolerance must lie in [0, 1e-6)");
         S->coupling_tol = tol;
         S->E_valid = false;
-        S->apply_ng[0] = S->apply_ng[1] = 0;  // the cut changes the work per tile: tune again
+        // the cut changes the work per tile: tune again (a fixed form stays)
+        S->apply_ng[0] = S->apply_ng[1] =
+            S->flags.apply_tile ? (S->flags.apply_tile == 8 ? gmk::TAPPLY_NG : S->flags.apply_tile / 16) : 0;
         if (S->initialized) {
             build_perturbation(*S, S->t_i);
             HIPCK(hipStreamSynchronize(S->st));
@@ -2163,7 +2174,7 @@ int gm_set_flags(gm_solver *S, const gm_flags *f)
             f->apply_tile != 64)
             throw std::invalid_argument("apply_tile must be 0 (autotune), 8 (polar form), 16, 32 or 64");
         if (f->apply_tile == 8 && gmk::tapply_class(S->l_max, 0) == 0)
-            throw std::invalid_argument("apply_tile 8 (the polar-quadrature form) needs l_max <= 24");
+            throw std::invalid_argument("apply_tile 8 (the polar-quadrature form) needs l_max <= 20");
         if (f->power_average_orbits != S->flags.power_average_orbits)
             S->avg = std::make_unique<gmh::RollingPhaseAverage>(f->power_average_orbits);
         if (f->ramp_secondary_potential != S->flags.ramp_secondary_potential) S->E_valid = false;

@@ -30,10 +30,13 @@
 
 namespace gmt {
 
+// The blocks follow the layout of the kernel's l_max class LM >= l_max (so
+// that its loops have compile-time bounds); entries of modes l > l_max and of
+// multipoles l* > 2 l_max are zero.
 struct ThetaTables {
-    int lmax = 0;
-    int ks = 0;     // (m, s) blocks of phases A and C: m = -l_max..l_max, s < ceil(n_m / 4)
-    int kd = 0;     // (d, s) blocks of the U sums: d = 0..2 l_max, s < ceil(n_l*(d) / 4)
+    int lmax = 0, LM = 0;
+    int ks = 0;     // (m, s) blocks of phases A and C: m = -LM..LM, s < ceil(n_m / 4), n_m = (LM - |m|) / 2 + 1
+    int kd = 0;     // (d, s) blocks of the U sums: d = 0..2 LM, s < ceil(n_l*(d) / 4), n_l* = (2 LM - d) / 2 + 1
     int ngmax = 0;  // rules ng = 1..ngmax (4 ng points on x > 0)
     // TA[rule][t][ks][16]: p_(m, 4 s + a)(x_(4 t + b)), rule ng at ks * 16 * ng (ng - 1) / 2
     std::vector<double> A;
@@ -47,7 +50,7 @@ struct ThetaTables {
 // smallest with 16 ng - 1 >= 2 l_max + lc.
 __host__ __device__ inline int theta_rule(int lmax, int lc) { return (2 * lmax + lc + 16) / 16; }
 
-ThetaTables build_theta_tables(int lmax);
+ThetaTables build_theta_tables(int lmax, int LM);
 
 // Gauss-Legendre rule of 2 n points on [-1, 1]: the n positive nodes
 // (ascending) and their weights, in long double (Newton on P_2n).

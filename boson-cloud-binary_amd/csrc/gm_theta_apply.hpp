@@ -1,0 +1,428 @@
+// gm_theta_apply.hpp - the operator apply with the coupling in polar-quadrature form
+// (kernel templates; gm_theta_f64.hip and gm_theta_f32.hip instantiate them).
+//
+// The coupling sum_k V_jk(r) x_k of the reference (hamiltonian.cpp:68-104,
+// 176-222) is evaluated through a Gauss-Legendre rule in x = cos(theta)
+// (gm_theta.hpp has the identity and the tables):
+//   U   U_d(x_i, r) = (1/2pi) sum_l* E_l*(r) p_l*^d(0) p_l*^d(x_i)   (d <= lc)
+//   A   X_i[m']     = sum_(k: m_k = m') p_k(x_i) xt_k                (xt = e^(i m theta) x)
+//   B   Y_i[m]      = sum_m' U_|m - m'|(x_i) X_i[m']                 (|m - m'| <= lc)
+//   C   (V x)_j     = e^(-i m_j theta) sum_i w_i p_j(x_i) Y_i[m_j]
+// A workgroup is one wavefront and one tile of 8 radial nodes, whose 16
+// columns are (node, real or imaginary part): every table is real, so the two
+// parts are transformed alike. U, A and C run on v_mfma_f64_4x4x4f64 in
+// quadrature groups of four points (four 4x4x4 blocks over four columns
+// each); B is a banded Toeplitz product on the VALU, one (point, column) per
+// lane, in the apply's arithmetic (FP64, or FP32 for the mixed-precision
+// Krylov applies). Work per node: about 2 (2 l_max + lc) n_pts K real
+// multiply-adds on the matrix core and n_pts (2 l_max + 1)(2 lc + 1) on the
+// VALU, against the 2 K^2 + 2 K (number of Gaunt terms) of the direct sum.
+// The epilogue is the Gaunt-form apply's (stencil, CN form, sinks, fused
+// reductions: gm_devfn.hpp).
+#pragma once
+#include "gm_devfn.hpp"
+#include "gm_theta.hpp"
+
+#include <type_traits>
+
+namespace gmk {
+
+namespace {
+
+constexpr int TT = 8;  // radial nodes of a wavefront's tile
+constexpr int TW = 4;  // wavefronts of a tile's workgroup (each takes a quarter of the output m)
+#ifndef GM_TEB
+#define GM_TEB 2
+#endif
+#ifndef GM_TWAVES
+#define GM_TWAVES 2
+#endif
+
+// Accumulator layout at l_max = LM: m = -LM .. LM in order, ceil(n_m / 4)
+// blocks each (n_m = (LM - |m|) / 2 + 1 modes); a compile-time table, so that
+// the indices of the register arrays fold to constants.
+__host__ __device__ constexpr int ns_of(int LM, int m)
+{
+    return ((LM - (m < 0 ? -m : m)) / 2 + 1 + 3) / 4;
+}
+__host__ __device__ constexpr int kcount(int LM)
+{
+    int o = 0;
+    for (int m = -LM; m <= LM; ++m) o += ns_of(LM, m);
+    return o;
+}
+template <int LM>
+struct TLay {
+    int ko[2 * LM + 2];    // first accumulator of m = i - LM
+    int me[kcount(LM)];    // m of accumulator e
+    int se[kcount(LM)];    // block s of accumulator e
+    constexpr TLay() : ko(), me(), se()
+    {
+        int o = 0;
+        for (int i = 0; i <= 2 * LM; ++i) {
+            ko[i] = o;
+            for (int s = 0; s < ns_of(LM, i - LM); ++s, ++o) {
+                me[o] = i - LM;
+                se[o] = s;
+            }
+        }
+        ko[2 * LM + 1] = o;
+    }
+};
+
+// Block layout of the U sums at class LM: d = 0 .. 2 LM, ceil(n_l*(d) / 4)
+// blocks each (n_l* = (2 LM - d) / 2 + 1 multipoles l* = d, d + 2, ...).
+template <int LM>
+struct DLay {
+    int ko[2 * LM + 2];  // first block of d
+    int ns[2 * LM + 1];  // blocks of d
+    constexpr DLay() : ko(), ns()
+    {
+        int o = 0;
+        for (int d = 0; d <= 2 * LM; ++d) {
+            ko[d] = o;
+            ns[d] = ((2 * LM - d) / 2 + 1 + 3) / 4;
+            o += ns[d];
+        }
+        ko[2 * LM + 1] = o;
+    }
+};
+
+__device__ __forceinline__ double mf(double a, double b, double c)
+{
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+
+// f(integral_constant<I>) for I = B .. E - 1 (unrolled at compile time)
+template <int I, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < E) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, E>(f);
+    }
+}
+// the same while f returns true
+template <int I, int E, class F>
+__device__ __forceinline__ void static_while(F &&f)
+{
+    if constexpr (I < E) {
+        if (f(std::integral_constant<int, I>{})) static_while<I + 1, E>(f);
+    }
+}
+
+}  // namespace
+
+// Doubles per tile of the U-sum table at class LM: every point group of the
+// largest rule and every diagonal d <= 2 LM, 4 points x TT nodes each.
+template <int LM>
+__host__ __device__ constexpr size_t ut_stride(int lmax)
+{
+    return (size_t)gmt::theta_rule(lmax, 2 * lmax) * (2 * LM + 1) * 4 * TT;
+}
+
+// The U sums of one tile (one wavefront): U_d(x_i, r) = sum_l* E_l*(r)
+// TU[i][d][l*] for the tile's rule and every point group, d <= lc (zero
+// above), on the matrix core, into Ut. They depend on the orbit's radial
+// profiles and the cut only, so they are rebuilt with those, not per apply.
+template <int LM>
+__device__ __forceinline__ void tapply_u_body(const gmd::Geometry g, double *Ut, int tile)
+{
+    constexpr DLay<LM> dl{};
+    constexpr int KD = dl.ko[2 * LM + 1];
+    __shared__ double sE[(2 * LM + 1) * TT];
+    const int lane = threadIdx.x;
+    const int a4 = lane >> 4, b4 = lane & 3;
+    const int la = 4 * a4 + b4;
+    const int col = 4 * ((lane >> 2) & 3) + b4;
+    const int nd = col >> 1;
+    const int N = g.N, lmax = g.lmax, Ls = g.Ls;
+    const int t0 = tile * TT;
+    const int lc = g.lcut ? g.lcut[t0 / gmd::TILE] : 2 * lmax;
+    for (int e = lane; e < (2 * LM + 1) * TT; e += 64) {
+        const int ls = e >> 3, i = t0 + (e & 7);
+        sE[e] = (ls <= lc && ls <= 2 * lmax && i < N) ? g.E[(size_t)((ls & 1) * Ls + ls) * N + i] : 0.0;
+    }
+    __syncthreads();
+    const int ng = gmt::theta_rule(lmax, lc);
+    const size_t tri = (size_t)ng * (ng - 1) / 2;
+    double *out = Ut + (size_t)tile * ut_stride<LM>(lmax) + a4 * TT + nd;
+    for (int t = 0; t < ng; ++t) {
+        const double *tu = g.TU + (tri + t) * (size_t)KD * 16 + la;
+        static_for<0, 2 * LM + 1>([&](auto dc) {
+            constexpr int d = dc();
+            double acc = 0.0;
+            if (d <= lc) {
+                static_for<0, dl.ns[d]>([&](auto sc) {
+                    constexpr int s = sc();
+                    const int ls = d + 2 * (4 * s + a4);
+                    const double e = sE[(ls < 2 * LM ? ls : 2 * LM) * TT + nd];
+                    acc = mf(tu[(dl.ko[d] + s) * 16], ls <= lc ? e : 0.0, acc);
+                });
+            }
+            // the two parts of a node hold the same value
+            if ((col & 1) == 0) out[(t * (2 * LM + 1) + d) * 4 * TT] = acc;
+        });
+    }
+}
+
+template <int LM>
+__global__ __launch_bounds__(64) void k_tapply_u(gmd::Geometry g, double *Ut)
+{
+    tapply_u_body<LM>(g, Ut, blockIdx.x);
+}
+
+// Output m of each wavefront: contiguous ranges [lo[w], lo[w + 1]) of
+// m + LM holding about a quarter of the accumulator blocks each.
+template <int LM>
+struct WSplit {
+    int lo[TW + 1];
+    constexpr WSplit() : lo()
+    {
+        const int total = kcount(LM);
+        int mi = 0, acc = 0;
+        for (int w = 0; w < TW; ++w) {
+            lo[w] = mi;
+            const int target = (total * (w + 1) + TW / 2) / TW;
+            while (mi < 2 * LM + 1 && (w == TW - 1 || acc + ns_of(LM, mi - LM) <= target)) acc += ns_of(LM, mi++ - LM);
+        }
+        lo[TW] = 2 * LM + 1;
+    }
+};
+
+// LDS of one workgroup (one tile of TT nodes).
+template <int LM>
+struct TShared {
+    static constexpr int KMAX = (LM + 1) * (LM + 2) / 2;
+    double x[KMAX * 16];           // phased x of the tile [mode][column], later its coupling
+    double X[(2 * LM + 1) * 64];   // X[m'] of the current point group, per lane
+    double A[kcount(LM) * 16];     // A operands of phase A, then of phase C
+};
+
+// The work of wavefront W: phase A for its m', phase B for its m (all X from
+// LDS, the U sums from their table), phase C for its m, then its
+// accumulators into the coupling tile.
+template <int LM, typename T, int W>
+__device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh, int lc, int ng, int lane, int tile)
+{
+    constexpr int NM = 2 * LM + 1;
+    constexpr TLay<LM> lay{};
+    constexpr int KL = kcount(LM);
+    constexpr DLay<LM> dl{};
+    constexpr int KD = dl.ko[2 * LM + 1];
+    constexpr WSplit<LM> ws{};
+    constexpr int M0 = ws.lo[W], M1 = ws.lo[W + 1];
+    constexpr int E0 = lay.ko[M0], E1 = lay.ko[M1];
+    const gmd::Geometry &g = a.g;
+    const int a4 = lane >> 4, b4 = lane & 3;
+    const int la = 4 * a4 + b4;                  // index in a table's 4x4 block
+    const int col = 4 * ((lane >> 2) & 3) + b4;  // column: node col >> 1, part col & 1
+    const int nd = col >> 1;
+    const int lmax = g.lmax;
+    const size_t tri = (size_t)ng * (ng - 1) / 2;
+    // U_d of this tile at (point 4 t + a4, node nd): Ut[((t (2 LM + 1) + d) 4 + a4) TT + nd]
+    const double *ut = g.Ut + (size_t)tile * ut_stride<LM>(g.lmax) + a4 * TT + nd;
+    double yacc[E1 - E0 > 0 ? E1 - E0 : 1];
+    static_for<E0, E1>([&](auto e) { yacc[e() - E0] = 0.0; });
+    for (int t = 0; t < ng; ++t) {
+        __syncthreads();
+        {
+            const double2 *ta = (const double2 *)(g.TA + (tri + t) * (size_t)KL * 16);
+            for (int e = threadIdx.x; e < KL * 8; e += 64 * TW) ((double2 *)sh.A)[e] = ta[e];
+        }
+        __syncthreads();
+        // A: X[m'] at (point 4 t + a4, column col) for this wavefront's m'
+        static_for<M0, M1>([&](auto mic) {
+            constexpr int mi = mic(), m = mi - LM, am = m < 0 ? -m : m;
+            constexpr int nb = lay.ko[mi + 1] - lay.ko[mi];
+            double av[nb], bv[nb];
+            static_for<0, nb>([&](auto sc) {
+                constexpr int s = sc();
+                const int l = am + 2 * (4 * s + a4);
+                const int j = l <= lmax ? l * (l + 1) / 2 + (m + l) / 2 : 0;  // zero in the table above l_max
+                av[s] = sh.A[(lay.ko[mi] + s) * 16 + la];
+                bv[s] = sh.x[j * 16 + col];
+            });
+            double acc = 0.0;
+            static_for<0, nb>([&](auto sc) { acc = mf(av[sc()], bv[sc()], acc); });
+            sh.X[mi * 64 + lane] = acc;
+        });
+        __syncthreads();
+        {
+            const double2 *tc = (const double2 *)(g.TC + (tri + t) * (size_t)KL * 16);
+            for (int e = threadIdx.x; e < KL * 8; e += 64 * TW) ((double2 *)sh.A)[e] = tc[e];
+        }
+        T X[NM];
+        static_for<0, NM>([&](auto mic) { X[mic()] = T(sh.X[mic() * 64 + lane]); });
+        // B: Y[m] = sum_d U_d (X[m + d] + X[m - d]) for this wavefront's m, the
+        // diagonals in groups of four (the table holds zeros above the cut)
+        T Y[M1 - M0 > 0 ? M1 - M0 : 1];
+        static_for<M0, M1>([&](auto mic) { Y[mic() - M0] = T(0); });
+        static_while<0, (2 * LM + 4) / 4>([&](auto gc) {
+            constexpr int d0 = 4 * gc();
+            if (d0 > lc) return false;
+            T u[4];
+            static_for<0, 4>([&](auto qc) {
+                constexpr int d = d0 + qc();
+                u[qc()] = T(0);
+                if constexpr (d <= 2 * LM) u[qc()] = T(ut[(t * (2 * LM + 1) + d) * 4 * TT]);
+            });
+            static_for<0, 4>([&](auto qc) {
+                constexpr int d = d0 + qc();
+                if constexpr (d <= 2 * LM) {
+                    static_for<M0, M1>([&](auto mic) {
+                        constexpr int mi = mic();
+                        if constexpr (d == 0) {
+                            Y[mi - M0] = u[0] * X[mi];
+                        } else {
+                            if constexpr (mi + d < NM) Y[mi - M0] = fma(u[qc()], X[mi + d], Y[mi - M0]);
+                            if constexpr (mi - d >= 0) Y[mi - M0] = fma(u[qc()], X[mi - d], Y[mi - M0]);
+                        }
+                    });
+                }
+            });
+            return true;
+        });
+        __syncthreads();
+        // C: the blocks of (V x) of this wavefront's m on the matrix core
+        static_for<M0, M1>([&](auto mic) {
+            constexpr int mi = mic();
+            constexpr int nb = lay.ko[mi + 1] - lay.ko[mi];
+            const double yd = (double)Y[mi - M0];
+            double av[nb];
+            static_for<0, nb>([&](auto sc) { av[sc()] = sh.A[(lay.ko[mi] + sc()) * 16 + la]; });
+            static_for<0, nb>([&](auto sc) {
+                constexpr int e = lay.ko[mi] + sc();
+                yacc[e - E0] = mf(av[sc()], yd, yacc[e - E0]);
+            });
+        });
+    }
+    // accumulator e: (V x) of mode (m_e, 4 s_e + a4) at column col, into the
+    // tile over the phased x (read for the last time before the barrier)
+    __syncthreads();
+    static_for<E0, E1>([&](auto ec) {
+        constexpr int e = ec(), m = lay.me[e], am = m < 0 ? -m : m, s = lay.se[e];
+        const int l = am + 2 * (4 * s + a4);
+        if (l <= lmax) sh.x[(l * (l + 1) / 2 + (m + l) / 2) * 16 + col] = yacc[e - E0];
+    });
+}
+
+template <int LM, typename T>
+__device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
+{
+    constexpr int KMAX = (LM + 1) * (LM + 2) / 2;
+    __shared__ TShared<LM> sh;
+    const gmd::Geometry &g = a.g;
+    gmd::Krylov *Kr = a.Kr;
+    if ((a.mode == AM_V || a.mode == AM_T) && Kr->done) return;
+    const gmd::StepParams *P = a.P;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int N = g.N, lmax = g.lmax, K = g.K;
+    const int t0 = B.id * TT;
+    const int lc = g.lcut ? g.lcut[t0 / gmd::TILE] : 2 * lmax;
+    // the tile's x, all loads in flight before the first use (element e:
+    // mode e >> 3, node e & 7; rows above K and nodes past N load node t0)
+    {
+        constexpr int NE = (KMAX * TT + 64 * TW - 1) / (64 * TW);
+        double2 v[NE];
+        static_for<0, NE>([&](auto ec) {
+            const int e = threadIdx.x + 64 * TW * ec(), j = e >> 3, i = t0 + (e & 7);
+            v[ec()] = a.X[(size_t)(j < K ? j : 0) * N + (i < N ? i : t0)];
+        });
+        static_for<0, NE>([&](auto ec) {
+            const int e = threadIdx.x + 64 * TW * ec(), j = e >> 3, n = e & 7;
+            if (j < K) {
+                const double2 w = t0 + n < N ? cmul(P->ph[mode_m(j) + lmax], v[ec()]) : make_double2(0.0, 0.0);
+                sh.x[j * 16 + 2 * n] = w.x;
+                sh.x[j * 16 + 2 * n + 1] = w.y;
+            }
+        });
+    }
+    const int ng = gmt::theta_rule(lmax, lc);
+    // Every register array is indexed by template constants only (static_for,
+    // one code path per wavefront); the point-group tables are shared in LDS.
+    static_assert(TW == 4, "four wavefronts per tile");
+    if (wv == 0) tapply_wave<LM, T, 0>(a, sh, lc, ng, lane, B.id);
+    else if (wv == 1) tapply_wave<LM, T, 1>(a, sh, lc, ng, lane, B.id);
+    else if (wv == 2) tapply_wave<LM, T, 2>(a, sh, lc, ng, lane, B.id);
+    else tapply_wave<LM, T, 3>(a, sh, lc, ng, lane, B.id);
+    __syncthreads();
+    // Epilogue over the tile's (mode, node), eight consecutive nodes of a mode
+    // per eight threads, in batches of EB whose loads are issued before their
+    // arithmetic and stores: C = e^(-i m theta) (V xt), then the stencil, CN
+    // form, sinks and the fused reductions.
+    const ApplyEpi ep = apply_epi(a);
+    double red[3] = {0.0, 0.0, 0.0};
+    constexpr int EB = GM_TEB;
+    for (int e0 = threadIdx.x; e0 < K * TT; e0 += 64 * TW * EB) {
+        ElemIn in[EB];
+        static_for<0, EB>([&](auto bc) {
+            const int e = e0 + 64 * TW * bc(), j = e >> 3, i = t0 + (e & 7);
+            in[bc()] = apply_element_load(a, e < K * TT ? j : 0, i < N ? i : t0);
+        });
+        static_for<0, EB>([&](auto bc) {
+            const int e = e0 + 64 * TW * bc(), j = e >> 3, n = e & 7, i = t0 + n;
+            if (e >= K * TT || i >= N) return;
+            const int l = mode_l(j), m = 2 * (j - l * (l + 1) / 2) - l;
+            const double2 Cv = make_double2(sh.x[j * 16 + 2 * n], sh.x[j * 16 + 2 * n + 1]);
+            apply_element(a, ep, j, l, i, cdotm(P->ph[m + lmax], Cv), in[bc()], red);
+        });
+    }
+    apply_reduce(a, red, B);
+}
+
+template <int LM, typename T>
+__global__ __launch_bounds__(64 * TW, GM_TWAVES) void k_tapply(ApplyArgs a)
+{
+    k_tapply_body<LM, T>(a, grid_blk());
+}
+// One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
+template <int LM, typename T>
+__global__ __launch_bounds__(64 * TW, GM_TWAVES) void k_tapply_batch(const ApplyArgs *__restrict__ args, const int *__restrict__ off,
+                                                     int M)
+{
+    const int m = batch_member(off, M);
+    const ApplyArgs a = args[m];
+    k_tapply_body<LM, T>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                              __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+}
+
+template <int LM, typename T>
+hipError_t launch_t(const ApplyArgs *a, const ApplyArgs *args, const int *off, int M, int total, hipStream_t st)
+{
+    if (args)
+        hipLaunchKernelGGL((k_tapply_batch<LM, T>), dim3(total), dim3(64 * TW), 0, st, args, off, M);
+    else
+        hipLaunchKernelGGL((k_tapply<LM, T>), dim3(tapply_grid(a->g.N)), dim3(64 * TW), 0, st, *a);
+    return hipGetLastError();
+}
+
+template <int LM>
+hipError_t launch_u(const gmd::Geometry &g, double *Ut, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_tapply_u<LM>), dim3((g.N + TT - 1) / TT), dim3(64), 0, st, g, Ut);
+    return hipGetLastError();
+}
+
+// Launch of the l_max class cls in arithmetic T (one instantiation unit per T).
+template <typename T>
+hipError_t launch_tt(int cls, const ApplyArgs *a, const ApplyArgs *args, const int *off, int M, int total,
+                     hipStream_t st)
+{
+    switch (cls) {
+    case 4: return launch_t<4, T>(a, args, off, M, total, st);
+    case 8: return launch_t<8, T>(a, args, off, M, total, st);
+    case 12: return launch_t<12, T>(a, args, off, M, total, st);
+    case 16: return launch_t<16, T>(a, args, off, M, total, st);
+    case 20: return launch_t<20, T>(a, args, off, M, total, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+extern template hipError_t launch_tt<double>(int, const ApplyArgs *, const ApplyArgs *, const int *, int, int,
+                                            hipStream_t);
+extern template hipError_t launch_tt<float>(int, const ApplyArgs *, const ApplyArgs *, const int *, int, int,
+                                           hipStream_t);
+
+}  // namespace gmk

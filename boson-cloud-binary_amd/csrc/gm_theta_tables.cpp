@@ -58,15 +58,17 @@ void assoc_legendre(int L, long double x, std::vector<long double> &out)
     }
 }
 
-ThetaTables build_theta_tables(int lmax)
+ThetaTables build_theta_tables(int lmax, int LM)
 {
     ThetaTables T;
     T.lmax = lmax;
+    T.LM = LM;
     const int L2 = 2 * lmax;
-    auto n_m = [&](int m) { return (lmax - std::abs(m)) / 2 + 1; };
-    auto n_l = [&](int d) { return (L2 - d) / 2 + 1; };
-    for (int m = -lmax; m <= lmax; ++m) T.ks += (n_m(m) + 3) / 4;
-    for (int d = 0; d <= L2; ++d) T.kd += (n_l(d) + 3) / 4;
+    // block layout of the class LM; entries of modes l > l_max (or l* > 2 l_max) are zero
+    auto n_m = [&](int m) { return (LM - std::abs(m)) / 2 + 1; };
+    auto n_l = [&](int d) { return (2 * LM - d) / 2 + 1; };
+    for (int m = -LM; m <= LM; ++m) T.ks += (n_m(m) + 3) / 4;
+    for (int d = 0; d <= 2 * LM; ++d) T.kd += (n_l(d) + 3) / 4;
     T.ngmax = theta_rule(lmax, L2);
     const size_t tri = static_cast<size_t>(T.ngmax) * (T.ngmax + 1) / 2;
     T.A.assign(tri * T.ks * 16, 0.0);
@@ -91,34 +93,31 @@ ThetaTables build_theta_tables(int lmax)
         const size_t baseU = static_cast<size_t>(T.kd) * 16 * ng * (ng - 1) / 2;
         for (int t = 0; t < ng; ++t) {
             int ks = 0;
-            for (int m = -lmax; m <= lmax; ++m) {
-                const int nm = n_m(m), ns = (nm + 3) / 4;
+            for (int m = -LM; m <= LM; ++m) {
+                const int ns = (n_m(m) + 3) / 4;
                 for (int s = 0; s < ns; ++s, ++ks) {
                     for (int a = 0; a < 4; ++a)
                         for (int b = 0; b < 4; ++b) {
                             const size_t at = baseA + (static_cast<size_t>(t) * T.ks + ks) * 16 + 4 * a + b;
                             // phase A: row b = point 4 t + b, k a = mode (m, 4 s + a)
-                            if (4 * s + a < nm) {
-                                const int l = std::abs(m) + 2 * (4 * s + a);
-                                T.A[at] = static_cast<double>(pf(P[4 * t + b], l, m));
-                            }
+                            const int la = std::abs(m) + 2 * (4 * s + a);
+                            if (la <= lmax) T.A[at] = static_cast<double>(pf(P[4 * t + b], la, m));
                             // phase C: row b = mode (m, 4 s + b), k a = point 4 t + a; the
                             // half rule's weight is twice the full rule's
-                            if (4 * s + b < nm) {
-                                const int l = std::abs(m) + 2 * (4 * s + b);
-                                T.C[at] = static_cast<double>(2.0L * w[4 * t + a] * pf(P[4 * t + a], l, m));
-                            }
+                            const int lb = std::abs(m) + 2 * (4 * s + b);
+                            if (lb <= lmax)
+                                T.C[at] = static_cast<double>(2.0L * w[4 * t + a] * pf(P[4 * t + a], lb, m));
                         }
                 }
             }
             int kd = 0;
-            for (int d = 0; d <= L2; ++d) {
-                const int nl = n_l(d), ns = (nl + 3) / 4;
+            for (int d = 0; d <= 2 * LM; ++d) {
+                const int ns = (n_l(d) + 3) / 4;
                 for (int s = 0; s < ns; ++s, ++kd)
                     for (int a = 0; a < 4; ++a)
                         for (int b = 0; b < 4; ++b) {
-                            if (4 * s + a >= nl) continue;
                             const int ls = d + 2 * (4 * s + a);
+                            if (ls > L2) continue;
                             T.U[baseU + (static_cast<size_t>(t) * T.kd + kd) * 16 + 4 * a + b] =
                                 static_cast<double>(inv2pi * pf(p0, ls, d) * pf(P[4 * t + b], ls, d));
                         }

@@ -78,7 +78,7 @@ typedef struct gm_flags {
                                          profiling (gm_profile_enable) runs without graphs */
     int32_t apply_tile;               /* operator-apply form: 0 = autotune (default); 16, 32, 64 = Gaunt-form
                                          tiles of that many nodes; 8 = the polar-quadrature form
-                                         (8-node tiles; l_max <= 24 in FP64, <= 32 mixed) */
+                                         (8-node tiles; l_max <= 20) */
 } gm_flags;
 
 /* Orbit state: the 13 doubles of the checkpoint (restart.cpp:153-163) in the

@@ -16,9 +16,9 @@ POLAR = 8
 
 
 @pytest.mark.parametrize("l_max,Nr", [(0, 65), (1, 65), (2, 129), (4, 129), (5, 97), (8, 129), (12, 193),
-                                      (17, 97), (20, 129), (24, 97)])
+                                      (17, 97), (20, 129)])
 def test_polar_apply_H_matches_oracle(l_max, Nr):
-    """H through the polar form at l_max in every class (4 .. 24, runtime l_max
+    """H through the polar form at l_max in every class (4 .. 20, runtime l_max
     below the class's as well), with absorbers, cut and untruncated."""
     announce(f"polar apply_H l_max={l_max} Nr={Nr}")
     objs = pair(l_max=l_max, absorption=True, Nr=Nr, L=float(8 * (Nr - 1)), R=40.0)
@@ -33,6 +33,14 @@ def test_polar_apply_H_matches_oracle(l_max, Nr):
     assert rel(g.apply_H(x), o.apply_H(x)) < 1e-13
     g.set_coupling_tolerance(0.0)
     assert rel(g.apply_H(x), o.apply_H(x)) < 1e-13
+
+
+def test_polar_form_needs_its_lmax_class():
+    """Above l_max 20 the polar form has no class: the flag is refused and the
+    autotuner times the Gaunt-form tiles only."""
+    g, _ = pair(l_max=21, Nr=65)
+    with pytest.raises(gm.InvalidArgument):
+        g.set_flags(apply_tile=POLAR)
 
 
 @pytest.mark.parametrize("l_max,Nr,tol", [(12, 1025, 1e-12), (20, 513, 1e-12), (12, 1025, 9e-7)])
