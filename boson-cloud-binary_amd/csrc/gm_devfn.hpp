@@ -217,6 +217,18 @@ struct ElemIn {
     double2 xi, xm, xp, cap, r0;
 };
 
+// The element's loads other than x (the CAP and R0), for callers holding x.
+__device__ __forceinline__ ElemIn apply_element_load_r0(const ApplyArgs &a, int j, int i)
+{
+    const gmd::Geometry &g = a.g;
+    const int MODE = a.mode;
+    ElemIn in;
+    in.cap = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
+    const bool uses_r0 = MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_RST;
+    in.r0 = uses_r0 ? a.R0[(size_t)j * g.N + i] : make_double2(0.0, 0.0);
+    return in;
+}
+
 __device__ __forceinline__ ElemIn apply_element_load(const ApplyArgs &a, int j, int i)
 {
     const gmd::Geometry &g = a.g;

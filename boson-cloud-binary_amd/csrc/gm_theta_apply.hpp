@@ -196,9 +196,15 @@ struct WSplit {
 template <int LM>
 struct TShared {
     static constexpr int KMAX = (LM + 1) * (LM + 2) / 2;
-    double x[KMAX * 16];           // phased x of the tile [mode][column], later its coupling
-    double X[(2 * LM + 1) * 64];   // X[m'] of the current point group, per lane
-    double A[kcount(LM) * 16];     // A operands of phase A, then of phase C
+    double x[KMAX * 16];               // x of the tile [mode][node][re, im]
+    double2 ph[2 * LM + 1];            // e^(i m theta), m = -LM .. LM (zero above l_max)
+    union {
+        struct {
+            double X[(2 * LM + 1) * 64];  // X[m'] of the current point group, per lane
+            double A[kcount(LM) * 16];    // A operands of phase A, then of phase C
+        };
+        double C[KMAX * 16];           // after the point groups: (V xt) [mode][node][re, im]
+    };
 };
 
 // The work of wavefront W: phase A for its m', phase B for its m (all X from
@@ -219,41 +225,67 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
     const int a4 = lane >> 4, b4 = lane & 3;
     const int la = 4 * a4 + b4;                  // index in a table's 4x4 block
     const int col = 4 * ((lane >> 2) & 3) + b4;  // column: node col >> 1, part col & 1
-    const int nd = col >> 1;
+    const int nd = col >> 1, part = col & 1;
     const int lmax = g.lmax;
     const size_t tri = (size_t)ng * (ng - 1) / 2;
     // U_d of this tile at (point 4 t + a4, node nd): Ut[((t (2 LM + 1) + d) 4 + a4) TT + nd]
     const double *ut = g.Ut + (size_t)tile * ut_stride<LM>(g.lmax) + a4 * TT + nd;
     double yacc[E1 - E0 > 0 ? E1 - E0 : 1];
     static_for<E0, E1>([&](auto e) { yacc[e() - E0] = 0.0; });
+    // Software pipeline: a table's global loads are issued a phase before its
+    // copy into LDS, and a diagonal group's U sums a group before their use.
+    constexpr int NT = (KL * 8 + 64 * TW - 1) / (64 * TW);
+    double2 rt[NT];
+    const auto load_table = [&](const double *base) {
+        static_for<0, NT>([&](auto k) {
+            const int e = threadIdx.x + 64 * TW * k();
+            rt[k()] = e < KL * 8 ? ((const double2 *)base)[e] : make_double2(0.0, 0.0);
+        });
+    };
+    const auto store_table = [&]() {
+        static_for<0, NT>([&](auto k) {
+            const int e = threadIdx.x + 64 * TW * k();
+            if (e < KL * 8) ((double2 *)sh.A)[e] = rt[k()];
+        });
+    };
+    T un[4];
+    const auto load_u = [&](int t, auto d0c) {
+        constexpr int d0 = d0c();
+        static_for<0, 4>([&](auto qc) {
+            constexpr int d = d0 + qc();
+            if constexpr (d <= 2 * LM) un[qc()] = T(ut[(t * (2 * LM + 1) + d) * 4 * TT]);
+            else un[qc()] = T(0);
+        });
+    };
+    load_table(g.TA + tri * (size_t)KL * 16);
     for (int t = 0; t < ng; ++t) {
         __syncthreads();
-        {
-            const double2 *ta = (const double2 *)(g.TA + (tri + t) * (size_t)KL * 16);
-            for (int e = threadIdx.x; e < KL * 8; e += 64 * TW) ((double2 *)sh.A)[e] = ta[e];
-        }
+        store_table();  // TA of point group t
+        load_u(t, std::integral_constant<int, 0>{});
+        load_table(g.TC + (tri + t) * (size_t)KL * 16);
         __syncthreads();
         // A: X[m'] at (point 4 t + a4, column col) for this wavefront's m'
         static_for<M0, M1>([&](auto mic) {
             constexpr int mi = mic(), m = mi - LM, am = m < 0 ? -m : m;
             constexpr int nb = lay.ko[mi + 1] - lay.ko[mi];
+            // the column's part of e^(i m theta) x: c x_re - s x_im or s x_re + c x_im
+            const double2 ph = sh.ph[mi];
+            const double pr = part ? ph.y : ph.x, pim = part ? ph.x : -ph.y;
             double av[nb], bv[nb];
             static_for<0, nb>([&](auto sc) {
                 constexpr int s = sc();
                 const int l = am + 2 * (4 * s + a4);
                 const int j = l <= lmax ? l * (l + 1) / 2 + (m + l) / 2 : 0;  // zero in the table above l_max
                 av[s] = sh.A[(lay.ko[mi] + s) * 16 + la];
-                bv[s] = sh.x[j * 16 + col];
+                bv[s] = pr * sh.x[j * 16 + 2 * nd] + pim * sh.x[j * 16 + 2 * nd + 1];
             });
             double acc = 0.0;
             static_for<0, nb>([&](auto sc) { acc = mf(av[sc()], bv[sc()], acc); });
             sh.X[mi * 64 + lane] = acc;
         });
         __syncthreads();
-        {
-            const double2 *tc = (const double2 *)(g.TC + (tri + t) * (size_t)KL * 16);
-            for (int e = threadIdx.x; e < KL * 8; e += 64 * TW) ((double2 *)sh.A)[e] = tc[e];
-        }
+        store_table();  // TC of point group t
+        if (t + 1 < ng) load_table(g.TA + (tri + t + 1) * (size_t)KL * 16);
         T X[NM];
         static_for<0, NM>([&](auto mic) { X[mic()] = T(sh.X[mic() * 64 + lane]); });
         // B: Y[m] = sum_d U_d (X[m + d] + X[m - d]) for this wavefront's m, the
@@ -264,11 +296,9 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
             constexpr int d0 = 4 * gc();
             if (d0 > lc) return false;
             T u[4];
-            static_for<0, 4>([&](auto qc) {
-                constexpr int d = d0 + qc();
-                u[qc()] = T(0);
-                if constexpr (d <= 2 * LM) u[qc()] = T(ut[(t * (2 * LM + 1) + d) * 4 * TT]);
-            });
+            static_for<0, 4>([&](auto qc) { u[qc()] = un[qc()]; });
+            if constexpr (d0 + 4 <= 2 * LM)
+                if (d0 + 4 <= lc) load_u(t, std::integral_constant<int, d0 + 4>{});
             static_for<0, 4>([&](auto qc) {
                 constexpr int d = d0 + qc();
                 if constexpr (d <= 2 * LM) {
@@ -299,13 +329,14 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
             });
         });
     }
-    // accumulator e: (V x) of mode (m_e, 4 s_e + a4) at column col, into the
-    // tile over the phased x (read for the last time before the barrier)
+    // accumulator e: (V xt) of mode (m_e, 4 s_e + a4) at column col, into
+    // the coupling tile over X and the tables (read for the last time before
+    // the barrier)
     __syncthreads();
     static_for<E0, E1>([&](auto ec) {
         constexpr int e = ec(), m = lay.me[e], am = m < 0 ? -m : m, s = lay.se[e];
         const int l = am + 2 * (4 * s + a4);
-        if (l <= lmax) sh.x[(l * (l + 1) / 2 + (m + l) / 2) * 16 + col] = yacc[e - E0];
+        if (l <= lmax) sh.C[(l * (l + 1) / 2 + (m + l) / 2) * 16 + col] = yacc[e - E0];
     });
 }
 
@@ -323,7 +354,13 @@ __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
     const int t0 = B.id * TT;
     const int lc = g.lcut ? g.lcut[t0 / gmd::TILE] : 2 * lmax;
     // the tile's x, all loads in flight before the first use (element e:
-    // mode e >> 3, node e & 7; rows above K and nodes past N load node t0)
+    // mode e >> 3, node e & 7; rows above K and nodes past N load node t0
+    // and store zeros)
+    static_assert(sizeof(sh.C) <= sizeof(sh.X) + sizeof(sh.A), "the coupling tile fits over X and A");
+    for (int e = threadIdx.x; e < 2 * LM + 1; e += 64 * TW) {
+        const int m = e - LM;
+        sh.ph[e] = (m >= -lmax && m <= lmax) ? P->ph[m + lmax] : make_double2(0.0, 0.0);
+    }
     {
         constexpr int NE = (KMAX * TT + 64 * TW - 1) / (64 * TW);
         double2 v[NE];
@@ -334,7 +371,7 @@ __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
         static_for<0, NE>([&](auto ec) {
             const int e = threadIdx.x + 64 * TW * ec(), j = e >> 3, n = e & 7;
             if (j < K) {
-                const double2 w = t0 + n < N ? cmul(P->ph[mode_m(j) + lmax], v[ec()]) : make_double2(0.0, 0.0);
+                const double2 w = t0 + n < N ? v[ec()] : make_double2(0.0, 0.0);
                 sh.x[j * 16 + 2 * n] = w.x;
                 sh.x[j * 16 + 2 * n + 1] = w.y;
             }
@@ -359,14 +396,22 @@ __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
     for (int e0 = threadIdx.x; e0 < K * TT; e0 += 64 * TW * EB) {
         ElemIn in[EB];
         static_for<0, EB>([&](auto bc) {
-            const int e = e0 + 64 * TW * bc(), j = e >> 3, i = t0 + (e & 7);
-            in[bc()] = apply_element_load(a, e < K * TT ? j : 0, i < N ? i : t0);
+            const int e = e0 + 64 * TW * bc(), j = e < K * TT ? e >> 3 : 0, n = e & 7;
+            const int i = t0 + n < N ? t0 + n : t0;
+            const double *xr = sh.x + j * 16;
+            // x and its interior neighbours from the tile, the neighbours
+            // across the tile's edges (n = 0, TT - 1) from memory
+            in[bc()] = apply_element_load_r0(a, j, i);
+            in[bc()].xi = make_double2(xr[2 * n], xr[2 * n + 1]);
+            in[bc()].xm = n > 0 ? make_double2(xr[2 * n - 2], xr[2 * n - 1]) : a.X[(size_t)j * N + (i > 0 ? i - 1 : 0)];
+            in[bc()].xp = n < TT - 1 ? make_double2(xr[2 * n + 2], xr[2 * n + 3])
+                                     : a.X[(size_t)j * N + (i < N - 1 ? i + 1 : N - 1)];
         });
         static_for<0, EB>([&](auto bc) {
             const int e = e0 + 64 * TW * bc(), j = e >> 3, n = e & 7, i = t0 + n;
             if (e >= K * TT || i >= N) return;
             const int l = mode_l(j), m = 2 * (j - l * (l + 1) / 2) - l;
-            const double2 Cv = make_double2(sh.x[j * 16 + 2 * n], sh.x[j * 16 + 2 * n + 1]);
+            const double2 Cv = make_double2(sh.C[j * 16 + 2 * n], sh.C[j * 16 + 2 * n + 1]);
             apply_element(a, ep, j, l, i, cdotm(P->ph[m + lmax], Cv), in[bc()], red);
         });
     }
