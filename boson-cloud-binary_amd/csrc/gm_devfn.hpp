@@ -193,6 +193,7 @@ struct ApplyEpi {
     double2 coef0, coef1;  // rank-one sink coefficients (dt/4) g_a (v_a^H x), or g_a (v_a^H psi) for AM_RHS
     double rhs_nrm;        // |rhs| (AM_RES, AM_RST)
     double alpha2, hdt;    // 1 / dr^2, dt / 2
+    double c0;             // alpha2 / mu: the stencil's part of H0_ii
     int nsink;
 };
 
@@ -205,6 +206,7 @@ __device__ __forceinline__ ApplyEpi apply_epi(const ApplyArgs &a)
     e.rhs_nrm = (a.mode == AM_RES || a.mode == AM_RST) ? a.Kr->rhs_nrm : 0.0;
     e.alpha2 = 1.0 / (a.g.dr * a.g.dr);
     e.hdt = a.P->half_dt;
+    e.c0 = e.alpha2 / a.g.mu;
     e.nsink = a.P->nsink;
     return e;
 }
@@ -244,13 +246,21 @@ __device__ __forceinline__ ElemIn apply_element_load(const ApplyArgs &a, int j, 
     return in;
 }
 
+// 1 / (2 mu r_i^2), the node's factor of the centrifugal term l (l + 1) / (2 mu r^2)
+__device__ __forceinline__ double centrifugal(const gmd::Geometry &g, int i)
+{
+    const double ri = g.eps + i * g.dr;
+    return i > 0 ? 1.0 / (2 * g.mu * ri * ri) : 0.0;
+}
+
 // One element (mode j = (l, m), node i) of the apply, given the coupling Cc
-// = (V x)_j at node i and the element's loads: the three-point stencil, the
-// centrifugal term and the CAP (hamiltonian.cpp:44-66,224-243), A = 1 + i dt/2
-// H (matrices.cpp:27-40) with the rank-one sinks (matrices.cpp:111-116), the
-// output of the mode and its terms of the fused reductions.
+// = (V x)_j at node i, the element's loads and the node's centrifugal factor
+// cr: the three-point stencil, the centrifugal term and the CAP
+// (hamiltonian.cpp:44-66,224-243), A = 1 + i dt/2 H (matrices.cpp:27-40)
+// with the rank-one sinks (matrices.cpp:111-116), the output of the mode and
+// its terms of the fused reductions.
 __device__ __forceinline__ void apply_element(const ApplyArgs &a, const ApplyEpi &ep, int j, int l, int i, double2 Cc,
-                                              const ElemIn &in, double (&red)[3])
+                                              const ElemIn &in, double cr, double (&red)[3])
 {
     const gmd::Geometry &g = a.g;
     const int N = g.N, MODE = a.mode;
@@ -259,8 +269,7 @@ __device__ __forceinline__ void apply_element(const ApplyArgs &a, const ApplyEpi
     const double2 xi = in.xi, capv = in.cap;
     double2 Hx;
     if (i > 0 && i < N - 1) {
-        const double ri = g.eps + i * g.dr;
-        const double h0d = -(-2.0 * ep.alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * g.mu);
+        const double h0d = ep.c0 + (double)(l * (l + 1)) * cr;
         Hx = cadd(cscale(g.off, cadd(in.xm, in.xp)), cadd(cmul(make_double2(h0d + capv.x, capv.y), xi), Cc));
     } else {
         Hx = cmul(capv, xi);

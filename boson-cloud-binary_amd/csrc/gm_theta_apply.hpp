@@ -392,6 +392,9 @@ __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
     // form, sinks and the fused reductions.
     const ApplyEpi ep = apply_epi(a);
     double red[3] = {0.0, 0.0, 0.0};
+    // every element of a thread lies at node t0 + (threadIdx.x & 7)
+    const int ie = t0 + (threadIdx.x & (TT - 1));
+    const double cr = ie < N ? centrifugal(g, ie) : 0.0;
     constexpr int EB = GM_TEB;
     for (int e0 = threadIdx.x; e0 < K * TT; e0 += 64 * TW * EB) {
         ElemIn in[EB];
@@ -410,9 +413,13 @@ __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
         static_for<0, EB>([&](auto bc) {
             const int e = e0 + 64 * TW * bc(), j = e >> 3, n = e & 7, i = t0 + n;
             if (e >= K * TT || i >= N) return;
-            const int l = mode_l(j), m = 2 * (j - l * (l + 1) / 2) - l;
+            // l from j = l (l + 1) / 2 + (m + l) / 2 (single-precision root, corrected)
+            int l = (int)((sqrtf(8.0f * j + 1.0f) - 1.0f) * 0.5f);
+            if ((l + 1) * (l + 2) / 2 <= j) ++l;
+            if (l * (l + 1) / 2 > j) --l;
+            const int m = 2 * (j - l * (l + 1) / 2) - l;
             const double2 Cv = make_double2(sh.C[j * 16 + 2 * n], sh.C[j * 16 + 2 * n + 1]);
-            apply_element(a, ep, j, l, i, cdotm(P->ph[m + lmax], Cv), in[bc()], red);
+            apply_element(a, ep, j, l, i, cdotm(sh.ph[m + LM], Cv), in[bc()], cr, red);
         });
     }
     apply_reduce(a, red, B);
