@@ -72,10 +72,24 @@ CPP_TESTS = os.path.join(REPO, "tests", "cpp")
 DRIVERS = (("cnm_driver.cpp", True), ("gm_main.cpp", True), ("config_probe.cpp", False), ("physics_ops.cpp", True))
 
 
+# host-only programs built from the library's own host sources (no GPU)
+HOST_DRIVERS = (("theta_tables.cpp", ("gm_theta_tables.cpp", "gm_math.cpp")),)
+
+
 def build_driver(verbose=True):
-    """The C++ test programs of tests/cpp, linked against libgm.so."""
+    """The C++ test programs of tests/cpp, linked against libgm.so, and the
+    host-only ones built from the library's host sources."""
     out_dir = os.path.join(CPP_TESTS, "_bin")
     os.makedirs(out_dir, exist_ok=True)
+    for src, deps in HOST_DRIVERS:
+        out = os.path.join(out_dir, os.path.splitext(src)[0])
+        cmd = [HIPCC, "--offload-host-only", "-O2", "-std=c++17", "-I", CSRC, os.path.join(CPP_TESTS, src)]
+        cmd += [os.path.join(CSRC, d) for d in deps] + ["-o", out]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), res.stdout + res.stderr))
+        if verbose:
+            print("built", out)
     for src, link in DRIVERS:
         out = os.path.join(out_dir, os.path.splitext(src)[0])
         cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"),
