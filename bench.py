@@ -315,17 +315,31 @@ def executed_flops(terms, pairs, K, N):
 
 def mfma_macs(kernel):
     """Matrix-core multiply-adds per launch from the committed SQ counter
-    summary (SQ_INSTS_MFMA x 1024 for 16x16x4 MFMAs; the first table of the
-    file), or None."""
+    summary (the first table of the file: SQ_INSTS_MFMA x 1024 for the Gaunt
+    form's 16x16x4 MFMAs, x 256 for the polar form's 4x4x4 ones), or None."""
     path = os.path.join(REPO, "profiles", "k_apply_sq_counters_l20_n16384.txt")
+    per = 256.0 if kernel.startswith("k_tapply<") else 1024.0
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith(kernel + " "):
-                    return float(line[len(kernel):].split()[2]) * 1024.0
+                    return float(line[len(kernel):].split()[2]) * per
     except (OSError, ValueError, IndexError):
         pass
     return None
+
+
+def apply_kernel(s, precision):
+    """The traced name of the operator apply the solver runs in `precision`:
+    the polar-quadrature form k_tapply<class, T> (8-node tiles) or the
+    Gaunt form k_apply<ng, T> (16 ng-node tiles), as the autotuner chose."""
+    tile = s.apply_config(precision)[0]
+    t = "float" if precision == gm.GM_PRECISION_MIXED else "double"
+    if tile == 8:
+        lmax = int(round((math.sqrt(8 * s.n_modes + 1) - 3) / 2))  # K = (l_max + 1)(l_max + 2) / 2
+        cls = max(4, (lmax + 3) // 4 * 4)
+        return "k_tapply<%d, %s>" % (cls, t), True
+    return "k_apply<%d, %s>" % (max(tile, 16) // 16, t), False
 
 
 def rooflines(s, prof, K, N, iters, mixed):
@@ -340,30 +354,41 @@ def rooflines(s, prof, K, N, iters, mixed):
     (about 5 us); they are not counted."""
     terms, pairs = s.coupling_work()
     krylov = min(prof["apply_krylov"][1], 2 * int(sum(iters)))
+    kernel, polar = apply_kernel(s, gm.GM_PRECISION_MIXED if mixed else gm.GM_PRECISION_FP64)
+    # the Gaunt form's FP32 apply runs on the FP32 matrix core; the polar
+    # form's transforms run on the FP64 matrix core in both precisions (its
+    # banded product is VALU work, FP32 and FP64 alike at 78.6 TFLOP/s)
     if mixed:
         ms, cnt = prof["apply_krylov"][0], krylov
-        peak, name, dt = FP32_PEAK_TFLOPS, "k_apply<fp32> (Krylov applies)", "f32"
+        peak = FP64_PEAK_TFLOPS if polar else FP32_PEAK_TFLOPS
+        name, dt = kernel + " (Krylov applies)", "f32"
     else:
         ms = prof["apply_krylov"][0] + prof["apply_other"][0]
         cnt = krylov + prof["apply_other"][1]
-        peak, name, dt = FP64_PEAK_TFLOPS, "k_apply", "f64"
+        peak, name, dt = FP64_PEAK_TFLOPS, kernel, "f64"
     avg_s = ms / max(cnt, 1) * 1e-3
     flops = algorithmic_flops(K, N)
-    executed = executed_flops(terms, pairs, K, N)
     roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": peak, "unit": "TFLOP/s",
             "kernel": name, "dtype": dt, "per_launch": flops, "per_launch_unit": "flop",
             "per_launch_formula": "4 K^2 N + 12 K N (real symmetric K x K coupling times a complex vector, "
                                   "plus stencil and phases)",
             "avg_launch_ms": avg_s * 1e3, "launches": cnt}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    # what the factored apply executes instead: Gaunt terms after the multipole
-    # cut, and the matrix-core work they are padded to
-    macs = mfma_macs("k_apply<2, float>" if mixed else "k_apply<2, double>")
-    roof["executed"] = {"flops_per_launch": executed, "gaunt_terms_per_launch": terms,
-                        "mode_pairs_per_launch": pairs, "achieved_TFLOPs": executed / avg_s / 1e12,
-                        "mfma_macs_per_launch": macs,
-                        "mfma_padding": (macs / terms) if macs else None}
-    roof["traffic"] = measured_traffic("k_apply_fp32" if mixed else "k_apply")
+    # what the apply executes instead: the Gaunt form pads the Gaunt terms
+    # kept by the multipole cut onto the matrix core; the polar form runs its
+    # transforms there (from the committed SQ counters)
+    macs = mfma_macs(kernel)
+    if polar:
+        roof["executed"] = {"form": "polar quadrature", "mfma_macs_per_launch": macs,
+                            "mfma_TFLOPs": (2 * macs / avg_s / 1e12) if macs else None,
+                            "gaunt_terms_per_launch": terms}
+    else:
+        executed = executed_flops(terms, pairs, K, N)
+        roof["executed"] = {"form": "Gaunt", "flops_per_launch": executed, "gaunt_terms_per_launch": terms,
+                            "mode_pairs_per_launch": pairs, "achieved_TFLOPs": executed / avg_s / 1e12,
+                            "mfma_macs_per_launch": macs,
+                            "mfma_padding": (macs / terms) if macs else None}
+    roof["traffic"] = measured_traffic(("k_tapply" if polar else "k_apply") + ("_fp32" if mixed else ""))
     # whole step: the sum over kernels of max(bytes / HBM peak, flops / MFMA
     # peak) against the measured time of the profiled steps
     nsteps = max(len(iters), 1)
@@ -371,7 +396,7 @@ def rooflines(s, prof, K, N, iters, mixed):
     vec = 16.0 * K * N
     apply_bytes = 3.0 * vec
     b64 = max(apply_bytes / (HBM_PEAK_GBS * 1e9), flops / (FP64_PEAK_TFLOPS * 1e12))
-    bk = max(apply_bytes / (HBM_PEAK_GBS * 1e9), flops / ((FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS) * 1e12))
+    bk = max(apply_bytes / (HBM_PEAK_GBS * 1e9), flops / (peak * 1e12))
     bound_s = 2 * nk * bk + 2 * nsteps * b64
     for kname, passes, launches in (("precond", HBM_PASSES["precond"], 2 * nk), ("update", HBM_PASSES["update"], nk),
                                     ("diag", HBM_PASSES["diag"], nsteps), ("init", 2.0, nsteps),
@@ -648,10 +673,11 @@ def summary_traffic(fetch_csv, write_csv, cfetch_csv, cwrite_csv):
         lines.append(f"{k:38s} {n:8d} {fk:14.1f} {wk:14.1f} {b:14.4g}")
     lines.append("")
     lines.append("working = launches that fetch more than 1 MiB (launches after convergence return at once)")
-    for key, dt in (("k_apply", "double"), ("k_apply_fp32", "float")):
-        k = _main_variant(rows, "k_apply", dt)
-        if k:
-            lines.append(f"traffic_bytes_per_launch {key} {rows[k][3]:.6g}")
+    for prefix in ("k_apply", "k_tapply"):
+        for key, dt in ((prefix, "double"), (prefix + "_fp32", "float")):
+            k = _main_variant(rows, prefix, dt)
+            if k:
+                lines.append(f"traffic_bytes_per_launch {key} {rows[k][3]:.6g}")
     return "\n".join(lines) + "\n"
 
 
@@ -691,7 +717,8 @@ def summary_sq(pass1_csv, pass2_csv, trace_csv):
                 continue
             mx = max(d[weight] for d in ds)
             ds = [d for d in ds if d[weight] >= 0.25 * mx]
-            if mx <= 0 or not (k.startswith("k_apply<") or k.startswith("k_precond<") or k in ("k_update", "k_diag_apply")):
+            if mx <= 0 or not (k.startswith("k_apply<") or k.startswith("k_tapply<") or k.startswith("k_precond<") or
+                               k in ("k_update", "k_diag_apply")):
                 continue
             w = working_us(t.get(k, []), 10.0)
             ms = _mean(w) / 1e3 if w else float("nan")
@@ -702,10 +729,12 @@ def summary_sq(pass1_csv, pass2_csv, trace_csv):
     notes = []
     simds, xcds = 1024, 8
     for k, v in sorted(vals.items()):
-        if not k.startswith("k_apply<") or "GRBM_GUI_ACTIVE" not in v or "SQ_INSTS_MFMA" not in v:
+        if not (k.startswith("k_apply<") or k.startswith("k_tapply<")) or "GRBM_GUI_ACTIVE" not in v or \
+                "SQ_INSTS_MFMA" not in v:
             continue
         cyc = v["GRBM_GUI_ACTIVE"] / xcds
-        per = 64 if k.endswith("double>") else 32
+        # issue cycles per MFMA: 16x16x4 f64 64, 16x16x4 f32 32, 4x4x4 f64 (polar form) 16
+        per = 16 if k.startswith("k_tapply<") else (64 if k.endswith("double>") else 32)
         notes.append(f"{k}: {cyc:.0f} cycles per XCD; matrix core busy {100 * v['SQ_INSTS_MFMA'] * per / (cyc * simds):.0f}% "
                      f"of each SIMD's cycles ({per} cycles per MFMA); VALU at 4 cycles per wave64 instruction "
                      f"{100 * v['SQ_INSTS_VALU'] * 4 / (cyc * simds):.0f}%; mean resident waves per SIMD "
