@@ -288,13 +288,14 @@ def cpu_baseline(w, runs=3, steps_per_run=8):
 
 # Algorithmic HBM bytes per mode-node of the bandwidth-bound kernel classes
 # (complex128 = 16 B per vector element; tables and per-node profiles excluded):
-#   precond  forward: read r, p, v, pivots, write p, y; backward: read pivots,
-#            y, write y (p-update variant 9 vectors; s-update variant 8: r, v,
-#            pivots / s, y / pivots, y / y) -> 8.5 on average
+#   precond  read r, p, v, pivots, write p, y (p-update variant, 6 vectors;
+#            s-update variant 5: r, v, pivots / s, z) -> 5.5 on average (the
+#            forward results stay on chip; the segment halos' extra reads are
+#            not algorithmic)
 #   update   read x, y, z, s, t, r0; write x, r -> 8
 #   diag     read psi, V x; per-tile sums only -> 2
 #   apply    read x (+ r0 / b), write y (+ A psi) -> 3 (coupling excluded)
-HBM_PASSES = {"precond": 8.5, "precond_setup": 3.0, "update": 8.0, "diag": 2.0}
+HBM_PASSES = {"precond": 5.5, "precond_setup": 3.0, "update": 8.0, "diag": 2.0}
 
 
 def algorithmic_flops(K, N):
@@ -717,7 +718,7 @@ def summary_sq(pass1_csv, pass2_csv, trace_csv):
                 continue
             mx = max(d[weight] for d in ds)
             ds = [d for d in ds if d[weight] >= 0.25 * mx]
-            if mx <= 0 or not (k.startswith("k_apply<") or k.startswith("k_tapply<") or k.startswith("k_precond<") or
+            if mx <= 0 or not (k.startswith("k_apply<") or k.startswith("k_tapply<") or k.startswith("k_psweep<") or
                                k in ("k_update", "k_diag_apply")):
                 continue
             w = working_us(t.get(k, []), 10.0)

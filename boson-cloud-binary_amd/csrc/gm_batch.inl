@@ -227,7 +227,7 @@ void build_plan(BatchLane &B)
             bool any = false;
             for (int m = 0; m < M; ++m)
                 if ((a[m].inv32 != nullptr && a[m].Y32 != nullptr) == (f32 != 0)) {
-                    b[m] = gmk::precond_blocks(a[m].K);
+                    b[m] = gmk::precond_blocks(a[m].K, a[m].N);
                     any = true;
                 }
             if (any) add_seg(B.iter, gmk::BK_PRECOND, a, b, 0, 0, var, f32);

@@ -34,6 +34,18 @@ constexpr int MAXLM = 40;  // largest supported l_max + 1
 constexpr int PT = GM_PT;  // threads of a preconditioner or factor workgroup
 constexpr int PE = GM_PE;  // consecutive nodes per preconditioner thread
 constexpr int RED_BLOCKS = 1024;  // max partial-sum slots per reduction
+#ifndef GM_PHALO
+#define GM_PHALO 192
+#endif
+// The p / s sweeps of the preconditioner split a mode's row into segments of
+// PSEG nodes, one workgroup each, solved over a window of one PT x PE tile:
+// the segment and PHALO nodes on either side, whose dropped carries decay
+// below rounding (k_factor checks it per mode).
+constexpr int PHALO = GM_PHALO;
+constexpr int PSEG = PT * PE - 2 * PHALO;
+static_assert(PSEG > PHALO, "preconditioner segments longer than their halos");
+// segments per mode: one (the whole row in one window) up to PT x PE nodes
+__host__ __device__ inline int precond_nseg(int N) { return N <= PT * PE ? 1 : (N + PSEG - 1) / PSEG; }
 
 constexpr int MR = 16;     // rows of an MFMA chunk
 
@@ -148,6 +160,7 @@ struct Krylov {
     int iter, maxit, done, restart, eigen_mode, restarts, zero_rhs, accepted;
     unsigned counter;
     int fresh;                  // no iteration yet since k_init: r is read from r0' / |rhs| and p = r
+    int pb;                     // p is held in the second p buffer (the p update writes the other one)
 };
 
 // Output of the diagnostics reduction.

@@ -7,10 +7,11 @@
 //               E and G tables (hamiltonian.cpp:44-66,176-243, matrices.cpp:27-40)
 //   k_factor    Thomas pivots of each mode's radial tridiagonal
 //               (include/RadialBlockPreconditioner.hpp:43-76)
-//   k_precond   forward/backward substitution as tile scans of affine maps,
-//               fused with the BiCGSTAB p / s updates and the Woodbury sink
-//               correction (include/RadialBlockPreconditioner.hpp:79-97,
-//               matrices.cpp:128-157)
+//   k_psweep    forward/backward substitution as block scans of affine maps
+//               over segments of each mode's row, fused with the BiCGSTAB
+//               p / s updates and the Woodbury sink correction
+//               (include/RadialBlockPreconditioner.hpp:79-97, matrices.cpp:128-157)
+//   k_wcol      the same recurrences for the Woodbury columns P^-1 v_a
 //   k_update    x += alpha y + w z, r = s - w t with fused dot products and the
 //               loop control of matrices.cpp:65-89 / Eigen BiCGSTAB
 //   k_sink      projections v_a^H x over the absorber supports (absorption.cpp:100-125)
@@ -792,6 +793,35 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
         }
     }
     if (failed) atomicOr(f.failed, 1);
+    if (!f.pseq) return;
+    // The segmented sweeps drop, at each end of a window, a carry that the
+    // recurrence then multiplies by a_i = -l inv_i over the halo: the forward
+    // carry into segment s + 1 by the product over its left halo, nodes
+    // [b - PHALO + 1, b], the backward carry into segment s by the product
+    // over its right halo, [b, b + PHALO - 1], b = (s + 1) PSEG. The mode is
+    // solved whole unless every such product is below 2^-56.
+    const int nseg = gmd::precond_nseg(N);
+    __shared__ int bad;
+    if (tid == 0) bad = 0;
+    __syncthreads();  // (also makes this block's pivots visible to its threads)
+    const double lell = 0.5 * log(cnorm2(ell)), lim = -56.0 * 0.6931471805599453;
+    for (int w = wv; w < 2 * (nseg - 1); w += T / 64) {
+        const int b = (w / 2 + 1) * gmd::PSEG;
+        const bool right = w & 1;
+        // a right halo that reaches node N - 1 drops nothing
+        if (right && b + gmd::PHALO >= N) continue;
+        const int lo = right ? b : b - gmd::PHALO + 1;
+        double sum = 0.0;
+        for (int i = lo + lane; i < lo + gmd::PHALO; i += 64) {
+            // a_i = 0 on the boundary rows (the product is then zero)
+            const double2 iv = f.inv[(size_t)j * N + i];
+            sum += (i >= 1 && i <= N - 2) ? lell + 0.5 * log(cnorm2(iv)) : -INFINITY;
+        }
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (lane == 0 && !(sum < lim)) atomicOr(&bad, 1);
+    }
+    __syncthreads();
+    if (tid == 0) f.pseq[j] = bad;
 }
 
 // ---------------------------------------------------------------- Thomas solve
@@ -880,52 +910,35 @@ __device__ __forceinline__ void store_phased(void *Xt, int f32, size_t q, double
     else ((double2 *)Xt)[q] = v;
 }
 
-template <int VAR, typename RT>
-__device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
+// Woodbury columns W_a = P^-1 v_a (matrices.cpp:133-145): block col K + j
+// solves mode j of the column of sink col by the Thomas recurrence
+// (RadialBlockPreconditioner.hpp:78-97), y_i = a_i y_(i-1) + inv_i v_i and
+// x_i = a_i x_(i+1) + y_i with a_i = -l inv_i (zero on the boundary rows), as
+// block scans of affine maps over the row's tiles of PT x PE nodes with exact
+// carries; the inputs of the next tile are loaded while the current one is
+// scanned. It also widens the column's non-zero node range. The sink columns,
+// read-only to every other kernel, are the output.
+__device__ __forceinline__ void k_wcol_body(const PrecArgs &a, Blk B)
 {
-    // RT = float: the two sweeps in FP32 on FP32 pivots (the forward result in
-    // an FP32 scratch vector); the p / s updates and the output stay FP64
-    typedef typename Arith<RT>::C C;
-    constexpr bool F32 = std::is_same<RT, float>::value;
-    Krylov *Kr = a.Kr;
-    if (VAR != PV_WCOL && Kr->done) return;
+    typedef double2 C;
     constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
     __shared__ C sa[TE + TE / E];
     __shared__ C sf[TE + TE / E];
-    // PV_WCOL: block col * K + j solves mode j of the column of sink col
-    const int col = VAR == PV_WCOL ? B.id / a.K : 0;
-    const int j = B.id - col * a.K;
-    // (the sink columns, read-only to every other kernel, are this variant's output)
-    double2 *const Y = VAR == PV_WCOL ? const_cast<double2 *>(a.sk[col].W) : a.Y;
-    int *const range = VAR == PV_WCOL ? a.range + 2 * col : nullptr;
-    const int N = a.N;
-    const int tid = threadIdx.x;
+    const int col = B.id / a.K, j = B.id - col * a.K;
+    const SinkDev &sk = a.sk[col];
+    double2 *const Y = const_cast<double2 *>(sk.W);
+    int *const range = a.range + 2 * col;
+    const int N = a.N, tid = threadIdx.x;
     const size_t base_j = (size_t)j * N;
-    const C ell = Arith<RT>::from(make_double2(0.0, a.Ps->half_dt * a.off));
-    auto pivot = [&](size_t q) -> C {
-        if constexpr (F32) return a.inv32[q];
-        else return a.inv[q];
-    };
+    const C ell = make_double2(0.0, a.Ps->half_dt * a.off);
     const int ntile = (N + TE - 1) / TE;
-    double2 beta = make_double2(0, 0), wv = make_double2(0, 0), alpha = make_double2(0, 0);
-    // first iteration: p = r + beta (p - w v) with p = v = 0 is r (a restart
-    // that resets the counter keeps p and v)
-    const bool first = VAR == PV_P && Kr->fresh;
-    // the first iteration reads r = r0' / |rhs| from the unnormalised shadow residual
-    const bool it0 = VAR != PV_WCOL && Kr->fresh;
-    const double nrm0 = it0 ? Kr->rhs_nrm : 1.0;
-    const double2 *Rin = it0 ? a.R0 : a.R;
-    if (VAR == PV_P) {
-        beta = Kr->beta;
-        wv = Kr->w;
-    }
-    if (VAR == PV_S) alpha = Kr->alpha;
-    // Both sweeps walk the row in tiles of TE nodes; the global inputs of the
-    // next tile are loaded (at clamped, always valid indices) while the
-    // current one is scanned, two tiles per loop turn so that the buffers
-    // need no copies.
+    auto coef = [&](const C iv) __attribute__((always_inline)) {
+        return C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)};
+    };
+    // forward: the sink vector v_col (zero outside its support) at clamped,
+    // always valid node indices
     struct FIn {
-        double2 r, p, v;
+        double2 v;
         C iv;
     };
     auto fload = [&](const int t, FIn (&b)[E]) __attribute__((always_inline)) {
@@ -933,45 +946,21 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int idx = min(tc * TE + tid + T * e, N - 1);
-            const size_t q = base_j + idx;
-            if (VAR == PV_WCOL) {
-                // the sink vector v_col, zero outside its support
-                const int d = idx - a.sk[col].lo;
-                b[e].r = d >= 0 && d < a.sk[col].n ? a.sk[col].v[(size_t)j * a.sk[col].n + d] : make_double2(0.0, 0.0);
-            } else {
-                b[e].r = Rin[q];
-                b[e].v = a.V[q];
-                if (VAR == PV_P) b[e].p = a.Pv[q];  // unused at the first iteration
-            }
-            b[e].iv = pivot(q);
+            const int d = idx - sk.lo;
+            b[e].v = d >= 0 && d < sk.n ? sk.v[(size_t)j * sk.n + d] : make_double2(0.0, 0.0);
+            b[e].iv = a.inv[base_j + idx];
         }
     };
-    // forward sweep
     C carry = czero<C>();
     auto fproc = [&](const int t, const FIn (&b)[E]) __attribute__((always_inline)) {
         const int t0 = t * TE;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e;
-            const int idx = t0 + pos;
+            const int pos = tid + T * e, idx = t0 + pos;
             C av = cone<C>(), fv = czero<C>();
             if (idx < N) {
-                const size_t q = base_j + idx;
-                double2 in;
-                const double2 rq = it0 ? make_double2(b[e].r.x / nrm0, b[e].r.y / nrm0) : b[e].r;
-                if (VAR == PV_P) {
-                    in = first ? rq : cadd(rq, cmul(beta, csub(b[e].p, cmul(wv, b[e].v))));
-                    a.Pv[q] = in;
-                } else if (VAR == PV_S) {
-                    in = csub(rq, cmul(alpha, b[e].v));
-                    a.Pv[q] = in;
-                } else {
-                    in = b[e].r;
-                }
-                const C iv = b[e].iv;
-                fv = cmul(iv, Arith<RT>::from(in));
-                av = (idx >= 1 && idx <= N - 2) ? C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)}
-                                                : czero<C>();
+                fv = cmul(b[e].iv, b[e].v);
+                av = (idx >= 1 && idx <= N - 2) ? coef(b[e].iv) : czero<C>();
             }
             sa[lds_slot(pos)] = av;
             sf[lds_slot(pos)] = fv;
@@ -1000,12 +989,8 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
         lds_barrier();
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e;
-            const int idx = t0 + pos;
-            if (idx < N) {
-                if constexpr (F32) a.Y32[base_j + idx] = sf[lds_slot(pos)];
-                else Y[base_j + idx] = sf[lds_slot(pos)];
-            }
+            const int pos = tid + T * e, idx = t0 + pos;
+            if (idx < N) Y[base_j + idx] = sf[lds_slot(pos)];
         }
         lds_barrier();
     };
@@ -1019,8 +1004,8 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
             if (t + 1 < ntile) fproc(t + 1, fb);
         }
     }
-    // backward sweep; the forward results written by the other threads of the
-    // block are read back after this barrier (with its global fence)
+    // backward; the forward results written by the other threads of the block
+    // are read back after this barrier (with its global fence)
     __syncthreads();
     struct BIn {
         C y, iv;
@@ -1030,33 +1015,21 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const size_t q = base_j + min(tc * TE + tid + T * e, N - 1);
-            if constexpr (F32) b[e].y = a.Y32[q];
-            else b[e].y = Y[q];
-            b[e].iv = pivot(q);
+            b[e].y = Y[q];
+            b[e].iv = a.inv[q];
         }
     };
     int nz_lo = N, nz_hi = 0;
     carry = czero<C>();
-    const double2 phj = a.Xt ? a.Ps->ph[mode_m(j) + a.lmax] : make_double2(1.0, 0.0);
-    // Woodbury epilogue: the last block reads y on the sink supports, so those
-    // values are stored at agent scope (written through to the device
-    // coherence point) instead of fencing the whole of each XCD's L2
-    const bool wood_out = VAR != PV_WCOL && a.wood;
-    const int ns_w = wood_out ? a.Ps->nsink : 0;
-    const int s0lo = ns_w > 0 ? a.sk[0].lo : 0, s0hi = ns_w > 0 ? a.sk[0].lo + a.sk[0].n : 0;
-    const int s1lo = ns_w > 1 ? a.sk[1].lo : 0, s1hi = ns_w > 1 ? a.sk[1].lo + a.sk[1].n : 0;
     auto bproc = [&](const int t, const BIn (&b)[E]) __attribute__((always_inline)) {
         const int t0 = t * TE;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e;
-            const int idx = t0 + pos;
+            const int pos = tid + T * e, idx = t0 + pos;
             C cv = cone<C>(), fv = czero<C>();
             if (idx < N) {
-                const C iv = b[e].iv;
                 fv = b[e].y;
-                cv = (idx >= 1 && idx <= N - 2) ? C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)}
-                                                : czero<C>();
+                cv = (idx >= 1 && idx <= N - 2) ? coef(b[e].iv) : czero<C>();
             }
             sa[lds_slot(pos)] = cv;
             sf[lds_slot(pos)] = fv;
@@ -1085,19 +1058,11 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
         lds_barrier();
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e;
-            const int idx = t0 + pos;
+            const int pos = tid + T * e, idx = t0 + pos;
             if (idx < N) {
-                const double2 yv = Arith<RT>::to_d(sf[lds_slot(pos)]);
-                if ((idx >= s0lo && idx < s0hi) || (idx >= s1lo && idx < s1hi)) {
-                    double *yp = reinterpret_cast<double *>(Y + base_j + idx);
-                    __hip_atomic_store(yp, yv.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(yp + 1, yv.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                } else {
-                    Y[base_j + idx] = yv;
-                }
-                if (a.Xt) store_phased(a.Xt, a.xt_f32, base_j + idx, phj, yv);
-                if (VAR == PV_WCOL && (yv.x != 0.0 || yv.y != 0.0)) {
+                const double2 yv = sf[lds_slot(pos)];
+                Y[base_j + idx] = yv;
+                if (yv.x != 0.0 || yv.y != 0.0) {
                     nz_lo = min(nz_lo, idx);
                     nz_hi = max(nz_hi, idx + 1);
                 }
@@ -1115,122 +1080,331 @@ __device__ __forceinline__ void k_precond_body(const PrecArgs &a, Blk B)
             if (t - 1 >= 0) bproc(t - 1, bb);
         }
     }
-    if (VAR == PV_WCOL) {
-        // exact non-zero node range of the solution over all modes
-        for (int o = 32; o > 0; o >>= 1) {
-            nz_lo = min(nz_lo, __shfl_xor(nz_lo, o));
-            nz_hi = max(nz_hi, __shfl_xor(nz_hi, o));
+    // exact non-zero node range of the solution over all modes
+    for (int o = 32; o > 0; o >>= 1) {
+        nz_lo = min(nz_lo, __shfl_xor(nz_lo, o));
+        nz_hi = max(nz_hi, __shfl_xor(nz_hi, o));
+    }
+    if ((tid & 63) == 0) {
+        atomicMin(range, nz_lo);
+        atomicMax(range + 1, nz_hi);
+    }
+}
+__global__ __launch_bounds__(gmd::PT) void k_wcol(PrecArgs a) { k_wcol_body(a, grid_blk()); }
+
+// The p / s sweeps of the preconditioner: p = r + beta (p - w v) (PV_P) or
+// s = r - alpha v (PV_S), then y = P^-1 p or z = P^-1 s by the Thomas
+// recurrence (RadialBlockPreconditioner.hpp:78-97) as two first-order affine
+// recurrences, y_i = a_i y_(i-1) + inv_i in_i and x_i = a_i x_(i+1) + y_i with
+// a_i = -l inv_i (zero on the boundary rows).
+//
+// Block j * nseg + g solves segment g of mode j: the window [w0, w1) of at
+// most one PT x PE tile, the segment [g0, g1) and up to PHALO nodes on either
+// side, is loaded once (each thread PE consecutive nodes), both recurrences
+// run as block scans with the carries into the window's ends dropped, and the
+// segment's nodes are stored. A dropped carry is multiplied over a halo by
+// the product of |a_i|, which k_factor checks to stay below 2^-56 (then the
+// segment's output is the whole-row solve's to rounding). A mode that fails
+// the check is solved whole by its segment-0 block: the same recurrences over
+// the row's tiles in order, with exact carries, the forward result kept in
+// the output vector. RT = float sweeps in FP32 on the FP32 pivots; the p / s
+// update and the output stay FP64.
+template <int VAR, typename RT>
+__device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
+{
+    static_assert(VAR == PV_P || VAR == PV_S, "p or s sweep");
+    typedef typename Arith<RT>::C C;
+    constexpr bool F32 = std::is_same<RT, float>::value;
+    constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
+    Krylov *Kr = a.Kr;
+    if (Kr->done) return;
+    const int nseg = a.nseg;
+    const int j = B.id / nseg, g = B.id - j * nseg;
+    const bool whole = nseg > 1 && a.pseq[j];
+    const int N = a.N, tid = threadIdx.x;
+    const size_t base_j = (size_t)j * N;
+    double2 *const Y = a.Y;
+    const C ell = Arith<RT>::from(make_double2(0.0, a.Ps->half_dt * a.off));
+    // the first iteration reads r = r0' / |rhs| from the unnormalised shadow
+    // residual and takes p = r
+    const bool it0 = Kr->fresh != 0;
+    const bool first = VAR == PV_P && it0;
+    const double nrm0 = it0 ? Kr->rhs_nrm : 1.0;
+    const double2 *Rin = it0 ? a.R0 : a.R;
+    double2 beta = make_double2(0, 0), wv = make_double2(0, 0), alpha = make_double2(0, 0);
+    if (VAR == PV_P) {
+        beta = Kr->beta;
+        wv = Kr->w;
+    }
+    if (VAR == PV_S) alpha = Kr->alpha;
+    // PV_P reads p from one buffer and writes the new p into the other (a
+    // window's halo reads p that the neighbouring segment updates)
+    const double2 *Pin = a.Pv;
+    double2 *Pout = a.Pv;
+    if (VAR == PV_P) {
+        const bool second = Kr->pb != 0;
+        Pin = second ? a.Pv2 : a.Pv;
+        Pout = second ? a.Pv : a.Pv2;
+    }
+    const double2 phj = a.Xt ? a.Ps->ph[mode_m(j) + a.lmax] : make_double2(1.0, 0.0);
+    // Woodbury (a.wood): the thread's share of the projections v_a^H y over
+    // the sink supports
+    const int ns_w = a.wood ? a.Ps->nsink : 0;
+    const int s0lo = ns_w > 0 ? a.sk[0].lo : 0, s0hi = ns_w > 0 ? a.sk[0].lo + a.sk[0].n : 0;
+    const int s1lo = ns_w > 1 ? a.sk[1].lo : 0, s1hi = ns_w > 1 ? a.sk[1].lo + a.sk[1].n : 0;
+    double wp[4] = {0.0, 0.0, 0.0, 0.0};
+    auto store_out = [&](const int idx, const double2 yv) __attribute__((always_inline)) {
+        Y[base_j + idx] = yv;
+        if (a.Xt) store_phased(a.Xt, a.xt_f32, base_j + idx, phj, yv);
+        if (idx >= s0lo && idx < s0hi) {
+            const double2 d = cdotm(a.sk[0].v[(size_t)j * a.sk[0].n + (idx - s0lo)], yv);
+            wp[0] += d.x;
+            wp[1] += d.y;
         }
-        if ((tid & 63) == 0) {
-            atomicMin(range, nz_lo);
-            atomicMax(range + 1, nz_hi);
+        if (idx >= s1lo && idx < s1hi) {
+            const double2 d = cdotm(a.sk[1].v[(size_t)j * a.sk[1].n + (idx - s1lo)], yv);
+            wp[2] += d.x;
+            wp[3] += d.y;
+        }
+    };
+    auto coef = [&](const C iv) __attribute__((always_inline)) {
+        return C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)};
+    };
+    // Forward over [w0, w1) (positions tid E + e): in_i, its store on [p0, p1),
+    // a_i and f_i = inv_i in_i; a_w0 = 0 when drop (the carry into the window
+    // is dropped), and the identity past w1. Returns y in fe, a in ae.
+    auto forward = [&](const int w0, const int w1, const int p0, const int p1, const bool drop, C &carry, C (&ae)[E],
+                       C (&fe)[E]) __attribute__((always_inline)) {
+        double2 r[E], v[E], p[E];
+        C iv[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const size_t q = base_j + min(w0 + tid * E + e, w1 - 1);
+            r[e] = Rin[q];
+            v[e] = a.V[q];
+            if (VAR == PV_P && !first) p[e] = Pin[q];
+            if constexpr (F32) iv[e] = a.inv32[q];
+            else iv[e] = a.inv[q];
+        }
+        Aff<C> loc;
+        loc.A = cone<C>();
+        loc.F = czero<C>();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int idx = w0 + tid * E + e;
+            ae[e] = cone<C>();
+            fe[e] = czero<C>();
+            if (idx < w1) {
+                const double2 rq = it0 ? make_double2(r[e].x / nrm0, r[e].y / nrm0) : r[e];
+                double2 in;
+                if (VAR == PV_P) in = first ? rq : cadd(rq, cmul(beta, csub(p[e], cmul(wv, v[e]))));
+                else in = csub(rq, cmul(alpha, v[e]));
+                if (idx >= p0 && idx < p1) Pout[base_j + idx] = in;
+                fe[e] = cmul(iv[e], Arith<RT>::from(in));
+                ae[e] = (idx >= 1 && idx <= N - 2 && !(drop && idx == w0)) ? coef(iv[e]) : czero<C>();
+            }
+            loc.A = cmul(ae[e], loc.A);
+            loc.F = cadd(cmul(ae[e], loc.F), fe[e]);
+        }
+        Aff<C> tot;
+        const Aff<C> ex = block_scan<false>(loc, tot);
+        C y = cadd(cmul(ex.A, carry), ex.F);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            y = cadd(cmul(ae[e], y), fe[e]);
+            fe[e] = y;
+        }
+        carry = cadd(cmul(tot.A, carry), tot.F);
+    };
+    // Backward over [w0, w1) from y in fe and a in ae: x_i = a_i x_(i+1) + y_i,
+    // a_(w1 - 1) = 0 when drop, the identity past w1. Returns x in fe.
+    auto backward = [&](const int w0, const int w1, const bool drop, C &carry, C (&ae)[E], C (&fe)[E])
+                        __attribute__((always_inline)) {
+        Aff<C> loc;
+        loc.A = cone<C>();
+        loc.F = czero<C>();
+#pragma unroll
+        for (int e = E - 1; e >= 0; --e) {
+            const int idx = w0 + tid * E + e;
+            if (idx >= w1) {
+                ae[e] = cone<C>();
+                fe[e] = czero<C>();
+            } else if (drop && idx == w1 - 1) {
+                ae[e] = czero<C>();
+            }
+            loc.A = cmul(ae[e], loc.A);
+            loc.F = cadd(cmul(ae[e], loc.F), fe[e]);
+        }
+        Aff<C> tot;
+        const Aff<C> ex = block_scan<true>(loc, tot);
+        C x = cadd(cmul(ex.A, carry), ex.F);
+#pragma unroll
+        for (int e = E - 1; e >= 0; --e) {
+            x = cadd(cmul(ae[e], x), fe[e]);
+            fe[e] = x;
+        }
+        carry = cadd(cmul(tot.A, carry), tot.F);
+    };
+    C ae[E], fe[E];
+    if (!whole) {
+        // The window's r, v, p and pivots are loaded coalesced (position
+        // tid + T e), f = inv in goes through LDS to the thread's consecutive
+        // positions, where the thread loads its pivots once more for a; the
+        // output returns through LDS to coalesced stores.
+        __shared__ C sf[TE + TE / E];
+        const int g0 = g * gmd::PSEG, g1 = nseg == 1 ? N : min(N, g0 + gmd::PSEG);
+        const int w0 = max(0, g0 - gmd::PHALO), w1 = nseg == 1 ? N : min(N, g1 + gmd::PHALO);
+        {
+            double2 r[E], v[E], p[E];
+            C iv[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const size_t q = base_j + min(w0 + tid + T * e, w1 - 1);
+                r[e] = Rin[q];
+                v[e] = a.V[q];
+                if (VAR == PV_P && !first) p[e] = Pin[q];
+                if constexpr (F32) iv[e] = a.inv32[q];
+                else iv[e] = a.inv[q];
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int pos = tid + T * e, idx = w0 + pos;
+                C fv = czero<C>();
+                if (idx < w1) {
+                    const double2 rq = it0 ? make_double2(r[e].x / nrm0, r[e].y / nrm0) : r[e];
+                    double2 in;
+                    if (VAR == PV_P) in = first ? rq : cadd(rq, cmul(beta, csub(p[e], cmul(wv, v[e]))));
+                    else in = csub(rq, cmul(alpha, v[e]));
+                    if (idx >= g0 && idx < g1) Pout[base_j + idx] = in;
+                    fv = cmul(iv[e], Arith<RT>::from(in));
+                }
+                sf[lds_slot(pos)] = fv;
+            }
+        }
+        C iv[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const size_t q = base_j + min(w0 + tid * E + e, w1 - 1);
+            if constexpr (F32) iv[e] = a.inv32[q];
+            else iv[e] = a.inv[q];
+        }
+        lds_barrier();
+        // forward, a_w0 = 0 (the carry into the window is dropped), the
+        // identity past w1
+        Aff<C> loc;
+        loc.A = cone<C>();
+        loc.F = czero<C>();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int idx = w0 + tid * E + e;
+            fe[e] = sf[lds_slot(tid * E + e)];
+            ae[e] = idx >= w1 ? cone<C>() : (idx >= 1 && idx <= N - 2 && idx != w0) ? coef(iv[e]) : czero<C>();
+            loc.A = cmul(ae[e], loc.A);
+            loc.F = cadd(cmul(ae[e], loc.F), fe[e]);
+        }
+        Aff<C> tot;
+        C y = block_scan<false>(loc, tot).F;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            y = cadd(cmul(ae[e], y), fe[e]);
+            fe[e] = y;
+        }
+        C carry = czero<C>();
+        backward(w0, w1, true, carry, ae, fe);
+        // (every thread read its slots before the scans' barriers)
+#pragma unroll
+        for (int e = 0; e < E; ++e) sf[lds_slot(tid * E + e)] = fe[e];
+        lds_barrier();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int idx = w0 + tid + T * e;
+            if (idx >= g0 && idx < g1) store_out(idx, Arith<RT>::to_d(sf[lds_slot(tid + T * e)]));
+        }
+    } else if (g == 0) {
+        // the whole row, tile by tile with exact carries; the forward result
+        // goes to the output vector and is read back by the backward pass
+        const int ntile = (N + TE - 1) / TE;
+        C carry = czero<C>();
+        for (int t = 0; t < ntile; ++t) {
+            const int w0 = t * TE, w1 = min(N, w0 + TE);
+            forward(w0, w1, 0, N, false, carry, ae, fe);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int idx = w0 + tid * E + e;
+                if (idx < w1) Y[base_j + idx] = Arith<RT>::to_d(fe[e]);
+            }
+        }
+        __syncthreads();  // (with its global fence: the block's forward results are visible)
+        carry = czero<C>();
+        for (int t = ntile - 1; t >= 0; --t) {
+            const int w0 = t * TE, w1 = min(N, w0 + TE);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int idx = w0 + tid * E + e;
+                const size_t q = base_j + min(idx, w1 - 1);
+                C iv;
+                if constexpr (F32) iv = a.inv32[q];
+                else iv = a.inv[q];
+                fe[e] = Arith<RT>::from(Y[q]);
+                ae[e] = (idx >= 1 && idx <= N - 2) ? coef(iv) : czero<C>();
+            }
+            backward(w0, w1, false, carry, ae, fe);
+            // (every thread of the block read its forward values before the scan's barriers)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int idx = w0 + tid * E + e;
+                if (idx < w1) store_out(idx, Arith<RT>::to_d(fe[e]));
+            }
         }
     }
-    if (VAR == PV_WCOL || !a.wood) return;
-    // Woodbury epilogue in the last block (matrices.cpp:146-153):
-    // c = Ginv (V^H y), then the rank-one coefficients of A applied to
-    // y - W c: coefA_a = (dt/4) g_a v_a^H (y - W c).
-    __shared__ bool last;
-    // every thread's support stores have completed before the block is
-    // counted (the same ordering argument as block_partials: agent-scope
-    // stores are at the coherence point once vmcnt is zero; the signal fences
-    // stop the compiler from moving them across the wait)
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __syncthreads();
-    if (tid == 0) last = __hip_atomic_fetch_add(&Kr->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == B.n - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const StepParams *P = a.Ps;
-    const int ns = P->nsink;
-    // (loops over the at most two sinks have constant bounds, so the small
-    // arrays stay in registers)
-    double v[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        if (s >= ns) break;
-        const SinkDev &sk = a.sk[s];
-        const int tot = a.K * sk.n;
-        for (int q = tid; q < tot; q += T) {
-            const int k = q / sk.n, d = q - k * sk.n;
-            const double2 d2 = cdotm(sk.v[q], vload2(Y + (size_t)k * N + sk.lo + d));
-            v[2 * s] += d2.x;
-            v[2 * s + 1] += d2.y;
-        }
+    if (!a.wood) return;
+    // Woodbury epilogue (matrices.cpp:146-153): c = Ginv (V^H y) and the
+    // rank-one coefficients of A applied to the corrected y - W c,
+    // (dt/4) g_a v_a^H (y - W c). With G = D + V^H W, D = diag(1/((dt/4) g)),
+    // V^H (y - W c) = V^H y - (G - D) Ginv V^H y = D c, so the coefficients
+    // are c itself. The blocks that hold sink-support nodes (and segment 0 of
+    // every mode, which stores a row solved whole) store partial projections;
+    // the last of them sums the partials in a fixed order.
+    auto holds = [&](int gg) {
+        const int lo = gg * gmd::PSEG, hi = nseg == 1 ? N : min(N, lo + gmd::PSEG);
+        return gg == 0 || (lo < s0hi && s0lo < hi) || (lo < s1hi && s1lo < hi);
+    };
+    int count = 0, rank = 0;
+    for (int gg = 0; gg < nseg; ++gg) {
+        const bool h = holds(gg);
+        count += h;
+        rank += h && gg < g;
     }
-    __shared__ double red[4];
-    __shared__ double ws[gmd::PT / 64][4];
-    for (int q = 0; q < 4; ++q) {
-        const double s = wave_sum(v[q]);
-        if ((tid & 63) == 0) ws[tid >> 6][q] = s;
-    }
-    __syncthreads();
-    if (tid == 0)
-        for (int q = 0; q < 4; ++q) {
-            double s = 0;
-            for (int w = 0; w < T / 64; ++w) s += ws[w][q];
-            red[q] = s;
-        }
-    __syncthreads();
-    double2 proj[2] = {make_double2(red[0], red[1]), make_double2(red[2], red[3])};
-    double2 c[2] = {make_double2(0, 0), make_double2(0, 0)};
+    if (!holds(g)) return;
+    const Blk P{j * count + rank, a.K * count};
+    if (!block_partials<4>(wp, a.wpart, &Kr->counter, P)) return;
+    double pr[4];
+    for (int q = 0; q < 4; ++q) pr[q] = sum_partials(a.wpart, q, P);
+    if (tid == 0) {
+        const double2 proj[2] = {make_double2(pr[0], pr[1]), make_double2(pr[2], pr[3])};
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (s < ns && u < ns) c[s] = cadd(c[s], cmul(Kr->Ginv[s * 2 + u], proj[u]));
-    __syncthreads();
-    for (int q = 0; q < 4; ++q) v[q] = 0.0;
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        if (s >= ns) break;
-        const SinkDev &sk = a.sk[s];
-        const int tot = a.K * sk.n;
-        for (int q = tid; q < tot; q += T) {
-            const int k = q / sk.n, d = q - k * sk.n;
-            const size_t gi = (size_t)k * N + sk.lo + d;
-            double2 y = vload2(Y + gi);
-            const double2 rot = P->wrot[mode_m(k) + a.lmax];
+        for (int s = 0; s < 2; ++s) {
+            if (s >= ns_w) break;
+            double2 c = make_double2(0.0, 0.0);
 #pragma unroll
             for (int u = 0; u < 2; ++u)
-                if (u < ns) y = csub(y, cmul(cmul(c[u], rot), a.sk[u].W[gi]));
-            const double2 d2 = cdotm(sk.v[q], y);
-            v[2 * s] += d2.x;
-            v[2 * s + 1] += d2.y;
+                if (u < ns_w) c = cadd(c, cmul(Kr->Ginv[s * 2 + u], proj[u]));
+            Kr->c[s] = c;
+            Kr->coefA[s] = c;
         }
-    }
-    for (int q = 0; q < 4; ++q) {
-        const double s = wave_sum(v[q]);
-        if ((tid & 63) == 0) ws[tid >> 6][q] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        for (int q = 0; q < 4; ++q) {
-            double s = 0;
-            for (int w = 0; w < T / 64; ++w) s += ws[w][q];
-            red[q] = s;
-        }
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-            if (s < ns) {
-                Kr->c[s] = c[s];
-                Kr->coefA[s] = cscale(0.5 * P->half_dt * P->g[s], make_double2(red[2 * s], red[2 * s + 1]));
-            }
         Kr->counter = 0;
     }
 }
 template <int VAR, typename RT>
-__global__ __launch_bounds__(gmd::PT) void k_precond(PrecArgs a) { k_precond_body<VAR, RT>(a, grid_blk()); }
+__global__ __launch_bounds__(gmd::PT) void k_psweep(PrecArgs a) { k_psweep_body<VAR, RT>(a, grid_blk()); }
 // One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
 template <int VAR, typename RT>
-__global__ __launch_bounds__(gmd::PT) void k_precond_batch(const PrecArgs *__restrict__ args, const int *__restrict__ off, int M)
+__global__ __launch_bounds__(gmd::PT) void k_psweep_batch(const PrecArgs *__restrict__ args, const int *__restrict__ off, int M)
 {
     const int m = batch_member(off, M);
     const PrecArgs a = args[m];
-    k_precond_body<VAR, RT>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
-                             __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
+    k_psweep_body<VAR, RT>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+                                  __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
 // ---------------------------------------------------------------- BiCGSTAB vector update
@@ -1276,6 +1450,7 @@ __device__ __forceinline__ void k_update_body(const UpdateArgs &a, Blk B)
         if (threadIdx.x == 0) {
             Kr->iter += 1;
             Kr->fresh = 0;
+            Kr->pb ^= 1;  // the p update of this iteration wrote the other buffer
             loop_top(Kr, r2, make_double2(rr, ri));
             Kr->counter = 0;
         }
@@ -1320,6 +1495,7 @@ __device__ __forceinline__ void k_init_body(const InitArgs &a, Blk B)
         Kr->restarts = 0;
         Kr->restart = 0;
         Kr->fresh = 1;
+        Kr->pb = 0;
         Kr->rst_eps2 = a.rst_eps2;
         Kr->rst_ref = a.rst_ref;
         Kr->accepted = 0;
@@ -1854,11 +2030,13 @@ hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st)
 {
     const bool f32 = a.inv32 != nullptr && a.Y32 != nullptr && var != PV_WCOL;
-    if (var == PV_P && f32) hipLaunchKernelGGL((k_precond<PV_P, float>), dim3(a.K), dim3(gmd::PT), 0, st, a);
-    else if (var == PV_S && f32) hipLaunchKernelGGL((k_precond<PV_S, float>), dim3(a.K), dim3(gmd::PT), 0, st, a);
-    else if (var == PV_P) hipLaunchKernelGGL((k_precond<PV_P, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
-    else if (var == PV_S) hipLaunchKernelGGL((k_precond<PV_S, double>), dim3(a.K), dim3(gmd::PT), 0, st, a);
-    else hipLaunchKernelGGL((k_precond<PV_WCOL, double>), dim3(a.K * a.ncol), dim3(gmd::PT), 0, st, a);
+    if ((var == PV_P || var == PV_S) && a.nseg != gmd::precond_nseg(a.N)) return hipErrorInvalidValue;
+    const dim3 g(precond_blocks(a.K, a.N));
+    if (var == PV_P && f32) hipLaunchKernelGGL((k_psweep<PV_P, float>), g, dim3(gmd::PT), 0, st, a);
+    else if (var == PV_S && f32) hipLaunchKernelGGL((k_psweep<PV_S, float>), g, dim3(gmd::PT), 0, st, a);
+    else if (var == PV_P) hipLaunchKernelGGL((k_psweep<PV_P, double>), g, dim3(gmd::PT), 0, st, a);
+    else if (var == PV_S) hipLaunchKernelGGL((k_psweep<PV_S, double>), g, dim3(gmd::PT), 0, st, a);
+    else hipLaunchKernelGGL(k_wcol, dim3(a.K * a.ncol), dim3(gmd::PT), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1869,7 +2047,7 @@ static int stream_grid(size_t n)
     return (int)g;
 }
 
-int precond_blocks(int K) { return K; }
+int precond_blocks(int K, int N) { return K * gmd::precond_nseg(N); }
 int stream_blocks(size_t n) { return stream_grid(n); }
 int phase_blocks(int N, int K) { return ((N + 255) / 256 < 16 ? (N + 255) / 256 : 16) * K; }
 int diag_apply_blocks(int N, int K) { return (K * ((N + gmd::DIAG_NODES - 1) / gmd::DIAG_NODES) + 3) / 4; }
@@ -1919,10 +2097,10 @@ hipError_t launch_batch(const BatchLaunch &b, hipStream_t st)
         break;
     case BK_PRECOND: {
         const PrecArgs *a = static_cast<const PrecArgs *>(b.args);
-        if (b.var == PV_P && b.f32) hipLaunchKernelGGL((k_precond_batch<PV_P, float>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
-        else if (b.var == PV_S && b.f32) hipLaunchKernelGGL((k_precond_batch<PV_S, float>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
-        else if (b.var == PV_P) hipLaunchKernelGGL((k_precond_batch<PV_P, double>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
-        else if (b.var == PV_S) hipLaunchKernelGGL((k_precond_batch<PV_S, double>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        if (b.var == PV_P && b.f32) hipLaunchKernelGGL((k_psweep_batch<PV_P, float>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        else if (b.var == PV_S && b.f32) hipLaunchKernelGGL((k_psweep_batch<PV_S, float>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        else if (b.var == PV_P) hipLaunchKernelGGL((k_psweep_batch<PV_P, double>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
+        else if (b.var == PV_S) hipLaunchKernelGGL((k_psweep_batch<PV_S, double>), g, dim3(gmd::PT), 0, st, a, b.off, b.M);
         else return hipErrorInvalidValue;
         break;
     }

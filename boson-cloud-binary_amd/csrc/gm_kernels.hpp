@@ -55,6 +55,7 @@ struct FactorArgs {
     double2 *inv;
     float2 *inv32;        // FP32 copy of the pivots, or nullptr
     int *failed;
+    int *pseq;            // per mode: 1 when a segment halo is too short (the row is solved whole)
 };
 
 struct PrecArgs {
@@ -64,7 +65,11 @@ struct PrecArgs {
     gmd::Krylov *Kr;
     const double2 *R;
     const double2 *R0;   // unnormalised shadow residual: r = R0 / |rhs| at the first iteration
-    double2 *Pv;         // p (PV_P, updated in place) or s (PV_S, written)
+    double2 *Pv;         // PV_P: p in the first buffer (Krylov pb = 0); PV_S: s (written)
+    double2 *Pv2;        // PV_P: the second p buffer (the update reads one buffer and writes the other)
+    const int *pseq;     // PV_P / PV_S: per mode, solve the whole row in one workgroup (k_factor's check)
+    int nseg;            // PV_P / PV_S: segments per mode (precond_nseg)
+    double *wpart;       // PV_P / PV_S with wood: 4 K nseg partial projections of the Woodbury epilogue
     const double2 *V;
     int ncol;            // PV_WCOL: number of sink columns
     double2 *Y;          // output (PV_P / PV_S)
@@ -213,7 +218,7 @@ void apply_pool_release(int device, int first, int count);
 hipError_t apply_pool_write(int first, const ApplyArgs *src, int count);  // synchronous, current device
 hipError_t launch_batch(const BatchLaunch &b, hipStream_t st);
 // Blocks of the plain launches, for laying out batched ones.
-int precond_blocks(int K);
+int precond_blocks(int K, int N);
 int stream_blocks(size_t n);
 int phase_blocks(int N, int K);
 int diag_apply_blocks(int N, int K);

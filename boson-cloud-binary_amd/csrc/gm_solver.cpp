@@ -165,7 +165,9 @@ struct gm_solver {
     int abs_nq = 0;
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
-    DevBuf<double2> psi, b, x, rr, r0, p, v, y, z, s, t, inv, tmp;
+    DevBuf<double2> psi, b, x, rr, r0, p, p2, v, y, z, s, t, inv, tmp;
+    DevBuf<int> pseq;  // per mode: the preconditioner's p / s sweeps solve the whole row (k_factor)
+    DevBuf<double> wpart;  // partial Woodbury projections of the p / s sweeps (4 per segment block)
     DevBuf<float2> inv32, y32;  // FP32 pivots and sweep scratch (mixed-precision preconditioner)
     DevBuf<double2> xt;         // phased column vectors of the coupling (FP64; FP32 uses the first half)
     DevBuf<double> partial, rowpart, diagout;
@@ -838,6 +840,9 @@ gmk::PrecArgs prec_args(gm_solver &S)
     fill_sinks(S, a.sk);
     a.Ps = S.dP.p;
     a.lmax = S.l_max;
+    a.pseq = S.pseq.p;
+    a.nseg = gmd::precond_nseg(S.Nr);
+    a.wpart = S.wpart.p;
     return a;
 }
 
@@ -870,6 +875,7 @@ void launch_factor(gm_solver &S)
     f.inv = S.inv.p;
     f.inv32 = S.inv32.p;
     f.failed = S.dflags.p;
+    f.pseq = S.pseq.p;
     launch(S, GM_PROF_FACTOR, [&] { return gmk::launch_factor(f, S.st); });
 }
 
@@ -1005,6 +1011,7 @@ gmk::PrecArgs iter_prec_args(gm_solver &S, bool wood, int var)
     pa.R0 = S.r0.p;
     pa.V = S.v.p;
     pa.Pv = var == gmk::PV_P ? S.p.p : S.s.p;
+    pa.Pv2 = var == gmk::PV_P ? S.p2.p : nullptr;
     pa.Y = var == gmk::PV_P ? S.y.p : S.z.p;
     return pa;
 }
@@ -1344,8 +1351,14 @@ void solve_tail_work(gm_solver &S)
 void alloc_device(gm_solver &S)
 {
     const size_t n = S.nvec();
-    for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.v, &S.y, &S.z, &S.s, &S.t, &S.inv, &S.tmp})
+    for (auto *bf : {&S.psi, &S.b, &S.x, &S.rr, &S.r0, &S.p, &S.p2, &S.v, &S.y, &S.z, &S.s, &S.t, &S.inv, &S.tmp})
         if (bf->n != n) bf->alloc(n);
+    const size_t nwp = static_cast<size_t>(4) * S.K * gmd::precond_nseg(S.Nr);
+    if (S.wpart.n != nwp) S.wpart.alloc(nwp);
+    if (S.pseq.n != static_cast<size_t>(S.K)) {
+        S.pseq.alloc(S.K);
+        HIPCK(hipMemsetAsync(S.pseq.p, 0, S.K * sizeof(int), S.st));
+    }
     for (auto *bf : {&S.inv32, &S.y32})
         if (bf->n != n) bf->alloc(n);
     if (S.xt.n != n) S.xt.alloc(n);
