@@ -240,8 +240,12 @@ void build_plan(BatchLane &B)
             gm_solver &S = *B.mem[m];
             if (!wood[m]) continue;
             a[m] = wcorr_args(S, z ? S.z.p : S.y.p, krylov_xt(S), S.flags.precision == GM_PRECISION_MIXED);
-            b[m] = gmk::stream_blocks(static_cast<size_t>(S.Nr) * S.K);
+            b[m] = 1;
         }
+        // the coefficients (one block per member), then the correction
+        add_seg(B.iter, gmk::BK_WCOEF, a, b);
+        for (int m = 0; m < M; ++m)
+            if (wood[m]) b[m] = gmk::stream_blocks(static_cast<size_t>(B.mem[m]->Nr) * B.mem[m]->K);
         add_seg(B.iter, gmk::BK_WCORR, a, b);
     };
     auto krylov_apply_seg = [&](int mode) {

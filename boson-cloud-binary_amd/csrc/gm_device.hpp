@@ -161,6 +161,7 @@ struct Krylov {
     unsigned counter;
     int fresh;                  // no iteration yet since k_init: r is read from r0' / |rhs| and p = r
     int pb;                     // p is held in the second p buffer (the p update writes the other one)
+    int wparts;                 // slots of the preconditioner's partial Woodbury projections
 };
 
 // Output of the diagnostics reduction.

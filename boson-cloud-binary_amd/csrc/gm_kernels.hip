@@ -1098,8 +1098,8 @@ __global__ __launch_bounds__(gmd::PT) void k_wcol(PrecArgs a) { k_wcol_body(a, g
 // recurrences, y_i = a_i y_(i-1) + inv_i in_i and x_i = a_i x_(i+1) + y_i with
 // a_i = -l inv_i (zero on the boundary rows).
 //
-// Block j * nseg + g solves segment g of mode j: the window [w0, w1) of at
-// most one PT x PE tile, the segment [g0, g1) and up to PHALO nodes on either
+// A block solves segment g of mode j: the window [w0, w1) of at most one
+// PT x PE tile, the segment [g0, g1) and up to PHALO nodes on either
 // side, is loaded once (each thread PE consecutive nodes), both recurrences
 // run as block scans with the carries into the window's ends dropped, and the
 // segment's nodes are stored. A dropped carry is multiplied over a halo by
@@ -1119,9 +1119,32 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     Krylov *Kr = a.Kr;
     if (Kr->done) return;
     const int nseg = a.nseg;
-    const int j = B.id / nseg, g = B.id - j * nseg;
-    const bool whole = nseg > 1 && a.pseq[j];
     const int N = a.N, tid = threadIdx.x;
+    // Woodbury (a.wood): the segments holding sink-support nodes, and segment
+    // 0 of every mode (a mode solved whole stores its row there), store
+    // partial projections v_a^H y for the epilogue. Their blocks come first
+    // in the grid (j = b / count), so that the last of them, which sums the
+    // partials, finishes while the other segments are still being solved.
+    const int ns_w = a.wood ? a.Ps->nsink : 0;
+    const int s0lo = ns_w > 0 ? a.sk[0].lo : 0, s0hi = ns_w > 0 ? a.sk[0].lo + a.sk[0].n : 0;
+    const int s1lo = ns_w > 1 ? a.sk[1].lo : 0, s1hi = ns_w > 1 ? a.sk[1].lo + a.sk[1].n : 0;
+    auto holds = [&](int gg) {
+        const int lo = gg * gmd::PSEG, hi = nseg == 1 ? N : min(N, lo + gmd::PSEG);
+        return gg == 0 || (lo < s0hi && s0lo < hi) || (lo < s1hi && s1lo < hi);
+    };
+    int count = 0;
+    for (int gg = 0; gg < nseg; ++gg) count += holds(gg);
+    int j, g = 0, rank;
+    {
+        const bool holder = B.id < a.K * count;
+        const int per = holder ? count : nseg - count, b = holder ? B.id : B.id - a.K * count;
+        j = b / per;
+        rank = b - j * per;
+        // the rank-th segment (in order) that holds / does not hold
+        for (int gg = 0, k = 0; gg < nseg; ++gg)
+            if (holds(gg) == holder && k++ == rank) g = gg;
+    }
+    const bool whole = nseg > 1 && a.pseq[j];
     const size_t base_j = (size_t)j * N;
     double2 *const Y = a.Y;
     const C ell = Arith<RT>::from(make_double2(0.0, a.Ps->half_dt * a.off));
@@ -1147,11 +1170,7 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
         Pout = second ? a.Pv : a.Pv2;
     }
     const double2 phj = a.Xt ? a.Ps->ph[mode_m(j) + a.lmax] : make_double2(1.0, 0.0);
-    // Woodbury (a.wood): the thread's share of the projections v_a^H y over
-    // the sink supports
-    const int ns_w = a.wood ? a.Ps->nsink : 0;
-    const int s0lo = ns_w > 0 ? a.sk[0].lo : 0, s0hi = ns_w > 0 ? a.sk[0].lo + a.sk[0].n : 0;
-    const int s1lo = ns_w > 1 ? a.sk[1].lo : 0, s1hi = ns_w > 1 ? a.sk[1].lo + a.sk[1].n : 0;
+    // the thread's share of the projections v_a^H y over the sink supports
     double wp[4] = {0.0, 0.0, 0.0, 0.0};
     auto store_out = [&](const int idx, const double2 yv) __attribute__((always_inline)) {
         Y[base_j + idx] = yv;
@@ -1357,42 +1376,25 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
             }
         }
     }
-    if (!a.wood) return;
-    // Woodbury epilogue (matrices.cpp:146-153): c = Ginv (V^H y) and the
-    // rank-one coefficients of A applied to the corrected y - W c,
-    // (dt/4) g_a v_a^H (y - W c). With G = D + V^H W, D = diag(1/((dt/4) g)),
-    // V^H (y - W c) = V^H y - (G - D) Ginv V^H y = D c, so the coefficients
-    // are c itself. The blocks that hold sink-support nodes (and segment 0 of
-    // every mode, which stores a row solved whole) store partial projections;
-    // the last of them sums the partials in a fixed order.
-    auto holds = [&](int gg) {
-        const int lo = gg * gmd::PSEG, hi = nseg == 1 ? N : min(N, lo + gmd::PSEG);
-        return gg == 0 || (lo < s0hi && s0lo < hi) || (lo < s1hi && s1lo < hi);
-    };
-    int count = 0, rank = 0;
-    for (int gg = 0; gg < nseg; ++gg) {
-        const bool h = holds(gg);
-        count += h;
-        rank += h && gg < g;
+    // Woodbury: the holding blocks (the first K count of the grid) store
+    // their partial projections v_a^H y in a fixed order; the correction
+    // kernel that follows sums them (no block of this launch waits for
+    // another)
+    if (!a.wood || B.id >= a.K * count) return;
+    __shared__ double ws[T / 64][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double v = wave_sum(wp[q]);
+        if ((tid & 63) == 0) ws[tid >> 6][q] = v;
     }
-    if (!holds(g)) return;
-    const Blk P{j * count + rank, a.K * count};
-    if (!block_partials<4>(wp, a.wpart, &Kr->counter, P)) return;
-    double pr[4];
-    for (int q = 0; q < 4; ++q) pr[q] = sum_partials(a.wpart, q, P);
+    __syncthreads();
     if (tid == 0) {
-        const double2 proj[2] = {make_double2(pr[0], pr[1]), make_double2(pr[2], pr[3])};
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            if (s >= ns_w) break;
-            double2 c = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-                if (u < ns_w) c = cadd(c, cmul(Kr->Ginv[s * 2 + u], proj[u]));
-            Kr->c[s] = c;
-            Kr->coefA[s] = c;
+        for (int q = 0; q < 4; ++q) {
+            double v = 0.0;
+            for (int w = 0; w < T / 64; ++w) v += ws[w][q];
+            a.wpart[(size_t)q * a.K * count + B.id] = v;
         }
-        Kr->counter = 0;
+        if (B.id == 0) Kr->wparts = a.K * count;
     }
 }
 template <int VAR, typename RT>
@@ -1662,6 +1664,49 @@ __global__ __launch_bounds__(1024) void k_sink_batch(const SinkArgs *__restrict_
 // Woodbury correction of a preconditioned vector, in place over the node
 // range where the columns W_a can be non-zero (matrices.cpp:151):
 // y -= sum_a c_a W_a.
+// Woodbury coefficients (matrices.cpp:146-153), one block of 256 threads
+// between a preconditioner apply and its correction: c = Ginv (V^H y) from the
+// preconditioner's partial projections, summed in a fixed order. The rank-one
+// coefficients of A applied to the corrected y - W c, (dt/4) g_a v_a^H
+// (y - W c), equal c: with G = D + V^H W and D = diag(1/((dt/4) g)),
+// V^H (y - W c) = V^H y - (G - D) Ginv V^H y = D c.
+__device__ __forceinline__ void k_wcoef_body(const WcorrArgs &a)
+{
+    Krylov *Kr = a.Kr;
+    if (Kr->done) return;
+    const int ns = a.nsink;
+    // a contiguous slice of the partials per thread, then a fixed-order
+    // combine across the block
+    __shared__ double ws[4][4];
+    const int nb = Kr->wparts, per = (nb + 255) / 256, b0 = threadIdx.x * per;
+    double pr[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = b0; b < b0 + per && b < nb; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pr[q] += a.wpart[(size_t)q * nb + b];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double v = wave_sum(pr[q]);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < 4; ++q) pr[q] = ws[0][q] + ws[1][q] + ws[2][q] + ws[3][q];
+    const double2 proj[2] = {make_double2(pr[0], pr[1]), make_double2(pr[2], pr[3])};
+    for (int s = 0; s < ns; ++s) {
+        double2 c = make_double2(0.0, 0.0);
+        for (int u = 0; u < ns; ++u) c = cadd(c, cmul(Kr->Ginv[s * 2 + u], proj[u]));
+        Kr->c[s] = c;
+        Kr->coefA[s] = c;
+    }
+}
+__global__ __launch_bounds__(256) void k_wcoef(WcorrArgs a) { k_wcoef_body(a); }
+// One launch over many solvers: member m owns block off[m] (one block each).
+__global__ __launch_bounds__(256) void k_wcoef_batch(const WcorrArgs *__restrict__ args, const int *__restrict__ off, int M)
+{
+    k_wcoef_body(args[batch_member(off, M)]);
+}
+
+// y -= W c (the stored columns rotated to the step's angle), and the phased copy
 __device__ __forceinline__ void k_wcorr_body(const WcorrArgs &a, Blk B)
 {
     const Krylov *Kr = a.Kr;
@@ -2116,6 +2161,9 @@ hipError_t launch_batch(const BatchLaunch &b, hipStream_t st)
     case BK_SINK:
         hipLaunchKernelGGL(k_sink_batch, g, dim3(1024), 0, st, static_cast<const SinkArgs *>(b.args), b.off, b.M);
         break;
+    case BK_WCOEF:
+        hipLaunchKernelGGL(k_wcoef_batch, g, dim3(256), 0, st, static_cast<const WcorrArgs *>(b.args), b.off, b.M);
+        break;
     case BK_WCORR:
         hipLaunchKernelGGL(k_wcorr_batch, g, dim3(256), 0, st, static_cast<const WcorrArgs *>(b.args), b.off, b.M);
         break;
@@ -2281,6 +2329,7 @@ hipError_t launch_cscale(double2 *y, size_t n, double2 c, hipStream_t st)
 
 hipError_t launch_wcorr(const WcorrArgs &a, hipStream_t st)
 {
+    hipLaunchKernelGGL(k_wcoef, dim3(1), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_wcorr, dim3(stream_grid((size_t)a.N * a.K)), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -69,7 +69,7 @@ struct PrecArgs {
     double2 *Pv2;        // PV_P: the second p buffer (the update reads one buffer and writes the other)
     const int *pseq;     // PV_P / PV_S: per mode, solve the whole row in one workgroup (k_factor's check)
     int nseg;            // PV_P / PV_S: segments per mode (precond_nseg)
-    double *wpart;       // PV_P / PV_S with wood: 4 K nseg partial projections of the Woodbury epilogue
+    double *wpart;       // PV_P / PV_S with wood: the partial projections v_a^H y (4 per holding block)
     const double2 *V;
     int ncol;            // PV_WCOL: number of sink columns
     double2 *Y;          // output (PV_P / PV_S)
@@ -87,7 +87,8 @@ struct PrecArgs {
 struct WcorrArgs {
     int N, K, nsink, lmax;
     const gmd::StepParams *P;  // wrot: rotation of the stored Woodbury columns
-    const gmd::Krylov *Kr;
+    gmd::Krylov *Kr;
+    const double *wpart;  // the preconditioner's partial projections v_a^H y (Krylov wparts slots)
     const double2 *W0, *W1;
     double2 *Y;
     void *Xt;            // phased copy of Y to correct as well (or nullptr)
@@ -198,7 +199,8 @@ hipError_t launch_absorbers(const AbsorberArgs &a, hipStream_t st);
 // [off[m], off[m + 1]) of a grid of `total` blocks. Kernels and their block
 // counts are those of the plain launches; a member with no blocks sits out.
 enum BatchKernel {
-    BK_APPLY = 0, BK_PRECOND, BK_UPDATE, BK_INIT, BK_ACCEPT, BK_SINK, BK_WCORR, BK_DIAG_APPLY, BK_DIAG_FINAL, BK_PHASE
+    BK_APPLY = 0, BK_PRECOND, BK_UPDATE, BK_INIT, BK_ACCEPT, BK_SINK, BK_WCORR, BK_DIAG_APPLY, BK_DIAG_FINAL, BK_PHASE,
+    BK_WCOEF
 };
 struct BatchLaunch {
     int kernel;        // BatchKernel

@@ -964,6 +964,7 @@ gmk::WcorrArgs wcorr_args(gm_solver &S, double2 *Y, void *Xt, int xt_f32)
     w.K = S.K;
     w.nsink = P.nsink;
     w.Kr = S.dK.p;
+    w.wpart = S.wpart.p;
     w.W0 = S.sink[P.sink_body[0]].W.p;
     w.W1 = P.nsink > 1 ? S.sink[P.sink_body[1]].W.p : nullptr;
     w.Y = Y;
