@@ -332,14 +332,15 @@ def mfma_macs(kernel):
 
 def apply_kernel(s, precision):
     """The traced name of the operator apply the solver runs in `precision`:
-    the polar-quadrature form k_tapply<class, T> (8-node tiles) or the
-    Gaunt form k_apply<ng, T> (16 ng-node tiles), as the autotuner chose."""
+    the polar-quadrature form k_tapply<class> (8-node tiles, FP64 in both
+    precisions) or the Gaunt form k_apply<ng, T> (16 ng-node tiles), as the
+    autotuner chose."""
     tile = s.apply_config(precision)[0]
     t = "float" if precision == gm.GM_PRECISION_MIXED else "double"
     if tile == 8:
         lmax = int(round((math.sqrt(8 * s.n_modes + 1) - 3) / 2))  # K = (l_max + 1)(l_max + 2) / 2
         cls = max(4, (lmax + 3) // 4 * 4)
-        return "k_tapply<%d, %s>" % (cls, t), True
+        return "k_tapply<%d>" % cls, True
     return "k_apply<%d, %s>" % (max(tile, 16) // 16, t), False
 
 
@@ -357,12 +358,12 @@ def rooflines(s, prof, K, N, iters, mixed):
     krylov = min(prof["apply_krylov"][1], 2 * int(sum(iters)))
     kernel, polar = apply_kernel(s, gm.GM_PRECISION_MIXED if mixed else gm.GM_PRECISION_FP64)
     # the Gaunt form's FP32 apply runs on the FP32 matrix core; the polar
-    # form's transforms run on the FP64 matrix core in both precisions (its
-    # banded product is VALU work, FP32 and FP64 alike at 78.6 TFLOP/s)
+    # form runs in FP64 in both precisions (its transforms on the FP64 matrix
+    # core, its banded product on the VALU)
     if mixed:
         ms, cnt = prof["apply_krylov"][0], krylov
         peak = FP64_PEAK_TFLOPS if polar else FP32_PEAK_TFLOPS
-        name, dt = kernel + " (Krylov applies)", "f32"
+        name, dt = kernel + " (Krylov applies)", "f64" if polar else "f32"
     else:
         ms = prof["apply_krylov"][0] + prof["apply_other"][0]
         cnt = krylov + prof["apply_other"][1]
@@ -389,7 +390,7 @@ def rooflines(s, prof, K, N, iters, mixed):
                             "mode_pairs_per_launch": pairs, "achieved_TFLOPs": executed / avg_s / 1e12,
                             "mfma_macs_per_launch": macs,
                             "mfma_padding": (macs / terms) if macs else None}
-    roof["traffic"] = measured_traffic(("k_tapply" if polar else "k_apply") + ("_fp32" if mixed else ""))
+    roof["traffic"] = measured_traffic("k_tapply" if polar else "k_apply" + ("_fp32" if mixed else ""))
     # whole step: the sum over kernels of max(bytes / HBM peak, flops / MFMA
     # peak) against the measured time of the profiled steps
     nsteps = max(len(iters), 1)
@@ -674,11 +675,11 @@ def summary_traffic(fetch_csv, write_csv, cfetch_csv, cwrite_csv):
         lines.append(f"{k:38s} {n:8d} {fk:14.1f} {wk:14.1f} {b:14.4g}")
     lines.append("")
     lines.append("working = launches that fetch more than 1 MiB (launches after convergence return at once)")
-    for prefix in ("k_apply", "k_tapply"):
-        for key, dt in ((prefix, "double"), (prefix + "_fp32", "float")):
-            k = _main_variant(rows, prefix, dt)
-            if k:
-                lines.append(f"traffic_bytes_per_launch {key} {rows[k][3]:.6g}")
+    for key, prefix, dt in (("k_apply", "k_apply", "double"), ("k_apply_fp32", "k_apply", "float"),
+                            ("k_tapply", "k_tapply", "")):
+        k = _main_variant(rows, prefix, dt)
+        if k:
+            lines.append(f"traffic_bytes_per_launch {key} {rows[k][3]:.6g}")
     return "\n".join(lines) + "\n"
 
 

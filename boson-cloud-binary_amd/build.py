@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "gm", "libgm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["gm_kernels.hip", "gm_theta_f64.hip", "gm_theta_f32.hip", "gm_math.cpp", "gm_theta_tables.cpp", "gm_ilut.cpp", "gm_solver.cpp"]
+SOURCES = ["gm_kernels.hip", "gm_theta_f64.hip", "gm_math.cpp", "gm_theta_tables.cpp", "gm_ilut.cpp", "gm_solver.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I", CSRC, "-I", os.path.join(REPO, "include")]
 # Packed FP32 VALU ops stay off in every device pass (v_pk_* have returned wrong

@@ -131,12 +131,14 @@ int apply_blocks(gm_solver &S, gmk::ApplyArgs &a, int mode)
     return grid;
 }
 
-// Ragged apply launches, one per (tile, precision) present among the members.
+// Ragged apply launches, one per (tile, precision) of the Gaunt form and one
+// per l_max class of the polar form (FP64 for both precisions) present among
+// the members.
 void add_apply_segs(std::vector<std::unique_ptr<BatchSeg>> &out, std::vector<gmk::ApplyArgs> &args,
                     std::vector<int> &blocks, const std::vector<int> &run)
 {
     const int M = static_cast<int>(args.size());
-    for (int mixed = 0; mixed < 2; ++mixed) {
+    for (int mixed = 0; mixed < 2; ++mixed)
         for (int ng : {1, 2, 4}) {
             std::vector<int> b(M, 0);
             bool any = false;
@@ -147,18 +149,15 @@ void add_apply_segs(std::vector<std::unique_ptr<BatchSeg>> &out, std::vector<gmk
                 }
             if (any) add_seg(out, gmk::BK_APPLY, args, b, ng, mixed);
         }
-        // polar-form members, one launch per l_max class
-        for (int cls = 4; cls <= 32; cls += 4) {
-            std::vector<int> b(M, 0);
-            bool any = false;
-            for (int m = 0; m < M; ++m)
-                if (run[m] && args[m].ng == gmk::TAPPLY_NG && args[m].mixed == mixed &&
-                    gmk::tapply_class(args[m].g.lmax, mixed) == cls) {
-                    b[m] = blocks[m];
-                    any = true;
-                }
-            if (any) add_seg(out, gmk::BK_APPLY, args, b, gmk::TAPPLY_NG, mixed, cls);
-        }
+    for (int cls = 4; cls <= 20; cls += 4) {
+        std::vector<int> b(M, 0);
+        bool any = false;
+        for (int m = 0; m < M; ++m)
+            if (run[m] && args[m].ng == gmk::TAPPLY_NG && gmk::tapply_class(args[m].g.lmax, 0) == cls) {
+                b[m] = blocks[m];
+                any = true;
+            }
+        if (any) add_seg(out, gmk::BK_APPLY, args, b, gmk::TAPPLY_NG, 0, cls);
     }
 }
 

@@ -2134,7 +2134,7 @@ hipError_t launch_batch(const BatchLaunch &b, hipStream_t st)
     switch (b.kernel) {
     case BK_APPLY:
         if (b.ng == TAPPLY_NG)
-            return launch_tapply_batch(static_cast<const ApplyArgs *>(b.args), b.off, b.M, b.total, b.var, b.mixed, st);
+            return launch_tapply_batch(static_cast<const ApplyArgs *>(b.args), b.off, b.M, b.total, b.var, st);
         if (b.ng == 1) launch_apply_batch_ng<1>(b, st);
         else if (b.ng == 2) launch_apply_batch_ng<2>(b, st);
         else if (b.ng == 4) launch_apply_batch_ng<4>(b, st);

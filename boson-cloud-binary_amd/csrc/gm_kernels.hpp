@@ -240,8 +240,7 @@ hipError_t launch_tapply(const ApplyArgs &a, hipStream_t st);
 // its build from the radial profiles and the cut (g.Ut unused here).
 size_t tapply_u_doubles(int N, int lmax);
 hipError_t launch_tapply_u(const gmd::Geometry &g, double *Ut, hipStream_t st);
-hipError_t launch_tapply_batch(const ApplyArgs *args, const int *off, int M, int total, int cls, int mixed,
-                               hipStream_t st);
+hipError_t launch_tapply_batch(const ApplyArgs *args, const int *off, int M, int total, int cls, hipStream_t st);
 hipError_t launch_apply(ApplyArgs a, int mode, hipStream_t st);
 PhaseArgs phase_args(const ApplyArgs &a);
 // radial profiles from the orbit in P (R1, R2, q, Mt, prf)

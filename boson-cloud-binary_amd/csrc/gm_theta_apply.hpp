@@ -1,5 +1,5 @@
 // gm_theta_apply.hpp - the operator apply with the coupling in polar-quadrature form
-// (kernel templates; gm_theta_f64.hip and gm_theta_f32.hip instantiate them).
+// (kernel templates; gm_theta_f64.hip instantiates them).
 //
 // The coupling sum_k V_jk(r) x_k of the reference (hamiltonian.cpp:68-104,
 // 176-222) is evaluated through a Gauss-Legendre rule in x = cos(theta)
@@ -8,13 +8,12 @@
 //   A   X_i[m']     = sum_(k: m_k = m') p_k(x_i) xt_k                (xt = e^(i m theta) x)
 //   B   Y_i[m]      = sum_m' U_|m - m'|(x_i) X_i[m']                 (|m - m'| <= lc)
 //   C   (V x)_j     = e^(-i m_j theta) sum_i w_i p_j(x_i) Y_i[m_j]
-// A workgroup is one wavefront and one tile of 8 radial nodes, whose 16
-// columns are (node, real or imaginary part): every table is real, so the two
-// parts are transformed alike. U, A and C run on v_mfma_f64_4x4x4f64 in
-// quadrature groups of four points (four 4x4x4 blocks over four columns
-// each); B is a banded Toeplitz product on the VALU, one (point, column) per
-// lane, in the apply's arithmetic (FP64, or FP32 for the mixed-precision
-// Krylov applies). Work per node: about 2 (2 l_max + lc) n_pts K real
+// A workgroup is one tile of 8 radial nodes, whose 16 columns are (node,
+// real or imaginary part): every table is real, so the two parts are
+// transformed alike. Its four wavefronts split the output m. U, A and C run
+// on v_mfma_f64_4x4x4f64 in quadrature groups of four points (four 4x4x4
+// blocks over four columns each); B is a banded Toeplitz product on the VALU
+// in FP64, one (point, column) per lane. Work per node: about 2 (2 l_max + lc) n_pts K real
 // multiply-adds on the matrix core and n_pts (2 l_max + 1)(2 lc + 1) on the
 // VALU, against the 2 K^2 + 2 K (number of Gaunt terms) of the direct sum.
 // The epilogue is the Gaunt-form apply's (stencil, CN form, sinks, fused
@@ -210,7 +209,7 @@ struct TShared {
 // The work of wavefront W: phase A for its m', phase B for its m (all X from
 // LDS, the U sums from their table), phase C for its m, then its
 // accumulators into the coupling tile.
-template <int LM, typename T, int W>
+template <int LM, int W>
 __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh, int lc, int ng, int lane, int tile)
 {
     constexpr int NM = 2 * LM + 1;
@@ -248,13 +247,13 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
             if (e < KL * 8) ((double2 *)sh.A)[e] = rt[k()];
         });
     };
-    T un[4];
+    double un[4];
     const auto load_u = [&](int t, auto d0c) {
         constexpr int d0 = d0c();
         static_for<0, 4>([&](auto qc) {
             constexpr int d = d0 + qc();
-            if constexpr (d <= 2 * LM) un[qc()] = T(ut[(t * (2 * LM + 1) + d) * 4 * TT]);
-            else un[qc()] = T(0);
+            if constexpr (d <= 2 * LM) un[qc()] = ut[(t * (2 * LM + 1) + d) * 4 * TT];
+            else un[qc()] = 0.0;
         });
     };
     load_table(g.TA + tri * (size_t)KL * 16);
@@ -286,16 +285,16 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
         __syncthreads();
         store_table();  // TC of point group t
         if (t + 1 < ng) load_table(g.TA + (tri + t + 1) * (size_t)KL * 16);
-        T X[NM];
-        static_for<0, NM>([&](auto mic) { X[mic()] = T(sh.X[mic() * 64 + lane]); });
+        double X[NM];
+        static_for<0, NM>([&](auto mic) { X[mic()] = sh.X[mic() * 64 + lane]; });
         // B: Y[m] = sum_d U_d (X[m + d] + X[m - d]) for this wavefront's m, the
         // diagonals in groups of four (the table holds zeros above the cut)
-        T Y[M1 - M0 > 0 ? M1 - M0 : 1];
-        static_for<M0, M1>([&](auto mic) { Y[mic() - M0] = T(0); });
+        double Y[M1 - M0 > 0 ? M1 - M0 : 1];
+        static_for<M0, M1>([&](auto mic) { Y[mic() - M0] = 0.0; });
         static_while<0, (2 * LM + 4) / 4>([&](auto gc) {
             constexpr int d0 = 4 * gc();
             if (d0 > lc) return false;
-            T u[4];
+            double u[4];
             static_for<0, 4>([&](auto qc) { u[qc()] = un[qc()]; });
             if constexpr (d0 + 4 <= 2 * LM)
                 if (d0 + 4 <= lc) load_u(t, std::integral_constant<int, d0 + 4>{});
@@ -320,7 +319,7 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
         static_for<M0, M1>([&](auto mic) {
             constexpr int mi = mic();
             constexpr int nb = lay.ko[mi + 1] - lay.ko[mi];
-            const double yd = (double)Y[mi - M0];
+            const double yd = Y[mi - M0];
             double av[nb];
             static_for<0, nb>([&](auto sc) { av[sc()] = sh.A[(lay.ko[mi] + sc()) * 16 + la]; });
             static_for<0, nb>([&](auto sc) {
@@ -340,7 +339,7 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
     });
 }
 
-template <int LM, typename T>
+template <int LM>
 __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
 {
     constexpr int KMAX = (LM + 1) * (LM + 2) / 2;
@@ -381,10 +380,10 @@ __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
     // Every register array is indexed by template constants only (static_for,
     // one code path per wavefront); the point-group tables are shared in LDS.
     static_assert(TW == 4, "four wavefronts per tile");
-    if (wv == 0) tapply_wave<LM, T, 0>(a, sh, lc, ng, lane, B.id);
-    else if (wv == 1) tapply_wave<LM, T, 1>(a, sh, lc, ng, lane, B.id);
-    else if (wv == 2) tapply_wave<LM, T, 2>(a, sh, lc, ng, lane, B.id);
-    else tapply_wave<LM, T, 3>(a, sh, lc, ng, lane, B.id);
+    if (wv == 0) tapply_wave<LM, 0>(a, sh, lc, ng, lane, B.id);
+    else if (wv == 1) tapply_wave<LM, 1>(a, sh, lc, ng, lane, B.id);
+    else if (wv == 2) tapply_wave<LM, 2>(a, sh, lc, ng, lane, B.id);
+    else tapply_wave<LM, 3>(a, sh, lc, ng, lane, B.id);
     __syncthreads();
     // Epilogue over the tile's (mode, node), eight consecutive nodes of a mode
     // per eight threads, in batches of EB whose loads are issued before their
@@ -425,29 +424,29 @@ __device__ __forceinline__ void k_tapply_body(const ApplyArgs &a, Blk B)
     apply_reduce(a, red, B);
 }
 
-template <int LM, typename T>
+template <int LM>
 __global__ __launch_bounds__(64 * TW, GM_TWAVES) void k_tapply(ApplyArgs a)
 {
-    k_tapply_body<LM, T>(a, grid_blk());
+    k_tapply_body<LM>(a, grid_blk());
 }
 // One launch over many solvers: member m owns blocks [off[m], off[m + 1]).
-template <int LM, typename T>
+template <int LM>
 __global__ __launch_bounds__(64 * TW, GM_TWAVES) void k_tapply_batch(const ApplyArgs *__restrict__ args, const int *__restrict__ off,
                                                      int M)
 {
     const int m = batch_member(off, M);
     const ApplyArgs a = args[m];
-    k_tapply_body<LM, T>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
+    k_tapply_body<LM>(a, Blk{(int)blockIdx.x - __builtin_amdgcn_readfirstlane(off[m]),
                               __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
-template <int LM, typename T>
+template <int LM>
 hipError_t launch_t(const ApplyArgs *a, const ApplyArgs *args, const int *off, int M, int total, hipStream_t st)
 {
     if (args)
-        hipLaunchKernelGGL((k_tapply_batch<LM, T>), dim3(total), dim3(64 * TW), 0, st, args, off, M);
+        hipLaunchKernelGGL((k_tapply_batch<LM>), dim3(total), dim3(64 * TW), 0, st, args, off, M);
     else
-        hipLaunchKernelGGL((k_tapply<LM, T>), dim3(tapply_grid(a->g.N)), dim3(64 * TW), 0, st, *a);
+        hipLaunchKernelGGL((k_tapply<LM>), dim3(tapply_grid(a->g.N)), dim3(64 * TW), 0, st, *a);
     return hipGetLastError();
 }
 
@@ -458,23 +457,8 @@ hipError_t launch_u(const gmd::Geometry &g, double *Ut, hipStream_t st)
     return hipGetLastError();
 }
 
-// Launch of the l_max class cls in arithmetic T (one instantiation unit per T).
-template <typename T>
+// Launch of the l_max class cls (gm_theta_f64.hip).
 hipError_t launch_tt(int cls, const ApplyArgs *a, const ApplyArgs *args, const int *off, int M, int total,
-                     hipStream_t st)
-{
-    switch (cls) {
-    case 4: return launch_t<4, T>(a, args, off, M, total, st);
-    case 8: return launch_t<8, T>(a, args, off, M, total, st);
-    case 12: return launch_t<12, T>(a, args, off, M, total, st);
-    case 16: return launch_t<16, T>(a, args, off, M, total, st);
-    case 20: return launch_t<20, T>(a, args, off, M, total, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-extern template hipError_t launch_tt<double>(int, const ApplyArgs *, const ApplyArgs *, const int *, int, int,
-                                            hipStream_t);
-extern template hipError_t launch_tt<float>(int, const ApplyArgs *, const ApplyArgs *, const int *, int, int,
-                                           hipStream_t);
+                     hipStream_t st);
 
 }  // namespace gmk

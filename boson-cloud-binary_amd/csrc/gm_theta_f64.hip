@@ -1,10 +1,26 @@
-// gm_theta_f64.hip - the polar-quadrature apply in FP64 (gm_theta_apply.hpp),
-// with the class selection and launch entry points.
+// gm_theta_f64.hip - the polar-quadrature apply (gm_theta_apply.hpp), with the
+// class selection and launch entry points. It runs in FP64 for both
+// precisions: its transforms are FP64 matrix-core work and its banded
+// product runs on the VALU, where FP32 (packed FP32 off) issues at the FP64
+// rate, so an FP32 form was measured slower (0.165 against 0.145 ms at
+// l_max 20, N 16384) and a mixed-precision solver's Krylov applies take the
+// FP64 one.
 #include "gm_theta_apply.hpp"
 
 namespace gmk {
 
-template hipError_t launch_tt<double>(int, const ApplyArgs *, const ApplyArgs *, const int *, int, int, hipStream_t);
+hipError_t launch_tt(int cls, const ApplyArgs *a, const ApplyArgs *args, const int *off, int M, int total,
+                     hipStream_t st)
+{
+    switch (cls) {
+    case 4: return launch_t<4>(a, args, off, M, total, st);
+    case 8: return launch_t<8>(a, args, off, M, total, st);
+    case 12: return launch_t<12>(a, args, off, M, total, st);
+    case 16: return launch_t<16>(a, args, off, M, total, st);
+    case 20: return launch_t<20>(a, args, off, M, total, st);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 int tapply_class(int lmax, int /*mixed*/)
 {
@@ -18,8 +34,7 @@ hipError_t launch_tapply(const ApplyArgs &a, hipStream_t st)
 {
     const int cls = tapply_class(a.g.lmax, a.mixed);
     if (!cls || !a.g.TA) return hipErrorInvalidValue;
-    return a.mixed ? launch_tt<float>(cls, &a, nullptr, nullptr, 0, 0, st)
-                   : launch_tt<double>(cls, &a, nullptr, nullptr, 0, 0, st);
+    return launch_tt(cls, &a, nullptr, nullptr, 0, 0, st);
 }
 
 size_t tapply_u_doubles(int N, int lmax)
@@ -42,11 +57,9 @@ hipError_t launch_tapply_u(const gmd::Geometry &g, double *Ut, hipStream_t st)
     }
 }
 
-hipError_t launch_tapply_batch(const ApplyArgs *args, const int *off, int M, int total, int cls, int mixed,
-                               hipStream_t st)
+hipError_t launch_tapply_batch(const ApplyArgs *args, const int *off, int M, int total, int cls, hipStream_t st)
 {
-    return mixed ? launch_tt<float>(cls, nullptr, args, off, M, total, st)
-                 : launch_tt<double>(cls, nullptr, args, off, M, total, st);
+    return launch_tt(cls, nullptr, args, off, M, total, st);
 }
 
 }  // namespace gmk
