@@ -122,36 +122,26 @@ __host__ __device__ constexpr size_t ut_stride(int lmax)
     return (size_t)gmt::theta_rule(lmax, 2 * lmax) * (2 * LM + 1) * 4 * TT;
 }
 
-// The U sums of one tile (one wavefront): U_d(x_i, r) = sum_l* E_l*(r)
-// TU[i][d][l*] for the tile's rule and every point group, d <= lc (zero
-// above), on the matrix core, into Ut. They depend on the orbit's radial
-// profiles and the cut only, so they are rebuilt with those, not per apply.
-template <int LM>
-__device__ __forceinline__ void tapply_u_body(const gmd::Geometry g, double *Ut, int tile)
+// The U sums of one tile: U_d(x_i, r) = sum_l* E_l*(r) TU[i][d][l*] for the
+// tile's rule and every point group, d <= lc (zero above), on the matrix
+// core, into Ut. They depend on the orbit's radial profiles and the cut only,
+// so they are rebuilt with those, not per apply. Wavefront W of the tile's
+// workgroup takes d = W, W + TW, ...
+template <int LM, int W>
+__device__ __forceinline__ void tapply_u_wave(const gmd::Geometry &g, const double *sE, double *out, int lc, int ng,
+                                              int lane)
 {
     constexpr DLay<LM> dl{};
     constexpr int KD = dl.ko[2 * LM + 1];
-    __shared__ double sE[(2 * LM + 1) * TT];
-    const int lane = threadIdx.x;
     const int a4 = lane >> 4, b4 = lane & 3;
     const int la = 4 * a4 + b4;
     const int col = 4 * ((lane >> 2) & 3) + b4;
     const int nd = col >> 1;
-    const int N = g.N, lmax = g.lmax, Ls = g.Ls;
-    const int t0 = tile * TT;
-    const int lc = g.lcut ? g.lcut[t0 / gmd::TILE] : 2 * lmax;
-    for (int e = lane; e < (2 * LM + 1) * TT; e += 64) {
-        const int ls = e >> 3, i = t0 + (e & 7);
-        sE[e] = (ls <= lc && ls <= 2 * lmax && i < N) ? g.E[(size_t)((ls & 1) * Ls + ls) * N + i] : 0.0;
-    }
-    __syncthreads();
-    const int ng = gmt::theta_rule(lmax, lc);
     const size_t tri = (size_t)ng * (ng - 1) / 2;
-    double *out = Ut + (size_t)tile * ut_stride<LM>(lmax) + a4 * TT + nd;
     for (int t = 0; t < ng; ++t) {
         const double *tu = g.TU + (tri + t) * (size_t)KD * 16 + la;
-        static_for<0, 2 * LM + 1>([&](auto dc) {
-            constexpr int d = dc();
+        static_for<0, (2 * LM + 1 - W + TW - 1) / TW>([&](auto kc) {
+            constexpr int d = W + TW * kc();
             double acc = 0.0;
             if (d <= lc) {
                 static_for<0, dl.ns[d]>([&](auto sc) {
@@ -168,26 +158,53 @@ __device__ __forceinline__ void tapply_u_body(const gmd::Geometry g, double *Ut,
 }
 
 template <int LM>
-__global__ __launch_bounds__(64) void k_tapply_u(gmd::Geometry g, double *Ut)
+__device__ __forceinline__ void tapply_u_body(const gmd::Geometry g, double *Ut, int tile)
+{
+    __shared__ double sE[(2 * LM + 1) * TT];
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int N = g.N, lmax = g.lmax, Ls = g.Ls;
+    const int t0 = tile * TT;
+    const int lc = g.lcut ? g.lcut[t0 / gmd::TILE] : 2 * lmax;
+    for (int e = threadIdx.x; e < (2 * LM + 1) * TT; e += 64 * TW) {
+        const int ls = e >> 3, i = t0 + (e & 7);
+        sE[e] = (ls <= lc && ls <= 2 * lmax && i < N) ? g.E[(size_t)((ls & 1) * Ls + ls) * N + i] : 0.0;
+    }
+    __syncthreads();
+    const int ng = gmt::theta_rule(lmax, lc);
+    const int a4 = lane >> 4, nd = (4 * ((lane >> 2) & 3) + (lane & 3)) >> 1;
+    double *out = Ut + (size_t)tile * ut_stride<LM>(lmax) + a4 * TT + nd;
+    static_assert(TW == 4, "four wavefronts per tile");
+    if (wv == 0) tapply_u_wave<LM, 0>(g, sE, out, lc, ng, lane);
+    else if (wv == 1) tapply_u_wave<LM, 1>(g, sE, out, lc, ng, lane);
+    else if (wv == 2) tapply_u_wave<LM, 2>(g, sE, out, lc, ng, lane);
+    else tapply_u_wave<LM, 3>(g, sE, out, lc, ng, lane);
+}
+
+template <int LM>
+__global__ __launch_bounds__(64 * TW) void k_tapply_u(gmd::Geometry g, double *Ut)
 {
     tapply_u_body<LM>(g, Ut, blockIdx.x);
 }
 
-// Output m of each wavefront: contiguous ranges [lo[w], lo[w + 1]) of
-// m + LM holding about a quarter of the accumulator blocks each.
-template <int LM>
-struct WSplit {
-    int lo[TW + 1];
-    constexpr WSplit() : lo()
+// Output m of wavefront W, interleaved: m + LM = W, W + TW, W + 2 TW, ...
+// (every wavefront takes near and far m alike, so the banded product of
+// phase B, whose cost per m depends on the cut, stays balanced); n m, their
+// m + LM and the first of their accumulators in the wavefront's local
+// layout.
+template <int LM, int W>
+struct WList {
+    static constexpr int n = (2 * LM + 1 - W + TW - 1) / TW;
+    int mi[n > 0 ? n : 1];
+    int ko[n + 1];
+    constexpr WList() : mi(), ko()
     {
-        const int total = kcount(LM);
-        int mi = 0, acc = 0;
-        for (int w = 0; w < TW; ++w) {
-            lo[w] = mi;
-            const int target = (total * (w + 1) + TW / 2) / TW;
-            while (mi < 2 * LM + 1 && (w == TW - 1 || acc + ns_of(LM, mi - LM) <= target)) acc += ns_of(LM, mi++ - LM);
+        int o = 0;
+        for (int k = 0; k < n; ++k) {
+            mi[k] = W + TW * k;
+            ko[k] = o;
+            o += ns_of(LM, mi[k] - LM);
         }
-        lo[TW] = 2 * LM + 1;
+        ko[n] = o;
     }
 };
 
@@ -208,7 +225,9 @@ struct TShared {
 
 // The work of wavefront W: phase A for its m', phase B for its m (all X from
 // LDS, the U sums from their table), phase C for its m, then its
-// accumulators into the coupling tile.
+// accumulators into the coupling tile. (Interleaving the m of the four
+// wavefronts instead of giving each a contiguous range made the apply 3 %
+// faster at l_max 20, N 16384.)
 template <int LM, int W>
 __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh, int lc, int ng, int lane, int tile)
 {
@@ -217,9 +236,8 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
     constexpr int KL = kcount(LM);
     constexpr DLay<LM> dl{};
     constexpr int KD = dl.ko[2 * LM + 1];
-    constexpr WSplit<LM> ws{};
-    constexpr int M0 = ws.lo[W], M1 = ws.lo[W + 1];
-    constexpr int E0 = lay.ko[M0], E1 = lay.ko[M1];
+    constexpr WList<LM, W> wl{};
+    constexpr int NW = WList<LM, W>::n, NACC = wl.ko[NW];
     const gmd::Geometry &g = a.g;
     const int a4 = lane >> 4, b4 = lane & 3;
     const int la = 4 * a4 + b4;                  // index in a table's 4x4 block
@@ -229,8 +247,8 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
     const size_t tri = (size_t)ng * (ng - 1) / 2;
     // U_d of this tile at (point 4 t + a4, node nd): Ut[((t (2 LM + 1) + d) 4 + a4) TT + nd]
     const double *ut = g.Ut + (size_t)tile * ut_stride<LM>(g.lmax) + a4 * TT + nd;
-    double yacc[E1 - E0 > 0 ? E1 - E0 : 1];
-    static_for<E0, E1>([&](auto e) { yacc[e() - E0] = 0.0; });
+    double yacc[NACC > 0 ? NACC : 1];
+    static_for<0, NACC>([&](auto e) { yacc[e()] = 0.0; });
     // Software pipeline: a table's global loads are issued a phase before its
     // copy into LDS, and a diagonal group's U sums a group before their use.
     constexpr int NT = (KL * 8 + 64 * TW - 1) / (64 * TW);
@@ -264,8 +282,8 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
         load_table(g.TC + (tri + t) * (size_t)KL * 16);
         __syncthreads();
         // A: X[m'] at (point 4 t + a4, column col) for this wavefront's m'
-        static_for<M0, M1>([&](auto mic) {
-            constexpr int mi = mic(), m = mi - LM, am = m < 0 ? -m : m;
+        static_for<0, NW>([&](auto kc) {
+            constexpr int mi = wl.mi[kc()], m = mi - LM, am = m < 0 ? -m : m;
             constexpr int nb = lay.ko[mi + 1] - lay.ko[mi];
             // the column's part of e^(i m theta) x: c x_re - s x_im or s x_re + c x_im
             const double2 ph = sh.ph[mi];
@@ -289,8 +307,8 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
         static_for<0, NM>([&](auto mic) { X[mic()] = sh.X[mic() * 64 + lane]; });
         // B: Y[m] = sum_d U_d (X[m + d] + X[m - d]) for this wavefront's m, the
         // diagonals in groups of four (the table holds zeros above the cut)
-        double Y[M1 - M0 > 0 ? M1 - M0 : 1];
-        static_for<M0, M1>([&](auto mic) { Y[mic() - M0] = 0.0; });
+        double Y[NW > 0 ? NW : 1];
+        static_for<0, NW>([&](auto kc) { Y[kc()] = 0.0; });
         static_while<0, (2 * LM + 4) / 4>([&](auto gc) {
             constexpr int d0 = 4 * gc();
             if (d0 > lc) return false;
@@ -301,13 +319,13 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
             static_for<0, 4>([&](auto qc) {
                 constexpr int d = d0 + qc();
                 if constexpr (d <= 2 * LM) {
-                    static_for<M0, M1>([&](auto mic) {
-                        constexpr int mi = mic();
+                    static_for<0, NW>([&](auto kc) {
+                        constexpr int k = kc(), mi = wl.mi[k];
                         if constexpr (d == 0) {
-                            Y[mi - M0] = u[0] * X[mi];
+                            Y[k] = u[0] * X[mi];
                         } else {
-                            if constexpr (mi + d < NM) Y[mi - M0] = fma(u[qc()], X[mi + d], Y[mi - M0]);
-                            if constexpr (mi - d >= 0) Y[mi - M0] = fma(u[qc()], X[mi - d], Y[mi - M0]);
+                            if constexpr (mi + d < NM) Y[k] = fma(u[qc()], X[mi + d], Y[k]);
+                            if constexpr (mi - d >= 0) Y[k] = fma(u[qc()], X[mi - d], Y[k]);
                         }
                     });
                 }
@@ -316,15 +334,15 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
         });
         __syncthreads();
         // C: the blocks of (V x) of this wavefront's m on the matrix core
-        static_for<M0, M1>([&](auto mic) {
-            constexpr int mi = mic();
+        static_for<0, NW>([&](auto kc) {
+            constexpr int k = kc(), mi = wl.mi[k];
             constexpr int nb = lay.ko[mi + 1] - lay.ko[mi];
-            const double yd = Y[mi - M0];
+            const double yd = Y[k];
             double av[nb];
             static_for<0, nb>([&](auto sc) { av[sc()] = sh.A[(lay.ko[mi] + sc()) * 16 + la]; });
             static_for<0, nb>([&](auto sc) {
-                constexpr int e = lay.ko[mi] + sc();
-                yacc[e - E0] = mf(av[sc()], yd, yacc[e - E0]);
+                constexpr int e = wl.ko[k] + sc();
+                yacc[e] = mf(av[sc()], yd, yacc[e]);
             });
         });
     }
@@ -332,10 +350,13 @@ __device__ __forceinline__ void tapply_wave(const ApplyArgs &a, TShared<LM> &sh,
     // the coupling tile over X and the tables (read for the last time before
     // the barrier)
     __syncthreads();
-    static_for<E0, E1>([&](auto ec) {
-        constexpr int e = ec(), m = lay.me[e], am = m < 0 ? -m : m, s = lay.se[e];
-        const int l = am + 2 * (4 * s + a4);
-        if (l <= lmax) sh.C[(l * (l + 1) / 2 + (m + l) / 2) * 16 + col] = yacc[e - E0];
+    static_for<0, NW>([&](auto kc) {
+        constexpr int k = kc(), mi = wl.mi[k], m = mi - LM, am = m < 0 ? -m : m;
+        static_for<0, wl.ko[k + 1] - wl.ko[k]>([&](auto sc) {
+            constexpr int s = sc();
+            const int l = am + 2 * (4 * s + a4);
+            if (l <= lmax) sh.C[(l * (l + 1) / 2 + (m + l) / 2) * 16 + col] = yacc[wl.ko[k] + s];
+        });
     });
 }
 
@@ -453,7 +474,7 @@ hipError_t launch_t(const ApplyArgs *a, const ApplyArgs *args, const int *off, i
 template <int LM>
 hipError_t launch_u(const gmd::Geometry &g, double *Ut, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_tapply_u<LM>), dim3((g.N + TT - 1) / TT), dim3(64), 0, st, g, Ut);
+    hipLaunchKernelGGL((k_tapply_u<LM>), dim3((g.N + TT - 1) / TT), dim3(64 * TW), 0, st, g, Ut);
     return hipGetLastError();
 }
 
