@@ -828,8 +828,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": mx["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": "CN step, l_max=20 (K=231), N=16384, L=4000, q=0.5, R=50, mu=0.15, dt=0.25, "
-                               "fixed orbit, absorbers on, mask; value = mixed precision (FP32 coupling in the "
-                               "Krylov applies), fp64 = plain FP64 drop-in", "K": K, "N": N,
+                               "fixed orbit, absorbers on, mask; value = mixed precision (FP32 preconditioner "
+                               "sweeps; the Krylov applies' coupling in FP32 in a Gaunt tile, in FP64 in the "
+                               "polar form the autotuner picks here), fp64 = plain FP64 drop-in", "K": K, "N": N,
                    "parallelism": "single GPU"},
         "roofline": mx["roofline"],
         "fp64": f64,

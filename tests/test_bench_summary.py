@@ -113,7 +113,5 @@ def test_sq_summary_and_the_roofline_reader(tmp_path, monkeypatch):
 
 def test_committed_profiles_feed_the_roofline():
     """The committed summaries hold the lines bench.py reads for the roofline."""
-    assert bench.measured_traffic("k_tapply_fp32") > 0
     assert bench.measured_traffic("k_tapply") > 0
-    assert bench.mfma_macs("k_tapply<20, double>") > 0
-    assert bench.mfma_macs("k_tapply<20, float>") > 0
+    assert bench.mfma_macs("k_tapply<20>") > 0
