@@ -610,6 +610,25 @@ __global__ __launch_bounds__(256) void k_diag_apply_batch(const ApplyArgs *__res
                              __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
+// Non-temporal loads and stores for the streaming passes (each element read
+// or written once per launch): they keep those streams from displacing what
+// the operator apply re-reads every launch (its U-sum table, the quadrature
+// tables, the pivots) from the caches. Measured at l_max 20, N 16384: the
+// apply 146 -> 138 us, k_update 84 -> 79 us, the sweeps 3-5 us faster each.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ntld(const double2 *p)
+{
+    const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void ntst(double2 *p, double2 v)
+{
+    nt_d2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<nt_d2 *>(p));
+}
+
 // ---------------------------------------------------------------- Thomas factor
 
 // Diagonal of mode j's CN matrix at node i: 1 + i dt/2 (H0_ii + V_jj(r_i) + CAP_i)
@@ -1173,7 +1192,7 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     // the thread's share of the projections v_a^H y over the sink supports
     double wp[4] = {0.0, 0.0, 0.0, 0.0};
     auto store_out = [&](const int idx, const double2 yv) __attribute__((always_inline)) {
-        Y[base_j + idx] = yv;
+        ntst(Y + base_j + idx, yv);
         if (a.Xt) store_phased(a.Xt, a.xt_f32, base_j + idx, phj, yv);
         if (idx >= s0lo && idx < s0hi) {
             const double2 d = cdotm(a.sk[0].v[(size_t)j * a.sk[0].n + (idx - s0lo)], yv);
@@ -1279,9 +1298,10 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const size_t q = base_j + min(w0 + tid + T * e, w1 - 1);
-                r[e] = Rin[q];
-                v[e] = a.V[q];
-                if (VAR == PV_P && !first) p[e] = Pin[q];
+                // (the pivots, read by every sweep, stay cacheable)
+                r[e] = ntld(Rin + q);
+                v[e] = ntld(a.V + q);
+                if (VAR == PV_P && !first) p[e] = ntld(Pin + q);
                 if constexpr (F32) iv[e] = a.inv32[q];
                 else iv[e] = a.inv[q];
             }
@@ -1294,7 +1314,7 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
                     double2 in;
                     if (VAR == PV_P) in = first ? rq : cadd(rq, cmul(beta, csub(p[e], cmul(wv, v[e]))));
                     else in = csub(rq, cmul(alpha, v[e]));
-                    if (idx >= g0 && idx < g1) Pout[base_j + idx] = in;
+                    if (idx >= g0 && idx < g1) ntst(Pout + base_j + idx, in);
                     fv = cmul(iv[e], Arith<RT>::from(in));
                 }
                 sf[lds_slot(pos)] = fv;
@@ -1437,11 +1457,11 @@ __device__ __forceinline__ void k_update_body(const UpdateArgs &a, Blk B)
     const double2 al = Kr->alpha, w = Kr->w;
     double v[3] = {0, 0, 0};
     for (size_t q = (size_t)B.id * blockDim.x + threadIdx.x; q < a.n; q += (size_t)B.n * blockDim.x) {
-        const double2 x = cadd(a.X[q], cadd(cmul(al, a.Yc[q]), cmul(w, a.Zc[q])));
-        a.X[q] = x;
-        const double2 r = csub(a.S[q], cmul(w, a.T[q]));
-        a.R[q] = r;
-        const double2 d = cdotm(a.R0[q], r);
+        const double2 x = cadd(ntld(a.X + q), cadd(cmul(al, ntld(a.Yc + q)), cmul(w, ntld(a.Zc + q))));
+        ntst(a.X + q, x);
+        const double2 r = csub(ntld(a.S + q), cmul(w, ntld(a.T + q)));
+        ntst(a.R + q, r);
+        const double2 d = cdotm(ntld(a.R0 + q), r);
         v[0] += d.x;
         v[1] += d.y;
         v[2] += cnorm2(r);
