@@ -164,6 +164,27 @@ __device__ inline double sum_partials(const double *partial, int q, Blk B)
 
 // ---------------------------------------------------------------- apply elements
 
+// Non-temporal loads and stores for the streaming passes (each element read
+// or written once per launch: the BiCGSTAB update, the sweeps' vectors, the
+// apply's shadow-residual reads and its outputs): they keep those streams
+// from displacing what the operator apply re-reads every launch (its U-sum
+// table, the quadrature tables, the pivots) from the caches. Measured at
+// l_max 20, N 16384: the apply 146 -> 131 us, k_update 84 -> 79 us, the
+// sweeps 3-5 us faster each.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ntld(const double2 *p)
+{
+    const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void ntst(double2 *p, double2 v)
+{
+    nt_d2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<nt_d2 *>(p));
+}
+
 __device__ __forceinline__ double2 load_x(const ApplyArgs &a, int k, int i)
 {
     return a.X[(size_t)k * a.g.N + i];
@@ -227,7 +248,7 @@ __device__ __forceinline__ ElemIn apply_element_load_r0(const ApplyArgs &a, int 
     ElemIn in;
     in.cap = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
     const bool uses_r0 = MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_RST;
-    in.r0 = uses_r0 ? a.R0[(size_t)j * g.N + i] : make_double2(0.0, 0.0);
+    in.r0 = uses_r0 ? ntld(a.R0 + (size_t)j * g.N + i) : make_double2(0.0, 0.0);
     return in;
 }
 
@@ -242,7 +263,7 @@ __device__ __forceinline__ ElemIn apply_element_load(const ApplyArgs &a, int j, 
     in.xp = a.X[row + (i < N - 1 ? 1 : 0)];
     in.cap = g.cap ? g.cap[i] : make_double2(0.0, 0.0);
     const bool uses_r0 = MODE == AM_V || MODE == AM_T || MODE == AM_RES || MODE == AM_RST;
-    in.r0 = uses_r0 ? a.R0[row] : make_double2(0.0, 0.0);
+    in.r0 = uses_r0 ? ntld(a.R0 + row) : make_double2(0.0, 0.0);
     return in;
 }
 
@@ -275,7 +296,7 @@ __device__ __forceinline__ void apply_element(const ApplyArgs &a, const ApplyEpi
         Hx = cmul(capv, xi);
     }
     if (MODE == AM_H) {
-        a.Y[row] = Hx;
+        ntst(a.Y + row, Hx);
     } else if (MODE == AM_RHS) {
         const double2 sHx = imul(ep.hdt, Hx);
         double2 st = make_double2(0.0, 0.0);
@@ -283,32 +304,32 @@ __device__ __forceinline__ void apply_element(const ApplyArgs &a, const ApplyEpi
         const double2 rhs = csub(csub(xi, sHx), st);
         const double2 ax = cadd(cadd(xi, sHx), st);
         const double2 r0 = make_double2(rhs.x - ax.x, rhs.y - ax.y);
-        a.Y[row] = rhs;
-        a.Y2[row] = r0;
+        ntst(a.Y + row, rhs);
+        ntst(a.Y2 + row, r0);
         red[0] += cnorm2(rhs);
         red[1] += cnorm2(r0);
     } else {
         double2 out = cadd(xi, imul(ep.hdt, Hx));
         if (ep.nsink) out = cadd(out, sink_term(a, ep.nsink, ep.coef0, ep.coef1, j, i));
         if (MODE == AM_V) {
-            a.Y[row] = out;
+            ntst(a.Y + row, out);
             const double2 d = cdotm(in.r0, out);
             red[0] += d.x;
             red[1] += d.y;
         } else if (MODE == AM_T) {
-            a.Y[row] = out;
+            ntst(a.Y + row, out);
             const double2 d = cdotm(out, in.r0);
             red[0] += d.x;
             red[1] += d.y;
             red[2] += cnorm2(out);
         } else if (MODE == AM_A) {
-            a.Y[row] = out;
+            ntst(a.Y + row, out);
         } else if (MODE == AM_RST) {
             // r = b - A x of the normalised system, r0' = |rhs| r
             const double2 bq = make_double2(in.r0.x / ep.rhs_nrm, in.r0.y / ep.rhs_nrm);
             const double2 rq = csub(bq, out);
-            a.Y[row] = rq;
-            a.Y2[row] = cscale(ep.rhs_nrm, rq);
+            ntst(a.Y + row, rq);
+            ntst(a.Y2 + row, cscale(ep.rhs_nrm, rq));
             red[0] += cnorm2(rq);
         } else {  // AM_RES
             double2 bq = in.r0;

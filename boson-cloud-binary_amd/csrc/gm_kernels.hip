@@ -610,25 +610,6 @@ __global__ __launch_bounds__(256) void k_diag_apply_batch(const ApplyArgs *__res
                              __builtin_amdgcn_readfirstlane(off[m + 1] - off[m])});
 }
 
-// Non-temporal loads and stores for the streaming passes (each element read
-// or written once per launch): they keep those streams from displacing what
-// the operator apply re-reads every launch (its U-sum table, the quadrature
-// tables, the pivots) from the caches. Measured at l_max 20, N 16384: the
-// apply 146 -> 138 us, k_update 84 -> 79 us, the sweeps 3-5 us faster each.
-typedef double nt_d2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ double2 ntld(const double2 *p)
-{
-    const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p));
-    return make_double2(v.x, v.y);
-}
-__device__ __forceinline__ void ntst(double2 *p, double2 v)
-{
-    nt_d2 w;
-    w.x = v.x;
-    w.y = v.y;
-    __builtin_nontemporal_store(w, reinterpret_cast<nt_d2 *>(p));
-}
-
 // ---------------------------------------------------------------- Thomas factor
 
 // Diagonal of mode j's CN matrix at node i: 1 + i dt/2 (H0_ii + V_jj(r_i) + CAP_i)
