@@ -11,7 +11,7 @@
 //               over segments of each mode's row, fused with the BiCGSTAB
 //               p / s updates and the Woodbury sink correction
 //               (include/RadialBlockPreconditioner.hpp:79-97, matrices.cpp:128-157)
-//   k_wcol      the same recurrences for the Woodbury columns P^-1 v_a
+//               (and the Woodbury columns P^-1 v_a)
 //   k_update    x += alpha y + w z, r = s - w t with fused dot products and the
 //               loop control of matrices.cpp:65-89 / Eigen BiCGSTAB
 //   k_sink      projections v_a^H x over the absorber supports (absorption.cpp:100-125)
@@ -910,187 +910,6 @@ __device__ __forceinline__ void store_phased(void *Xt, int f32, size_t q, double
     else ((double2 *)Xt)[q] = v;
 }
 
-// Woodbury columns W_a = P^-1 v_a (matrices.cpp:133-145): block col K + j
-// solves mode j of the column of sink col by the Thomas recurrence
-// (RadialBlockPreconditioner.hpp:78-97), y_i = a_i y_(i-1) + inv_i v_i and
-// x_i = a_i x_(i+1) + y_i with a_i = -l inv_i (zero on the boundary rows), as
-// block scans of affine maps over the row's tiles of PT x PE nodes with exact
-// carries; the inputs of the next tile are loaded while the current one is
-// scanned. It also widens the column's non-zero node range. The sink columns,
-// read-only to every other kernel, are the output.
-__device__ __forceinline__ void k_wcol_body(const PrecArgs &a, Blk B)
-{
-    typedef double2 C;
-    constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
-    __shared__ C sa[TE + TE / E];
-    __shared__ C sf[TE + TE / E];
-    const int col = B.id / a.K, j = B.id - col * a.K;
-    const SinkDev &sk = a.sk[col];
-    double2 *const Y = const_cast<double2 *>(sk.W);
-    int *const range = a.range + 2 * col;
-    const int N = a.N, tid = threadIdx.x;
-    const size_t base_j = (size_t)j * N;
-    const C ell = make_double2(0.0, a.Ps->half_dt * a.off);
-    const int ntile = (N + TE - 1) / TE;
-    auto coef = [&](const C iv) __attribute__((always_inline)) {
-        return C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)};
-    };
-    // forward: the sink vector v_col (zero outside its support) at clamped,
-    // always valid node indices
-    struct FIn {
-        double2 v;
-        C iv;
-    };
-    auto fload = [&](const int t, FIn (&b)[E]) __attribute__((always_inline)) {
-        const int tc = min(t, ntile - 1);
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int idx = min(tc * TE + tid + T * e, N - 1);
-            const int d = idx - sk.lo;
-            b[e].v = d >= 0 && d < sk.n ? sk.v[(size_t)j * sk.n + d] : make_double2(0.0, 0.0);
-            b[e].iv = a.inv[base_j + idx];
-        }
-    };
-    C carry = czero<C>();
-    auto fproc = [&](const int t, const FIn (&b)[E]) __attribute__((always_inline)) {
-        const int t0 = t * TE;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e, idx = t0 + pos;
-            C av = cone<C>(), fv = czero<C>();
-            if (idx < N) {
-                fv = cmul(b[e].iv, b[e].v);
-                av = (idx >= 1 && idx <= N - 2) ? coef(b[e].iv) : czero<C>();
-            }
-            sa[lds_slot(pos)] = av;
-            sf[lds_slot(pos)] = fv;
-        }
-        lds_barrier();
-        Aff<C> loc;
-        loc.A = cone<C>();
-        loc.F = czero<C>();
-        C ae[E], fe[E];
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            ae[e] = sa[lds_slot(tid * E + e)];
-            fe[e] = sf[lds_slot(tid * E + e)];
-            loc.A = cmul(ae[e], loc.A);
-            loc.F = cadd(cmul(ae[e], loc.F), fe[e]);
-        }
-        Aff<C> tot;
-        const Aff<C> ex = block_scan<false>(loc, tot);
-        C y = cadd(cmul(ex.A, carry), ex.F);
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            y = cadd(cmul(ae[e], y), fe[e]);
-            sf[lds_slot(tid * E + e)] = y;
-        }
-        carry = cadd(cmul(tot.A, carry), tot.F);
-        lds_barrier();
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e, idx = t0 + pos;
-            if (idx < N) Y[base_j + idx] = sf[lds_slot(pos)];
-        }
-        lds_barrier();
-    };
-    {
-        FIn fa[E], fb[E];
-        fload(0, fa);
-        for (int t = 0; t < ntile; t += 2) {
-            fload(t + 1, fb);
-            fproc(t, fa);
-            fload(t + 2, fa);
-            if (t + 1 < ntile) fproc(t + 1, fb);
-        }
-    }
-    // backward; the forward results written by the other threads of the block
-    // are read back after this barrier (with its global fence)
-    __syncthreads();
-    struct BIn {
-        C y, iv;
-    };
-    auto bload = [&](const int t, BIn (&b)[E]) __attribute__((always_inline)) {
-        const int tc = max(t, 0);
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const size_t q = base_j + min(tc * TE + tid + T * e, N - 1);
-            b[e].y = Y[q];
-            b[e].iv = a.inv[q];
-        }
-    };
-    int nz_lo = N, nz_hi = 0;
-    carry = czero<C>();
-    auto bproc = [&](const int t, const BIn (&b)[E]) __attribute__((always_inline)) {
-        const int t0 = t * TE;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e, idx = t0 + pos;
-            C cv = cone<C>(), fv = czero<C>();
-            if (idx < N) {
-                fv = b[e].y;
-                cv = (idx >= 1 && idx <= N - 2) ? coef(b[e].iv) : czero<C>();
-            }
-            sa[lds_slot(pos)] = cv;
-            sf[lds_slot(pos)] = fv;
-        }
-        lds_barrier();
-        Aff<C> loc;
-        loc.A = cone<C>();
-        loc.F = czero<C>();
-        C ce[E], fe[E];
-#pragma unroll
-        for (int e = E - 1; e >= 0; --e) {
-            ce[e] = sa[lds_slot(tid * E + e)];
-            fe[e] = sf[lds_slot(tid * E + e)];
-            loc.A = cmul(ce[e], loc.A);
-            loc.F = cadd(cmul(ce[e], loc.F), fe[e]);
-        }
-        Aff<C> tot;
-        const Aff<C> ex = block_scan<true>(loc, tot);
-        C y = cadd(cmul(ex.A, carry), ex.F);
-#pragma unroll
-        for (int e = E - 1; e >= 0; --e) {
-            y = cadd(cmul(ce[e], y), fe[e]);
-            sf[lds_slot(tid * E + e)] = y;
-        }
-        carry = cadd(cmul(tot.A, carry), tot.F);
-        lds_barrier();
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int pos = tid + T * e, idx = t0 + pos;
-            if (idx < N) {
-                const double2 yv = sf[lds_slot(pos)];
-                Y[base_j + idx] = yv;
-                if (yv.x != 0.0 || yv.y != 0.0) {
-                    nz_lo = min(nz_lo, idx);
-                    nz_hi = max(nz_hi, idx + 1);
-                }
-            }
-        }
-        lds_barrier();
-    };
-    {
-        BIn ba[E], bb[E];
-        bload(ntile - 1, ba);
-        for (int t = ntile - 1; t >= 0; t -= 2) {
-            bload(t - 1, bb);
-            bproc(t, ba);
-            bload(t - 2, ba);
-            if (t - 1 >= 0) bproc(t - 1, bb);
-        }
-    }
-    // exact non-zero node range of the solution over all modes
-    for (int o = 32; o > 0; o >>= 1) {
-        nz_lo = min(nz_lo, __shfl_xor(nz_lo, o));
-        nz_hi = max(nz_hi, __shfl_xor(nz_hi, o));
-    }
-    if ((tid & 63) == 0) {
-        atomicMin(range, nz_lo);
-        atomicMax(range + 1, nz_hi);
-    }
-}
-__global__ __launch_bounds__(gmd::PT) void k_wcol(PrecArgs a) { k_wcol_body(a, grid_blk()); }
 
 // The p / s sweeps of the preconditioner: p = r + beta (p - w v) (PV_P) or
 // s = r - alpha v (PV_S), then y = P^-1 p or z = P^-1 s by the Thomas
@@ -1112,14 +931,20 @@ __global__ __launch_bounds__(gmd::PT) void k_wcol(PrecArgs a) { k_wcol_body(a, g
 template <int VAR, typename RT>
 __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
 {
-    static_assert(VAR == PV_P || VAR == PV_S, "p or s sweep");
+    static_assert(VAR == PV_P || VAR == PV_S || (VAR == PV_WCOL && std::is_same<RT, double>::value),
+                  "p or s sweep, or the FP64 Woodbury columns");
+    constexpr bool WCOL = VAR == PV_WCOL;
     typedef typename Arith<RT>::C C;
     constexpr bool F32 = std::is_same<RT, float>::value;
     constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
     Krylov *Kr = a.Kr;
-    if (Kr->done) return;
+    if (!WCOL && Kr->done) return;
     const int nseg = a.nseg;
     const int N = a.N, tid = threadIdx.x;
+    // PV_WCOL: the launch is ncol column grids of K nseg blocks; column col
+    // solves P^-1 v_col into the sink's Woodbury column
+    const int col = WCOL ? B.id / (a.K * nseg) : 0;
+    const int bid = B.id - col * a.K * nseg;
     // Woodbury (a.wood): the segments holding sink-support nodes, and segment
     // 0 of every mode (a mode solved whole stores its row there), store
     // partial projections v_a^H y for the epilogue. Their blocks come first
@@ -1136,8 +961,8 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     for (int gg = 0; gg < nseg; ++gg) count += holds(gg);
     int j, g = 0, rank;
     {
-        const bool holder = B.id < a.K * count;
-        const int per = holder ? count : nseg - count, b = holder ? B.id : B.id - a.K * count;
+        const bool holder = bid < a.K * count;
+        const int per = holder ? count : nseg - count, b = holder ? bid : bid - a.K * count;
         j = b / per;
         rank = b - j * per;
         // the rank-th segment (in order) that holds / does not hold
@@ -1146,11 +971,13 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     }
     const bool whole = nseg > 1 && a.pseq[j];
     const size_t base_j = (size_t)j * N;
-    double2 *const Y = a.Y;
+    const SinkDev &skc = a.sk[col];
+    // (the sink columns, read-only to every other kernel, are PV_WCOL's output)
+    double2 *const Y = WCOL ? const_cast<double2 *>(skc.W) : a.Y;
     const C ell = Arith<RT>::from(make_double2(0.0, a.Ps->half_dt * a.off));
     // the first iteration reads r = r0' / |rhs| from the unnormalised shadow
     // residual and takes p = r
-    const bool it0 = Kr->fresh != 0;
+    const bool it0 = !WCOL && Kr->fresh != 0;
     const bool first = VAR == PV_P && it0;
     const double nrm0 = it0 ? Kr->rhs_nrm : 1.0;
     const double2 *Rin = it0 ? a.R0 : a.R;
@@ -1172,8 +999,13 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     const double2 phj = a.Xt ? a.Ps->ph[mode_m(j) + a.lmax] : make_double2(1.0, 0.0);
     // the thread's share of the projections v_a^H y over the sink supports
     double wp[4] = {0.0, 0.0, 0.0, 0.0};
+    int nz_lo = N, nz_hi = 0;  // PV_WCOL: the column's non-zero node range
     auto store_out = [&](const int idx, const double2 yv) __attribute__((always_inline)) {
         ntst(Y + base_j + idx, yv);
+        if (WCOL && (yv.x != 0.0 || yv.y != 0.0)) {
+            nz_lo = min(nz_lo, idx);
+            nz_hi = max(nz_hi, idx + 1);
+        }
         if (a.Xt) store_phased(a.Xt, a.xt_f32, base_j + idx, phj, yv);
         if (idx >= s0lo && idx < s0hi) {
             const double2 d = cdotm(a.sk[0].v[(size_t)j * a.sk[0].n + (idx - s0lo)], yv);
@@ -1189,19 +1021,36 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     auto coef = [&](const C iv) __attribute__((always_inline)) {
         return C{-(ell.x * iv.x - ell.y * iv.y), -(ell.x * iv.y + ell.y * iv.x)};
     };
+    // PV_WCOL's right-hand side: the sink vector v_col, zero outside its support
+    auto vcol = [&](const int idx) __attribute__((always_inline)) {
+        const int d = idx - skc.lo;
+        return d >= 0 && d < skc.n ? skc.v[(size_t)j * skc.n + d] : make_double2(0.0, 0.0);
+    };
+    // the p / s update (PV_WCOL: the column's right-hand side as it is)
+    auto update = [&](const double2 r, const double2 v, const double2 p) __attribute__((always_inline)) {
+        if constexpr (WCOL) return r;
+        const double2 rq = it0 ? make_double2(r.x / nrm0, r.y / nrm0) : r;
+        if constexpr (VAR == PV_P) return first ? rq : cadd(rq, cmul(beta, csub(p, cmul(wv, v))));
+        return csub(rq, cmul(alpha, v));
+    };
     // Forward over [w0, w1) (positions tid E + e): in_i, its store on [p0, p1),
     // a_i and f_i = inv_i in_i; a_w0 = 0 when drop (the carry into the window
     // is dropped), and the identity past w1. Returns y in fe, a in ae.
     auto forward = [&](const int w0, const int w1, const int p0, const int p1, const bool drop, C &carry, C (&ae)[E],
                        C (&fe)[E]) __attribute__((always_inline)) {
-        double2 r[E], v[E], p[E];
+        double2 r[E] = {}, v[E] = {}, p[E] = {};
         C iv[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const size_t q = base_j + min(w0 + tid * E + e, w1 - 1);
-            r[e] = Rin[q];
-            v[e] = a.V[q];
-            if (VAR == PV_P && !first) p[e] = Pin[q];
+            const int idx = min(w0 + tid * E + e, w1 - 1);
+            const size_t q = base_j + idx;
+            if constexpr (WCOL) {
+                r[e] = vcol(idx);
+            } else {
+                r[e] = Rin[q];
+                v[e] = a.V[q];
+                if (VAR == PV_P && !first) p[e] = Pin[q];
+            }
             if constexpr (F32) iv[e] = a.inv32[q];
             else iv[e] = a.inv[q];
         }
@@ -1214,11 +1063,8 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
             ae[e] = cone<C>();
             fe[e] = czero<C>();
             if (idx < w1) {
-                const double2 rq = it0 ? make_double2(r[e].x / nrm0, r[e].y / nrm0) : r[e];
-                double2 in;
-                if (VAR == PV_P) in = first ? rq : cadd(rq, cmul(beta, csub(p[e], cmul(wv, v[e]))));
-                else in = csub(rq, cmul(alpha, v[e]));
-                if (idx >= p0 && idx < p1) Pout[base_j + idx] = in;
+                const double2 in = update(r[e], v[e], p[e]);
+                if (!WCOL && idx >= p0 && idx < p1) Pout[base_j + idx] = in;
                 fe[e] = cmul(iv[e], Arith<RT>::from(in));
                 ae[e] = (idx >= 1 && idx <= N - 2 && !(drop && idx == w0)) ? coef(iv[e]) : czero<C>();
             }
@@ -1266,23 +1112,29 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     };
     C ae[E], fe[E];
     if (!whole) {
-        // The window's r, v, p and pivots are loaded coalesced (position
-        // tid + T e), f = inv in goes through LDS to the thread's consecutive
+        // The window's r, v, p (PV_WCOL: v_col) and pivots are loaded
+        // coalesced (position tid + T e), f = inv in goes through LDS to the
+        // thread's consecutive
         // positions, where the thread loads its pivots once more for a; the
         // output returns through LDS to coalesced stores.
         __shared__ C sf[TE + TE / E];
         const int g0 = g * gmd::PSEG, g1 = nseg == 1 ? N : min(N, g0 + gmd::PSEG);
         const int w0 = max(0, g0 - gmd::PHALO), w1 = nseg == 1 ? N : min(N, g1 + gmd::PHALO);
         {
-            double2 r[E], v[E], p[E];
+            double2 r[E] = {}, v[E] = {}, p[E] = {};
             C iv[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const size_t q = base_j + min(w0 + tid + T * e, w1 - 1);
+                const int idx = min(w0 + tid + T * e, w1 - 1);
+                const size_t q = base_j + idx;
                 // (the pivots, read by every sweep, stay cacheable)
-                r[e] = ntld(Rin + q);
-                v[e] = ntld(a.V + q);
-                if (VAR == PV_P && !first) p[e] = ntld(Pin + q);
+                if constexpr (WCOL) {
+                    r[e] = vcol(idx);
+                } else {
+                    r[e] = ntld(Rin + q);
+                    v[e] = ntld(a.V + q);
+                    if (VAR == PV_P && !first) p[e] = ntld(Pin + q);
+                }
                 if constexpr (F32) iv[e] = a.inv32[q];
                 else iv[e] = a.inv[q];
             }
@@ -1291,11 +1143,8 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
                 const int pos = tid + T * e, idx = w0 + pos;
                 C fv = czero<C>();
                 if (idx < w1) {
-                    const double2 rq = it0 ? make_double2(r[e].x / nrm0, r[e].y / nrm0) : r[e];
-                    double2 in;
-                    if (VAR == PV_P) in = first ? rq : cadd(rq, cmul(beta, csub(p[e], cmul(wv, v[e]))));
-                    else in = csub(rq, cmul(alpha, v[e]));
-                    if (idx >= g0 && idx < g1) ntst(Pout + base_j + idx, in);
+                    const double2 in = update(r[e], v[e], p[e]);
+                    if (!WCOL && idx >= g0 && idx < g1) ntst(Pout + base_j + idx, in);
                     fv = cmul(iv[e], Arith<RT>::from(in));
                 }
                 sf[lds_slot(pos)] = fv;
@@ -1376,6 +1225,31 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
                 if (idx < w1) store_out(idx, Arith<RT>::to_d(fe[e]));
             }
         }
+    }
+    if constexpr (WCOL) {
+        // the column's exact non-zero node range over all modes: one atomic
+        // pair per block that stored a non-zero (far segments store zeros)
+        __shared__ int nzw[T / 64][2];
+        for (int o = 32; o > 0; o >>= 1) {
+            nz_lo = min(nz_lo, __shfl_xor(nz_lo, o));
+            nz_hi = max(nz_hi, __shfl_xor(nz_hi, o));
+        }
+        if ((tid & 63) == 0) {
+            nzw[tid >> 6][0] = nz_lo;
+            nzw[tid >> 6][1] = nz_hi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < T / 64; ++w) {
+                nz_lo = min(nz_lo, nzw[w][0]);
+                nz_hi = max(nz_hi, nzw[w][1]);
+            }
+            if (nz_lo < nz_hi) {
+                atomicMin(a.range + 2 * col, nz_lo);
+                atomicMax(a.range + 2 * col + 1, nz_hi);
+            }
+        }
+        return;
     }
     // Woodbury: the holding blocks (the first K count of the grid) store
     // their partial projections v_a^H y in a fixed order; the correction
@@ -2076,13 +1950,13 @@ hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st)
 {
     const bool f32 = a.inv32 != nullptr && a.Y32 != nullptr && var != PV_WCOL;
-    if ((var == PV_P || var == PV_S) && a.nseg != gmd::precond_nseg(a.N)) return hipErrorInvalidValue;
+    if (a.nseg != gmd::precond_nseg(a.N)) return hipErrorInvalidValue;
     const dim3 g(precond_blocks(a.K, a.N));
     if (var == PV_P && f32) hipLaunchKernelGGL((k_psweep<PV_P, float>), g, dim3(gmd::PT), 0, st, a);
     else if (var == PV_S && f32) hipLaunchKernelGGL((k_psweep<PV_S, float>), g, dim3(gmd::PT), 0, st, a);
     else if (var == PV_P) hipLaunchKernelGGL((k_psweep<PV_P, double>), g, dim3(gmd::PT), 0, st, a);
     else if (var == PV_S) hipLaunchKernelGGL((k_psweep<PV_S, double>), g, dim3(gmd::PT), 0, st, a);
-    else hipLaunchKernelGGL(k_wcol, dim3(a.K * a.ncol), dim3(gmd::PT), 0, st, a);
+    else hipLaunchKernelGGL((k_psweep<PV_WCOL, double>), dim3(precond_blocks(a.K, a.N) * a.ncol), dim3(gmd::PT), 0, st, a);
     return hipGetLastError();
 }
 
