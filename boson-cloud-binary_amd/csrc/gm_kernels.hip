@@ -692,18 +692,173 @@ __device__ __forceinline__ Mob shfl_mob(const Mob &v, int src)
     return r;
 }
 
+// Block exclusive scan of Moebius maps in increasing thread order; returns
+// the thread's prefix and leaves the block total in `tot`.
+__device__ __forceinline__ Mob mob_block_scan(const Mob &loc, Mob &tot)
+{
+    constexpr int T = gmd::PT;
+    __shared__ Mob wt[T / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    Mob inc = loc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const Mob o = shfl_mob(inc, lane >= d ? lane - d : 0);
+        if (lane >= d) inc = mob_compose(o, inc);
+    }
+    Mob ex = shfl_mob(inc, lane > 0 ? lane - 1 : 0);
+    if (lane == 0) {
+        ex.a = make_double2(1.0, 0.0);
+        ex.b = make_double2(0.0, 0.0);
+        ex.c = make_double2(0.0, 0.0);
+        ex.d = make_double2(1.0, 0.0);
+    }
+    if (lane == 63) wt[wv] = inc;
+    __syncthreads();
+    Mob pre;
+    pre.a = make_double2(1.0, 0.0);
+    pre.b = make_double2(0.0, 0.0);
+    pre.c = make_double2(0.0, 0.0);
+    pre.d = make_double2(1.0, 0.0);
+    tot = pre;
+    for (int w = 0; w < T / 64; ++w) {
+        if (w == wv) pre = tot;
+        tot = mob_compose(tot, wt[w]);
+    }
+    __syncthreads();
+    return mob_compose(pre, ex);
+}
+
 // Thomas pivots of every mode (include/RadialBlockPreconditioner.hpp:43-76):
 // pivot_i = d_i - lower_i inv_{i-1} upper_{i-1}, with lower = upper = l (the
 // stencil entry of A) on interior rows, so pivot_i = d_i - l^2 / pivot_{i-1}
 // for 2 <= i <= N-2 and pivot_i = d_i otherwise. The recurrence is a Moebius
-// map; one workgroup per mode scans the maps of PE consecutive nodes per
-// thread (2x2 products, rescaled), then every thread replays its nodes with
-// the reference's arithmetic from its exact incoming pivot.
+// map: the workgroup scans the maps of PE consecutive nodes per thread (2x2
+// products, rescaled), then every thread replays its nodes with the
+// reference's arithmetic from its incoming pivot.
+//
+// k_factor_win: block j nseg + g computes the pivots of segment g of mode j
+// over the window of the preconditioner sweeps (the segment and PHALO nodes
+// on either side) with the pivot at the window's first node reset to its
+// diagonal. A pivot's error is multiplied by |l / pivot|^2 per node, so over
+// the left halo by the square of the product that bounds the sweeps' dropped
+// carries. From the window's pivots on [b - PHALO + 1, b + PHALO - 1] (the
+// segment's own and its right halo's, which follow from the segment with an
+// exact carry) the block also checks boundary b = g1 for the sweeps: flag
+// pseq[j nseg + g] unless both halo products are below 2^-56.
+__global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
+{
+    constexpr int T = gmd::PT, E = gmd::PE;
+    const int N = f.N, nseg = gmd::precond_nseg(N);
+    const int j = blockIdx.x / nseg, g = blockIdx.x - j * nseg;
+    const int tid = threadIdx.x;
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= j) ++l;
+    const double *Gjj = f.Gdiag + (size_t)j * (f.lmax + 1);
+    const double2 ell = make_double2(0.0, f.P->half_dt * f.off);
+    const double2 ell2 = cmul(ell, ell);
+    const double2 nl2 = make_double2(-ell2.x, -ell2.y);
+    const int g0 = g * gmd::PSEG, g1 = nseg == 1 ? N : min(N, g0 + gmd::PSEG);
+    const int w0 = max(0, g0 - gmd::PHALO), w1 = nseg == 1 ? N : min(N, g1 + gmd::PHALO);
+    const int i0 = w0 + tid * E;
+    // interior rows 2 .. N - 2 apply the recurrence; the window's first node
+    // and the boundary rows reset the pivot to the diagonal
+    auto interior = [&](int i) { return i >= 2 && i <= N - 2 && i != w0; };
+    double2 dv[E];
+    Mob loc;
+    loc.a = make_double2(1.0, 0.0);
+    loc.b = make_double2(0.0, 0.0);
+    loc.c = make_double2(0.0, 0.0);
+    loc.d = make_double2(1.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = i0 + e;
+        if (i >= w1) {
+            dv[e] = make_double2(1.0, 0.0);
+        } else {
+            dv[e] = cn_diag(f, Gjj, l, i);
+            loc = interior(i) ? mob_interior(loc, dv[e], nl2) : mob_reset(loc, dv[e]);
+        }
+    }
+    mob_rescale(loc);
+    Mob tot;
+    const Mob P = mob_block_scan(loc, tot);
+    // incoming pivot: P applied to (1, 1) (the window starts with a reset, so
+    // the value entering it is arbitrary)
+    const double2 u0 = cadd(P.a, P.b), v0 = cadd(P.c, P.d);
+    double2 prev_inv = cdiv(make_double2(1.0, 0.0), cdiv(u0, v0));
+    int failed = 0;
+    double2 iv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = i0 + e;
+        iv[e] = make_double2(0.0, 0.0);
+        if (i >= w1) continue;
+        double2 piv = dv[e];
+        if (interior(i)) piv = csub(piv, cmul(cmul(ell, prev_inv), ell));
+        const double ap = sqrt(cnorm2(piv));
+        if (ap == 0.0 || !isfinite(ap)) {
+            failed = 1;
+            iv[e] = make_double2(0.0, 0.0);
+        } else {
+            iv[e] = cdiv(make_double2(1.0, 0.0), piv);
+        }
+        if (i >= g0 && i < g1) {
+            f.inv[(size_t)j * N + i] = iv[e];
+            if (f.inv32) f.inv32[(size_t)j * N + i] = make_float2((float)iv[e].x, (float)iv[e].y);
+        }
+        prev_inv = iv[e];
+    }
+    if (failed) atomicOr(f.failed, 1);
+    if (!f.pseq || nseg == 1) return;
+    // the halo check of boundary b = g1 (none after the last segment)
+    const int b = g1;
+    const double lell = 0.5 * log(cnorm2(ell)), lim = -56.0 * 0.6931471805599453;
+    double sl = 0.0, sr = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = i0 + e;
+        if (i >= w1) continue;
+        // a_i = 0 on the boundary rows (the product is then zero)
+        const double t = (i >= 1 && i <= N - 2) ? lell + 0.5 * log(cnorm2(iv[e])) : -INFINITY;
+        if (i >= b - gmd::PHALO + 1 && i <= b) sl += t;
+        if (i >= b && i <= b + gmd::PHALO - 1) sr += t;
+    }
+    __shared__ double hs[T / 64][2];
+    sl = wave_sum(sl);
+    sr = wave_sum(sr);
+    if ((tid & 63) == 0) {
+        hs[tid >> 6][0] = sl;
+        hs[tid >> 6][1] = sr;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sl = sr = 0.0;
+        for (int w = 0; w < T / 64; ++w) {
+            sl += hs[w][0];
+            sr += hs[w][1];
+        }
+        // a right halo that reaches node N - 1 drops nothing
+        const bool bad = g < nseg - 1 && (!(sl < lim) || (b + gmd::PHALO < N && !(sr < lim)));
+        f.pseq[j * nseg + g] = bad;
+    }
+}
+
+// k_factor: one workgroup per mode over the whole row, its tiles in order
+// with exact carries, for the modes whose segments k_factor_win flagged (or
+// every mode when given no segment flags); it sets all of the mode's flags to
+// its own check.
 __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
 {
     constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
-    __shared__ Mob wt[T / 64];
     const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nseg = gmd::precond_nseg(f.N);
+    if (f.pseq) {
+        // k_factor_win's pivots stand unless a boundary of this mode failed
+        // its check (one window per row is exact)
+        int any = 0;
+        for (int g = 0; g + 1 < nseg; ++g) any |= f.pseq[j * nseg + g];
+        if (!any) return;
+    }
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= j) ++l;
     const int mi = j - l * (l + 1) / 2;
@@ -738,33 +893,8 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
         }
         mob_rescale(loc);
         // block exclusive scan of the thread maps (increasing node order)
-        Mob inc = loc;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const Mob o = shfl_mob(inc, lane >= d ? lane - d : 0);
-            if (lane >= d) inc = mob_compose(o, inc);
-        }
-        Mob ex = shfl_mob(inc, lane > 0 ? lane - 1 : 0);
-        if (lane == 0) {
-            ex.a = make_double2(1.0, 0.0);
-            ex.b = make_double2(0.0, 0.0);
-            ex.c = make_double2(0.0, 0.0);
-            ex.d = make_double2(1.0, 0.0);
-        }
-        if (lane == 63) wt[wv] = inc;
-        __syncthreads();
-        Mob pre, tot;
-        pre.a = make_double2(1.0, 0.0);
-        pre.b = make_double2(0.0, 0.0);
-        pre.c = make_double2(0.0, 0.0);
-        pre.d = make_double2(1.0, 0.0);
-        tot = pre;
-        for (int w = 0; w < T / 64; ++w) {
-            if (w == wv) pre = tot;
-            tot = mob_compose(tot, wt[w]);
-        }
-        __syncthreads();
-        const Mob P = mob_compose(pre, ex);
+        Mob tot;
+        const Mob P = mob_block_scan(loc, tot);
         // incoming pivot of this thread: P applied to (carry, 1)
         const double2 u = cadd(cmul(P.a, carry), P.b), v = cadd(cmul(P.c, carry), P.d);
         double2 prev_inv = cdiv(make_double2(1.0, 0.0), cdiv(u, v));
@@ -800,7 +930,6 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
     // [b - PHALO + 1, b], the backward carry into segment s by the product
     // over its right halo, [b, b + PHALO - 1], b = (s + 1) PSEG. The mode is
     // solved whole unless every such product is below 2^-56.
-    const int nseg = gmd::precond_nseg(N);
     __shared__ int bad;
     if (tid == 0) bad = 0;
     __syncthreads();  // (also makes this block's pivots visible to its threads)
@@ -821,7 +950,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
         if (lane == 0 && !(sum < lim)) atomicOr(&bad, 1);
     }
     __syncthreads();
-    if (tid == 0) f.pseq[j] = bad;
+    if (tid < nseg) f.pseq[j * nseg + tid] = tid + 1 < nseg ? bad : 0;
 }
 
 // ---------------------------------------------------------------- Thomas solve
@@ -969,7 +1098,10 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
         for (int gg = 0, k = 0; gg < nseg; ++gg)
             if (holds(gg) == holder && k++ == rank) g = gg;
     }
-    const bool whole = nseg > 1 && a.pseq[j];
+    // a mode is solved whole when any of its segment boundaries failed the
+    // halo check (k_factor_win, k_factor)
+    bool whole = false;
+    for (int gg = 0; gg + 1 < nseg; ++gg) whole |= a.pseq[j * nseg + gg] != 0;
     const size_t base_j = (size_t)j * N;
     const SinkDev &skc = a.sk[col];
     // (the sink columns, read-only to every other kernel, are PV_WCOL's output)
@@ -1943,6 +2075,9 @@ hipError_t launch_lcut(int N, int Ls, double tau, const double *E, int *lcut, hi
 
 hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 {
+    // the segments' pivots and checks, then the whole-row factor of the
+    // flagged modes (without segment flags: every mode whole)
+    if (f.pseq) hipLaunchKernelGGL(k_factor_win, dim3(f.K * gmd::precond_nseg(f.N)), dim3(gmd::PT), 0, st, f);
     hipLaunchKernelGGL(k_factor, dim3(f.K), dim3(gmd::PT), 0, st, f);
     return hipGetLastError();
 }
