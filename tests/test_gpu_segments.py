@@ -104,3 +104,14 @@ def test_whole_row_fallback_for_slow_decay():
                 counter=False, absorption=True, save_every=1000, ramp_periods=0.0, epsilon_smear=2.0)
     assert kinetic_ratio(args["dt"], args["mu"], dr) ** PHALO > 2.0 ** -40
     step_against_oracle(args, 3, seed=12)
+
+
+@pytest.mark.parametrize("Nr", [2049, 3330])
+def test_short_last_segment(Nr):
+    """A last segment shorter than a halo (2049 nodes: segments of 1664 and
+    385) and one that ends exactly on a segment size (3330 = 2 x 1664 + 2)."""
+    print(f"short last segment, Nr={Nr}", flush=True)
+    dr = 0.3
+    args = dict(Nr=Nr, L=(Nr - 1) * dr, dt=0.5, epsilon=0.0, l_max=3, mu=0.2, M_c=0.01, q=0.5, a=0.0, R=30.0,
+                counter=False, absorption=True, save_every=1000, ramp_periods=0.0, epsilon_smear=2.0)
+    step_against_oracle(args, 3, seed=13 + Nr)
