@@ -38,7 +38,8 @@ enum {
 
 enum {
     GM_PRECISION_FP64 = 0,  /* plain FP64 drop-in */
-    GM_PRECISION_MIXED = 1  /* FP32 coupling / preconditioner inside FP64 BiCGSTAB; a step
+    GM_PRECISION_MIXED = 1  /* FP32 preconditioner sweeps (and FP32 coupling in the Gaunt
+                               form; the polar form is FP64) inside FP64 BiCGSTAB; a step
                                that fails the FP64 residual check is solved again in FP64 */
 };
 
