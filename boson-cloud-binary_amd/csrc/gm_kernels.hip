@@ -1051,12 +1051,14 @@ __device__ __forceinline__ void store_phased(void *Xt, int f32, size_t q, double
 // side, is loaded once (each thread PE consecutive nodes), both recurrences
 // run as block scans with the carries into the window's ends dropped, and the
 // segment's nodes are stored. A dropped carry is multiplied over a halo by
-// the product of |a_i|, which k_factor checks to stay below 2^-56 (then the
-// segment's output is the whole-row solve's to rounding). A mode that fails
-// the check is solved whole by its segment-0 block: the same recurrences over
-// the row's tiles in order, with exact carries, the forward result kept in
-// the output vector. RT = float sweeps in FP32 on the FP32 pivots; the p / s
-// update and the output stay FP64.
+// the product of |a_i|, which k_factor_win checks to stay below 2^-56 at
+// every segment boundary (then the segment's output is the whole-row solve's
+// to rounding). A mode that fails the check is solved whole by its segment-0
+// block: the same recurrences over the row's tiles in order, with exact
+// carries, the forward result kept in the output vector. RT = float sweeps
+// in FP32 on the FP32 pivots; the p / s update and the output stay FP64.
+// PV_WCOL solves the Woodbury columns P^-1 v_a the same way, with the compact
+// sink vector as the right-hand side.
 template <int VAR, typename RT>
 __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
 {
@@ -1076,9 +1078,8 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
     const int bid = B.id - col * a.K * nseg;
     // Woodbury (a.wood): the segments holding sink-support nodes, and segment
     // 0 of every mode (a mode solved whole stores its row there), store
-    // partial projections v_a^H y for the epilogue. Their blocks come first
-    // in the grid (j = b / count), so that the last of them, which sums the
-    // partials, finishes while the other segments are still being solved.
+    // partial projections v_a^H y for k_wcoef. Their blocks come first in the
+    // grid (j = b / count); the partial of holding block b is slot b.
     const int ns_w = a.wood ? a.Ps->nsink : 0;
     const int s0lo = ns_w > 0 ? a.sk[0].lo : 0, s0hi = ns_w > 0 ? a.sk[0].lo + a.sk[0].n : 0;
     const int s1lo = ns_w > 1 ? a.sk[1].lo : 0, s1hi = ns_w > 1 ? a.sk[1].lo + a.sk[1].n : 0;
