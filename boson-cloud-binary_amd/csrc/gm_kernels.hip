@@ -846,7 +846,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
 // k_factor: one workgroup per mode over the whole row, its tiles in order
 // with exact carries, for the modes whose segments k_factor_win flagged (or
 // every mode when given no segment flags); it sets all of the mode's flags to
-// its own check.
+// its own check, and the mode's verdict pseq[K nseg + j] that the sweeps read.
 __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
 {
     constexpr int T = gmd::PT, E = gmd::PE, TE = T * E;
@@ -857,7 +857,10 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
         // its check (one window per row is exact)
         int any = 0;
         for (int g = 0; g + 1 < nseg; ++g) any |= f.pseq[j * nseg + g];
-        if (!any) return;
+        if (!any) {
+            if (tid == 0) f.pseq[f.K * nseg + j] = 0;
+            return;
+        }
     }
     int l = 0;
     while ((l + 1) * (l + 2) / 2 <= j) ++l;
@@ -951,6 +954,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
     }
     __syncthreads();
     if (tid < nseg) f.pseq[j * nseg + tid] = tid + 1 < nseg ? bad : 0;
+    if (tid == 0) f.pseq[f.K * nseg + j] = bad;
 }
 
 // ---------------------------------------------------------------- Thomas solve
@@ -1100,9 +1104,9 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
             if (holds(gg) == holder && k++ == rank) g = gg;
     }
     // a mode is solved whole when any of its segment boundaries failed the
-    // halo check (k_factor_win, k_factor)
-    bool whole = false;
-    for (int gg = 0; gg + 1 < nseg; ++gg) whole |= a.pseq[j * nseg + gg] != 0;
+    // halo check (k_factor_win; k_factor stores the mode's verdict after the
+    // boundary flags, one load here)
+    const bool whole = nseg > 1 && a.pseq[a.K * nseg + j] != 0;
     const size_t base_j = (size_t)j * N;
     const SinkDev &skc = a.sk[col];
     // (the sink columns, read-only to every other kernel, are PV_WCOL's output)
@@ -1244,33 +1248,35 @@ __device__ __forceinline__ void k_psweep_body(const PrecArgs &a, Blk B)
         carry = cadd(cmul(tot.A, carry), tot.F);
     };
     C ae[E], fe[E];
-    if (!whole) {
-        // The window's r, v, p (PV_WCOL: v_col) and pivots are loaded
-        // coalesced (position tid + T e), f = inv in goes through LDS to the
-        // thread's consecutive
-        // positions, where the thread loads its pivots once more for a; the
-        // output returns through LDS to coalesced stores.
-        __shared__ C sf[TE + TE / E];
-        const int g0 = g * gmd::PSEG, g1 = nseg == 1 ? N : min(N, g0 + gmd::PSEG);
-        const int w0 = max(0, g0 - gmd::PHALO), w1 = nseg == 1 ? N : min(N, g1 + gmd::PHALO);
-        {
-            double2 r[E] = {}, v[E] = {}, p[E] = {};
-            C iv[E];
+    // The window's r, v, p (PV_WCOL: v_col) and pivots are loaded coalesced
+    // (position tid + T e), before the mode's verdict is known (the indices
+    // are valid either way, and the loads overlap its latency); f = inv in
+    // goes through LDS to the thread's consecutive positions, where the
+    // thread loads its pivots once more for a; the output returns through
+    // LDS to coalesced stores.
+    const int g0 = g * gmd::PSEG, g1 = nseg == 1 ? N : min(N, g0 + gmd::PSEG);
+    const int w0 = max(0, g0 - gmd::PHALO), w1 = nseg == 1 ? N : min(N, g1 + gmd::PHALO);
+    double2 r[E] = {}, v[E] = {}, p[E] = {};
+    C ivw[E];
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int idx = min(w0 + tid + T * e, w1 - 1);
-                const size_t q = base_j + idx;
-                // (the pivots, read by every sweep, stay cacheable)
-                if constexpr (WCOL) {
-                    r[e] = vcol(idx);
-                } else {
-                    r[e] = ntld(Rin + q);
-                    v[e] = ntld(a.V + q);
-                    if (VAR == PV_P && !first) p[e] = ntld(Pin + q);
-                }
-                if constexpr (F32) iv[e] = a.inv32[q];
-                else iv[e] = a.inv[q];
-            }
+    for (int e = 0; e < E; ++e) {
+        const int idx = min(w0 + tid + T * e, w1 - 1);
+        const size_t q = base_j + idx;
+        // (the pivots, read by every sweep, stay cacheable)
+        if constexpr (WCOL) {
+            r[e] = vcol(idx);
+        } else {
+            r[e] = ntld(Rin + q);
+            v[e] = ntld(a.V + q);
+            if (VAR == PV_P && !first) p[e] = ntld(Pin + q);
+        }
+        if constexpr (F32) ivw[e] = a.inv32[q];
+        else ivw[e] = a.inv[q];
+    }
+    if (!whole) {
+        __shared__ C sf[TE + TE / E];
+        {
+            const C (&iv)[E] = ivw;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int pos = tid + T * e, idx = w0 + pos;

@@ -55,7 +55,7 @@ struct FactorArgs {
     double2 *inv;
     float2 *inv32;        // FP32 copy of the pivots, or nullptr
     int *failed;
-    int *pseq;            // per mode and segment boundary (K nseg): 1 when a halo is too short (the row is solved whole)
+    int *pseq;            // per mode and segment boundary (K nseg): 1 when a halo is too short; then per mode (K): solve whole
 };
 
 struct PrecArgs {
@@ -67,7 +67,7 @@ struct PrecArgs {
     const double2 *R0;   // unnormalised shadow residual: r = R0 / |rhs| at the first iteration
     double2 *Pv;         // PV_P: p in the first buffer (Krylov pb = 0); PV_S: s (written)
     double2 *Pv2;        // PV_P: the second p buffer (the update reads one buffer and writes the other)
-    const int *pseq;     // per mode and segment boundary (K nseg): a halo check failed, solve the row whole
+    const int *pseq;     // K nseg boundary flags, then per mode (K): solve the row whole
     int nseg;            // PV_P / PV_S: segments per mode (precond_nseg)
     double *wpart;       // PV_P / PV_S with wood: the partial projections v_a^H y (4 per holding block)
     const double2 *V;

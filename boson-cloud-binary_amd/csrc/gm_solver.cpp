@@ -166,7 +166,7 @@ struct gm_solver {
     double coupling_tol = 5.421010862427522e-20;  // 2^-64
     DevBuf<double2> dcap;
     DevBuf<double2> psi, b, x, rr, r0, p, p2, v, y, z, s, t, inv, tmp;
-    DevBuf<int> pseq;  // per mode and segment boundary: the halo check failed (the sweeps solve the row whole)
+    DevBuf<int> pseq;  // per mode and segment boundary: the halo check failed; then per mode: solve the row whole
     DevBuf<double> wpart;  // partial Woodbury projections of the p / s sweeps (4 per segment block)
     DevBuf<float2> inv32, y32;  // FP32 pivots and sweep scratch (mixed-precision preconditioner)
     DevBuf<double2> xt;         // phased column vectors of the coupling (FP64; FP32 uses the first half)
@@ -1356,7 +1356,7 @@ void alloc_device(gm_solver &S)
         if (bf->n != n) bf->alloc(n);
     const size_t nwp = static_cast<size_t>(4) * S.K * gmd::precond_nseg(S.Nr);
     if (S.wpart.n != nwp) S.wpart.alloc(nwp);
-    const size_t nflags = static_cast<size_t>(S.K) * gmd::precond_nseg(S.Nr);
+    const size_t nflags = static_cast<size_t>(S.K) * (gmd::precond_nseg(S.Nr) + 1);
     if (S.pseq.n != nflags) {
         S.pseq.alloc(nflags);
         HIPCK(hipMemsetAsync(S.pseq.p, 0, nflags * sizeof(int), S.st));
