@@ -225,7 +225,7 @@ void build_plan(BatchLane &B)
             std::vector<int> b(M, 0);
             bool any = false;
             for (int m = 0; m < M; ++m)
-                if ((a[m].inv32 != nullptr && a[m].Y32 != nullptr) == (f32 != 0)) {
+                if ((a[m].inv32 != nullptr) == (f32 != 0)) {
                     b[m] = gmk::precond_blocks(a[m].K, a[m].N);
                     any = true;
                 }

@@ -2091,7 +2091,7 @@ hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 
 hipError_t launch_precond(const PrecArgs &a, int var, hipStream_t st)
 {
-    const bool f32 = a.inv32 != nullptr && a.Y32 != nullptr && var != PV_WCOL;
+    const bool f32 = a.inv32 != nullptr && var != PV_WCOL;
     if (a.nseg != gmd::precond_nseg(a.N)) return hipErrorInvalidValue;
     const dim3 g(precond_blocks(a.K, a.N));
     if (var == PV_P && f32) hipLaunchKernelGGL((k_psweep<PV_P, float>), g, dim3(gmd::PT), 0, st, a);

@@ -78,8 +78,7 @@ struct PrecArgs {
     const gmd::StepParams *Ps;
     int *range;          // PV_WCOL: per column, atomically widened [lo, hi) of non-zero output nodes
     int lmax;            // Woodbury epilogue: mode -> m for the column rotation
-    const float2 *inv32; // FP32 pivots: with Y32, the p / s variants sweep in FP32 (mixed precision)
-    float2 *Y32;         // FP32 forward-sweep scratch
+    const float2 *inv32; // FP32 pivots: the p / s variants sweep in FP32 on them (mixed precision), or nullptr
     void *Xt;            // also write exp(i m theta) y here for the next apply (or nullptr)
     int xt_f32;          // Xt holds float2 (FP32 coupling) instead of double2
 };

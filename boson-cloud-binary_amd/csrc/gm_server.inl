@@ -50,14 +50,14 @@ struct ServeResult {
 };
 
 // Device bytes of one solver: 14 state-sized work vectors, the phased copy,
-// the FP32 pivots and sweep scratch, up to two full-length Woodbury columns,
+// the FP32 pivots, up to two full-length Woodbury columns,
 // the profiles (FP64 + FP32), the Gaunt tables, the U sums of the
 // polar-quadrature apply, the per-tile reduction rows and small buffers.
 double footprint_bytes(const gm_params &p)
 {
     const double K = (p.l_max + 1.0) * (p.l_max + 2.0) / 2.0, N = p.Nr;
     const double Ls = 2.0 * p.l_max + 1.0;
-    const double vectors = (14.0 + 1.0 + 1.0 + 2.0) * K * N * 16.0;
+    const double vectors = (14.0 + 1.0 + 0.5 + 2.0) * K * N * 16.0;
     const double profiles = 2.0 * Ls * N * (8.0 + 4.0);
     const double tables = 12.0 * K * K * (p.l_max + 1.0);
     const double usums = 8.0 * static_cast<double>(gmk::tapply_u_doubles(p.Nr, p.l_max));

@@ -168,7 +168,7 @@ struct gm_solver {
     DevBuf<double2> psi, b, x, rr, r0, p, p2, v, y, z, s, t, inv, tmp;
     DevBuf<int> pseq;  // per mode and segment boundary: the halo check failed; then per mode: solve the row whole
     DevBuf<double> wpart;  // partial Woodbury projections of the p / s sweeps (4 per segment block)
-    DevBuf<float2> inv32, y32;  // FP32 pivots and sweep scratch (mixed-precision preconditioner)
+    DevBuf<float2> inv32;  // FP32 pivots (mixed-precision preconditioner sweeps)
     DevBuf<double2> xt;         // phased column vectors of the coupling (FP64; FP32 uses the first half)
     DevBuf<double> partial, rowpart, diagout;
     DevBuf<gmk::SinkScratch> sinkscr;  // reduction scratch of the device apply_sink
@@ -1002,10 +1002,7 @@ gmk::PrecArgs iter_prec_args(gm_solver &S, bool wood, int var)
     gmk::PrecArgs pa = prec_args(S);
     pa.wood = wood;
     const int mixed = S.flags.precision == GM_PRECISION_MIXED;
-    if (mixed) {
-        pa.inv32 = S.inv32.p;
-        pa.Y32 = S.y32.p;
-    }
+    if (mixed) pa.inv32 = S.inv32.p;
     pa.Xt = krylov_xt(S);
     pa.xt_f32 = mixed;
     pa.R = S.rr.p;
@@ -1361,8 +1358,7 @@ void alloc_device(gm_solver &S)
         S.pseq.alloc(nflags);
         HIPCK(hipMemsetAsync(S.pseq.p, 0, nflags * sizeof(int), S.st));
     }
-    for (auto *bf : {&S.inv32, &S.y32})
-        if (bf->n != n) bf->alloc(n);
+    if (S.inv32.n != n) S.inv32.alloc(n);
     if (S.xt.n != n) S.xt.alloc(n);
     S.E.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
     S.E32.alloc(static_cast<size_t>(2) * (2 * S.l_max + 1) * S.Nr);
