@@ -763,6 +763,17 @@ __global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
     // interior rows 2 .. N - 2 apply the recurrence; the window's first node
     // and the boundary rows reset the pivot to the diagonal
     auto interior = [&](int i) { return i >= 2 && i <= N - 2 && i != w0; };
+    // the window's diagonals, computed at consecutive nodes per thread index
+    // (coalesced reads of the radial profiles) and handed over through LDS
+    // (window node q at sdv[q + q / E], so that the E consecutive nodes of a
+    // thread sit at a stride of E + 1 entries)
+    __shared__ double2 sdv[T * E + T];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int q = e * T + tid, i = w0 + q;
+        if (i < w1) sdv[q + q / E] = cn_diag(f, Gjj, l, i);
+    }
+    __syncthreads();
     double2 dv[E];
     Mob loc;
     loc.a = make_double2(1.0, 0.0);
@@ -775,7 +786,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
         if (i >= w1) {
             dv[e] = make_double2(1.0, 0.0);
         } else {
-            dv[e] = cn_diag(f, Gjj, l, i);
+            dv[e] = sdv[tid * (E + 1) + e];
             loc = interior(i) ? mob_interior(loc, dv[e], nl2) : mob_reset(loc, dv[e]);
         }
     }
@@ -795,8 +806,9 @@ __global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
         if (i >= w1) continue;
         double2 piv = dv[e];
         if (interior(i)) piv = csub(piv, cmul(cmul(ell, prev_inv), ell));
-        const double ap = sqrt(cnorm2(piv));
-        if (ap == 0.0 || !isfinite(ap)) {
+        // |piv|^2 is zero or not finite exactly when |piv| is
+        const double ap2 = cnorm2(piv);
+        if (ap2 == 0.0 || !isfinite(ap2)) {
             failed = 1;
             iv[e] = make_double2(0.0, 0.0);
         } else {
@@ -817,11 +829,12 @@ __global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int i = i0 + e;
-        if (i >= w1) continue;
+        // only the nodes within PHALO of b enter the products
+        if (i >= w1 || i < b - gmd::PHALO + 1 || i > b + gmd::PHALO - 1) continue;
         // a_i = 0 on the boundary rows (the product is then zero)
         const double t = (i >= 1 && i <= N - 2) ? lell + 0.5 * log(cnorm2(iv[e])) : -INFINITY;
-        if (i >= b - gmd::PHALO + 1 && i <= b) sl += t;
-        if (i >= b && i <= b + gmd::PHALO - 1) sr += t;
+        if (i <= b) sl += t;
+        if (i >= b) sr += t;
     }
     __shared__ double hs[T / 64][2];
     sl = wave_sum(sl);
