@@ -621,8 +621,17 @@ __device__ __forceinline__ double2 cn_diag(const FactorArgs &f, const double *Gj
     if (i > 0 && i < N - 1) {
         const double ri = f.eps + i * f.dr;
         const double alpha2 = 1.0 / (f.dr * f.dr);
+        // sum_s G_jj[s] E_2s(r_i) in order of s, the profile loads issued
+        // eight at a time (one memory latency per eight terms)
         double v = 0.0;
-        for (int s = 0; s <= l; ++s) v += Gjj[s] * f.E[(size_t)(2 * s) * N + i];
+        for (int s0 = 0; s0 <= l; s0 += 8) {
+            double ev[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ev[u] = s0 + u <= l ? f.E[(size_t)(2 * (s0 + u)) * N + i] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (s0 + u <= l) v += Gjj[s0 + u] * ev[u];
+        }
         h.x = -(-2.0 * alpha2 - (double)(l * (l + 1)) / (ri * ri)) / (2 * f.mu) + v;
     }
     if (f.cap) h = cadd(h, f.cap[i]);
@@ -692,11 +701,11 @@ __device__ __forceinline__ Mob shfl_mob(const Mob &v, int src)
     return r;
 }
 
-// Block exclusive scan of Moebius maps in increasing thread order; returns
-// the thread's prefix and leaves the block total in `tot`.
+// Block exclusive scan of Moebius maps in increasing thread order (T
+// threads); returns the thread's prefix and leaves the block total in `tot`.
+template <int T>
 __device__ __forceinline__ Mob mob_block_scan(const Mob &loc, Mob &tot)
 {
-    constexpr int T = gmd::PT;
     __shared__ Mob wt[T / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     Mob inc = loc;
@@ -744,10 +753,15 @@ __device__ __forceinline__ Mob mob_block_scan(const Mob &loc, Mob &tot)
 // carries. From the window's pivots on [b - PHALO + 1, b + PHALO - 1] (the
 // segment's own and its right halo's, which follow from the segment with an
 // exact carry) the block also checks boundary b = g1 for the sweeps: flag
-// pseq[j nseg + g] unless both halo products are below 2^-56.
-__global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
+// pseq[j nseg + g] unless both halo products are below 2^-56. FT threads
+// take the window's PT x PE nodes, FE consecutive ones each (fewer threads
+// with more nodes each spend less of the block on the scan, whose cost is
+// per thread).
+constexpr int FT = 256, FE = gmd::PT * gmd::PE / FT;
+static_assert(FT * FE == gmd::PT * gmd::PE, "the factor window covers the sweeps' window");
+__global__ __launch_bounds__(FT) void k_factor_win(FactorArgs f)
 {
-    constexpr int T = gmd::PT, E = gmd::PE;
+    constexpr int T = FT, E = FE;
     const int N = f.N, nseg = gmd::precond_nseg(N);
     const int j = blockIdx.x / nseg, g = blockIdx.x - j * nseg;
     const int tid = threadIdx.x;
@@ -792,7 +806,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor_win(FactorArgs f)
     }
     mob_rescale(loc);
     Mob tot;
-    const Mob P = mob_block_scan(loc, tot);
+    const Mob P = mob_block_scan<T>(loc, tot);
     // incoming pivot: P applied to (1, 1) (the window starts with a reset, so
     // the value entering it is arbitrary)
     const double2 u0 = cadd(P.a, P.b), v0 = cadd(P.c, P.d);
@@ -910,7 +924,7 @@ __global__ __launch_bounds__(gmd::PT) void k_factor(FactorArgs f)
         mob_rescale(loc);
         // block exclusive scan of the thread maps (increasing node order)
         Mob tot;
-        const Mob P = mob_block_scan(loc, tot);
+        const Mob P = mob_block_scan<T>(loc, tot);
         // incoming pivot of this thread: P applied to (carry, 1)
         const double2 u = cadd(cmul(P.a, carry), P.b), v = cadd(cmul(P.c, carry), P.d);
         double2 prev_inv = cdiv(make_double2(1.0, 0.0), cdiv(u, v));
@@ -2097,7 +2111,7 @@ hipError_t launch_factor(const FactorArgs &f, hipStream_t st)
 {
     // the segments' pivots and checks, then the whole-row factor of the
     // flagged modes (without segment flags: every mode whole)
-    if (f.pseq) hipLaunchKernelGGL(k_factor_win, dim3(f.K * gmd::precond_nseg(f.N)), dim3(gmd::PT), 0, st, f);
+    if (f.pseq) hipLaunchKernelGGL(k_factor_win, dim3(f.K * gmd::precond_nseg(f.N)), dim3(FT), 0, st, f);
     hipLaunchKernelGGL(k_factor, dim3(f.K), dim3(gmd::PT), 0, st, f);
     return hipGetLastError();
 }
